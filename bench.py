@@ -1,0 +1,167 @@
+#!/usr/bin/env python3
+"""Benchmark of the forward time-stepping hot path on one MI355X.
+
+Metric: Mpt-updates/s per RK step = nx*ny*nz*steps / wall-seconds, one step
+being the reference's full 5-stage LDDRK STEP (5 RHS stages, a += dt*scr,
+z filter, periodic spectral filter; step.f90:43-128), on the 256x256x300
+polytrope box (200 Mm, z in [-12, +0.5] Mm, PML 10+10, Gaussian x Ricker
+source, option 4 = MP_QUIET_PML_3D_DISPL with damping), FP32 state.
+Inputs are resident in HBM before the timed region.
+
+Prints ONE JSON line.  Besides the metric it carries:
+  roofline      the dominant kernel class, its algorithmic bytes per launch /
+                its mean launch time (HIP events on the engine's stream,
+                recorded over a second pass of the same steps), vs 8 TB/s;
+  cpu_baseline  the FP64 CPU oracle (a C restatement of the reference, OpenMP)
+                on a bounded sample of the same workload (one RK step).
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=100)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--workload", default="box2")
+    ap.add_argument("--precision", default="fp32", choices=("fp32", "fp64"))
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--traffic-json", default=os.path.join(ROOT, "profiles", "traffic.json"),
+                    help="HBM bytes per launch measured by rocprofv3 --pmc (optional)")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only (this build has no multi-GPU path); "
+              f"--gpus {args.gpus} is not supported", file=sys.stderr)
+        return 2
+
+    import numpy as np
+
+    from sparc_amd import abi
+    from sparc_amd import driver as D
+
+    prec = abi.FP32 if args.precision == "fp32" else abi.FP64
+    W = D.workload(args.workload, precision=prec)
+    P = W.params
+    S = D.setup_for(W)
+    total = args.warmup + args.steps
+    eng, cfg = D.make_engine(W, S, nsteps=max(total, W.nsteps))
+    mask, series = D.forward_source(W, max(total, W.nsteps) + 8)
+    eng.set_source(abi.SRC_SERIES, mask, series, 10 ** 9)
+    N = P.nx * P.ny * P.nz
+    print(f"[bench] workload={W.name} grid={P.nx}x{P.ny}x{P.nz} dt={P.timestep:.4f}s "
+          f"precision={args.precision} warmup={args.warmup} steps={args.steps}",
+          file=sys.stderr, flush=True)
+
+    eng.step(0, args.warmup)
+    eng.sync()
+    t0 = time.perf_counter()
+    eng.step(args.warmup, args.steps)
+    eng.sync()
+    t1 = time.perf_counter()
+    wall = t1 - t0
+    mpts = N * args.steps / wall / 1e6
+    finite = bool(np.isfinite(eng.norm(abi.V_Z)))
+    print(f"[bench] timed {args.steps} steps in {wall:.3f}s -> {mpts:.1f} Mpt/s", file=sys.stderr,
+          flush=True)
+
+    # second pass of the same steps with HIP-event brackets per launch
+    eng.profile(True)
+    eng.step(args.warmup + args.steps, args.steps)
+    prof = eng.profile_read()
+    eng.profile(False)
+    kernels = {k: {"ms_total": v[0], "launches": v[2],
+                   "us_per_launch": 1e3 * v[0] / v[2] if v[2] else None,
+                   "gbs": v[1] / (v[0] * 1e-3) / 1e9 if v[0] > 0 else None}
+               for k, v in prof.items()}
+    dom = max(prof, key=lambda k: prof[k][0])
+    ms, by, nl = prof[dom]
+    achieved = by / (ms * 1e-3) / 1e9
+    traffic = None
+    try:
+        with open(args.traffic_json) as f:
+            tj = json.load(f)
+        ent = tj.get(f"{W.name}:{args.precision}:{dom}")
+        if ent:
+            traffic = ent["hbm_bytes_per_launch"]
+    except (OSError, ValueError):
+        pass
+    roof = {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1),
+            "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+            "traffic": traffic, "algorithmic_bytes_per_launch": by / nl,
+            "us_per_launch": 1e3 * ms / nl, "kernels": kernels}
+    step_algo = (18 * 6 + 5 * 5) * (4 if prec == abi.FP32 else 8)  # bytes/pt-step, 5 3-D bg
+    eng.close()
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        cpu = cpu_baseline(W)
+
+    line = {
+        "metric": "Mpt-updates/s per RK step (forward, option 4, 256x256x300)",
+        "value": round(mpts, 2),
+        "unit": "Mpt/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(1e3 * wall / args.steps, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32" if prec == abi.FP32 else "f64",
+        "data": "synthetic (polytrope background, Gaussian x Ricker source)",
+        "config": {"workload": f"{W.name}: {P.nx}x{P.ny}x{P.nz} polytrope box, "
+                               f"{P.xlength / 1e8:.0f} Mm, option 4 with PML 10+10, damping, "
+                               f"filters, dt={P.timestep:.4f}s",
+                   "nx": P.nx, "ny": P.ny, "nz": P.nz, "parallelism": "single GPU",
+                   "finite": finite,
+                   "step_roofline_frac_532B": round(mpts * 1e6 * step_algo / 1e9 / HBM_PEAK_GBS,
+                                                    4)},
+        "roofline": roof,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line))
+    return 0
+
+
+def cpu_baseline(W):
+    """FP64 CPU oracle (C restatement of the reference step, OpenMP) timed on
+    one RK step of the same workload."""
+    try:
+        from oracle import oracle as O
+        from sparc_amd import abi
+        from sparc_amd import driver as D
+        S = D.setup_for(W)
+        cfg = D.make_config(W.params, S, abi.FORWARD, W.nsteps)
+        cfg.precision = abi.FP64
+        o = O.Oracle(cfg)
+        D.load_setup(o, S)
+        mask, series = D.forward_source(W, W.nsteps + 8)
+        o.set_source(abi.SRC_SERIES, mask, series, 10 ** 9)
+        o.step(0, 1)  # warm (time 0 also runs the spectral filter)
+        t0 = time.perf_counter()
+        o.step(1, 1)
+        dt = time.perf_counter() - t0
+        o.close()
+        P = W.params
+        threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+        return {"value": round(P.nx * P.ny * P.nz / dt / 1e6, 4), "unit": "Mpt/s",
+                "cores": threads, "kind": "port",
+                "sample": f"1 RK step of {P.nx}x{P.ny}x{P.nz} (FP64, {dt:.1f} s)"}
+    except Exception as e:  # the baseline is reported, never required
+        return {"value": None, "unit": "Mpt/s", "cores": None, "kind": "port",
+                "sample": f"unavailable: {e}"}
+
+
+if __name__ == "__main__":
+    sys.exit(main())

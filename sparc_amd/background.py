@@ -1,0 +1,355 @@
+"""Host-side FP64 set-up of one solve: the work SPARC does once before its
+time loop (Initialize_all, solar_data, mp_Initialize_RHS[_2d], INIT_DAMPING,
+FORWARD_SOURCE).  The results are handed to the GPU engine through
+``sparc_set_background`` / ``sparc_set_profile`` exactly as the reference's
+module globals would be.
+
+Single-precision literals of the reference (no -r8, makefile:27) are kept
+through ``sp()`` so set-up values match the reference to the last bit where
+the operation order allows.
+"""
+from __future__ import annotations
+
+import math
+from dataclasses import dataclass, field
+
+import numpy as np
+
+
+def sp(x: float) -> float:
+    """A Fortran default-real literal stored into a REAL*8."""
+    return float(np.float32(x))
+
+
+PI_R = sp(3.14159265358979)      # initialize.f90:38
+RSUN_R = sp(69598946770.0)       # initialize.f90:38
+DIMC = 10.0 ** 6                 # solar_data, initialize.f90:751
+DIMRHO = sp(1e-3)                # 10.0**(-3.0), initialize.f90:755
+DIML = RSUN_R                    # initialize.f90:756
+DIMB = (4.0 * PI_R * DIMC ** 2.0 * DIMRHO) ** 0.5
+
+
+# --------------------------------------------------------------------------
+# DBYD2 (dbyd2.f90:8-237) + MTRIDAG ispecial=0 (mtridag.f90:94-111), vectorised
+# over the first index.  Used for set-up gradients only.
+# --------------------------------------------------------------------------
+def dbyd2(B: np.ndarray, ibc: int, A: np.ndarray | None = None) -> np.ndarray:
+    """Sixth-order compact derivative of B(n1, n2) along axis 1 (w.r.t.
+    index/(n2-1)); A carries caller boundary values for ibc 2, 3, 4."""
+    B = np.asarray(B, dtype=np.float64)
+    n1, n2 = B.shape
+    if not 0 <= ibc <= 4 or n2 < 5:
+        raise ValueError("dbyd2: ibc must be 0..4 and n2 >= 5")
+    A = np.zeros_like(B) if A is None else np.array(A, dtype=np.float64)
+    d2i = float(n2 - 1)
+    C1, C2 = d2i * 7.0 / 9.0, d2i / 36.0
+    A[:, 2:n2 - 2] = C1 * (B[:, 3:n2 - 1] - B[:, 1:n2 - 3]) + C2 * (B[:, 4:n2] - B[:, 0:n2 - 4])
+    c1, c2, c3 = sp(-43. / 96.) * d2i, sp(-5. / 6.) * d2i, 9. / 8. * d2i
+    c4, c5 = sp(1. / 6.) * d2i, sp(-1. / 96.) * d2i
+    A[:, 1] = c1 * B[:, 0] + c2 * B[:, 1] + c3 * B[:, 2] + c4 * B[:, 3] + c5 * B[:, 4]
+    A[:, n2 - 2] = -(c1 * B[:, n2 - 1] + c2 * B[:, n2 - 2] + c3 * B[:, n2 - 3]
+                     + c4 * B[:, n2 - 4] + c5 * B[:, n2 - 5])
+    b1, b2, b3, b4 = sp(-10 / 3.) * d2i, -3.0 * d2i, 6. * d2i, sp(1. / 3.) * d2i
+    if ibc == 0:
+        A[:, 0] = 0.0
+        A[:, n2 - 1] = 0.0
+    if ibc in (1, 3):
+        A[:, 0] = b1 * B[:, 0] + b2 * B[:, 1] + b3 * B[:, 2] + b4 * B[:, 3]
+    if ibc in (1, 2):
+        A[:, n2 - 1] = -(b1 * B[:, n2 - 1] + b2 * B[:, n2 - 2] + b3 * B[:, n2 - 3]
+                         + b4 * B[:, n2 - 4])
+    alpha = sp(1. / 3.)
+    alpha2, gamma2, alpha1, beta1 = 3. / 4., 1. / 8., 6.0, 3.
+    cst = 1. / (alpha2 - beta1 * gamma2)
+    up1 = (alpha1 * alpha2 - beta1) * cst
+    if ibc % 2:
+        A[:, 0] = (alpha2 * A[:, 0] - beta1 * A[:, 1]) * cst
+    if ibc in (1, 2):
+        A[:, n2 - 1] = (alpha2 * A[:, n2 - 1] - beta1 * A[:, n2 - 2]) * cst
+    uppr = np.zeros(n2)
+    uppr[2:n2 - 2] = alpha
+    uppr[0] = 0.0 if ibc % 2 == 0 else up1
+    uppr[1] = alpha2
+    uppr[n2 - 2] = gamma2
+    lowr = uppr[::-1].copy()
+    lowr[n2 - 1] = up1 if ibc in (1, 2) else 0.0
+    # MTRIDAG ispecial=0
+    betad = np.empty(n2)
+    gam = np.zeros(n2)
+    betad[0] = 1.0
+    for j in range(1, n2):
+        gam[j] = uppr[j - 1] / betad[j - 1]
+        betad[j] = 1.0 - lowr[j] * gam[j]
+    U = np.empty_like(A)
+    U[:, 0] = A[:, 0] / betad[0]
+    for j in range(1, n2):
+        U[:, j] = (A[:, j] - lowr[j] * U[:, j - 1]) / betad[j]
+    for j in range(n2 - 2, -1, -1):
+        U[:, j] = U[:, j] - gam[j + 1] * U[:, j + 1]
+    return U
+
+
+def ddz(f: np.ndarray, stretch: np.ndarray, ibc: int = 1) -> np.ndarray:
+    """ddz (derivatives.f90:10-24) of a (nx, ny, nz) field."""
+    nx, ny, nz = f.shape
+    d = dbyd2(f.reshape(nx * ny, nz, order="F"), ibc)
+    return (d * stretch[None, :]).reshape((nx, ny, nz), order="F")
+
+
+# --------------------------------------------------------------------------
+# Background model
+# --------------------------------------------------------------------------
+M_CM = 1.0e8  # 1 Mm in cm
+
+
+def polytrope_table(nz: int, z_top_mm: float, z_bot_mm: float, d_top_mm: float = 0.5,
+                    m: float = 1.5, gamma1: float = 5.0 / 3.0, g: float = 2.775841e4,
+                    rho_ref: float = 2.78e-7, d_ref_cm: float = 3.92e7) -> np.ndarray:
+    """Adiabatic polytrope with temperature linear in depth, in the column
+    format of the reference's ``polytrope`` file (solar_data,
+    initialize.f90:735-769): z/Rsun, c, rho, p, g (cgs), Gamma_1; row 1 at the
+    bottom.
+
+    Depth d = z_s - z below the T=0 level z_s = z_top + d_top.  Hydrostatic
+    balance gives rho = rho_ref (d/d_ref)^m, p = rho g d/(m+1),
+    c^2 = Gamma_1 p / rho.  The grid is uniform in sqrt(d), i.e. uniform in
+    acoustic travel time, as in the shipped ``polytrope`` (its c decreases by
+    a constant step per row).  rho_ref and d_ref are the shipped file's top
+    row (rho = 2.78e-7 g/cc where c^2 (m+1)/(Gamma_1 g) = 3.92e7 cm).
+    """
+    zs = (z_top_mm + d_top_mm) * M_CM
+    d_top = d_top_mm * M_CM
+    d_bot = zs - z_bot_mm * M_CM
+    s = np.linspace(math.sqrt(d_bot), math.sqrt(d_top), nz)
+    d = s * s
+    z_cm = zs - d
+    rho = rho_ref * (d / d_ref_cm) ** m
+    p = rho * g * d / (m + 1.0)
+    c = np.sqrt(gamma1 * p / rho)
+    tab = np.empty((nz, 6))
+    tab[:, 0] = 1.0 + z_cm / RSUN_R
+    tab[:, 1] = c
+    tab[:, 2] = rho
+    tab[:, 3] = p
+    tab[:, 4] = g
+    tab[:, 5] = gamma1
+    return tab
+
+
+def solar_data(tab: np.ndarray) -> dict:
+    """Nondimensionalisation of solar_data (initialize.f90:728-774)."""
+    z = tab[:, 0].copy()
+    c = tab[:, 1] / DIMC
+    return {
+        "z": z,
+        "c2": (tab[:, 1] / DIMC) ** 2.0,
+        "c": c,
+        "rho0": tab[:, 2] / DIMRHO,
+        "p0": tab[:, 3] / (DIMRHO * DIMC ** 2.0),
+        "g": tab[:, 4] * DIML / DIMC ** 2.0,
+        "gamma": tab[:, 5].copy(),
+    }
+
+
+def nearest_index(z: np.ndarray, target: float) -> int:
+    """1-based index of the last grid point nearest ``target``
+    (physics.f90:44-63: a scan with ``.LE.`` keeps the last tie)."""
+    temp = 1.0
+    best = 1
+    for k in range(z.size):
+        if abs(z[k] - target) <= temp:
+            temp = abs(z[k] - target)
+            best = k + 1
+    return best
+
+
+def init_rhs(z, c2, rho0, p0, g, npmlbot: int, npmltop: int, is2d: bool) -> dict:
+    """Derived background of mp_Initialize_RHS (physics.f90:204-303) or
+    mp_Initialize_RHS_2d (physics2d.f90:235-342), PML profiles included.
+    c2, rho0, p0 are (nx, ny, nz); z, g are (nz,)."""
+    nx, ny, nz = c2.shape
+    unstretch = dbyd2(np.asarray(z)[None, :], 1)[0]
+    stretch = 1.0 / unstretch
+    rhoinv = 1.0 / rho0
+    c2rho0 = c2 * rho0
+    gradrho0_z = ddz(rho0, stretch, 1)
+    gradp0_z = ddz(p0, stretch, 1)
+    g = np.array(g, dtype=np.float64)
+    tempin = c2 ** 0.5
+    dampz = np.zeros_like(c2)
+    pd = PI_R * sp(0.005) * DIML / DIMC
+    alphaz = np.full(nz, pd)
+    botfac = 3. * 3. if is2d else 2. * 3. * 3.
+    ktop = nz - 6 if is2d else nz - 1
+    for k1 in range(1, nz + 1):
+        k = k1 - 1
+        if k1 < npmlbot + 1:
+            L = z[npmlbot - 1] - z[0]
+            alphaz[k] = PI_R * sp(0.005) * DIML / DIMC * (z[k] - z[0]) / L
+            dampz[:, :, k] = botfac * tempin[:, :, 0] / L * ((z[npmlbot - 1] - z[k]) / L) ** 2.0
+            fd = alphaz[k] / (dampz[:, :, k] + alphaz[k])
+            gradp0_z[:, :, k] = gradp0_z[:, :, k] * fd
+            gradrho0_z[:, :, k] = gradrho0_z[:, :, k] * fd
+            g[k] = g[k] * fd[0, 0]
+        if k1 > nz - npmltop:
+            L = z[nz - 1] - z[nz - npmltop]
+            alphaz[k] = PI_R * sp(0.005) * DIML / DIMC * (z[nz - 1] - z[k]) / L
+            dampz[:, :, k] = 3. * 3. * tempin[:, :, ktop] / L * ((z[nz - npmltop] - z[k]) / L) ** 2.0
+            fd = alphaz[k] / (dampz[:, :, k] + alphaz[k])
+            gradp0_z[:, :, k] = gradp0_z[:, :, k] * fd
+            gradrho0_z[:, :, k] = gradrho0_z[:, :, k] * fd
+            g[k] = g[k] * fd[0, 0]
+    bzpml = -alphaz[None, None, :] - dampz
+    az = -dampz
+    return dict(unstretch=unstretch, stretch=stretch, rhoinv=rhoinv, c2rho0=c2rho0,
+                gradrho0_z=gradrho0_z, gradp0_z=gradp0_z, g=g, az=az, bzpml=bzpml)
+
+
+def damping_rates(nx: int, ny: int, xlength: float, ylength: float, is2d: bool) -> np.ndarray:
+    """INIT_DAMPING (damping.f90:10-65) / INIT_DAMPING_2D (71-120):
+    rates((nx/2+1), ny) with normx*normy folded."""
+    constx = 2. * PI_R / xlength * RSUN_R
+    consty = 2. * PI_R / ylength * RSUN_R
+    i = np.arange(nx // 2 + 1, dtype=np.float64)
+    kay = np.zeros((nx // 2 + 1, ny))
+    if is2d:
+        kay[:, 0] = (constx ** 2. * i ** 2. + consty ** 2. * 0.0 ** 2.) ** 0.5
+    else:
+        for j in range(1, ny + 1):
+            jj = (j - 1.0) if j <= ny // 2 + 1 else (ny - j + 1.0)
+            kay[:, j - 1] = (constx ** 2. * i ** 2. + consty ** 2. * jj ** 2.) ** 0.5
+    kayref = 902.0
+    normx, normy = 1.0 / nx, 1.0 / ny
+    return DIML / DIMC * sp(1e-6) * 150.0 * (kay / kayref) ** sp(2.2) * normx * normy
+
+
+def decay_coeff(n: int) -> np.ndarray:
+    """decay_coeff_x (initialize.f90:215-221): evaluated in single precision
+    (``i - n/3.0`` and ``exp`` of a default real), times normx."""
+    i = np.arange(1, n // 2 + 2, dtype=np.float32)
+    t = np.float32(1.0) / (np.float32(1.0) + np.exp((i - np.float32(n) / np.float32(3.0))
+                                                     * np.float32(1.0)))
+    return t.astype(np.float64) * (1.0 / n)
+
+
+def fwdsource_reference(nt: int, timestep: float, nupeak: float = sp(0.0017),
+                        nuwidth: float = sp(0.0055)) -> np.ndarray:
+    """FORWARD_SOURCE (driver.f90:1046-1071): 1e-5 cos(2 pi nu t)
+    exp(-2 (t - 0.5/nu)^2 w^2), zeroed below 1e-5 before scaling."""
+    t = (np.arange(1, nt + 1) - 1.0) * timestep
+    f = np.cos(2. * PI_R * nupeak * t) * np.exp(-2. * (t - 0.5 / nupeak) ** 2. * nuwidth ** 2.)
+    f[np.abs(f) < sp(0.00001)] = 0.0
+    return f * 10. ** (-5.)
+
+
+def ricker_source(nt: int, timestep: float, nu: float = 3.0e-3, t0: float | None = None,
+                  amp: float = 1e-5) -> np.ndarray:
+    """Ricker wavelet peaked at ``nu`` (Hz), sampled at t = (k-1)*timestep,
+    with the reference's convention of zeroing |f| < 1e-5 before scaling by
+    1e-5 (driver.f90:1066-1068)."""
+    if t0 is None:
+        t0 = 1.2 / nu
+    t = (np.arange(1, nt + 1) - 1.0) * timestep
+    a = (np.pi * nu * (t - t0)) ** 2
+    f = (1.0 - 2.0 * a) * np.exp(-a)
+    f[np.abs(f) < 1e-5] = 0.0
+    return f * amp
+
+
+def gaussian_mask(nx: int, ny: int, xlength: float, ylength: float, x0_mm: float, y0_mm: float,
+                  sigma_mm: float) -> np.ndarray:
+    """Horizontal Gaussian exp(-r^2 / (2 sigma^2)) on the periodic grid
+    x = (i-1)/nx (initialize.f90:197-203), centred at (x0, y0) Mm measured from
+    the box centre like ``(x-0.5)*xlength`` (driver.f90:612)."""
+    x = ((np.arange(nx) / nx) - 0.5) * xlength / M_CM
+    y = ((np.arange(ny) / ny) - 0.5) * ylength / M_CM if ny > 1 else np.zeros(1)
+    dx = (x[:, None] - x0_mm)
+    dy = (y[None, :] - y0_mm)
+    return np.exp(-(dx ** 2 + dy ** 2) / (2.0 * sigma_mm ** 2))
+
+
+def sinc_mask_2d(nx: int, xlength: float, xloc_mm: float) -> np.ndarray:
+    """The 2-D forward source mask of DETERMINE_STATUS (driver.f90:608-624):
+    exp(-d^2/(400 dx^2)) sinc(0.43 pi d)."""
+    x = np.arange(nx) / nx
+    dx = (x[1] - x[0]) * xlength * 10. ** (-8)
+    xdist = (x - 0.5) * xlength * 10. ** (-8) - xloc_mm
+    with np.errstate(divide="ignore", invalid="ignore"):
+        sincx = np.sin(0.43 * PI_R * xdist) / (0.43 * PI_R * xdist)
+    sincx[np.argmin(np.abs(xdist))] = 1.0
+    return (np.exp(-xdist ** 2. / (400. * dx ** 2.)) * sincx)[:, None]
+
+
+@dataclass
+class Setup:
+    """Everything one solve needs, in reference units (nondimensional)."""
+    nx: int
+    ny: int
+    nz: int
+    option: int
+    xlength: float
+    ylength: float
+    timestep: float
+    npmlbot: int
+    npmltop: int
+    z: np.ndarray
+    c2: np.ndarray
+    rho0: np.ndarray
+    p0: np.ndarray
+    g_raw: np.ndarray
+    derived: dict = field(default_factory=dict)
+    rates: np.ndarray | None = None
+    decay_x: np.ndarray | None = None
+    decay_y: np.ndarray | None = None
+
+    @property
+    def deltat(self) -> float:
+        return self.timestep * DIMC / DIML  # initialize.f90:472
+
+    @property
+    def stretchx(self) -> float:
+        return RSUN_R / self.xlength  # initialize.f90:223
+
+    @property
+    def stretchy(self) -> float:
+        return RSUN_R / self.ylength
+
+
+def build_setup(tab: np.ndarray, nx: int, ny: int, option: int, xlength: float, ylength: float,
+                timestep: float, npmlbot: int, npmltop: int, c2_3d: np.ndarray | None = None,
+                rho0_3d: np.ndarray | None = None) -> Setup:
+    """solar_data + mp_Initialize_RHS[_2d] for a z-stratified model, with
+    optional 3-D c^2 / rho0 replacing the profiles (e.g. model_c, the
+    sound-speed perturbation of driver.f90:151-155)."""
+    nz = tab.shape[0]
+    sd = solar_data(tab)
+    shape = (nx, ny, nz)
+    c2 = np.broadcast_to(sd["c2"], shape).copy(order="F") if c2_3d is None else np.asfortranarray(c2_3d)
+    rho0 = np.broadcast_to(sd["rho0"], shape).copy(order="F") if rho0_3d is None else np.asfortranarray(rho0_3d)
+    p0 = np.broadcast_to(sd["p0"], shape).copy(order="F")
+    is2d = option == 17
+    der = init_rhs(sd["z"], c2, rho0, p0, sd["g"], npmlbot, npmltop, is2d)
+    S = Setup(nx=nx, ny=ny, nz=nz, option=option, xlength=xlength, ylength=ylength,
+              timestep=timestep, npmlbot=npmlbot, npmltop=npmltop, z=sd["z"], c2=c2,
+              rho0=rho0, p0=p0, g_raw=sd["g"], derived=der)
+    S.rates = damping_rates(nx, ny, xlength, ylength, is2d)
+    S.decay_x = decay_coeff(nx)
+    S.decay_y = decay_coeff(ny)
+    return S
+
+
+def is_uniform(f: np.ndarray) -> bool:
+    return bool(np.all(f == f[:1, :1, :]))
+
+
+def background_arrays(S: Setup) -> dict:
+    """Background id -> array (compact z-profile when horizontally uniform)."""
+    from . import abi
+    d = S.derived
+    out = {}
+    for bid, arr in ((abi.BG_RHO0, S.rho0), (abi.BG_RHOINV, d["rhoinv"]),
+                     (abi.BG_C2RHO0, d["c2rho0"]), (abi.BG_GRADP0_Z, d["gradp0_z"]),
+                     (abi.BG_GRADRHO0_Z, d["gradrho0_z"]), (abi.BG_AZ, d["az"]),
+                     (abi.BG_BZPML, d["bzpml"]), (abi.BG_C2, S.c2)):
+        out[bid] = np.ascontiguousarray(arr[0, 0, :]) if is_uniform(arr) else arr
+    return out

@@ -1,0 +1,55 @@
+"""In-tree build of the HIP engine (gfx950) and, for the tests, the CPU oracle.
+
+``python -m sparc_amd.build`` compiles ``sparc_amd/libsparc_gpu.so`` with hipcc;
+``__graft_entry__.build()`` calls :func:`build_all`.
+"""
+from __future__ import annotations
+
+import os
+import shutil
+import subprocess
+import sys
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(HERE)
+HIPCC = shutil.which("hipcc") or "/opt/rocm/bin/hipcc"
+# packed FP32 instructions are kept out of the generated code
+HIPFLAGS = ["--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-shared",
+            "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
+SOURCES = [os.path.join(HERE, "csrc", "sparc_gpu.hip")]
+DEPS = SOURCES + [os.path.join(HERE, "csrc", f) for f in ("kernels.h", "fft_lds.h")] + [
+    os.path.join(ROOT, "include", "sparc_gpu.h")]
+
+
+def build_engine(verbose: bool = False) -> str:
+    out = os.path.join(HERE, "libsparc_gpu.so")
+    tmp = out + ".tmp"
+    cmd = [HIPCC, *HIPFLAGS, "-o", tmp, *SOURCES]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        sys.stderr.write(r.stdout + r.stderr)
+        raise RuntimeError("hipcc failed building libsparc_gpu.so")
+    if verbose:
+        sys.stderr.write(r.stderr)
+    os.replace(tmp, out)
+    return out
+
+
+def build_oracle() -> None:
+    """Builds oracle/liboracle.so and, when the reference tree is present,
+    oracle/_ref (test infrastructure only)."""
+    r = subprocess.run(["make", "-C", os.path.join(ROOT, "oracle"), "all"],
+                       capture_output=True, text=True)
+    if r.returncode != 0:
+        sys.stderr.write(r.stdout + r.stderr)
+        raise RuntimeError("oracle build failed")
+
+
+def build_all() -> None:
+    build_engine()
+    build_oracle()
+
+
+if __name__ == "__main__":
+    build_all()
+    print("built", os.path.join(HERE, "libsparc_gpu.so"))

@@ -1,0 +1,1178 @@
+// sparc_gpu.hip -- context, set-up and C ABI of the MI355X stepping engine.
+// See include/sparc_gpu.h for the entry points and the reference seams they
+// replace.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/sparc_gpu.h"
+#include "kernels.h"
+
+using namespace sparc;
+
+// single-precision literal of initialize.f90:38 (no -r8 in makefile:27)
+static const double PI_R = (double)(float)3.14159265358979;
+
+struct sparc_ctx {
+  sparc_config cfg{};
+  int fp32 = 0;
+  size_t ts = 8;
+  hipStream_t stream = nullptr;
+  int device = 0;
+  long N = 0, nxy = 0, NP = 0;
+  int nzpml = 0;
+  void* a[SPARC_NFIELDS] = {};
+  void* scr[SPARC_NFIELDS] = {};
+  void* pmlvars[2] = {};
+  void* scrpml[2] = {};
+  void* bg[SPARC_BG_COUNT] = {};
+  int bg3d[SPARC_BG_COUNT] = {};
+  bool bgset[SPARC_BG_COUNT] = {};
+  void* prof[SPARC_PROF_COUNT] = {};
+  bool profset[SPARC_PROF_COUNT] = {};
+  std::vector<double> unstretch_h;
+  void* W[8] = {};
+  void* spec[3] = {};
+  void* twx = nullptr;
+  void* twy = nullptr;
+  void* dmulx = nullptr;
+  void* dmuly = nullptr;
+  void* fmulx = nullptr;
+  void* fmuly = nullptr;
+  void* rates = nullptr;
+  bool damping_set = false, filter_set = false;
+  void* tri[4] = {};  // per ibc 0..3: lowr | invb | gam, nz each
+  double tc[14] = {};
+  FFTPlan px{}, py{};
+  double betas[5] = {};
+  // source
+  int src_kind = SPARC_SRC_NONE, src_nt = 0, src_active_until = -1;
+  void* src_mask = nullptr;
+  std::vector<double> src_series;
+  void* forcing = nullptr;
+  // kernels
+  int nslots = 0;
+  void* snap = nullptr;
+  double snap_scale = 1.0;
+  double* kern[SPARC_KERN_COUNT] = {};
+  double* grav = nullptr;
+  std::vector<void*> allocs;
+  size_t bytes = 0;
+  std::string err;
+  // instrumentation
+  bool profiling = false;
+  struct Rec {
+    int cls;
+    double bytes;
+    hipEvent_t e0, e1;
+  };
+  std::vector<Rec> recs;
+  std::vector<hipEvent_t> pool;
+  double pms[SPARC_KC_COUNT] = {}, pbytes[SPARC_KC_COUNT] = {};
+  int64_t plaunch[SPARC_KC_COUNT] = {};
+};
+
+// Event bracket around one launch (only while profiling).
+struct ProfScope {
+  sparc_ctx* c;
+  int idx = -1;
+  static hipEvent_t ev(sparc_ctx* c) {
+    if (!c->pool.empty()) {
+      hipEvent_t e = c->pool.back();
+      c->pool.pop_back();
+      return e;
+    }
+    hipEvent_t e = nullptr;
+    if (hipEventCreate(&e) != hipSuccess) return nullptr;
+    return e;
+  }
+  ProfScope(sparc_ctx* cc, int cls, double bytes) : c(cc) {
+    if (!c->profiling) return;
+    sparc_ctx::Rec r{cls, bytes, ev(c), ev(c)};
+    if (!r.e0 || !r.e1) return;
+    (void)hipEventRecord(r.e0, c->stream);
+    c->recs.push_back(r);
+    idx = (int)c->recs.size() - 1;
+  }
+  ~ProfScope() {
+    if (idx >= 0) (void)hipEventRecord(c->recs[idx].e1, c->stream);
+  }
+};
+
+static int fail(sparc_ctx* c, int code, const char* fmt, ...) {
+  if (c) {
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof(buf), fmt, ap);
+    va_end(ap);
+    c->err = buf;
+  }
+  return code;
+}
+
+#define HIPCHK(ctx, call)                                                             \
+  do {                                                                                \
+    hipError_t e_ = (call);                                                           \
+    if (e_ != hipSuccess)                                                             \
+      return fail(ctx, SPARC_EHIP, "%s failed: %s", #call, hipGetErrorString(e_));    \
+  } while (0)
+
+#define LAUNCHCHK(ctx)                                                                \
+  do {                                                                                \
+    hipError_t e_ = hipGetLastError();                                                \
+    if (e_ != hipSuccess)                                                             \
+      return fail(ctx, SPARC_EHIP, "kernel launch failed at %s:%d: %s", __FILE__,     \
+                  __LINE__, hipGetErrorString(e_));                                   \
+  } while (0)
+
+static int dalloc(sparc_ctx* c, void** p, size_t bytes) {
+  *p = nullptr;
+  if (bytes == 0) bytes = 16;
+  hipError_t e = hipMalloc(p, bytes);
+  if (e != hipSuccess) {
+    *p = nullptr;
+    return fail(c, SPARC_ENOMEM, "hipMalloc(%zu) failed: %s", bytes, hipGetErrorString(e));
+  }
+  c->allocs.push_back(*p);
+  c->bytes += bytes;
+  return SPARC_OK;
+}
+
+static bool make_plan(int n, FFTPlan* p) {
+  p->n = n;
+  p->nrad = 0;
+  int m = n;
+  while (m % 4 == 0 && p->nrad < 20) {
+    p->rad[p->nrad++] = 4;
+    m /= 4;
+  }
+  while (m % 2 == 0 && p->nrad < 20) {
+    p->rad[p->nrad++] = 2;
+    m /= 2;
+  }
+  while (m % 3 == 0 && p->nrad < 20) {
+    p->rad[p->nrad++] = 3;
+    m /= 3;
+  }
+  return m == 1;
+}
+
+// host double -> device T
+static int upload(sparc_ctx* c, void* dst, const double* h, size_t n) {
+  if (c->fp32) {
+    std::vector<float> t(n);
+    for (size_t i = 0; i < n; ++i) t[i] = (float)h[i];
+    HIPCHK(c, hipMemcpy(dst, t.data(), n * 4, hipMemcpyHostToDevice));
+  } else {
+    HIPCHK(c, hipMemcpy(dst, h, n * 8, hipMemcpyHostToDevice));
+  }
+  return SPARC_OK;
+}
+
+static int download(sparc_ctx* c, double* h, const void* src, size_t n, double scale) {
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  if (c->fp32) {
+    std::vector<float> t(n);
+    HIPCHK(c, hipMemcpy(t.data(), src, n * 4, hipMemcpyDeviceToHost));
+    for (size_t i = 0; i < n; ++i) h[i] = (double)t[i] * scale;
+  } else {
+    HIPCHK(c, hipMemcpy(h, src, n * 8, hipMemcpyDeviceToHost));
+    if (scale != 1.0)
+      for (size_t i = 0; i < n; ++i) h[i] *= scale;
+  }
+  return SPARC_OK;
+}
+
+static size_t spec_elems(const sparc_config* cfg) {
+  // complex elements of the x-spectrum of nz/2 plane pairs
+  return (size_t)((cfg->nz + 1) / 2) * cfg->nx * cfg->ny;
+}
+
+static int check_config(const sparc_config* cfg, std::string* why) {
+  char b[256];
+  if (cfg->abi_version != SPARC_ABI_VERSION) {
+    snprintf(b, sizeof b, "abi_version %d != %d", cfg->abi_version, SPARC_ABI_VERSION);
+    *why = b;
+    return SPARC_EINVAL;
+  }
+  if (cfg->option != SPARC_OPT_QUIET_PML_3D_DISPL && cfg->option != SPARC_OPT_QUIET_PML_2D_DISPL) {
+    snprintf(b, sizeof b, "option %d not supported (4 or 17)", cfg->option);
+    *why = b;
+    return SPARC_EINVAL;
+  }
+  if (cfg->precision != SPARC_FP64 && cfg->precision != SPARC_FP32) {
+    *why = "precision must be SPARC_FP64 or SPARC_FP32";
+    return SPARC_EINVAL;
+  }
+  if (cfg->nz < 12 || cfg->nz > 4096 || cfg->nx < 2 || cfg->ny < 1) {
+    *why = "grid out of range (nz in [12,4096], nx >= 2)";
+    return SPARC_EINVAL;
+  }
+  FFTPlan p;
+  if (!make_plan(cfg->nx, &p)) {
+    *why = "nx must be 2^a*3^b";
+    return SPARC_EINVAL;
+  }
+  if (cfg->option == SPARC_OPT_QUIET_PML_2D_DISPL && cfg->ny != 1) {
+    *why = "option 17 needs ny == 1";
+    return SPARC_EINVAL;
+  }
+  if (cfg->option == SPARC_OPT_QUIET_PML_3D_DISPL) {
+    if (!make_plan(cfg->ny, &p) || cfg->ny < 2) {
+      *why = "ny must be 2^a*3^b >= 2 for option 4";
+      return SPARC_EINVAL;
+    }
+    if (cfg->nx % 32 != 0) {
+      *why = "option 4 needs nx % 32 == 0";
+      return SPARC_EINVAL;
+    }
+  }
+  if (cfg->npmlbot < 0 || cfg->npmltop < 0 || cfg->npmlbot + cfg->npmltop > cfg->nz / 2) {
+    *why = "PML widths out of range";
+    return SPARC_EINVAL;
+  }
+  if (cfg->npmlbot == 1 || cfg->npmltop == 1) {
+    *why = "PML width 1 makes the profile length zero (z(npml)-z(1))";
+    return SPARC_EINVAL;
+  }
+  if (cfg->deltat <= 0) {
+    *why = "deltat must be > 0";
+    return SPARC_EINVAL;
+  }
+  if (cfg->precision == SPARC_FP64 && cfg->field_scale != 1.0 && cfg->field_scale != 0.0) {
+    *why = "field_scale is an FP32 option";
+    return SPARC_EINVAL;
+  }
+  return SPARC_OK;
+}
+
+static size_t footprint_bytes(const sparc_config* cfg) {
+  size_t ts = cfg->precision == SPARC_FP32 ? 4 : 8;
+  size_t N = (size_t)cfg->nx * cfg->ny * cfg->nz;
+  size_t nxy = (size_t)cfg->nx * cfg->ny;
+  size_t NP = nxy * (size_t)std::max(cfg->npmlbot + cfg->npmltop, 1);
+  size_t b = 0;
+  b += 12 * N * ts;                         // a, scr
+  b += 4 * NP * ts;                         // pmlvars, scrpml
+  b += SPARC_BG_COUNT * N * ts;             // backgrounds (upper bound: all 3-D)
+  b += 8 * N * ts;                          // work
+  if (cfg->option == SPARC_OPT_QUIET_PML_3D_DISPL && cfg->damp_waves)
+    b += 3 * spec_elems(cfg) * 2 * ts;      // damping spectra
+  b += (size_t)(cfg->nx + cfg->ny) * 6 * ts + 4 * 3 * cfg->nz * ts + SPARC_PROF_COUNT * cfg->nz * ts;
+  b += (size_t)(cfg->nx / 2 + 1) * cfg->ny * ts + 2 * nxy * ts;
+  return b;
+}
+
+extern "C" int sparc_footprint(const sparc_config* cfg, size_t* bytes) {
+  if (!cfg || !bytes) return SPARC_EINVAL;
+  std::string why;
+  if (check_config(cfg, &why) != SPARC_OK) return SPARC_EINVAL;
+  *bytes = footprint_bytes(cfg);
+  return SPARC_OK;
+}
+
+// MTRIDAG ispecial=0 factors of DBYD2's matrix for one ibc (dbyd2.f90:151-196,
+// mtridag.f90:94-99), FP64.
+static void tri_factors(int n2, int ibc, std::vector<double>& lowr, std::vector<double>& invb,
+                        std::vector<double>& gam) {
+  std::vector<double> uppr(n2, 0.0), diag(n2, 1.0);
+  lowr.assign(n2, 0.0);
+  invb.assign(n2, 0.0);
+  gam.assign(n2, 0.0);
+  const double alpha = (double)(float)(1. / 3.);
+  for (int j = 2; j < n2 - 2; ++j) uppr[j] = alpha;
+  const double alpha2 = 3. / 4., gamma2 = 1. / 8., alpha1 = 6.0, beta1 = 3.;
+  const double cst = 1. / (alpha2 - beta1 * gamma2);
+  const double up1 = (alpha1 * alpha2 - beta1) * cst;
+  uppr[0] = (ibc % 2 == 0) ? 0.0 : up1;
+  uppr[1] = alpha2;
+  uppr[n2 - 2] = gamma2;
+  for (int i = 0; i < n2; ++i) lowr[i] = uppr[n2 - 1 - i];
+  lowr[n2 - 1] = (ibc != 1 && ibc != 2) ? 0.0 : up1;
+  std::vector<double> betad(n2);
+  betad[0] = diag[0];
+  for (int j = 1; j < n2; ++j) {
+    gam[j] = uppr[j - 1] / betad[j - 1];
+    betad[j] = diag[j] - lowr[j] * gam[j];
+  }
+  for (int j = 0; j < n2; ++j) invb[j] = 1.0 / betad[j];
+}
+
+extern "C" int sparc_create(const sparc_config* cfg, sparc_ctx** out) {
+  if (!cfg || !out) return SPARC_EINVAL;
+  *out = nullptr;
+  sparc_ctx* c = new sparc_ctx();
+  c->cfg = *cfg;
+  if (c->cfg.field_scale == 0.0) c->cfg.field_scale = 1.0;
+  std::string why;
+  if (check_config(cfg, &why) != SPARC_OK) {
+    c->err = why;
+    *out = c;  // the caller can read the reason, then destroy
+    return SPARC_EINVAL;
+  }
+  *out = c;
+  c->fp32 = cfg->precision == SPARC_FP32;
+  c->ts = c->fp32 ? 4 : 8;
+  c->nxy = (long)cfg->nx * cfg->ny;
+  c->N = c->nxy * cfg->nz;
+  c->nzpml = cfg->npmlbot + cfg->npmltop;
+  c->NP = c->nxy * std::max(c->nzpml, 1);
+  HIPCHK(c, hipGetDevice(&c->device));
+  HIPCHK(c, hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+  const size_t ts = c->ts, N = c->N;
+  int rc;
+  for (int f = 0; f < SPARC_NFIELDS; ++f) {
+    if ((rc = dalloc(c, &c->a[f], N * ts))) return rc;
+    if ((rc = dalloc(c, &c->scr[f], N * ts))) return rc;
+  }
+  for (int f = 0; f < 2; ++f) {
+    if ((rc = dalloc(c, &c->pmlvars[f], c->NP * ts))) return rc;
+    if ((rc = dalloc(c, &c->scrpml[f], c->NP * ts))) return rc;
+  }
+  for (int w = 0; w < 8; ++w)
+    if ((rc = dalloc(c, &c->W[w], N * ts))) return rc;
+  if (cfg->option == SPARC_OPT_QUIET_PML_3D_DISPL && cfg->damp_waves)
+    for (int f = 0; f < 3; ++f)
+      if ((rc = dalloc(c, &c->spec[f], spec_elems(cfg) * 2 * ts))) return rc;
+  for (int p = 0; p < SPARC_PROF_COUNT; ++p)
+    if ((rc = dalloc(c, &c->prof[p], cfg->nz * ts))) return rc;
+  if ((rc = dalloc(c, &c->forcing, c->nxy * ts))) return rc;
+  // FFT plans, twiddles, derivative multipliers (initialize.f90:204-210)
+  make_plan(cfg->nx, &c->px);
+  make_plan(cfg->ny, &c->py);
+  auto twiddles = [&](int n, void** dst) -> int {
+    std::vector<double> t(2 * n);
+    for (int m = 0; m < n; ++m) {
+      double ang = -2.0 * M_PI * (double)m / (double)n;
+      t[2 * m] = cos(ang);
+      t[2 * m + 1] = sin(ang);
+    }
+    int r = dalloc(c, dst, 2 * n * ts);
+    if (r) return r;
+    return upload(c, *dst, t.data(), 2 * n);
+  };
+  auto dmult = [&](int n, double stretch, void** dst) -> int {
+    std::vector<double> t(n);
+    for (int m = 0; m < n; ++m) {
+      int ms = (m <= n / 2) ? m : m - n;
+      if (2 * m == n) ms = 0;
+      t[m] = (double)ms * 2.0 * PI_R * stretch / (double)n;
+    }
+    int r = dalloc(c, dst, n * ts);
+    if (r) return r;
+    return upload(c, *dst, t.data(), n);
+  };
+  if ((rc = twiddles(cfg->nx, &c->twx))) return rc;
+  if ((rc = twiddles(cfg->ny, &c->twy))) return rc;
+  if ((rc = dmult(cfg->nx, cfg->stretchx, &c->dmulx))) return rc;
+  if ((rc = dmult(cfg->ny, cfg->stretchy, &c->dmuly))) return rc;
+  if ((rc = dalloc(c, &c->fmulx, cfg->nx * ts))) return rc;
+  if ((rc = dalloc(c, &c->fmuly, cfg->ny * ts))) return rc;
+  if ((rc = dalloc(c, &c->rates, (size_t)(cfg->nx / 2 + 1) * cfg->ny * ts))) return rc;
+  // tridiagonal factors for ibc 0..3 and RHS coefficients (dbyd2.f90:88-146)
+  for (int ibc = 0; ibc < 4; ++ibc) {
+    std::vector<double> lowr, invb, gam;
+    tri_factors(cfg->nz, ibc, lowr, invb, gam);
+    std::vector<double> t(3 * cfg->nz);
+    for (int k = 0; k < cfg->nz; ++k) {
+      t[k] = lowr[k];
+      t[cfg->nz + k] = invb[k];
+      t[2 * cfg->nz + k] = gam[k];
+    }
+    if ((rc = dalloc(c, &c->tri[ibc], 3 * cfg->nz * ts))) return rc;
+    if ((rc = upload(c, c->tri[ibc], t.data(), 3 * cfg->nz))) return rc;
+  }
+  const double d2i = (double)(cfg->nz - 1);
+  const double alpha2 = 3. / 4., gamma2 = 1. / 8., beta1 = 3.;
+  double tcv[14] = {d2i * 7.0 / 9.0,
+                    d2i / 36.0,
+                    (double)(float)(-43. / 96.) * d2i,
+                    (double)(float)(-5. / 6.) * d2i,
+                    9. / 8. * d2i,
+                    (double)(float)(1. / 6.) * d2i,
+                    (double)(float)(-1. / 96.) * d2i,
+                    (double)(float)(-10 / 3.) * d2i,
+                    -3.0 * d2i,
+                    6. * d2i,
+                    (double)(float)(1. / 3.) * d2i,
+                    alpha2,
+                    beta1,
+                    1. / (alpha2 - beta1 * gamma2)};
+  memcpy(c->tc, tcv, sizeof(tcv));
+  // INITIALIZE_STEP (step.f90:24-41); optimals(3..5) are default-real literals
+  double optimals[5] = {1.0, 0.5, (double)(float)0.1665579725791151184,
+                        (double)(float)0.03950410250586729987,
+                        (double)(float)0.007810706393337838236};
+  c->betas[4] = optimals[1];
+  c->betas[3] = optimals[2] / c->betas[4];
+  c->betas[2] = optimals[3] / (c->betas[4] * c->betas[3]);
+  c->betas[1] = optimals[4] / (c->betas[4] * c->betas[3] * c->betas[2]);
+  c->betas[0] = 0.0;
+  for (int s = 0; s < 5; ++s) c->betas[s] *= cfg->deltat;
+  int r2 = sparc_reset(c);
+  if (r2) return r2;
+  c->err.clear();
+  return SPARC_OK;
+}
+
+extern "C" void sparc_destroy(sparc_ctx* c) {
+  if (!c) return;
+  if (c->stream) (void)hipStreamSynchronize(c->stream);
+  for (auto& r : c->recs) {
+    (void)hipEventDestroy(r.e0);
+    (void)hipEventDestroy(r.e1);
+  }
+  for (auto e : c->pool) (void)hipEventDestroy(e);
+  for (void* p : c->allocs)
+    if (p) (void)hipFree(p);
+  if (c->stream) (void)hipStreamDestroy(c->stream);
+  delete c;
+}
+
+extern "C" const char* sparc_last_error(const sparc_ctx* c) {
+  return c ? c->err.c_str() : "null context";
+}
+
+extern "C" int sparc_device_bytes(const sparc_ctx* c, size_t* bytes) {
+  if (!c || !bytes) return SPARC_EINVAL;
+  *bytes = c->bytes;
+  return SPARC_OK;
+}
+
+extern "C" int sparc_reset(sparc_ctx* c) {
+  if (!c || !c->stream) return SPARC_EINVAL;
+  for (int f = 0; f < SPARC_NFIELDS; ++f) {
+    HIPCHK(c, hipMemsetAsync(c->a[f], 0, c->N * c->ts, c->stream));
+    HIPCHK(c, hipMemsetAsync(c->scr[f], 0, c->N * c->ts, c->stream));
+  }
+  for (int f = 0; f < 2; ++f) {
+    HIPCHK(c, hipMemsetAsync(c->pmlvars[f], 0, c->NP * c->ts, c->stream));
+    HIPCHK(c, hipMemsetAsync(c->scrpml[f], 0, c->NP * c->ts, c->stream));
+  }
+  // W[1] is the y-derivative slot; it stays zero when the y part is absent
+  for (int w = 0; w < 8; ++w) HIPCHK(c, hipMemsetAsync(c->W[w], 0, c->N * c->ts, c->stream));
+  HIPCHK(c, hipMemsetAsync(c->forcing, 0, c->nxy * c->ts, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return SPARC_OK;
+}
+
+extern "C" int sparc_set_background(sparc_ctx* c, int id, const double* h, size_t n) {
+  if (!c || !h || id < 0 || id >= SPARC_BG_COUNT)
+    return fail(c, SPARC_EINVAL, "bad background id %d", id);
+  int is3d;
+  if (n == (size_t)c->N)
+    is3d = 1;
+  else if (n == (size_t)c->cfg.nz)
+    is3d = 0;
+  else
+    return fail(c, SPARC_EINVAL, "background %d: n=%zu is neither nx*ny*nz nor nz", id, n);
+  if (c->bg[id] && c->bg3d[id] != is3d) {
+    // shape changed: the old buffer stays in allocs, a new one is made
+    c->bg[id] = nullptr;
+  }
+  if (!c->bg[id]) {
+    int rc = dalloc(c, &c->bg[id], n * c->ts);
+    if (rc) return rc;
+  }
+  c->bg3d[id] = is3d;
+  c->bgset[id] = true;
+  return upload(c, c->bg[id], h, n);
+}
+
+extern "C" int sparc_set_profile(sparc_ctx* c, int id, const double* h, int n) {
+  if (!c || !h || id < 0 || id >= SPARC_PROF_COUNT || n != c->cfg.nz)
+    return fail(c, SPARC_EINVAL, "bad profile id %d or length %d", id, n);
+  c->profset[id] = true;
+  if (id == SPARC_PROF_UNSTRETCH) c->unstretch_h.assign(h, h + n);
+  return upload(c, c->prof[id], h, n);
+}
+
+extern "C" int sparc_set_damping(sparc_ctx* c, const double* r, size_t n) {
+  if (!c || !r || n != (size_t)(c->cfg.nx / 2 + 1) * c->cfg.ny)
+    return fail(c, SPARC_EINVAL, "damping rates need (nx/2+1)*ny values");
+  c->damping_set = true;
+  return upload(c, c->rates, r, n);
+}
+
+extern "C" int sparc_set_filter(sparc_ctx* c, const double* dx, const double* dy) {
+  if (!c || !dx || !dy) return fail(c, SPARC_EINVAL, "null filter");
+  // full-spectrum multipliers decay(min(m, n-m)); decay folds 1/n (initialize.f90:215-221)
+  auto mk = [&](int n, const double* d, void* dst) -> int {
+    std::vector<double> t(n);
+    for (int m = 0; m < n; ++m) t[m] = d[(m <= n / 2) ? m : n - m];
+    return upload(c, dst, t.data(), n);
+  };
+  int rc = mk(c->cfg.nx, dx, c->fmulx);
+  if (rc) return rc;
+  rc = mk(c->cfg.ny, dy, c->fmuly);
+  if (rc) return rc;
+  c->filter_set = true;
+  return SPARC_OK;
+}
+
+extern "C" int sparc_set_source(sparc_ctx* c, int kind, const double* mask, size_t nmask,
+                                const double* series, int nt, int active_until) {
+  if (!c) return SPARC_EINVAL;
+  c->src_kind = SPARC_SRC_NONE;
+  if (kind == SPARC_SRC_NONE) return SPARC_OK;
+  if (kind == SPARC_SRC_SERIES) {
+    if (!mask || nmask != (size_t)c->nxy || !series || nt < 1)
+      return fail(c, SPARC_EINVAL, "SERIES source needs mask(nx*ny) and a series");
+    int rc = dalloc(c, &c->src_mask, c->nxy * c->ts);
+    if (rc) return rc;
+    if ((rc = upload(c, c->src_mask, mask, c->nxy))) return rc;
+    c->src_series.assign(series, series + nt);
+  } else if (kind == SPARC_SRC_LAGRANGE) {
+    if (!mask || nt < 1 || nmask != (size_t)c->nxy * nt)
+      return fail(c, SPARC_EINVAL, "LAGRANGE source needs vr(nx*ny*nt)");
+    int rc = dalloc(c, &c->src_mask, nmask * c->ts);
+    if (rc) return rc;
+    if ((rc = upload(c, c->src_mask, mask, nmask))) return rc;
+  } else {
+    return fail(c, SPARC_EINVAL, "unknown source kind %d", kind);
+  }
+  c->src_kind = kind;
+  c->src_nt = nt;
+  c->src_active_until = active_until;
+  return SPARC_OK;
+}
+
+// ---------------------------------------------------------------------------
+// stepping
+// ---------------------------------------------------------------------------
+template <typename T>
+static Fld<T> make_fld(sparc_ctx* c) {
+  Fld<T> F{};
+  for (int f = 0; f < SPARC_NFIELDS; ++f) {
+    F.a[f] = (T*)c->a[f];
+    F.scr[f] = (T*)c->scr[f];
+  }
+  for (int f = 0; f < 2; ++f) {
+    F.pmlvars[f] = (T*)c->pmlvars[f];
+    F.scrpml[f] = (T*)c->scrpml[f];
+  }
+  for (int b = 0; b < SPARC_BG_COUNT; ++b) {
+    F.bg[b] = (const T*)c->bg[b];
+    F.bg3d[b] = c->bg3d[b];
+  }
+  F.stretch = (const T*)c->prof[SPARC_PROF_STRETCH];
+  F.g = (const T*)c->prof[SPARC_PROF_G];
+  for (int w = 0; w < 8; ++w) F.W[w] = (T*)c->W[w];
+  F.forcing = (const T*)c->forcing;
+  F.nx = c->cfg.nx;
+  F.ny = c->cfg.ny;
+  F.nz = c->cfg.nz;
+  F.nxy = (int)c->nxy;
+  F.npmlbot = c->cfg.npmlbot;
+  F.npmltop = c->cfg.npmltop;
+  F.src_plane = -1;
+  F.delta_width = (T)c->cfg.delta_width;
+  return F;
+}
+
+template <typename T>
+static TriCoef<T> make_tc(const sparc_ctx* c) {
+  const double* v = c->tc;
+  TriCoef<T> t;
+  t.ci1 = (T)v[0];
+  t.ci2 = (T)v[1];
+  t.c1 = (T)v[2];
+  t.c2 = (T)v[3];
+  t.c3 = (T)v[4];
+  t.c4 = (T)v[5];
+  t.c5 = (T)v[6];
+  t.b1 = (T)v[7];
+  t.b2 = (T)v[8];
+  t.b3 = (T)v[9];
+  t.b4 = (T)v[10];
+  t.alpha2 = (T)v[11];
+  t.beta1 = (T)v[12];
+  t.cst = (T)v[13];
+  return t;
+}
+
+static inline int cdiv(long a, long b) { return (int)((a + b - 1) / b); }
+
+// x-line launch: lines per block so a block holds ~2 KiB complex values (FP32)
+template <typename T, int MODE>
+static int launch_x(sparc_ctx* c, const T* a, const T* s, T beta, T* out, const T* mult,
+                    C2<T>* spec) {
+  const int n = c->cfg.nx;
+  const long nrows = (long)c->cfg.ny * c->cfg.nz;
+  long nlines;
+  if (MODE == XM_DERIV || MODE == XM_FILTER)
+    nlines = (nrows + 1) / 2;
+  else
+    nlines = (long)((c->cfg.nz + 1) / 2) * c->cfg.ny;
+  int LPB = std::max(1, (int)((sizeof(T) == 4 ? 2048 : 1024) / n));
+  const size_t lds = 2 * (size_t)LPB * n * sizeof(C2<T>);
+  // algorithmic arrays: inputs (a, s), outputs; the damping spectrum counts as one field
+  const double fb = (double)c->N * sizeof(T);
+  double nb;
+  if (MODE == XM_DERIV || MODE == XM_FILTER || MODE == XM_DAMP_FWD)
+    nb = (s ? 2 : 1) + 1;
+  else
+    nb = 3;
+  ProfScope ps(c, (MODE == XM_DAMP_FWD || MODE == XM_DAMP_INV) ? SPARC_KC_DAMP : SPARC_KC_XLINES,
+               nb * fb);
+  hipLaunchKernelGGL((k_xlines<T, MODE>), dim3(cdiv(nlines, LPB)), dim3(256), lds, c->stream, n,
+                     (int)nrows, (int)nlines, LPB, a, s, beta, out, mult, c->px,
+                     (const C2<T>*)c->twx, spec, c->cfg.ny, c->cfg.nz, c->nxy);
+  LAUNCHCHK(c);
+  return SPARC_OK;
+}
+
+template <typename T, int MODE>
+static int launch_y(sparc_ctx* c, const T* a, const T* s, T beta, T* out, const T* mult,
+                    C2<T>* spec) {
+  const int nx = c->cfg.nx, ny = c->cfg.ny;
+  // tile width: 128 bytes of x per row, LDS capped at 64 KiB
+  int TX = (int)(128 / sizeof(T));
+  if (MODE == YM_DAMP) TX /= 2;
+  while (TX > 2 && (size_t)TX * ny * sizeof(C2<T>) * ((MODE == YM_DAMP) ? 2 : 1) > 65536) TX /= 2;
+  const size_t lds = (MODE == YM_DAMP) ? 2 * (size_t)TX * ny * sizeof(C2<T>)
+                                       : (size_t)TX * ny * sizeof(C2<T>);
+  const int planes = (MODE == YM_DAMP) ? (c->cfg.nz + 1) / 2 : c->cfg.nz;
+  const double fb = (double)c->N * sizeof(T);
+  ProfScope ps(c, MODE == YM_DAMP ? SPARC_KC_DAMP : SPARC_KC_YLINES,
+               (MODE == YM_DAMP ? 2.0 : (s ? 3.0 : 2.0)) * fb);
+  hipLaunchKernelGGL((k_ylines<T, MODE>), dim3(nx / TX, planes), dim3(256), lds, c->stream, nx,
+                     ny, c->cfg.nz, TX, a, s, beta, out, mult, c->py, (const C2<T>*)c->twy, spec,
+                     (const T*)c->rates);
+  LAUNCHCHK(c);
+  return SPARC_OK;
+}
+
+template <typename T, int EPI>
+static int launch_z(sparc_ctx* c, const Fld<T>& F, int ibc, const T* inA, const T* inS, T beta,
+                    const T* prov, T provscale, T* out) {
+  const T* tri = (const T*)c->tri[ibc];
+  const int nz = c->cfg.nz;
+  // algorithmic arrays of the solve + epilogue (the ud scratch is excluded)
+  auto nbg = [&](std::initializer_list<int> ids) {
+    double n = 0;
+    for (int id : ids) n += c->bg3d[id] ? 1.0 : 0.0;
+    return n;
+  };
+  const double in = inS ? 2.0 : 1.0;
+  double nb;
+  if (EPI == EPI_STORE)
+    nb = in + 1;
+  else if (EPI == EPI_P3D)
+    nb = in + 2 + nbg({SPARC_BG_C2RHO0, SPARC_BG_GRADP0_Z, SPARC_BG_RHO0, SPARC_BG_GRADRHO0_Z}) + 2;
+  else if (EPI == EPI_P2D)
+    nb = in + 1 + nbg({SPARC_BG_C2RHO0, SPARC_BG_GRADP0_Z, SPARC_BG_RHO0, SPARC_BG_GRADRHO0_Z}) + 2;
+  else if (EPI == EPI_RHS3D)
+    nb = in + 3 + (F.beta != (T)0 ? 6 : 3) + nbg({SPARC_BG_RHOINV}) + 6;
+  else
+    nb = in + 2 + (F.beta != (T)0 ? 4 : 2) + nbg({SPARC_BG_RHOINV}) + 4;
+  ProfScope ps(c, SPARC_KC_ZSOLVE, nb * (double)c->N * sizeof(T));
+  hipLaunchKernelGGL((k_zsolve<T, EPI>), dim3(cdiv(c->nxy, 64)), dim3(64), 0, c->stream, F,
+                     make_tc<T>(c), tri, tri + nz, tri + 2 * nz, ibc, inA, inS, beta,
+                     (T*)c->W[7], prov, provscale, out);
+  LAUNCHCHK(c);
+  return SPARC_OK;
+}
+
+// Lagrange weights of LAGRANGE_INTERP (physics.f90:680-734) for one time.
+static LagArgs lagrange_args(const sparc_ctx* c, int time) {
+  LagArgs la{};
+  const double timestep = c->cfg.timestep, cad = c->cfg.cadforcing;
+  const int num_steps = c->cfg.num_steps;
+  double xt = (double)time * timestep / cad;
+  const int time1 = (int)floor(xt) + 1;
+  xt = time * timestep;
+  double xs[7];
+  for (int q = 0; q < 7; ++q) xs[q] = (time1 - 4 + q) * cad;
+  bool use[7];
+  for (int q = 0; q < 4; ++q) use[q] = xs[q] >= 0;
+  use[4] = true;
+  use[5] = time1 < num_steps;
+  use[6] = time1 < num_steps - 1;
+  for (int q = 0; q < 7; ++q) {
+    if (!use[q]) continue;
+    const int samp = time1 - 3 + q;  // 1-based sample of vr
+    if (samp < 1 || samp > c->src_nt) continue;
+    double den = 1.0, num = 1.0;
+    for (int r = 0; r < 7; ++r)
+      if (r != q) {
+        den *= (xs[q] - xs[r]);
+        num *= (xt - xs[r]);
+      }
+    la.idx[la.nq] = samp - 1;
+    la.w[la.nq] = num / den;
+    la.nq++;
+  }
+  return la;
+}
+
+template <typename T>
+static int run_stage(sparc_ctx* c, Fld<T>& F, int s) {
+  const T beta = (T)c->betas[s];
+  F.beta = beta;
+  const bool first = (s == 0);
+  auto S = [&](int f) -> const T* { return first ? nullptr : (const T*)c->scr[f]; };
+  const T* dmx = (const T*)c->dmulx;
+  const T* dmy = (const T*)c->dmuly;
+  int rc;
+  if (c->cfg.option == SPARC_OPT_QUIET_PML_3D_DISPL) {
+    // ddxyz(xi_x, xi_y, xi_z, bc=1) + PML + div, p, rho (displacement.f90:491-535)
+    if ((rc = launch_x<T, XM_DERIV>(c, F.a[0], S(0), beta, F.W[0], dmx, nullptr))) return rc;
+    if ((rc = launch_y<T, YM_DERIV>(c, F.a[1], S(1), beta, F.W[1], dmy, nullptr))) return rc;
+    if ((rc = launch_z<T, EPI_P3D>(c, F, 1, F.a[2], S(2), beta, nullptr, 0, nullptr))) return rc;
+    // ddxyz(p, bc=0) (displacement.f90:547)
+    if ((rc = launch_x<T, XM_DERIV>(c, F.W[3], nullptr, 0, F.W[0], dmx, nullptr))) return rc;
+    if ((rc = launch_y<T, YM_DERIV>(c, F.W[3], nullptr, 0, F.W[1], dmy, nullptr))) return rc;
+    const bool damp = c->cfg.damp_waves && c->damping_set;
+    if (damp)  // DAMP_VELOCITY forward half, on the stage's temp v (damping.f90:135-160)
+      for (int f = 0; f < 3; ++f) {
+        if ((rc = launch_x<T, XM_DAMP_FWD>(c, F.a[3 + f], S(3 + f), beta, nullptr, nullptr,
+                                           (C2<T>*)c->spec[f])))
+          return rc;
+        if ((rc = launch_y<T, YM_DAMP>(c, nullptr, nullptr, 0, nullptr, nullptr,
+                                       (C2<T>*)c->spec[f])))
+          return rc;
+      }
+    if ((rc = launch_z<T, EPI_RHS3D>(c, F, 0, F.W[3], nullptr, 0, nullptr, 0, nullptr))) return rc;
+    if (damp)
+      for (int f = 0; f < 3; ++f)
+        if ((rc = launch_x<T, XM_DAMP_INV>(c, nullptr, nullptr, 0, F.scr[3 + f], nullptr,
+                                           (C2<T>*)c->spec[f])))
+          return rc;
+  } else {
+    // option 17 (physics2d.f90:959-1069)
+    if ((rc = launch_x<T, XM_DERIV>(c, F.a[0], S(0), beta, F.W[0], dmx, nullptr))) return rc;
+    const int nz = c->cfg.nz;
+    const T us = (T)(-c->unstretch_h[nz - 1]);
+    if ((rc = launch_z<T, EPI_P2D>(c, F, 3, F.a[2], S(2), beta, F.W[0] + c->nxy * (nz - 1), us,
+                                   nullptr)))
+      return rc;
+    if ((rc = launch_x<T, XM_DERIV>(c, F.W[3], nullptr, 0, F.W[0], dmx, nullptr))) return rc;
+    // ibc=2: gradp_z(:,:,1) is the caller's value, which stays zero
+    if ((rc = launch_z<T, EPI_RHS2D>(c, F, 2, F.W[3], nullptr, 0, nullptr, 0, nullptr))) return rc;
+  }
+  return SPARC_OK;
+}
+
+template <typename T>
+static int step_impl(sparc_ctx* c, int t0, int nsteps) {
+  Fld<T> F = make_fld<T>(c);
+  const int nxy = (int)c->nxy;
+  const bool is2d = c->cfg.option == SPARC_OPT_QUIET_PML_2D_DISPL;
+  const T scale = (T)c->cfg.field_scale;
+  int rc;
+  for (int i = t0; i < t0 + nsteps; ++i) {
+    // source for this step (constant over the 5 stages)
+    F.src_plane = -1;
+    if (c->src_kind != SPARC_SRC_NONE && i <= c->src_active_until) {
+      F.src_plane = (c->cfg.mode == SPARC_ADJOINT) ? c->cfg.o_rad - 1 : c->cfg.e_rad - 1;
+      if (c->src_kind == SPARC_SRC_SERIES) {
+        const double sv = (i < c->src_nt) ? c->src_series[i] : 0.0;
+        hipLaunchKernelGGL((k_forcing_series<T>), dim3(cdiv(nxy, 256)), dim3(256), 0, c->stream,
+                           (T*)c->forcing, (const T*)c->src_mask, (T)(sv * c->cfg.field_scale),
+                           nxy);
+      } else {
+        LagArgs la = lagrange_args(c, i);
+        hipLaunchKernelGGL((k_forcing_lagrange<T>), dim3(cdiv(nxy, 256)), dim3(256), 0,
+                           c->stream, (T*)c->forcing, (const T*)c->src_mask, nxy, la, scale);
+      }
+      LAUNCHCHK(c);
+    }
+    for (int s = 0; s < 5; ++s)
+      if ((rc = run_stage<T>(c, F, s))) return rc;
+    // a += deltat*scr with the z filter (step.f90:114-118)
+    const T dt = (T)c->cfg.deltat;
+    for (int f = 0; f < SPARC_NFIELDS; ++f) {
+      ProfScope ps(c, SPARC_KC_UPDATE, 3.0 * (double)c->N * sizeof(T));
+      hipLaunchKernelGGL((k_update_filter<T>), dim3(cdiv(nxy, 64)), dim3(64), 0, c->stream, F.a[f],
+                         (const T*)F.scr[f], dt, nxy, c->cfg.nz, is2d ? 6 : 2);
+      LAUNCHCHK(c);
+    }
+    if (c->nzpml > 0)
+      for (int f = 0; f < 2; ++f) {
+        ProfScope ps(c, SPARC_KC_OTHER, 3.0 * (double)c->NP * sizeof(T));
+        hipLaunchKernelGGL((k_axpy<T>), dim3(cdiv(c->NP, 256)), dim3(256), 0, c->stream,
+                           F.pmlvars[f], (const T*)F.scrpml[f], dt, c->NP);
+        LAUNCHCHK(c);
+      }
+    // spectral low-pass (physics.f90:792-830; physics2d.f90:563-585)
+    const int ff = c->cfg.freq_filtering;
+    if (ff > 0 && c->filter_set) {
+      if (!is2d && i % ff == 0) {
+        for (int f = 0; f < 5; ++f)
+          if ((rc = launch_x<T, XM_FILTER>(c, F.a[f], nullptr, 0, F.a[f], (const T*)c->fmulx,
+                                           nullptr)))
+            return rc;
+        for (int f = 0; f < 5; ++f)
+          if ((rc = launch_y<T, YM_FILTER>(c, F.a[f], nullptr, 0, F.a[f], (const T*)c->fmuly,
+                                           nullptr)))
+            return rc;
+      } else if (is2d && (i + 1) % ff == 0) {
+        for (int f = 0; f < SPARC_NFIELDS; ++f)
+          if ((rc = launch_x<T, XM_FILTER>(c, F.a[f], nullptr, 0, F.a[f], (const T*)c->fmulx,
+                                           nullptr)))
+            return rc;
+      }
+    }
+  }
+  return SPARC_OK;
+}
+
+extern "C" int sparc_step(sparc_ctx* c, int t0, int nsteps) {
+  if (!c || !c->stream) return SPARC_EINVAL;
+  if (nsteps < 0) return fail(c, SPARC_EINVAL, "nsteps < 0");
+  static const int need[] = {SPARC_BG_RHO0,       SPARC_BG_RHOINV, SPARC_BG_C2RHO0,
+                             SPARC_BG_GRADP0_Z,   SPARC_BG_GRADRHO0_Z, SPARC_BG_AZ,
+                             SPARC_BG_BZPML};
+  for (int id : need)
+    if (!c->bgset[id]) return fail(c, SPARC_ESTATE, "background %d not set", id);
+  if (!c->profset[SPARC_PROF_STRETCH] || !c->profset[SPARC_PROF_G] ||
+      (c->cfg.option == SPARC_OPT_QUIET_PML_2D_DISPL && !c->profset[SPARC_PROF_UNSTRETCH]))
+    return fail(c, SPARC_ESTATE, "stretch/g/unstretch profiles not set");
+  if (c->cfg.option == SPARC_OPT_QUIET_PML_3D_DISPL && c->cfg.damp_waves && !c->damping_set)
+    return fail(c, SPARC_ESTATE, "damp_waves set but damping rates not given");
+  if (c->cfg.freq_filtering > 0 && !c->filter_set)
+    return fail(c, SPARC_ESTATE, "freq_filtering > 0 but filter not given");
+  return c->fp32 ? step_impl<float>(c, t0, nsteps) : step_impl<double>(c, t0, nsteps);
+}
+
+extern "C" int sparc_sync(sparc_ctx* c) {
+  if (!c) return SPARC_EINVAL;
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return SPARC_OK;
+}
+
+// ---------------------------------------------------------------------------
+// state access
+// ---------------------------------------------------------------------------
+static void* arr_ptr(sparc_ctx* c, int arr, int field, size_t* n) {
+  if ((arr == SPARC_ARR_A || arr == SPARC_ARR_SCR) && field >= 0 && field < SPARC_NFIELDS) {
+    *n = c->N;
+    return arr == SPARC_ARR_A ? c->a[field] : c->scr[field];
+  }
+  if ((arr == SPARC_ARR_PMLVARS || arr == SPARC_ARR_SCRPML) && field >= 0 && field < 2) {
+    *n = c->NP;
+    return arr == SPARC_ARR_PMLVARS ? c->pmlvars[field] : c->scrpml[field];
+  }
+  return nullptr;
+}
+
+extern "C" int sparc_get_state(sparc_ctx* c, int arr, int field, double* h) {
+  if (!c || !h) return SPARC_EINVAL;
+  size_t n;
+  void* p = arr_ptr(c, arr, field, &n);
+  if (!p) return fail(c, SPARC_EINVAL, "bad array %d / field %d", arr, field);
+  return download(c, h, p, n, 1.0 / c->cfg.field_scale);
+}
+
+extern "C" int sparc_put_state(sparc_ctx* c, int arr, int field, const double* h) {
+  if (!c || !h) return SPARC_EINVAL;
+  size_t n;
+  void* p = arr_ptr(c, arr, field, &n);
+  if (!p) return fail(c, SPARC_EINVAL, "bad array %d / field %d", arr, field);
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  std::vector<double> t(h, h + n);
+  if (c->cfg.field_scale != 1.0)
+    for (auto& v : t) v *= c->cfg.field_scale;
+  return upload(c, p, t.data(), n);
+}
+
+extern "C" int sparc_get_plane(sparc_ctx* c, int field, int k, double* h) {
+  if (!c || !h || field < 0 || field >= SPARC_NFIELDS || k < 1 || k > c->cfg.nz)
+    return fail(c, SPARC_EINVAL, "bad field/plane");
+  return download(c, h, (char*)c->a[field] + (size_t)(k - 1) * c->nxy * c->ts, c->nxy,
+                  1.0 / c->cfg.field_scale);
+}
+
+extern "C" int sparc_norm(sparc_ctx* c, int field, double* out) {
+  if (!c || !out || field < 0 || field >= SPARC_NFIELDS) return SPARC_EINVAL;
+  std::vector<double> h(c->N);
+  int rc = download(c, h.data(), c->a[field], c->N, 1.0 / c->cfg.field_scale);
+  if (rc) return rc;
+  double s = 0.0;
+  for (double v : h) s += v * v;
+  *out = sqrt(s / ((double)c->cfg.nx * (double)c->cfg.ny * (double)c->cfg.nz));
+  return SPARC_OK;
+}
+
+template <typename T>
+__global__ void k_gather(double* out, const T* __restrict__ f, const int32_t* __restrict__ ijk,
+                         int npts, int nx, long nxy, double scale) {
+  int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= npts) return;
+  const long q = ijk[3 * p] + (long)nx * ijk[3 * p + 1] + nxy * (ijk[3 * p + 2] - 1);
+  out[p] = (double)f[q] * scale;
+}
+
+extern "C" int sparc_sample(sparc_ctx* c, int field, int npts, const int32_t* ijk, double* h) {
+  if (!c || !h || !ijk || npts < 0 || field < 0 || field >= SPARC_NFIELDS)
+    return SPARC_EINVAL;
+  if (npts == 0) return SPARC_OK;
+  for (int p = 0; p < npts; ++p)
+    if (ijk[3 * p] < 0 || ijk[3 * p] >= c->cfg.nx || ijk[3 * p + 1] < 0 ||
+        ijk[3 * p + 1] >= c->cfg.ny || ijk[3 * p + 2] < 1 || ijk[3 * p + 2] > c->cfg.nz)
+      return fail(c, SPARC_EINVAL, "sample point %d out of the grid", p);
+  int32_t* dijk;
+  double* dout;
+  HIPCHK(c, hipMallocAsync((void**)&dijk, 12 * (size_t)npts, c->stream));
+  HIPCHK(c, hipMallocAsync((void**)&dout, 8 * (size_t)npts, c->stream));
+  HIPCHK(c, hipMemcpyAsync(dijk, ijk, 12 * (size_t)npts, hipMemcpyHostToDevice, c->stream));
+  const double sc = 1.0 / c->cfg.field_scale;
+  if (c->fp32)
+    hipLaunchKernelGGL(k_gather<float>, dim3(cdiv(npts, 64)), dim3(64), 0, c->stream, dout,
+                       (const float*)c->a[field], dijk, npts, c->cfg.nx, c->nxy, sc);
+  else
+    hipLaunchKernelGGL(k_gather<double>, dim3(cdiv(npts, 64)), dim3(64), 0, c->stream, dout,
+                       (const double*)c->a[field], dijk, npts, c->cfg.nx, c->nxy, sc);
+  LAUNCHCHK(c);
+  HIPCHK(c, hipMemcpyAsync(h, dout, 8 * (size_t)npts, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipFreeAsync(dijk, c->stream));
+  HIPCHK(c, hipFreeAsync(dout, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return SPARC_OK;
+}
+
+// ---------------------------------------------------------------------------
+// kernels
+// ---------------------------------------------------------------------------
+extern "C" int sparc_snapshots_alloc(sparc_ctx* c, int nslots) {
+  if (!c || nslots < 0) return SPARC_EINVAL;
+  if (nslots <= c->nslots) return SPARC_OK;
+  void* p;
+  int rc = dalloc(c, &p, (size_t)nslots * 6 * c->N * c->ts);
+  if (rc) return rc;
+  c->snap = p;
+  c->nslots = nslots;
+  c->snap_scale = c->cfg.field_scale;
+  return SPARC_OK;
+}
+
+extern "C" int sparc_snapshot_store(sparc_ctx* c, int slot) {
+  if (!c || slot < 0 || slot >= c->nslots) return fail(c, SPARC_EINVAL, "bad slot %d", slot);
+  char* s = (char*)c->snap + (size_t)slot * 6 * c->N * c->ts;
+  const size_t fb = c->N * c->ts;
+  for (int f = 0; f < 3; ++f) {
+    HIPCHK(c, hipMemcpyAsync(s + f * fb, c->a[f], fb, hipMemcpyDeviceToDevice, c->stream));
+    HIPCHK(c, hipMemcpyAsync(s + (3 + f) * fb, c->scr[3 + f], fb, hipMemcpyDeviceToDevice,
+                             c->stream));
+  }
+  c->snap_scale = c->cfg.field_scale;
+  return SPARC_OK;
+}
+
+extern "C" int sparc_snapshots_copy(sparc_ctx* dst, const sparc_ctx* src) {
+  if (!dst || !src) return SPARC_EINVAL;
+  if (dst->N != src->N || dst->fp32 != src->fp32)
+    return fail(dst, SPARC_EINVAL, "snapshot banks differ in shape or precision");
+  int rc = sparc_snapshots_alloc(dst, src->nslots);
+  if (rc) return rc;
+  HIPCHK(dst, hipStreamSynchronize(src->stream));
+  HIPCHK(dst, hipMemcpyAsync(dst->snap, src->snap, (size_t)src->nslots * 6 * src->N * src->ts,
+                             hipMemcpyDeviceToDevice, dst->stream));
+  dst->snap_scale = src->snap_scale;
+  return SPARC_OK;
+}
+
+extern "C" int sparc_kernel_begin(sparc_ctx* c, const double* grav, int nz) {
+  if (!c || !grav || nz != c->cfg.nz) return fail(c, SPARC_EINVAL, "grav needs nz values");
+  int rc;
+  if (!c->kern[0]) {
+    for (int q = 0; q < SPARC_KERN_COUNT; ++q)
+      if ((rc = dalloc(c, (void**)&c->kern[q], c->N * 8))) return rc;
+    if ((rc = dalloc(c, (void**)&c->grav, nz * 8))) return rc;
+  }
+  for (int q = 0; q < SPARC_KERN_COUNT; ++q)
+    HIPCHK(c, hipMemsetAsync(c->kern[q], 0, c->N * 8, c->stream));
+  HIPCHK(c, hipMemcpyAsync(c->grav, grav, nz * 8, hipMemcpyHostToDevice, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return SPARC_OK;
+}
+
+template <typename T>
+static int kernel_acc_impl(sparc_ctx* c, int slot) {
+  const long N = c->N, nxy = c->nxy;
+  const int nz = c->cfg.nz;
+  Fld<T> F = make_fld<T>(c);
+  T* s = (T*)c->snap + (size_t)slot * 6 * N;
+  const T *fxx = s, *fxy = s + N, *fxz = s + 2 * N, *fax = s + 3 * N, *fay = s + 4 * N,
+          *faz = s + 5 * N;
+  const T *avx = F.a[3], *avy = F.a[4], *avz = F.a[5];
+  const T* dmx = (const T*)c->dmulx;
+  const T* dmy = (const T*)c->dmuly;
+  const bool has_y = c->cfg.ny > 4;
+  int rc;
+  // COMPUTE_DIV (kernels.f90:798-816), z part through ddzkern + FILTER_VAR_Z
+  auto div = [&](const T* fx, const T* fy, const T* fz, T* outdiv) -> int {
+    int r;
+    if ((r = launch_x<T, XM_DERIV>(c, fx, nullptr, 0, F.W[0], dmx, nullptr))) return r;
+    if (has_y && (r = launch_y<T, YM_DERIV>(c, fy, nullptr, 0, F.W[1], dmy, nullptr))) return r;
+    if ((r = launch_z<T, EPI_STORE>(c, F, 1, fz, nullptr, 0, nullptr, 0, F.W[2]))) return r;
+    hipLaunchKernelGGL((k_update_filter<T>), dim3(cdiv(nxy, 64)), dim3(64), 0, c->stream, F.W[2],
+                       (const T*)nullptr, (T)0, (int)nxy, nz, 2);
+    LAUNCHCHK(c);
+    hipLaunchKernelGGL((k_sum3<T>), dim3(cdiv(N, 256)), dim3(256), 0, c->stream, outdiv,
+                       (const T*)F.W[0], has_y ? (const T*)F.W[1] : nullptr, (const T*)F.W[2], N);
+    LAUNCHCHK(c);
+    return SPARC_OK;
+  };
+  if ((rc = div(fxx, fxy, fxz, F.W[5]))) return rc;
+  if ((rc = div(avx, avy, avz, F.W[6]))) return rc;
+  // grad(a_vel_z * grav) (kernels.f90:294-301)
+  hipLaunchKernelGGL((k_scale_z<T>), dim3(cdiv(N, 256)), dim3(256), 0, c->stream, F.W[3], avz,
+                     (const double*)c->grav, nxy, N);
+  LAUNCHCHK(c);
+  if ((rc = launch_x<T, XM_DERIV>(c, F.W[3], nullptr, 0, F.W[0], dmx, nullptr))) return rc;
+  const bool has_y2 = c->cfg.ny > 1;
+  if (has_y2 && (rc = launch_y<T, YM_DERIV>(c, F.W[3], nullptr, 0, F.W[1], dmy, nullptr)))
+    return rc;
+  if ((rc = launch_z<T, EPI_STORE>(c, F, 1, F.W[3], nullptr, 0, nullptr, 0, F.W[2]))) return rc;
+  hipLaunchKernelGGL((k_update_filter<T>), dim3(cdiv(nxy, 64)), dim3(64), 0, c->stream, F.W[2],
+                     (const T*)nullptr, (T)0, (int)nxy, nz, 2);
+  LAUNCHCHK(c);
+  const double sff = 1.0 / c->snap_scale, sfa = 1.0 / c->cfg.field_scale;
+  hipLaunchKernelGGL((k_kern_accum<T>), dim3(cdiv(N, 256)), dim3(256), 0, c->stream,
+                     c->kern[SPARC_KERN_HESSIAN], c->kern[SPARC_KERN_C2], c->kern[SPARC_KERN_RHO],
+                     fxx, fxy, fxz, fax, fay, faz, avx, avy, avz, (const T*)F.W[5],
+                     (const T*)F.W[6], (const T*)F.W[0], has_y2 ? (const T*)F.W[1] : nullptr,
+                     (const T*)F.W[2], (const T*)c->bg[SPARC_BG_C2], c->bg3d[SPARC_BG_C2],
+                     (const double*)c->grav, sff, sfa, nxy, N);
+  LAUNCHCHK(c);
+  return SPARC_OK;
+}
+
+extern "C" int sparc_kernel_accumulate(sparc_ctx* c, int slot) {
+  if (!c || slot < 0 || slot >= c->nslots) return fail(c, SPARC_EINVAL, "bad slot %d", slot);
+  if (!c->kern[0]) return fail(c, SPARC_ESTATE, "sparc_kernel_begin not called");
+  if (!c->bgset[SPARC_BG_C2]) return fail(c, SPARC_ESTATE, "background C2 not set");
+  return c->fp32 ? kernel_acc_impl<float>(c, slot) : kernel_acc_impl<double>(c, slot);
+}
+
+extern "C" int sparc_kernel_get(sparc_ctx* c, int which, double* h) {
+  if (!c || !h || which < 0 || which >= SPARC_KERN_COUNT || !c->kern[0])
+    return fail(c, SPARC_EINVAL, "bad kernel id or kernels not begun");
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  HIPCHK(c, hipMemcpy(h, c->kern[which], c->N * 8, hipMemcpyDeviceToHost));
+  return SPARC_OK;
+}
+
+// ---------------------------------------------------------------------------
+// single operators
+// ---------------------------------------------------------------------------
+template <typename T>
+static int apply_op_impl(sparc_ctx* c, int op, int ibc, const double* in, double* out) {
+  const long N = c->N, nxy = c->nxy;
+  const int nz = c->cfg.nz;
+  Fld<T> F = make_fld<T>(c);
+  T* src = F.W[5];
+  T* dst = F.W[6];
+  int rc;
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  if ((rc = upload(c, src, in, N))) return rc;
+  switch (op) {
+    case SPARC_OP_DDX:
+      rc = launch_x<T, XM_DERIV>(c, src, nullptr, 0, dst, (const T*)c->dmulx, nullptr);
+      break;
+    case SPARC_OP_DDY:
+      if (c->cfg.ny < 2) return fail(c, SPARC_EINVAL, "DDY needs ny >= 2");
+      rc = launch_y<T, YM_DERIV>(c, src, nullptr, 0, dst, (const T*)c->dmuly, nullptr);
+      break;
+    case SPARC_OP_DDZ: {
+      if (ibc < 0 || ibc > 3) return fail(c, SPARC_EINVAL, "ibc %d not supported (0..3)", ibc);
+      if (!c->profset[SPARC_PROF_STRETCH]) return fail(c, SPARC_ESTATE, "stretch not set");
+      const T* prov = nullptr;
+      if (ibc == 2 || ibc == 3) {
+        if ((rc = upload(c, F.W[4], out, N))) return rc;
+        prov = F.W[4] + (ibc == 2 ? 0 : nxy * (nz - 1));
+      }
+      rc = launch_z<T, EPI_STORE>(c, F, ibc, src, nullptr, 0, prov, (T)1, dst);
+      break;
+    }
+    case SPARC_OP_ZFILTER:
+      HIPCHK(c, hipMemcpyAsync(dst, src, N * sizeof(T), hipMemcpyDeviceToDevice, c->stream));
+      hipLaunchKernelGGL((k_update_filter<T>), dim3(cdiv(nxy, 64)), dim3(64), 0, c->stream, dst,
+                         (const T*)nullptr, (T)0, (int)nxy, nz, 2);
+      LAUNCHCHK(c);
+      rc = SPARC_OK;
+      break;
+    case SPARC_OP_DAMP:
+      if (!c->spec[0] || !c->damping_set)
+        return fail(c, SPARC_ESTATE, "DAMP needs option 4, damp_waves and damping rates");
+      HIPCHK(c, hipMemsetAsync(dst, 0, N * sizeof(T), c->stream));
+      if ((rc = launch_x<T, XM_DAMP_FWD>(c, src, nullptr, 0, nullptr, nullptr, (C2<T>*)c->spec[0])))
+        return rc;
+      if ((rc = launch_y<T, YM_DAMP>(c, nullptr, nullptr, 0, nullptr, nullptr, (C2<T>*)c->spec[0])))
+        return rc;
+      rc = launch_x<T, XM_DAMP_INV>(c, nullptr, nullptr, 0, dst, nullptr, (C2<T>*)c->spec[0]);
+      break;
+    case SPARC_OP_LOWPASS_X:
+      if (!c->filter_set) return fail(c, SPARC_ESTATE, "filter not set");
+      rc = launch_x<T, XM_FILTER>(c, src, nullptr, 0, dst, (const T*)c->fmulx, nullptr);
+      break;
+    case SPARC_OP_LOWPASS_Y:
+      if (!c->filter_set || c->cfg.ny < 2) return fail(c, SPARC_ESTATE, "filter not set / ny < 2");
+      rc = launch_y<T, YM_FILTER>(c, src, nullptr, 0, dst, (const T*)c->fmuly, nullptr);
+      break;
+    default:
+      return fail(c, SPARC_EINVAL, "unknown op %d", op);
+  }
+  if (rc) return rc;
+  // DAMP leaves -term in dst
+  return download(c, out, dst, N, op == SPARC_OP_DAMP ? -1.0 : 1.0);
+}
+
+extern "C" int sparc_apply_op(sparc_ctx* c, int op, int ibc, const double* in, double* out) {
+  if (!c || !in || !out) return SPARC_EINVAL;
+  return c->fp32 ? apply_op_impl<float>(c, op, ibc, in, out)
+                 : apply_op_impl<double>(c, op, ibc, in, out);
+}
+
+// ---------------------------------------------------------------------------
+// instrumentation
+// ---------------------------------------------------------------------------
+static int prof_collect(sparc_ctx* c) {
+  if (c->recs.empty()) return SPARC_OK;
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  for (auto& r : c->recs) {
+    float ms = 0.f;
+    HIPCHK(c, hipEventElapsedTime(&ms, r.e0, r.e1));
+    c->pms[r.cls] += ms;
+    c->pbytes[r.cls] += r.bytes;
+    c->plaunch[r.cls] += 1;
+    c->pool.push_back(r.e0);
+    c->pool.push_back(r.e1);
+  }
+  c->recs.clear();
+  return SPARC_OK;
+}
+
+extern "C" int sparc_profile_enable(sparc_ctx* c, int on) {
+  if (!c || !c->stream) return SPARC_EINVAL;
+  int rc = prof_collect(c);
+  if (rc) return rc;
+  if (on) {
+    for (int k = 0; k < SPARC_KC_COUNT; ++k) {
+      c->pms[k] = 0;
+      c->pbytes[k] = 0;
+      c->plaunch[k] = 0;
+    }
+  }
+  c->profiling = on != 0;
+  return SPARC_OK;
+}
+
+extern "C" int sparc_profile_read(sparc_ctx* c, int cls, double* ms, double* bytes,
+                                  int64_t* launches) {
+  if (!c || cls < 0 || cls >= SPARC_KC_COUNT) return SPARC_EINVAL;
+  int rc = prof_collect(c);
+  if (rc) return rc;
+  if (ms) *ms = c->pms[cls];
+  if (bytes) *bytes = c->pbytes[cls];
+  if (launches) *launches = c->plaunch[cls];
+  return SPARC_OK;
+}

@@ -1,0 +1,201 @@
+"""Host driver of one solve: the kernel-mode sequence of driver.f90
+(DETERMINE_STATUS, TIMESTEPPING, FORWARD_SOURCE) over the GPU engine.
+
+``Params`` mirrors the compile-time switches of params.i as runtime values;
+``workload()`` builds the synthetic configurations the benchmark and tests
+use (adiabatic polytrope boxes with a Gaussian x Ricker source).
+"""
+from __future__ import annotations
+
+import math
+from dataclasses import dataclass, field, replace
+
+import numpy as np
+
+from . import abi
+from . import background as bgm
+from .background import M_CM, RSUN_R
+
+
+@dataclass
+class Params:
+    """Runtime mirror of params.i."""
+    nx: int
+    ny: int
+    nz: int
+    xlength: float                 # cm (params.i:27)
+    ylength: float
+    timestep: float                # s
+    option: int = abi.OPT_3D
+    npmlbot: int = 10              # params.i:120
+    npmltop: int = 10
+    damp_waves: bool = True        # params.i:52
+    outputcad: float = 30.0        # params.i:140
+    cadforcing: float = 30.0       # params.i:36
+    excitdep: float = -1.5e7       # cm below the photosphere
+    obsheight: float = 0.0         # cm above the photosphere
+    precision: int = abi.FP32
+    field_scale: float = 1.0
+
+    @property
+    def is2d(self) -> bool:
+        return self.option == abi.OPT_2D
+
+    @property
+    def steps(self) -> int:
+        """Output cadence in steps, floor(outputcad/timestep) (driver.f90:269);
+        a 1e-9 guard keeps exact multiples exact."""
+        return int(math.floor(self.outputcad / self.timestep + 1e-9))
+
+    @property
+    def cadforcing_step(self) -> int:
+        return int(math.floor(self.cadforcing / self.timestep + 1e-9))
+
+    @property
+    def freq_filtering(self) -> int:
+        return int(math.floor(60.0 / self.timestep + 1e-9))  # driver.f90:270
+
+
+def cfl_timestep(tab: np.ndarray, dx_cm: float, dy_cm: float, cfl: float,
+                 cadence: float = 30.0) -> float:
+    """Largest timestep <= cfl * min(dz/c, dx/c_max, dy/c_max) that divides
+    the output cadence an integer number of times."""
+    z_cm = (tab[:, 0] - 1.0) * RSUN_R
+    c = tab[:, 1]
+    dz = np.diff(z_cm)
+    cmid = np.maximum(c[1:], c[:-1])
+    lim = min(float(np.min(dz / cmid)), dx_cm / float(c.max()), dy_cm / float(c.max()))
+    dt = cfl * lim
+    m = math.ceil(cadence / dt - 1e-12)
+    return cadence / m
+
+
+@dataclass
+class Workload:
+    name: str
+    params: Params
+    tab: np.ndarray
+    nsteps: int
+    src_xy_mm: tuple = (0.0, 0.0)
+    src_sigma_mm: float = 0.4
+    ricker_nu: float = 3.0e-3
+    extra: dict = field(default_factory=dict)
+
+
+def workload(name: str, precision: int | None = None) -> Workload:
+    """The synthetic cases of the benchmark configurations.
+
+    box1   64 x 64 x 100, 50 Mm, z in [-12, +0.5] Mm, PML 10+10, CFL 0.5, 400 steps
+    box2   256 x 256 x 300, 200 Mm, same polytrope, 2000 steps
+    box2s  128 x 128 x 150, 100 Mm (the small grid of the serving mix)
+    box2l  384 x 384 x 300, 300 Mm (the large grid of the serving mix)
+    tiny   32 x 32 x 40, 25 Mm (parity tests)
+    two_d  256 x 1 x 300, 400 Mm, option 17 (the reference's wired 2-D case)
+    """
+    if name == "box1":
+        nx = ny = 64
+        nz, L, nsteps, prec = 100, 50.0, 400, abi.FP64
+    elif name == "box2":
+        nx = ny = 256
+        nz, L, nsteps, prec = 300, 200.0, 2000, abi.FP32
+    elif name == "box2s":
+        nx = ny = 128
+        nz, L, nsteps, prec = 150, 100.0, 1000, abi.FP32
+    elif name == "box2l":
+        nx = ny = 384
+        nz, L, nsteps, prec = 300, 300.0, 1000, abi.FP32
+    elif name == "tiny":
+        nx = ny = 32
+        nz, L, nsteps, prec = 40, 25.0, 20, abi.FP64
+    elif name == "two_d":
+        tab = bgm.polytrope_table(300, 0.5, -12.0)
+        P = Params(nx=256, ny=1, nz=300, xlength=400.0 * M_CM, ylength=400.0 * M_CM,
+                   timestep=cfl_timestep(tab, 400.0 * M_CM / 256, 1e30, 0.5),
+                   option=abi.OPT_2D, npmlbot=10, npmltop=10, damp_waves=False,
+                   precision=abi.FP64 if precision is None else precision)
+        return Workload("two_d", P, tab, 400)
+    else:
+        raise ValueError(f"unknown workload {name}")
+    tab = bgm.polytrope_table(nz, 0.5, -12.0)
+    dx = L * M_CM / nx
+    dt = cfl_timestep(tab, dx, dx, 0.5)
+    P = Params(nx=nx, ny=ny, nz=nz, xlength=L * M_CM, ylength=L * M_CM, timestep=dt,
+               precision=prec if precision is None else precision)
+    return Workload(name, P, tab, nsteps)
+
+
+def make_config(P: Params, S: bgm.Setup, mode: int, nsteps_total: int) -> abi.SparcConfig:
+    c = abi.SparcConfig()
+    c.abi_version = abi.ABI_VERSION
+    c.nx, c.ny, c.nz = P.nx, P.ny, P.nz
+    c.option = P.option
+    c.precision = P.precision
+    c.npmlbot, c.npmltop = P.npmlbot, P.npmltop
+    c.damp_waves = int(P.damp_waves and not P.is2d)
+    c.freq_filtering = P.freq_filtering
+    if P.is2d:  # physics2d.f90:42,54 override the nearest-point search
+        e_rad, o_rad = P.nz - 5, P.nz - 1
+    else:
+        e_rad = bgm.nearest_index(S.z, 1.0 + P.excitdep / RSUN_R)
+        o_rad = bgm.nearest_index(S.z, 1.0 + P.obsheight / RSUN_R)
+    c.e_rad, c.o_rad = e_rad, o_rad
+    c.mode = mode
+    c.cadforcing_step = P.cadforcing_step
+    # initialize.f90:978-981 with maxtime = the solve length
+    maxtime = P.cadforcing_step * (nsteps_total // P.cadforcing_step)
+    c.forcing_length = maxtime // P.cadforcing_step + 1
+    # driver.f90:280-281
+    ns = int(math.floor(maxtime * P.timestep / P.cadforcing)) + 2
+    c.num_steps = 2 * (ns // 2) + 2
+    c.xlength, c.ylength = P.xlength, P.ylength
+    c.timestep = P.timestep
+    c.deltat = S.deltat
+    c.stretchx, c.stretchy = S.stretchx, S.stretchy
+    k = (o_rad if mode == abi.ADJOINT else e_rad) - 1
+    c.delta_width = 2.0 / (S.z[k + 1] - S.z[k - 1])  # DETERMINE_STATUS (driver.f90:644,690)
+    c.cadforcing = P.cadforcing
+    c.field_scale = P.field_scale if P.precision == abi.FP32 else 1.0
+    return c
+
+
+def load_setup(eng, S: bgm.Setup) -> None:
+    """Hands the set-up arrays to an engine (or an oracle with the same calls)."""
+    for bid, arr in bgm.background_arrays(S).items():
+        eng.set_background(bid, arr)
+    d = S.derived
+    eng.set_profile(abi.PROF_STRETCH, d["stretch"])
+    eng.set_profile(abi.PROF_UNSTRETCH, d["unstretch"])
+    eng.set_profile(abi.PROF_G, d["g"])
+    eng.set_profile(abi.PROF_Z, S.z)
+    eng.set_damping(S.rates)
+    eng.set_filter(S.decay_x, S.decay_y)
+
+
+def setup_for(W: Workload, c2_3d=None, rho0_3d=None) -> bgm.Setup:
+    P = W.params
+    return bgm.build_setup(W.tab, P.nx, P.ny, P.option, P.xlength, P.ylength, P.timestep,
+                           P.npmlbot, P.npmltop, c2_3d=c2_3d, rho0_3d=rho0_3d)
+
+
+def forward_source(W: Workload, nt: int):
+    """mask(x,y) and time series of the forward source.  3-D: Gaussian
+    (sigma 0.4 Mm) x Ricker (3 mHz); 2-D: the reference's sinc mask at the
+    master pixel x = -5 Mm (data/master.pixels) x FORWARD_SOURCE."""
+    P = W.params
+    if P.is2d:
+        mask = bgm.sinc_mask_2d(P.nx, P.xlength, -5.0)
+        series = bgm.fwdsource_reference(nt, P.timestep)
+    else:
+        mask = bgm.gaussian_mask(P.nx, P.ny, P.xlength, P.ylength, W.src_xy_mm[0],
+                                 W.src_xy_mm[1], W.src_sigma_mm)
+        series = bgm.ricker_source(nt, P.timestep, nu=W.ricker_nu)
+    return mask, series
+
+
+def make_engine(W: Workload, S: bgm.Setup, mode: int = abi.FORWARD, nsteps: int | None = None,
+                precision: int | None = None):
+    P = W.params if precision is None else replace(W.params, precision=precision)
+    cfg = make_config(P, S, mode, nsteps if nsteps is not None else W.nsteps)
+    eng = abi.Engine(cfg)
+    load_setup(eng, S)
+    return eng, cfg

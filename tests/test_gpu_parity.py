@@ -1,0 +1,203 @@
+"""GPU engine (through the C ABI) against the FP64 oracle on the same seeded
+inputs.  FP64 engine: agreement to FFT/tridiagonal rounding (<= 1e-10
+relative); FP32 engine: wavefields within 1e-5 relative L2 (the stated
+tolerance of the FP32 path)."""
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from sparc_amd import abi
+from sparc_amd import driver as D
+from tests.helpers import make_oracle, rel_err, rel_l2
+
+pytestmark = pytest.mark.gpu
+
+TOL64 = 1e-10
+TOL32_OP = 2e-6     # one operator in FP32 (max-abs relative)
+TOL32_L2 = 1e-5     # wavefield after a run, relative L2
+
+
+@pytest.fixture(scope="module")
+def tiny():
+    W = D.workload("tiny")
+    S = D.setup_for(W)
+    return W, S
+
+
+@pytest.fixture(scope="module")
+def two_d():
+    W = D.workload("two_d")
+    W.params.nx = 64
+    W.params.nz = 60
+    W.tab = __import__("sparc_amd.background", fromlist=["x"]).polytrope_table(60, 0.5, -12.0)
+    W.params.xlength = 100e8
+    W.params.ylength = 100e8
+    S = D.setup_for(W)
+    return W, S
+
+
+def _field(shape, seed):
+    rng = np.random.default_rng(seed)
+    nx, ny, nz = shape
+    x = np.arange(nx) / nx
+    y = np.arange(ny) / ny
+    f = rng.standard_normal(shape)
+    # smooth enough to have meaningful derivatives, random enough to exercise all modes
+    f += 3 * np.cos(2 * np.pi * 2 * x)[:, None, None] * np.sin(2 * np.pi * y)[None, :, None]
+    return np.asfortranarray(f)
+
+
+@pytest.mark.parametrize("prec,tol", [(abi.FP64, TOL64), (abi.FP32, TOL32_OP)])
+def test_operators_match_oracle(tiny, prec, tol):
+    W, S = tiny
+    eng, cfg = D.make_engine(W, S, precision=prec)
+    try:
+        f = _field(eng.shape, 1)
+        st = S.derived["stretch"]
+        assert rel_err(eng.apply_op(abi.OP_DDX, f), O.ddx(f, cfg.stretchx)) < tol
+        assert rel_err(eng.apply_op(abi.OP_DDY, f), O.ddy(f, cfg.stretchy)) < tol
+        for ibc in (0, 1, 2, 3):
+            preset = _field(eng.shape, 5 + ibc)
+            want = O.ddz(f, ibc, st, preset=preset if ibc in (2, 3) else None)
+            got = eng.apply_op(abi.OP_DDZ, f, ibc=ibc, out=preset if ibc in (2, 3) else None)
+            assert rel_err(got, want) < tol * 10, ibc
+        assert rel_err(eng.apply_op(abi.OP_ZFILTER, f), O.zfilter(f, 2)) < tol
+        o, _ = make_oracle(W, S)
+        want = o.damp_field(f, np.zeros_like(f))
+        got = eng.apply_op(abi.OP_DAMP, f)
+        assert rel_err(-got[:, :, 1:-1], want[:, :, 1:-1]) < tol
+    finally:
+        eng.close()
+
+
+def _run_pair(W, S, nsteps, prec, t0=0):
+    eng, cfg = D.make_engine(W, S, precision=prec, nsteps=nsteps)
+    o, _ = make_oracle(W, S, nsteps=nsteps)
+    mask, series = D.forward_source(W, nsteps + 8)
+    for x in (eng, o):
+        x.set_source(abi.SRC_SERIES, mask, series, 10 ** 9)
+    eng.step(t0, nsteps)
+    o.step(t0, nsteps)
+    return eng, o
+
+
+@pytest.mark.parametrize("prec", [abi.FP64, abi.FP32])
+def test_step_3d_matches_oracle(tiny, prec):
+    W, S = tiny
+    eng, o = _run_pair(W, S, 14, prec)  # freq_filtering=12: spectral filter at time 0 and 12
+    try:
+        for f in range(6):
+            a, b = eng.get(abi.ARR_A, f), o.get(abi.ARR_A, f)
+            assert np.linalg.norm(b) > 0, f
+            if prec == abi.FP64:
+                assert rel_err(a, b) < TOL64, f
+            else:
+                assert rel_l2(a, b) < TOL32_L2, f
+        # acc = scr(4..6) of the last stage, the snapshot contract
+        for f in range(3, 6):
+            a, b = eng.get(abi.ARR_SCR, f), o.get(abi.ARR_SCR, f)
+            assert (rel_err(a, b) < TOL64) if prec == abi.FP64 else (rel_l2(a, b) < TOL32_L2)
+        for f in range(2):
+            a, b = eng.get(abi.ARR_PMLVARS, f), o.get(abi.ARR_PMLVARS, f)
+            assert (rel_err(a, b) < TOL64) if prec == abi.FP64 else (rel_l2(a, b) < 1e-4)
+    finally:
+        eng.close()
+        o.close()
+
+
+@pytest.mark.parametrize("prec", [abi.FP64, abi.FP32])
+def test_step_2d_option17_matches_oracle(two_d, prec):
+    W, S = two_d
+    eng, o = _run_pair(W, S, 90, prec)  # (time+1) % 86 == 0 fires the 2-D spectral filter
+    try:
+        for f in (0, 2, 3, 5):
+            a, b = eng.get(abi.ARR_A, f), o.get(abi.ARR_A, f)
+            assert np.linalg.norm(b) > 0
+            if prec == abi.FP64:
+                assert rel_err(a, b) < TOL64, f
+            else:
+                assert rel_l2(a, b) < TOL32_L2, f
+        for f in (1, 4):  # y components stay identically zero in 2-D
+            assert not np.any(eng.get(abi.ARR_A, f))
+    finally:
+        eng.close()
+        o.close()
+
+
+def test_adjoint_lagrange_source_and_kernels_match_oracle(tiny):
+    """Forward run storing snapshots at the output cadence, adjoint run driven
+    by a 7-point-Lagrange source at o_rad, kernel accumulation in FP64."""
+    W, S = tiny
+    nsteps = 18
+    steps = W.params.steps
+    nslots = nsteps // steps
+    fe, _ = D.make_engine(W, S, nsteps=nsteps)
+    fo, _ = make_oracle(W, S, nsteps=nsteps)
+    mask, series = D.forward_source(W, nsteps + 8)
+    for x in (fe, fo):
+        x.set_source(abi.SRC_SERIES, mask, series, 10 ** 9)
+        x.snapshots_alloc(nslots)
+    for s in range(nslots):
+        for x in (fe, fo):
+            x.step(s * steps, steps)
+            x.snapshot_store(s)
+    ae, acfg = D.make_engine(W, S, mode=abi.ADJOINT, nsteps=nsteps)
+    ao, _ = make_oracle(W, S, mode=abi.ADJOINT, nsteps=nsteps)
+    rng = np.random.default_rng(9)
+    nt = acfg.forcing_length + 8
+    vr = np.zeros((W.params.nx, W.params.ny, nt))
+    vr[:, :, :acfg.forcing_length] = rng.standard_normal((W.params.nx, W.params.ny,
+                                                          acfg.forcing_length)) * 1e-6
+    ae.snapshots_copy_from(fe)
+    ao.snapshots_alloc(nslots)
+    for s in range(nslots):
+        for f in range(6):
+            ao.snapshot_put(s, f, fo.snapshot_get(s, f))
+    grav = S.g_raw
+    for x in (ae, ao):
+        x.set_source(abi.SRC_LAGRANGE, vr, None, 10 ** 9)
+        x.kernel_begin(grav)
+    for s in range(nslots):
+        for x in (ae, ao):
+            x.step(s * steps, steps)
+            x.kernel_accumulate(nslots - 1 - s)
+    try:
+        for f in range(3, 6):
+            assert rel_err(ae.get(abi.ARR_A, f), ao.get(abi.ARR_A, f)) < TOL64
+        for k in (abi.KERN_C2, abi.KERN_RHO, abi.KERN_HESSIAN):
+            a, b = ae.kernel_get(k), ao.kernel_get(k)
+            assert np.linalg.norm(b) > 0
+            assert rel_err(a, b) < TOL64, k
+    finally:
+        for x in (fe, fo, ae, ao):
+            x.close()
+
+
+def test_sample_plane_norm(tiny):
+    W, S = tiny
+    eng, o = _run_pair(W, S, 6, abi.FP64)
+    try:
+        a = eng.get(abi.ARR_A, abi.V_Z)
+        k = eng.cfg.o_rad
+        assert np.array_equal(eng.plane(abi.V_Z, k), a[:, :, k - 1])
+        ijk = np.array([[0, 0, 1], [3, 5, k], [31, 31, 40]])
+        assert np.array_equal(eng.sample(abi.V_Z, ijk), a[ijk[:, 0], ijk[:, 1], ijk[:, 2] - 1])
+        assert abs(eng.norm(abi.V_Z) - o.norm(abi.V_Z)) <= 1e-10 * o.norm(abi.V_Z)
+        with pytest.raises(abi.SparcError):
+            eng.sample(abi.V_Z, np.array([[32, 0, 1]]))
+    finally:
+        eng.close()
+        o.close()
+
+
+def test_put_get_roundtrip_and_reset(tiny):
+    W, S = tiny
+    eng, _ = D.make_engine(W, S, precision=abi.FP32)
+    try:
+        f = _field(eng.shape, 3)
+        eng.put(abi.ARR_A, abi.XI_Z, f)
+        assert rel_err(eng.get(abi.ARR_A, abi.XI_Z), f) < 1e-7
+        eng.reset()
+        assert not np.any(eng.get(abi.ARR_A, abi.XI_Z))
+    finally:
+        eng.close()
