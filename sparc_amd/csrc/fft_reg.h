@@ -1,0 +1,406 @@
+// Compile-time-sized batched complex FFT in LDS with register butterflies of
+// radix 16/8/4/3 (Stockham autosort, natural order in and out).  Line length
+// N is a template parameter so every index computation is by constants; the
+// twiddle table is staged in LDS once per block.  Used for the sizes of the
+// benchmark grids (32..512); other sizes use the generic core of fft_lds.h.
+// Included inside namespace sparc by kernels.h.
+#pragma once
+
+template <int N>
+struct RPlan;
+template <>
+struct RPlan<32> {
+  static constexpr int P = 2;
+  static constexpr int R[3] = {8, 4, 1};
+};
+template <>
+struct RPlan<64> {
+  static constexpr int P = 2;
+  static constexpr int R[3] = {8, 8, 1};
+};
+template <>
+struct RPlan<96> {
+  static constexpr int P = 3;
+  static constexpr int R[3] = {8, 4, 3};
+};
+template <>
+struct RPlan<128> {
+  static constexpr int P = 2;
+  static constexpr int R[3] = {16, 8, 1};
+};
+template <>
+struct RPlan<192> {
+  static constexpr int P = 3;
+  static constexpr int R[3] = {8, 8, 3};
+};
+template <>
+struct RPlan<256> {
+  static constexpr int P = 2;
+  static constexpr int R[3] = {16, 16, 1};
+};
+template <>
+struct RPlan<384> {
+  static constexpr int P = 3;
+  static constexpr int R[3] = {16, 8, 3};
+};
+template <>
+struct RPlan<512> {
+  static constexpr int P = 3;
+  static constexpr int R[3] = {8, 8, 8};
+};
+
+// cos/sin(2 pi m / 16), m = 0..15
+__device__ constexpr double kC16[16] = {
+    1.0, 0.92387953251128675613, 0.70710678118654752440, 0.38268343236508977173,
+    0.0, -0.38268343236508977173, -0.70710678118654752440, -0.92387953251128675613,
+    -1.0, -0.92387953251128675613, -0.70710678118654752440, -0.38268343236508977173,
+    0.0, 0.38268343236508977173, 0.70710678118654752440, 0.92387953251128675613};
+__device__ constexpr double kS16[16] = {
+    0.0, 0.38268343236508977173, 0.70710678118654752440, 0.92387953251128675613,
+    1.0, 0.92387953251128675613, 0.70710678118654752440, 0.38268343236508977173,
+    0.0, -0.38268343236508977173, -0.70710678118654752440, -0.92387953251128675613,
+    -1.0, -0.92387953251128675613, -0.70710678118654752440, -0.38268343236508977173};
+
+// x * exp(DIR * 2 pi i M / R), M and R compile-time (R a power of two <= 16)
+template <typename T, int R, int M, int DIR>
+__device__ __forceinline__ C2<T> twc(C2<T> x) {
+  constexpr int m16 = ((M % R) * (16 / R)) % 16;
+  if constexpr (m16 == 0) {
+    return x;
+  } else if constexpr (m16 == 8) {
+    return {-x.x, -x.y};
+  } else if constexpr (m16 == 4) {  // exp(DIR i pi/2) = DIR*i
+    return DIR > 0 ? C2<T>{-x.y, x.x} : C2<T>{x.y, -x.x};
+  } else if constexpr (m16 == 12) {
+    return DIR > 0 ? C2<T>{x.y, -x.x} : C2<T>{-x.y, x.x};
+  } else {
+    const T c = (T)kC16[m16];
+    const T s = (T)(DIR > 0 ? kS16[m16] : -kS16[m16]);
+    return {x.x * c - x.y * s, x.x * s + x.y * c};
+  }
+}
+
+__host__ __device__ constexpr int bitrev(int i, int bits) {
+  int r = 0;
+  for (int b = 0; b < bits; ++b)
+    if (i & (1 << b)) r |= 1 << (bits - 1 - b);
+  return r;
+}
+__host__ __device__ constexpr int ilog2(int n) { return n <= 1 ? 0 : 1 + ilog2(n / 2); }
+
+template <typename T, int R, int DIR, int LEN, int I, int K>
+__device__ __forceinline__ void bfly(C2<T> (&v)[R]) {
+  const C2<T> a = v[I + K];
+  const C2<T> b = twc<T, LEN, K, DIR>(v[I + K + LEN / 2]);
+  v[I + K] = {a.x + b.x, a.y + b.y};
+  v[I + K + LEN / 2] = {a.x - b.x, a.y - b.y};
+}
+
+template <typename T, int R, int DIR, int LEN, int I, int K>
+__device__ __forceinline__ void bfly_k(C2<T> (&v)[R]) {
+  if constexpr (K < LEN / 2) {
+    bfly<T, R, DIR, LEN, I, K>(v);
+    bfly_k<T, R, DIR, LEN, I, K + 1>(v);
+  }
+}
+template <typename T, int R, int DIR, int LEN, int I>
+__device__ __forceinline__ void bfly_i(C2<T> (&v)[R]) {
+  if constexpr (I < R) {
+    bfly_k<T, R, DIR, LEN, I, 0>(v);
+    bfly_i<T, R, DIR, LEN, I + LEN>(v);
+  }
+}
+template <typename T, int R, int DIR, int LEN>
+__device__ __forceinline__ void bfly_len(C2<T> (&v)[R]) {
+  if constexpr (LEN <= R) {
+    bfly_i<T, R, DIR, LEN, 0>(v);
+    bfly_len<T, R, DIR, LEN * 2>(v);
+  }
+}
+
+// In-register DFT of R points (R = 2, 4, 8, 16: radix-2 DIT; R = 3: direct).
+template <typename T, int R, int DIR>
+__device__ __forceinline__ void dft_reg(C2<T> (&v)[R]) {
+  if constexpr (R == 3) {
+    const T h = (T)0.5, s3 = (T)0.86602540378443864676;
+    const C2<T> s = {v[1].x + v[2].x, v[1].y + v[2].y};
+    const C2<T> d = {v[1].x - v[2].x, v[1].y - v[2].y};
+    const C2<T> m = {v[0].x - h * s.x, v[0].y - h * s.y};
+    const C2<T> id = (DIR < 0) ? C2<T>{s3 * d.y, -s3 * d.x} : C2<T>{-s3 * d.y, s3 * d.x};
+    v[0] = {v[0].x + s.x, v[0].y + s.y};
+    v[1] = {m.x + id.x, m.y + id.y};
+    v[2] = {m.x - id.x, m.y - id.y};
+  } else {
+    constexpr int bits = ilog2(R);
+    C2<T> t[R];
+#pragma unroll
+    for (int i = 0; i < R; ++i) t[i] = v[bitrev(i, bits)];
+    bfly_len<T, R, DIR, 2>(t);
+#pragma unroll
+    for (int i = 0; i < R; ++i) v[i] = t[i];
+  }
+}
+
+// One in-place Stockham pass of radix R over LPB lines of length N in LDS:
+// every thread loads its butterflies into registers, the block synchronizes,
+// then the results are written back.
+template <typename T, int N, int R, int NS, int DIR, int LPB, int NT>
+__device__ __forceinline__ void rpass(C2<T>* buf, const C2<T>* __restrict__ tw) {
+  constexpr int NB = N / R;
+  constexpr int TOT = LPB * NB;
+  constexpr int IT = (TOT + NT - 1) / NT;
+  C2<T> v[IT][R];
+#pragma unroll
+  for (int it = 0; it < IT; ++it) {
+    const int w = it * NT + (int)threadIdx.x;
+    if (TOT % NT == 0 || w < TOT) {
+      const int line = w / NB;
+      const int j = w - line * NB;
+#pragma unroll
+      for (int r = 0; r < R; ++r) v[it][r] = buf[line * N + j + r * NB];
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int it = 0; it < IT; ++it) {
+    const int w = it * NT + (int)threadIdx.x;
+    if (TOT % NT == 0 || w < TOT) {
+      const int line = w / NB;
+      const int j = w - line * NB;
+      const int k = j % NS;
+      if constexpr (NS > 1) {
+#pragma unroll
+        for (int r = 1; r < R; ++r) {
+          C2<T> t = tw[r * k * (N / (NS * R))];
+          if (DIR > 0) t.y = -t.y;
+          v[it][r] = cmul(v[it][r], t);
+        }
+      }
+      dft_reg<T, R, DIR>(v[it]);
+      const int idxD = (j / NS) * NS * R + k;
+#pragma unroll
+      for (int r = 0; r < R; ++r) buf[line * N + idxD + r * NS] = v[it][r];
+    }
+  }
+  __syncthreads();
+}
+
+// Full in-place FFT of LPB lines in buf.
+template <typename T, int N, int DIR, int LPB, int NT>
+__device__ __forceinline__ void rfft(C2<T>* buf, const C2<T>* __restrict__ tw) {
+  using PL = RPlan<N>;
+  constexpr int R0 = PL::R[0], R1 = PL::R[1], R2 = PL::R[2];
+  rpass<T, N, R0, 1, DIR, LPB, NT>(buf, tw);
+  rpass<T, N, R1, R0, DIR, LPB, NT>(buf, tw);
+  if constexpr (PL::P == 3) rpass<T, N, R2, R0 * R1, DIR, LPB, NT>(buf, tw);
+}
+
+// lines per block: 32 KiB of complex values per block
+template <typename T, int N>
+struct XCfg {
+  static constexpr int LPB = (32768 / (int)sizeof(C2<T>)) / N > 0
+                                 ? (32768 / (int)sizeof(C2<T>)) / N
+                                 : 1;
+  static constexpr int NT = 256;
+};
+
+// x-lines with compile-time N: same contract as k_xlines (kernels.h).
+// blockIdx.y selects one of up to 3 fields (a/s/out/spec given per field).
+template <typename T>
+struct XArgs {
+  const T* a[3];
+  const T* s[3];
+  T* out[3];
+  C2<T>* spec[3];
+};
+
+template <typename T, int N, int MODE>
+__global__ __launch_bounds__(256) void k_xlines_n(int nrows, int nlines, XArgs<T> X, T beta,
+                                                  const T* __restrict__ mult,
+                                                  const C2<T>* __restrict__ twg, int ny, int nz,
+                                                  long nxy) {
+  using CF = XCfg<T, N>;
+  constexpr int LPB = CF::LPB, NT = CF::NT;
+  __shared__ C2<T> b0[LPB * N];
+  __shared__ C2<T> tw[N];
+  const int fld = blockIdx.y;
+  const T* __restrict__ a = X.a[fld];
+  const T* __restrict__ s = X.s[fld];
+  T* out = X.out[fld];
+  C2<T>* spec = X.spec[fld];
+  for (int m = threadIdx.x; m < N; m += NT) tw[m] = twg[m];
+  const int L0 = blockIdx.x * LPB;
+  const int nl = min(LPB, nlines - L0);
+#pragma unroll 4
+  for (int e = threadIdx.x; e < LPB * N; e += NT) {
+    const int l = e / N, x = e - l * N;
+    C2<T> v = {0, 0};
+    if (l < nl) {
+      const long L = L0 + l;
+      if (MODE == XM_DERIV || MODE == XM_FILTER) {
+        const long r0 = 2 * L, r1 = r0 + 1;
+        T re = a[r0 * N + x];
+        if (s) re = re + beta * s[r0 * N + x];
+        T im = 0;
+        if (r1 < nrows) {
+          im = a[r1 * N + x];
+          if (s) im = im + beta * s[r1 * N + x];
+        }
+        v = {re, im};
+      } else if (MODE == XM_DAMP_FWD) {
+        const long q = L / ny, j = L - q * ny;
+        const long k0 = 2 * q, k1 = k0 + 1;
+        const long o0 = x + (long)N * j + nxy * k0;
+        T re = a[o0];
+        if (s) re = re + beta * s[o0];
+        T im = 0;
+        if (k1 < nz) {
+          im = a[o0 + nxy];
+          if (s) im = im + beta * s[o0 + nxy];
+        }
+        v = {re, im};
+      } else {
+        v = spec[L * N + x];
+      }
+    }
+    b0[e] = v;
+  }
+  __syncthreads();
+  if (MODE == XM_DAMP_INV)
+    rfft<T, N, +1, LPB, NT>(b0, tw);
+  else
+    rfft<T, N, -1, LPB, NT>(b0, tw);
+  C2<T>* r = b0;
+  if (MODE == XM_DAMP_FWD) {
+    for (int e = threadIdx.x; e < nl * N; e += NT) spec[(long)L0 * N + e] = r[e];
+    return;
+  }
+  if (MODE == XM_DERIV || MODE == XM_FILTER) {
+#pragma unroll 4
+    for (int e = threadIdx.x; e < LPB * N; e += NT) {
+      const int x = e % N;
+      const T c = mult[x];
+      C2<T> v = r[e];
+      if (MODE == XM_DERIV)
+        v = {-c * v.y, c * v.x};
+      else
+        v = {c * v.x, c * v.y};
+      r[e] = v;
+    }
+    __syncthreads();
+    rfft<T, N, +1, LPB, NT>(b0, tw);
+#pragma unroll 4
+    for (int e = threadIdx.x; e < nl * N; e += NT) {
+      const int l = e / N, x = e - l * N;
+      const long L = L0 + l;
+      const long r0 = 2 * L, r1 = r0 + 1;
+      out[r0 * N + x] = r[e].x;
+      if (r1 < nrows) out[r1 * N + x] = r[e].y;
+    }
+  } else {
+#pragma unroll 4
+    for (int e = threadIdx.x; e < nl * N; e += NT) {
+      const int l = e / N, x = e - l * N;
+      const long L = L0 + l;
+      const long q = L / ny, j = L - q * ny;
+      const long k0 = 2 * q, k1 = k0 + 1;
+      const long o0 = x + (long)N * j + nxy * k0;
+      if (k0 != 0 && k0 != nz - 1) out[o0] = out[o0] - r[e].x;
+      if (k1 < nz - 1) out[o0 + nxy] = out[o0 + nxy] - r[e].y;
+    }
+  }
+}
+
+// y-lines with compile-time NY: tile of TX columns of one plane (DERIV,
+// FILTER: TX real columns packed in TX/2 complex lines) or TX complex x-bins of
+// a plane-pair spectrum (DAMP).  blockIdx.z selects the field.
+__host__ __device__ constexpr int pow2_floor(int v) { return v >= 2 ? 2 * pow2_floor(v / 2) : 1; }
+
+template <typename T, int NY, int MODE>
+struct YCfg {
+  // 32 KiB of complex values per block; row segments of 128 bytes at most
+  static constexpr int LINES = pow2_floor((32768 / (int)sizeof(C2<T>)) / NY);
+  static constexpr int CAP = 128 / (int)sizeof(C2<T>);  // complex lines of 128 B
+  static constexpr int NL = LINES < CAP ? LINES : CAP;
+  static constexpr int TX = (MODE == YM_DAMP) ? NL : 2 * NL;
+};
+
+template <typename T, int NY, int MODE>
+__global__ __launch_bounds__(256) void k_ylines_n(int nx, int nz, XArgs<T> X, T beta,
+                                                  const T* __restrict__ mult,
+                                                  const C2<T>* __restrict__ twg,
+                                                  const T* __restrict__ rates) {
+  using CF = YCfg<T, NY, MODE>;
+  constexpr int TX = CF::TX, NL = CF::NL, NT = 256;
+  __shared__ C2<T> b0[NL * NY];
+  __shared__ C2<T> tw[NY];
+  const int fld = blockIdx.z;
+  const T* __restrict__ a = X.a[fld];
+  const T* __restrict__ s = X.s[fld];
+  T* out = X.out[fld];
+  C2<T>* spec = X.spec[fld];
+  for (int m = threadIdx.x; m < NY; m += NT) tw[m] = twg[m];
+  const long nxy = (long)nx * NY;
+  const int i0 = blockIdx.x * TX;
+  const long k = blockIdx.y;
+  if (MODE == YM_DERIV || MODE == YM_FILTER) {
+    T* b0r = reinterpret_cast<T*>(b0);
+#pragma unroll 4
+    for (int e = threadIdx.x; e < TX * NY; e += NT) {
+      const int y = e / TX, xx = e - y * TX;
+      const long o = (i0 + xx) + (long)nx * y + nxy * k;
+      T v = a[o];
+      if (s) v = v + beta * s[o];
+      b0r[2 * ((xx >> 1) * NY + y) + (xx & 1)] = v;
+    }
+    __syncthreads();
+    rfft<T, NY, -1, NL, NT>(b0, tw);
+    C2<T>* r = b0;
+#pragma unroll 4
+    for (int e = threadIdx.x; e < NL * NY; e += NT) {
+      const int y = e % NY;
+      const T c = mult[y];
+      C2<T> v = r[e];
+      if (MODE == YM_DERIV)
+        v = {-c * v.y, c * v.x};
+      else
+        v = {c * v.x, c * v.y};
+      r[e] = v;
+    }
+    __syncthreads();
+    rfft<T, NY, +1, NL, NT>(b0, tw);
+    const T* rr = reinterpret_cast<const T*>(b0);
+#pragma unroll 4
+    for (int e = threadIdx.x; e < TX * NY; e += NT) {
+      const int y = e / TX, xx = e - y * TX;
+      const long o = (i0 + xx) + (long)nx * y + nxy * k;
+      out[o] = rr[2 * ((xx >> 1) * NY + y) + (xx & 1)];
+    }
+  } else {
+    const long q = k;
+#pragma unroll 4
+    for (int e = threadIdx.x; e < NL * NY; e += NT) {
+      const int y = e / NL, mm = e - y * NL;
+      b0[mm * NY + y] = spec[(q * NY + y) * nx + i0 + mm];
+    }
+    __syncthreads();
+    rfft<T, NY, -1, NL, NT>(b0, tw);
+    C2<T>* r = b0;
+    const int nh = nx / 2 + 1;
+#pragma unroll 4
+    for (int e = threadIdx.x; e < NL * NY; e += NT) {
+      const int mm = e / NY, y = e - mm * NY;
+      const int m = i0 + mm;
+      const int mi = (m <= nx / 2) ? m : nx - m;
+      const T c = rates[mi + (long)nh * y];
+      r[e] = {c * r[e].x, c * r[e].y};
+    }
+    __syncthreads();
+    rfft<T, NY, +1, NL, NT>(b0, tw);
+#pragma unroll 4
+    for (int e = threadIdx.x; e < NL * NY; e += NT) {
+      const int y = e / NL, mm = e - y * NL;
+      spec[(q * NY + y) * nx + i0 + mm] = b0[mm * NY + y];
+    }
+  }
+}

@@ -55,6 +55,8 @@ __device__ __forceinline__ int pml_index(int k, int nz, int nbot, int ntop) {
   return -1;
 }
 
+#include "fft_reg.h"
+
 // ---------------------------------------------------------------------------
 // x-lines: batched FFT along the contiguous index.  Real rows are packed two
 // per complex line.  DERIV: out = IFFT(i k * FFT(a + beta*s)) (ddx,
@@ -224,162 +226,7 @@ __global__ __launch_bounds__(256) void k_ylines(int nx, int ny, int nz, int TX,
   }
 }
 
-// ---------------------------------------------------------------------------
-// z-columns: DBYD2 + MTRIDAG ispecial=0 (dbyd2.f90:88-199, mtridag.f90:94-111)
-// with the LU factors of the constant matrix precomputed in FP64 on the host,
-// one thread per (x,y) column, lanes coalesced along x.  The forward sweep
-// forms the right-hand side from a 6-value register window of B = a + beta*s;
-// the backward sweep hands d/dz = x*stretch(k) to a fused epilogue (the
-// pointwise part of the RHS operator of one LDDRK stage).
-// ---------------------------------------------------------------------------
-template <typename T, int EPI>
-__global__ __launch_bounds__(256) void k_zsolve(Fld<T> F, TriCoef<T> tc,
-                                                const T* __restrict__ lowr,
-                                                const T* __restrict__ invb,
-                                                const T* __restrict__ gam, int ibc,
-                                                const T* __restrict__ inA,
-                                                const T* __restrict__ inS, T beta_in,
-                                                T* __restrict__ ud,
-                                                const T* __restrict__ prov, T provscale,
-                                                T* __restrict__ out) {
-  const int c = blockIdx.x * blockDim.x + threadIdx.x;
-  if (c >= F.nxy) return;
-  const long nxy = F.nxy;
-  const int n = F.nz;
-  auto B = [&](int k) -> T {
-    const long q = c + nxy * k;
-    T v = inA[q];
-    if (inS) v = v + beta_in * inS[q];
-    return v;
-  };
-  // ---- forward sweep
-  T wm1 = 0, w0 = B(0), w1 = B(1), w2 = B(2), w3 = B(3), w4 = B(4);
-  T r0;
-  if (ibc == 0)
-    r0 = 0;
-  else if (ibc == 1 || ibc == 3)
-    r0 = tc.b1 * w0 + tc.b2 * w1 + tc.b3 * w2 + tc.b4 * w3;
-  else
-    r0 = prov ? prov[c] * provscale : (T)0;
-  const T r1 = tc.c1 * w0 + tc.c2 * w1 + tc.c3 * w2 + tc.c4 * w3 + tc.c5 * w4;
-  if (ibc & 1) r0 = (tc.alpha2 * r0 - tc.beta1 * r1) * tc.cst;
-  T u = r0 * invb[0];
-  ud[c] = u;
-  u = (r1 - lowr[1] * u) * invb[1];
-  ud[c + nxy] = u;
-  for (int k = 2; k <= n - 3; ++k) {
-    const T r = tc.ci1 * (w3 - w1) + tc.ci2 * (w4 - w0);
-    u = (r - lowr[k] * u) * invb[k];
-    ud[c + nxy * k] = u;
-    wm1 = w0;
-    w0 = w1;
-    w1 = w2;
-    w2 = w3;
-    w3 = w4;
-    if (k + 3 <= n - 1) w4 = B(k + 3);
-  }
-  // window now holds B(n-5..n-1) in wm1..w3
-  const T rn2 = -(tc.c1 * w3 + tc.c2 * w2 + tc.c3 * w1 + tc.c4 * w0 + tc.c5 * wm1);
-  u = (rn2 - lowr[n - 2] * u) * invb[n - 2];
-  ud[c + nxy * (n - 2)] = u;
-  T rn1;
-  if (ibc == 0)
-    rn1 = 0;
-  else if (ibc == 1 || ibc == 2) {
-    rn1 = -(tc.b1 * w3 + tc.b2 * w2 + tc.b3 * w1 + tc.b4 * w0);
-    rn1 = (tc.alpha2 * rn1 - tc.beta1 * rn2) * tc.cst;
-  } else {
-    rn1 = prov ? prov[c] * provscale : (T)0;
-  }
-  u = (rn1 - lowr[n - 1] * u) * invb[n - 1];
-  // ---- backward sweep + epilogue
-  T x = u;
-  for (int k = n - 1; k >= 0; --k) {
-    if (k < n - 1) x = ud[c + nxy * k] - gam[k + 1] * x;
-    const T val = x * F.stretch[k];
-    const long q = c + nxy * k;
-    if (EPI == EPI_STORE) {
-      out[q] = val;
-    } else if (EPI == EPI_P3D || EPI == EPI_P2D) {
-      T dz = val;
-      const int pi = pml_index(k, n, F.npmlbot, F.npmltop);
-      if (pi >= 0) {
-        const long pq = c + nxy * pi;
-        T psi = F.pmlvars[0][pq];
-        if (F.beta != (T)0) psi = psi + F.beta * F.scrpml[0][pq];
-        F.scrpml[0][pq] = bgv(F, SPARC_BG_AZ, c, k) * dz + bgv(F, SPARC_BG_BZPML, c, k) * psi;
-        dz = dz + psi;
-      }
-      T div;
-      if (EPI == EPI_P3D)
-        div = F.W[0][q] + F.W[1][q] + dz;
-      else
-        div = F.W[0][q] + dz;
-      T xz = F.a[SPARC_XI_Z][q];
-      if (F.beta != (T)0) xz = xz + F.beta * F.scr[SPARC_XI_Z][q];
-      F.W[3][q] = -bgv(F, SPARC_BG_C2RHO0, c, k) * div - xz * bgv(F, SPARC_BG_GRADP0_Z, c, k);
-      F.W[4][q] = -bgv(F, SPARC_BG_RHO0, c, k) * div - xz * bgv(F, SPARC_BG_GRADRHO0_Z, c, k);
-    } else {  // EPI_RHS3D / EPI_RHS2D
-      T gz = val;
-      const int pi = pml_index(k, n, F.npmlbot, F.npmltop);
-      if (pi >= 0) {
-        const long pq = c + nxy * pi;
-        T psi = F.pmlvars[1][pq];
-        if (F.beta != (T)0) psi = psi + F.beta * F.scrpml[1][pq];
-        F.scrpml[1][pq] = bgv(F, SPARC_BG_AZ, c, k) * gz + bgv(F, SPARC_BG_BZPML, c, k) * psi;
-        gz = gz + psi;
-      }
-      const T ri = bgv(F, SPARC_BG_RHOINV, c, k);
-      const T rho = F.W[4][q];
-      if (EPI == EPI_RHS3D) {
-        T rvx = -ri * F.W[0][q];
-        T rvy = -ri * F.W[1][q];
-        T rvz = -ri * gz;
-        rvz = rvz - rho * F.g[k] * ri;
-        if (k == F.src_plane)
-          rvz = rvz + F.forcing[c] / bgv(F, SPARC_BG_RHO0, c, k) * F.delta_width;
-        if (k == 0 || k == n - 1) {
-#pragma unroll
-          for (int f = 0; f < SPARC_NFIELDS; ++f) F.scr[f][q] = 0;
-        } else {
-          T tv[3];
-#pragma unroll
-          for (int f = 0; f < 3; ++f) {
-            tv[f] = F.a[SPARC_V_X + f][q];
-            if (F.beta != (T)0) tv[f] = tv[f] + F.beta * F.scr[SPARC_V_X + f][q];
-          }
-          F.scr[SPARC_XI_X][q] = tv[0];
-          F.scr[SPARC_XI_Y][q] = tv[1];
-          F.scr[SPARC_XI_Z][q] = tv[2];
-          F.scr[SPARC_V_X][q] = rvx;
-          F.scr[SPARC_V_Y][q] = rvy;
-          F.scr[SPARC_V_Z][q] = rvz;
-        }
-      } else {
-        T rvx = -ri * F.W[0][q];
-        T rvz = -ri * (gz + rho * F.g[k]);
-        if (k == F.src_plane)
-          rvz = rvz + F.forcing[c] * F.delta_width / bgv(F, SPARC_BG_RHO0, c, k);
-        if (k == 0) {
-          F.scr[SPARC_XI_X][q] = 0;
-          F.scr[SPARC_XI_Z][q] = 0;
-          F.scr[SPARC_V_X][q] = 0;
-          F.scr[SPARC_V_Z][q] = 0;
-        } else {
-          T tvx = F.a[SPARC_V_X][q], tvz = F.a[SPARC_V_Z][q];
-          if (F.beta != (T)0) {
-            tvx = tvx + F.beta * F.scr[SPARC_V_X][q];
-            tvz = tvz + F.beta * F.scr[SPARC_V_Z][q];
-          }
-          F.scr[SPARC_XI_X][q] = tvx;
-          F.scr[SPARC_XI_Z][q] = tvz;
-          F.scr[SPARC_V_X][q] = rvx;
-          F.scr[SPARC_V_Z][q] = rvz;
-        }
-      }
-    }
-  }
-}
+#include "zsolve.h"
 
 // ---------------------------------------------------------------------------
 // a += dt*scr fused with the 11-point z filter (step.f90:114-118,
@@ -389,8 +236,9 @@ __global__ __launch_bounds__(256) void k_zsolve(Fld<T> F, TriCoef<T> tc,
 // input plane k+5 is read, so the update is in place.
 // ---------------------------------------------------------------------------
 template <typename T>
-__global__ __launch_bounds__(256) void k_update_filter(T* a, const T* __restrict__ scr, T dt,
-                                                       int nxy, int nz, int kst) {
+__global__ __launch_bounds__(64) void k_update_filter(T* a, const T* __restrict__ scr, T dt,
+                                                      int nxy, int nz, int kst) {
+  constexpr int U = 16;  // planes loaded one chunk ahead of use
   const int c = blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= nxy) return;
   const T c0 = (T)0.75390625, h1 = (T)(0.5 * 0.41015625), h2 = (T)(0.5 * -0.234375),
@@ -408,21 +256,47 @@ __global__ __launch_bounds__(256) void k_update_filter(T* a, const T* __restrict
   for (int i = 0; i < 5; ++i) w[i] = 0;
 #pragma unroll
   for (int i = 5; i < 11; ++i) w[i] = L(i - 5);
-  for (int k = 0; k < nz; ++k) {
-    T o;
-    if (k == 0 || k == nz - 1)
-      o = w[5];
-    else if (k <= 4)
-      o = (kst == 2) ? (T)0.5 * (w[6] + w[4]) : w[5];
-    else if (k >= nz - 5)
-      o = (T)0.5 * (w[6] + w[4]);
-    else
-      o = c0 * w[5] + h1 * (w[4] + w[6]) + h2 * (w[3] + w[7]) + h3 * (w[2] + w[8]) +
-          h4 * (w[1] + w[9]) + h5 * (w[0] + w[10]);
-    a[c + (long)nxy * k] = o;
+  T na[U], ns[U];
+  auto load_chunk = [&](int kb) {
 #pragma unroll
-    for (int i = 0; i < 10; ++i) w[i] = w[i + 1];
-    w[10] = L(k + 6);
+    for (int j = 0; j < U; ++j) {
+      const int kk = kb + j;
+      if (kk < nz) {
+        const long q = c + (long)nxy * kk;
+        na[j] = a[q];
+        if (scr) ns[j] = scr[q];
+      }
+    }
+  };
+  load_chunk(6);
+  for (int k0 = 0; k0 < nz; k0 += U) {
+    T ca[U], cs[U];
+#pragma unroll
+    for (int j = 0; j < U; ++j) {
+      ca[j] = na[j];
+      cs[j] = ns[j];
+    }
+    if (k0 + U + 6 < nz) load_chunk(k0 + U + 6);
+#pragma unroll
+    for (int j = 0; j < U; ++j) {
+      const int k = k0 + j;
+      if (k < nz) {
+        T o;
+        if (k == 0 || k == nz - 1)
+          o = w[5];
+        else if (k <= 4)
+          o = (kst == 2) ? (T)0.5 * (w[6] + w[4]) : w[5];
+        else if (k >= nz - 5)
+          o = (T)0.5 * (w[6] + w[4]);
+        else
+          o = c0 * w[5] + h1 * (w[4] + w[6]) + h2 * (w[3] + w[7]) + h3 * (w[2] + w[8]) +
+              h4 * (w[1] + w[9]) + h5 * (w[0] + w[10]);
+        a[c + (long)nxy * k] = o;
+#pragma unroll
+        for (int i = 0; i < 10; ++i) w[i] = w[i + 1];
+        w[10] = (k + 6 < nz) ? (scr ? ca[j] + dt * cs[j] : ca[j]) : (T)0;
+      }
+    }
   }
 }
 

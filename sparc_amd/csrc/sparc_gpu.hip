@@ -599,10 +599,23 @@ static TriCoef<T> make_tc(const sparc_ctx* c) {
 
 static inline int cdiv(long a, long b) { return (int)((a + b - 1) / b); }
 
-// x-line launch: lines per block so a block holds ~2 KiB complex values (FP32)
+static bool fast_size(int n) {
+  return n == 32 || n == 64 || n == 96 || n == 128 || n == 192 || n == 256 || n == 384 ||
+         n == 512;
+}
+
+template <typename T, int MODE, int N>
+static void launch_xn(sparc_ctx* c, int nf, const XArgs<T>& X, T beta, const T* mult, int nrows,
+                      int nlines) {
+  constexpr int LPB = XCfg<T, N>::LPB;
+  hipLaunchKernelGGL((k_xlines_n<T, N, MODE>), dim3(cdiv(nlines, LPB), nf), dim3(256), 0,
+                     c->stream, nrows, nlines, X, beta, mult, (const C2<T>*)c->twx, c->cfg.ny,
+                     c->cfg.nz, c->nxy);
+}
+
+// x-lines over nf (1..3) fields that share the mode and beta.
 template <typename T, int MODE>
-static int launch_x(sparc_ctx* c, const T* a, const T* s, T beta, T* out, const T* mult,
-                    C2<T>* spec) {
+static int launch_xm(sparc_ctx* c, int nf, const XArgs<T>& X, T beta, const T* mult) {
   const int n = c->cfg.nx;
   const long nrows = (long)c->cfg.ny * c->cfg.nz;
   long nlines;
@@ -610,43 +623,108 @@ static int launch_x(sparc_ctx* c, const T* a, const T* s, T beta, T* out, const 
     nlines = (nrows + 1) / 2;
   else
     nlines = (long)((c->cfg.nz + 1) / 2) * c->cfg.ny;
-  int LPB = std::max(1, (int)((sizeof(T) == 4 ? 2048 : 1024) / n));
-  const size_t lds = 2 * (size_t)LPB * n * sizeof(C2<T>);
   // algorithmic arrays: inputs (a, s), outputs; the damping spectrum counts as one field
   const double fb = (double)c->N * sizeof(T);
   double nb;
   if (MODE == XM_DERIV || MODE == XM_FILTER || MODE == XM_DAMP_FWD)
-    nb = (s ? 2 : 1) + 1;
+    nb = (X.s[0] ? 2 : 1) + 1;
   else
     nb = 3;
   ProfScope ps(c, (MODE == XM_DAMP_FWD || MODE == XM_DAMP_INV) ? SPARC_KC_DAMP : SPARC_KC_XLINES,
-               nb * fb);
-  hipLaunchKernelGGL((k_xlines<T, MODE>), dim3(cdiv(nlines, LPB)), dim3(256), lds, c->stream, n,
-                     (int)nrows, (int)nlines, LPB, a, s, beta, out, mult, c->px,
-                     (const C2<T>*)c->twx, spec, c->cfg.ny, c->cfg.nz, c->nxy);
-  LAUNCHCHK(c);
+               nf * nb * fb);
+  if (fast_size(n)) {
+    switch (n) {
+      case 32: launch_xn<T, MODE, 32>(c, nf, X, beta, mult, (int)nrows, (int)nlines); break;
+      case 64: launch_xn<T, MODE, 64>(c, nf, X, beta, mult, (int)nrows, (int)nlines); break;
+      case 96: launch_xn<T, MODE, 96>(c, nf, X, beta, mult, (int)nrows, (int)nlines); break;
+      case 128: launch_xn<T, MODE, 128>(c, nf, X, beta, mult, (int)nrows, (int)nlines); break;
+      case 192: launch_xn<T, MODE, 192>(c, nf, X, beta, mult, (int)nrows, (int)nlines); break;
+      case 256: launch_xn<T, MODE, 256>(c, nf, X, beta, mult, (int)nrows, (int)nlines); break;
+      case 384: launch_xn<T, MODE, 384>(c, nf, X, beta, mult, (int)nrows, (int)nlines); break;
+      default: launch_xn<T, MODE, 512>(c, nf, X, beta, mult, (int)nrows, (int)nlines); break;
+    }
+    LAUNCHCHK(c);
+    return SPARC_OK;
+  }
+  int LPB = std::max(1, (int)((sizeof(T) == 4 ? 2048 : 1024) / n));
+  const size_t lds = 2 * (size_t)LPB * n * sizeof(C2<T>);
+  for (int f = 0; f < nf; ++f) {
+    hipLaunchKernelGGL((k_xlines<T, MODE>), dim3(cdiv(nlines, LPB)), dim3(256), lds, c->stream,
+                       n, (int)nrows, (int)nlines, LPB, X.a[f], X.s[f], beta, X.out[f], mult,
+                       c->px, (const C2<T>*)c->twx, X.spec[f], c->cfg.ny, c->cfg.nz, c->nxy);
+    LAUNCHCHK(c);
+  }
   return SPARC_OK;
 }
 
 template <typename T, int MODE>
-static int launch_y(sparc_ctx* c, const T* a, const T* s, T beta, T* out, const T* mult,
+static int launch_x(sparc_ctx* c, const T* a, const T* s, T beta, T* out, const T* mult,
                     C2<T>* spec) {
+  XArgs<T> X{};
+  X.a[0] = a;
+  X.s[0] = s;
+  X.out[0] = out;
+  X.spec[0] = spec;
+  return launch_xm<T, MODE>(c, 1, X, beta, mult);
+}
+
+template <typename T, int MODE, int NY>
+static void launch_yn(sparc_ctx* c, int nf, const XArgs<T>& X, T beta, const T* mult) {
+  constexpr int TX = YCfg<T, NY, MODE>::TX;
+  const int planes = (MODE == YM_DAMP) ? (c->cfg.nz + 1) / 2 : c->cfg.nz;
+  hipLaunchKernelGGL((k_ylines_n<T, NY, MODE>), dim3(c->cfg.nx / TX, planes, nf), dim3(256), 0,
+                     c->stream, c->cfg.nx, c->cfg.nz, X, beta, mult, (const C2<T>*)c->twy,
+                     (const T*)c->rates);
+}
+
+template <typename T, int MODE, int NY>
+static bool y_fits(int nx) {
+  return nx % YCfg<T, NY, MODE>::TX == 0;
+}
+
+template <typename T, int MODE>
+static int launch_ym(sparc_ctx* c, int nf, const XArgs<T>& X, T beta, const T* mult) {
   const int nx = c->cfg.nx, ny = c->cfg.ny;
-  // tile width: 128 bytes of x per row, LDS capped at 64 KiB
+  const double fb = (double)c->N * sizeof(T);
+  ProfScope ps(c, MODE == YM_DAMP ? SPARC_KC_DAMP : SPARC_KC_YLINES,
+               nf * (MODE == YM_DAMP ? 2.0 : (X.s[0] ? 3.0 : 2.0)) * fb);
+  bool done = true;
+  switch (ny) {
+#define YCASE(NYV)                                              case NYV:                                                       if (y_fits<T, MODE, NYV>(nx))                                   launch_yn<T, MODE, NYV>(c, nf, X, beta, mult);              else                                                            done = false;                                               break;
+    YCASE(32) YCASE(64) YCASE(96) YCASE(128) YCASE(192) YCASE(256) YCASE(384) YCASE(512)
+#undef YCASE
+    default:
+      done = false;
+  }
+  if (done) {
+    LAUNCHCHK(c);
+    return SPARC_OK;
+  }
+  // generic path: tile width 128 bytes of x per row, LDS capped at 64 KiB
   int TX = (int)(128 / sizeof(T));
   if (MODE == YM_DAMP) TX /= 2;
   while (TX > 2 && (size_t)TX * ny * sizeof(C2<T>) * ((MODE == YM_DAMP) ? 2 : 1) > 65536) TX /= 2;
   const size_t lds = (MODE == YM_DAMP) ? 2 * (size_t)TX * ny * sizeof(C2<T>)
                                        : (size_t)TX * ny * sizeof(C2<T>);
   const int planes = (MODE == YM_DAMP) ? (c->cfg.nz + 1) / 2 : c->cfg.nz;
-  const double fb = (double)c->N * sizeof(T);
-  ProfScope ps(c, MODE == YM_DAMP ? SPARC_KC_DAMP : SPARC_KC_YLINES,
-               (MODE == YM_DAMP ? 2.0 : (s ? 3.0 : 2.0)) * fb);
-  hipLaunchKernelGGL((k_ylines<T, MODE>), dim3(nx / TX, planes), dim3(256), lds, c->stream, nx,
-                     ny, c->cfg.nz, TX, a, s, beta, out, mult, c->py, (const C2<T>*)c->twy, spec,
-                     (const T*)c->rates);
-  LAUNCHCHK(c);
+  for (int f = 0; f < nf; ++f) {
+    hipLaunchKernelGGL((k_ylines<T, MODE>), dim3(nx / TX, planes), dim3(256), lds, c->stream, nx,
+                       ny, c->cfg.nz, TX, X.a[f], X.s[f], beta, X.out[f], mult, c->py,
+                       (const C2<T>*)c->twy, X.spec[f], (const T*)c->rates);
+    LAUNCHCHK(c);
+  }
   return SPARC_OK;
+}
+
+template <typename T, int MODE>
+static int launch_y(sparc_ctx* c, const T* a, const T* s, T beta, T* out, const T* mult,
+                    C2<T>* spec) {
+  XArgs<T> X{};
+  X.a[0] = a;
+  X.s[0] = s;
+  X.out[0] = out;
+  X.spec[0] = spec;
+  return launch_ym<T, MODE>(c, 1, X, beta, mult);
 }
 
 template <typename T, int EPI>
@@ -730,21 +808,19 @@ static int run_stage(sparc_ctx* c, Fld<T>& F, int s) {
     if ((rc = launch_x<T, XM_DERIV>(c, F.W[3], nullptr, 0, F.W[0], dmx, nullptr))) return rc;
     if ((rc = launch_y<T, YM_DERIV>(c, F.W[3], nullptr, 0, F.W[1], dmy, nullptr))) return rc;
     const bool damp = c->cfg.damp_waves && c->damping_set;
-    if (damp)  // DAMP_VELOCITY forward half, on the stage's temp v (damping.f90:135-160)
-      for (int f = 0; f < 3; ++f) {
-        if ((rc = launch_x<T, XM_DAMP_FWD>(c, F.a[3 + f], S(3 + f), beta, nullptr, nullptr,
-                                           (C2<T>*)c->spec[f])))
-          return rc;
-        if ((rc = launch_y<T, YM_DAMP>(c, nullptr, nullptr, 0, nullptr, nullptr,
-                                       (C2<T>*)c->spec[f])))
-          return rc;
-      }
+    XArgs<T> DX{};
+    for (int f = 0; f < 3; ++f) {
+      DX.a[f] = F.a[3 + f];
+      DX.s[f] = S(3 + f);
+      DX.out[f] = F.scr[3 + f];
+      DX.spec[f] = (C2<T>*)c->spec[f];
+    }
+    if (damp) {  // DAMP_VELOCITY forward half, on the stage's temp v (damping.f90:135-160)
+      if ((rc = launch_xm<T, XM_DAMP_FWD>(c, 3, DX, beta, nullptr))) return rc;
+      if ((rc = launch_ym<T, YM_DAMP>(c, 3, DX, 0, nullptr))) return rc;
+    }
     if ((rc = launch_z<T, EPI_RHS3D>(c, F, 0, F.W[3], nullptr, 0, nullptr, 0, nullptr))) return rc;
-    if (damp)
-      for (int f = 0; f < 3; ++f)
-        if ((rc = launch_x<T, XM_DAMP_INV>(c, nullptr, nullptr, 0, F.scr[3 + f], nullptr,
-                                           (C2<T>*)c->spec[f])))
-          return rc;
+    if (damp && (rc = launch_xm<T, XM_DAMP_INV>(c, 3, DX, 0, nullptr))) return rc;
   } else {
     // option 17 (physics2d.f90:959-1069)
     if ((rc = launch_x<T, XM_DERIV>(c, F.a[0], S(0), beta, F.W[0], dmx, nullptr))) return rc;

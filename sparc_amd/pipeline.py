@@ -1,0 +1,219 @@
+"""One source's forward + adjoint + kernel solve on the engine, following the
+kernel-mode sequence of the reference:
+
+  forward  TIMESTEPPING with the forward source (driver.f90:326 -> 398-547):
+           maxtime+1 steps; after step i with mod(i, steps) == 0 the v_z
+           samples at the receivers are recorded (vz_cc.bin, driver.f90:460-476)
+           and the partial state is kept (WRITE_OUT_PARTIAL_STATE: xi, acc) --
+           here in device memory instead of FITS files;
+  measure  travel times and the adjoint source (ADJOINT_SOURCE_COMP,
+           driver.f90:729-917) on the host in FP64;
+  adjoint  TIMESTEPPING with the time-reversed adjoint source interpolated by
+           LAGRANGE_INTERP at the receivers (driver.f90:350, physics.f90:678);
+           the kernel sums of PRODUCE_KERNELS (kernels.f90:197-313) are
+           accumulated in the loop, pairing forward stamp (k-1)*steps with
+           adjoint stamp maxtime-(k-1)*steps (kernels.f90:203,241).
+
+Scaling (nondimensional time t_n = t_s dimc/diml, volume in Rsun^3):
+  misfit chi = 0.5 sum tau^2 [s^2];  a_i(t) = d chi / d u_i(t);
+  the adjoint is driven at the receivers by a_i(T - t) (diml/dimc) delta^3 / rho0;
+  K_lnc(x) = -2 rho0 c^2 sum_k div(xi_f) div(v_a) (steps * deltat)
+  so that d chi = sum_cells K_lnc dlnc dV  (kernels.f90:537 with dimen = 1).
+"""
+from __future__ import annotations
+
+from dataclasses import dataclass
+
+import numpy as np
+
+from . import abi
+from . import background as bgm
+from . import driver as D
+from .measure import RingMeasurement
+
+
+@dataclass
+class SolveSpec:
+    W: D.Workload
+    receivers: np.ndarray  # (nrec, 2) 0-based (i, j)
+    maxtime: int           # last time index (maxtime + 1 steps per solve)
+    precision: int = abi.FP32
+    window_width: float = 120.0  # s, Gaussian time window
+
+    @property
+    def steps(self) -> int:
+        return self.W.params.steps
+
+    @property
+    def nt(self) -> int:
+        return self.maxtime // self.steps + 1
+
+    @property
+    def cad(self) -> float:
+        return self.steps * self.W.params.timestep
+
+
+def _ijk(spec: SolveSpec, k: int) -> np.ndarray:
+    r = spec.receivers
+    return np.stack([r[:, 0], r[:, 1], np.full(len(r), k)], axis=1).astype(np.int32)
+
+
+def sample(eng, field: int, ijk: np.ndarray) -> np.ndarray:
+    if hasattr(eng, "sample"):
+        return eng.sample(field, ijk)
+    a = eng.get(abi.ARR_A, field)
+    return a[ijk[:, 0], ijk[:, 1], ijk[:, 2] - 1]
+
+
+def forward_solve(eng, spec: SolveSpec, store: bool = True) -> np.ndarray:
+    """Runs the forward loop on an engine whose source is set; returns the
+    receiver traces (nrec, nt).  With store, forward snapshot k (stamp
+    k*steps) is kept in slot k."""
+    steps = spec.steps
+    ijk = _ijk(spec, eng.cfg.o_rad)
+    if store:
+        eng.snapshots_alloc(spec.nt)
+    traces = np.zeros((len(spec.receivers), spec.nt))
+    t = 0
+    for m in range(spec.nt):
+        stamp = m * steps
+        eng.step(t, stamp - t + 1)
+        t = stamp + 1
+        traces[:, m] = sample(eng, abi.V_Z, ijk)
+        if store:
+            eng.snapshot_store(m)
+    return traces
+
+
+def adjoint_vr(spec: SolveSpec, adj: np.ndarray, S: bgm.Setup) -> np.ndarray:
+    """vr(nx, ny, nt+8) of the adjoint run: the time-reversed adjoint source
+    (adj(:,1,nt-j+1) = ..., driver.f90:876) at the receiver pixels, as a
+    force density (delta^3 through 1/(dx dy) here and delta_width in the
+    engine) in nondimensional time units."""
+    P = spec.W.params
+    nt = spec.nt
+    vr = np.zeros((P.nx, P.ny, nt + 8))
+    dxn = 1.0 / (P.nx * S.stretchx)
+    dyn = 1.0 / (P.ny * S.stretchy)
+    tconv = bgm.DIML / bgm.DIMC
+    for r, (i, j) in enumerate(spec.receivers):
+        vr[i, j, :nt] += adj[r, ::-1] * tconv / (dxn * dyn)
+    return vr
+
+
+def adjoint_solve(eng, spec: SolveSpec, vr: np.ndarray, grav: np.ndarray) -> None:
+    """Adjoint loop with the Lagrange source; accumulates the kernel sums
+    against the forward snapshots held by ``eng`` (slot nt-1-m at adjoint
+    stamp m*steps)."""
+    steps = spec.steps
+    eng.set_source(abi.SRC_LAGRANGE, vr, None, 10 ** 9)
+    eng.kernel_begin(grav)
+    t = 0
+    for m in range(spec.nt):
+        stamp = m * steps
+        eng.step(t, stamp - t + 1)
+        t = stamp + 1
+        eng.kernel_accumulate(spec.nt - 1 - m)
+
+
+def cell_volume(S: bgm.Setup) -> np.ndarray:
+    """Nondimensional cell volume dx dy dz(k) (dz = unstretch/(nz-1))."""
+    dxn = 1.0 / (S.nx * S.stretchx)
+    dyn = 1.0 / (S.ny * S.stretchy)
+    dz = S.derived["unstretch"] / (S.nz - 1)
+    return dxn * dyn * dz
+
+
+def kernel_lnc(raw_c2: np.ndarray, S: bgm.Setup, spec: SolveSpec) -> np.ndarray:
+    """Sound-speed kernel K_lnc (per unit nondimensional volume) from the
+    accumulated sum (kernels.f90:537: -2 rho0 c^2 sum * stepskern*timestep)."""
+    dtn = spec.steps * S.deltat
+    return -2.0 * S.rho0 * S.c2 * raw_c2 * dtn
+
+
+def make_engines(spec: SolveSpec, S: bgm.Setup, eng_factory=None):
+    """Forward and adjoint engines (or oracles via eng_factory(cfg, S))."""
+    W = spec.W
+    P = W.params
+    nsteps = spec.maxtime + 1
+    fcfg = D.make_config(P, S, abi.FORWARD, nsteps)
+    acfg = D.make_config(P, S, abi.ADJOINT, nsteps)
+    for c in (fcfg, acfg):
+        c.precision = spec.precision
+        if spec.precision == abi.FP64:
+            c.field_scale = 1.0
+    if eng_factory is None:
+        fe, ae = abi.Engine(fcfg), abi.Engine(acfg)
+    else:
+        fe, ae = eng_factory(fcfg), eng_factory(acfg)
+    for e in (fe, ae):
+        D.load_setup(e, S)
+    mask, series = D.forward_source(W, nsteps + 8)
+    fe.set_source(abi.SRC_SERIES, mask, series, 10 ** 9)
+    return fe, ae
+
+
+def copy_snapshots(dst, src, nslots: int) -> None:
+    if hasattr(dst, "snapshots_copy_from"):
+        dst.snapshots_copy_from(src)
+        return
+    dst.snapshots_alloc(nslots)
+    for s in range(nslots):
+        for f in range(6):
+            dst.snapshot_put(s, f, src.snapshot_get(s, f))
+
+
+@dataclass
+class SolveResult:
+    traces: np.ndarray
+    tau: np.ndarray
+    misfit: float
+    adj: np.ndarray
+    kernel_raw: dict
+    K_lnc: np.ndarray
+
+
+def full_solve(spec: SolveSpec, S: bgm.Setup, obs: np.ndarray, meas: RingMeasurement | None = None,
+               eng_factory=None) -> tuple[SolveResult, RingMeasurement]:
+    """forward -> travel times vs obs -> adjoint source -> adjoint + kernels."""
+    fe, ae = make_engines(spec, S, eng_factory)
+    try:
+        syn = forward_solve(fe, spec, store=True)
+        if meas is None:
+            meas = RingMeasurement(syn, spec.cad, spec.window_width)
+        tau = meas.travel_times(obs, syn)
+        adj = meas.adjoint_source(syn, tau)
+        vr = adjoint_vr(spec, adj, S)
+        copy_snapshots(ae, fe, spec.nt)
+        adjoint_solve(ae, spec, vr, S.g_raw)
+        raw = {k: ae.kernel_get(k) for k in (abi.KERN_C2, abi.KERN_RHO, abi.KERN_HESSIAN)}
+        K = kernel_lnc(raw[abi.KERN_C2], S, spec)
+        res = SolveResult(traces=syn, tau=tau, misfit=meas.misfit(tau), adj=adj, kernel_raw=raw,
+                          K_lnc=K)
+        return res, meas
+    finally:
+        fe.close()
+        ae.close()
+
+
+def observe(spec: SolveSpec, S: bgm.Setup, eng_factory=None) -> np.ndarray:
+    """Forward traces of a (true) model: the synthetic 'observed' data."""
+    fe, ae = make_engines(spec, S, eng_factory)
+    try:
+        return forward_solve(fe, spec, store=False)
+    finally:
+        fe.close()
+        ae.close()
+
+
+def bump_c2(S0: bgm.Setup, W: D.Workload, amp: float = 0.02, radius_mm: float = 5.0,
+            depth_mm: float = 3.0, x0_mm: float = 0.0, y0_mm: float = 0.0) -> np.ndarray:
+    """c^2 of a +amp Gaussian sound-speed bump: c -> c (1 + amp exp(-r^2/R^2))
+    centred at (x0, y0, -depth) Mm."""
+    P = W.params
+    x = ((np.arange(P.nx) / P.nx) - 0.5) * P.xlength / 1e8
+    y = ((np.arange(P.ny) / P.ny) - 0.5) * P.ylength / 1e8 if P.ny > 1 else np.zeros(1)
+    zmm = (S0.z - 1.0) * bgm.RSUN_R / 1e8
+    r2 = ((x[:, None, None] - x0_mm) ** 2 + (y[None, :, None] - y0_mm) ** 2
+          + (zmm[None, None, :] + depth_mm) ** 2)
+    return S0.c2 * (1.0 + amp * np.exp(-r2 / radius_mm ** 2)) ** 2
