@@ -141,6 +141,15 @@ __device__ __forceinline__ void dft_reg(C2<T> (&v)[R]) {
   }
 }
 
+// LDS layout of a batch of lines: element i of a line at pad(i) = i + i/16
+// (breaks the stride-16 bank conflicts of radix-16 Stockham writes), lines
+// LS = pad(N) + 8 elements apart (rotates banks between lines).
+__host__ __device__ constexpr int lpad(int i) { return i + (i >> 4); }
+template <int N>
+struct LineStride {
+  static constexpr int LS = lpad(N) + 8;
+};
+
 // One in-place Stockham pass of radix R over LPB lines of length N in LDS:
 // every thread loads its butterflies into registers, the block synchronizes,
 // then the results are written back.
@@ -157,7 +166,7 @@ __device__ __forceinline__ void rpass(C2<T>* buf, const C2<T>* __restrict__ tw) 
       const int line = w / NB;
       const int j = w - line * NB;
 #pragma unroll
-      for (int r = 0; r < R; ++r) v[it][r] = buf[line * N + j + r * NB];
+      for (int r = 0; r < R; ++r) v[it][r] = buf[line * LineStride<N>::LS + lpad(j + r * NB)];
     }
   }
   __syncthreads();
@@ -179,7 +188,7 @@ __device__ __forceinline__ void rpass(C2<T>* buf, const C2<T>* __restrict__ tw) 
       dft_reg<T, R, DIR>(v[it]);
       const int idxD = (j / NS) * NS * R + k;
 #pragma unroll
-      for (int r = 0; r < R; ++r) buf[line * N + idxD + r * NS] = v[it][r];
+      for (int r = 0; r < R; ++r) buf[line * LineStride<N>::LS + lpad(idxD + r * NS)] = v[it][r];
     }
   }
   __syncthreads();
@@ -202,6 +211,7 @@ struct XCfg {
                                  ? (32768 / (int)sizeof(C2<T>)) / N
                                  : 1;
   static constexpr int NT = 256;
+  static constexpr int LS = LineStride<N>::LS;
 };
 
 // x-lines with compile-time N: same contract as k_xlines (kernels.h).
@@ -220,8 +230,8 @@ __global__ __launch_bounds__(256) void k_xlines_n(int nrows, int nlines, XArgs<T
                                                   const C2<T>* __restrict__ twg, int ny, int nz,
                                                   long nxy) {
   using CF = XCfg<T, N>;
-  constexpr int LPB = CF::LPB, NT = CF::NT;
-  __shared__ C2<T> b0[LPB * N];
+  constexpr int LPB = CF::LPB, NT = CF::NT, LS = CF::LS;
+  __shared__ C2<T> b0[LPB * LS];
   __shared__ C2<T> tw[N];
   const int fld = blockIdx.y;
   const T* __restrict__ a = X.a[fld];
@@ -263,7 +273,7 @@ __global__ __launch_bounds__(256) void k_xlines_n(int nrows, int nlines, XArgs<T
         v = spec[L * N + x];
       }
     }
-    b0[e] = v;
+    b0[l * LS + lpad(x)] = v;
   }
   __syncthreads();
   if (MODE == XM_DAMP_INV)
@@ -272,20 +282,23 @@ __global__ __launch_bounds__(256) void k_xlines_n(int nrows, int nlines, XArgs<T
     rfft<T, N, -1, LPB, NT>(b0, tw);
   C2<T>* r = b0;
   if (MODE == XM_DAMP_FWD) {
-    for (int e = threadIdx.x; e < nl * N; e += NT) spec[(long)L0 * N + e] = r[e];
+    for (int e = threadIdx.x; e < nl * N; e += NT) {
+      const int l = e / N, x = e - l * N;
+      spec[(long)L0 * N + e] = r[l * LS + lpad(x)];
+    }
     return;
   }
   if (MODE == XM_DERIV || MODE == XM_FILTER) {
 #pragma unroll 4
     for (int e = threadIdx.x; e < LPB * N; e += NT) {
-      const int x = e % N;
+      const int l = e / N, x = e - l * N;
       const T c = mult[x];
-      C2<T> v = r[e];
+      C2<T> v = r[l * LS + lpad(x)];
       if (MODE == XM_DERIV)
         v = {-c * v.y, c * v.x};
       else
         v = {c * v.x, c * v.y};
-      r[e] = v;
+      r[l * LS + lpad(x)] = v;
     }
     __syncthreads();
     rfft<T, N, +1, LPB, NT>(b0, tw);
@@ -294,8 +307,9 @@ __global__ __launch_bounds__(256) void k_xlines_n(int nrows, int nlines, XArgs<T
       const int l = e / N, x = e - l * N;
       const long L = L0 + l;
       const long r0 = 2 * L, r1 = r0 + 1;
-      out[r0 * N + x] = r[e].x;
-      if (r1 < nrows) out[r1 * N + x] = r[e].y;
+      const C2<T> v = r[l * LS + lpad(x)];
+      out[r0 * N + x] = v.x;
+      if (r1 < nrows) out[r1 * N + x] = v.y;
     }
   } else {
 #pragma unroll 4
@@ -305,8 +319,9 @@ __global__ __launch_bounds__(256) void k_xlines_n(int nrows, int nlines, XArgs<T
       const long q = L / ny, j = L - q * ny;
       const long k0 = 2 * q, k1 = k0 + 1;
       const long o0 = x + (long)N * j + nxy * k0;
-      if (k0 != 0 && k0 != nz - 1) out[o0] = out[o0] - r[e].x;
-      if (k1 < nz - 1) out[o0 + nxy] = out[o0 + nxy] - r[e].y;
+      const C2<T> v = r[l * LS + lpad(x)];
+      if (k0 != 0 && k0 != nz - 1) out[o0] = out[o0] - v.x;
+      if (k1 < nz - 1) out[o0 + nxy] = out[o0 + nxy] - v.y;
     }
   }
 }
@@ -331,8 +346,8 @@ __global__ __launch_bounds__(256) void k_ylines_n(int nx, int nz, XArgs<T> X, T 
                                                   const C2<T>* __restrict__ twg,
                                                   const T* __restrict__ rates) {
   using CF = YCfg<T, NY, MODE>;
-  constexpr int TX = CF::TX, NL = CF::NL, NT = 256;
-  __shared__ C2<T> b0[NL * NY];
+  constexpr int TX = CF::TX, NL = CF::NL, NT = 256, LS = LineStride<NY>::LS;
+  __shared__ C2<T> b0[NL * LS];
   __shared__ C2<T> tw[NY];
   const int fld = blockIdx.z;
   const T* __restrict__ a = X.a[fld];
@@ -351,21 +366,21 @@ __global__ __launch_bounds__(256) void k_ylines_n(int nx, int nz, XArgs<T> X, T 
       const long o = (i0 + xx) + (long)nx * y + nxy * k;
       T v = a[o];
       if (s) v = v + beta * s[o];
-      b0r[2 * ((xx >> 1) * NY + y) + (xx & 1)] = v;
+      b0r[2 * ((xx >> 1) * LS + lpad(y)) + (xx & 1)] = v;
     }
     __syncthreads();
     rfft<T, NY, -1, NL, NT>(b0, tw);
     C2<T>* r = b0;
 #pragma unroll 4
     for (int e = threadIdx.x; e < NL * NY; e += NT) {
-      const int y = e % NY;
+      const int l = e / NY, y = e - l * NY;
       const T c = mult[y];
-      C2<T> v = r[e];
+      C2<T> v = r[l * LS + lpad(y)];
       if (MODE == YM_DERIV)
         v = {-c * v.y, c * v.x};
       else
         v = {c * v.x, c * v.y};
-      r[e] = v;
+      r[l * LS + lpad(y)] = v;
     }
     __syncthreads();
     rfft<T, NY, +1, NL, NT>(b0, tw);
@@ -374,14 +389,14 @@ __global__ __launch_bounds__(256) void k_ylines_n(int nx, int nz, XArgs<T> X, T 
     for (int e = threadIdx.x; e < TX * NY; e += NT) {
       const int y = e / TX, xx = e - y * TX;
       const long o = (i0 + xx) + (long)nx * y + nxy * k;
-      out[o] = rr[2 * ((xx >> 1) * NY + y) + (xx & 1)];
+      out[o] = rr[2 * ((xx >> 1) * LS + lpad(y)) + (xx & 1)];
     }
   } else {
     const long q = k;
 #pragma unroll 4
     for (int e = threadIdx.x; e < NL * NY; e += NT) {
       const int y = e / NL, mm = e - y * NL;
-      b0[mm * NY + y] = spec[(q * NY + y) * nx + i0 + mm];
+      b0[mm * LS + lpad(y)] = spec[(q * NY + y) * nx + i0 + mm];
     }
     __syncthreads();
     rfft<T, NY, -1, NL, NT>(b0, tw);
@@ -393,14 +408,15 @@ __global__ __launch_bounds__(256) void k_ylines_n(int nx, int nz, XArgs<T> X, T 
       const int m = i0 + mm;
       const int mi = (m <= nx / 2) ? m : nx - m;
       const T c = rates[mi + (long)nh * y];
-      r[e] = {c * r[e].x, c * r[e].y};
+      const int o = mm * LS + lpad(y);
+      r[o] = {c * r[o].x, c * r[o].y};
     }
     __syncthreads();
     rfft<T, NY, +1, NL, NT>(b0, tw);
 #pragma unroll 4
     for (int e = threadIdx.x; e < NL * NY; e += NT) {
       const int y = e / NL, mm = e - y * NL;
-      spec[(q * NY + y) * nx + i0 + mm] = b0[mm * NY + y];
+      spec[(q * NY + y) * nx + i0 + mm] = b0[mm * LS + lpad(y)];
     }
   }
 }

@@ -42,6 +42,11 @@ struct TriCoef {
   T c1, c2, c3, c4, c5;    // 5th order rows 2 and N2-1
   T b1, b2, b3, b4;        // 4th order rows 1 and N2
   T alpha2, beta1, cst;    // boundary preconditioning
+  // Rows kc0 <= k < kc1 where the LU factor tables hold one value in T
+  // (the factors converge geometrically away from the boundary rows):
+  // there lowr = lw, 1/betad = ib, gam = gm (bitwise the table values).
+  int kc0, kc1;
+  T lw, ib, gm;
 };
 
 template <typename T>

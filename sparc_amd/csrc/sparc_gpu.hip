@@ -48,6 +48,9 @@ struct sparc_ctx {
   bool damping_set = false, filter_set = false;
   void* tri[4] = {};  // per ibc 0..3: lowr | invb | gam, nz each
   double tc[14] = {};
+  // per ibc and precision (0 fp64, 1 fp32): [kc0, kc1) rows with constant factors
+  int kc[4][2] = {};
+  double kcv[4][3] = {};
   FFTPlan px{}, py{};
   double betas[5] = {};
   // source
@@ -387,6 +390,24 @@ extern "C" int sparc_create(const sparc_config* cfg, sparc_ctx** out) {
     }
     if ((rc = dalloc(c, &c->tri[ibc], 3 * cfg->nz * ts))) return rc;
     if ((rc = upload(c, c->tri[ibc], t.data(), 3 * cfg->nz))) return rc;
+    // the widest range around the middle row where lowr, invb and gam (as
+    // stored in the working precision) equal the middle row's values bitwise
+    const int n = cfg->nz, mid = n / 2;
+    auto same = [&](int k) {
+      if (c->fp32)
+        return (float)lowr[k] == (float)lowr[mid] && (float)invb[k] == (float)invb[mid] &&
+               (float)gam[k] == (float)gam[mid];
+      return lowr[k] == lowr[mid] && invb[k] == invb[mid] && gam[k] == gam[mid];
+    };
+    int k0 = mid, k1 = mid + 1;
+    while (k0 - 1 >= 0 && same(k0 - 1)) --k0;
+    while (k1 < n && same(k1)) ++k1;
+    if (!same(mid)) k0 = k1 = 0;
+    c->kc[ibc][0] = k0;
+    c->kc[ibc][1] = k1;
+    c->kcv[ibc][0] = lowr[mid];
+    c->kcv[ibc][1] = invb[mid];
+    c->kcv[ibc][2] = gam[mid];
   }
   const double d2i = (double)(cfg->nz - 1);
   const double alpha2 = 3. / 4., gamma2 = 1. / 8., beta1 = 3.;
@@ -577,9 +598,14 @@ static Fld<T> make_fld(sparc_ctx* c) {
 }
 
 template <typename T>
-static TriCoef<T> make_tc(const sparc_ctx* c) {
+static TriCoef<T> make_tc(const sparc_ctx* c, int ibc) {
   const double* v = c->tc;
   TriCoef<T> t;
+  t.kc0 = c->kc[ibc][0];
+  t.kc1 = c->kc[ibc][1];
+  t.lw = (T)c->kcv[ibc][0];
+  t.ib = (T)c->kcv[ibc][1];
+  t.gm = (T)c->kcv[ibc][2];
   t.ci1 = (T)v[0];
   t.ci2 = (T)v[1];
   t.c1 = (T)v[2];
@@ -751,9 +777,22 @@ static int launch_z(sparc_ctx* c, const Fld<T>& F, int ibc, const T* inA, const 
   else
     nb = in + 2 + (F.beta != (T)0 ? 4 : 2) + nbg({SPARC_BG_RHOINV}) + 4;
   ProfScope ps(c, SPARC_KC_ZSOLVE, nb * (double)c->N * sizeof(T));
-  hipLaunchKernelGGL((k_zsolve<T, EPI>), dim3(cdiv(c->nxy, 64)), dim3(64), 0, c->stream, F,
-                     make_tc<T>(c), tri, tri + nz, tri + 2 * nz, ibc, inA, inS, beta,
-                     (T*)c->W[7], prov, provscale, out);
+  // segments per column: L = nz/NSEG planes each, at least ~48 (>= 2x warm-up)
+  const int nseg = nz >= 192 ? 4 : (nz >= 96 ? 2 : 1);
+  const size_t lds = 5 * (size_t)nz * sizeof(T);
+  const TriCoef<T> tcv = make_tc<T>(c, ibc);
+  const dim3 grid(cdiv(c->nxy, 64));
+#define ZL(NS)                                                                                 \
+  hipLaunchKernelGGL((k_zsolve<T, EPI, NS>), grid, dim3(64 * NS), lds, c->stream, F, tcv, tri, \
+                     tri + nz, tri + 2 * nz, ibc, inA, inS, beta, (T*)c->W[7], prov, provscale,  \
+                     out)
+  if (nseg == 4)
+    ZL(4);
+  else if (nseg == 2)
+    ZL(2);
+  else
+    ZL(1);
+#undef ZL
   LAUNCHCHK(c);
   return SPARC_OK;
 }

@@ -6,10 +6,12 @@
 // part of the RHS operator of one LDDRK stage).
 //
 // A grid of nx*ny columns gives one wave per SIMD at 256x256, so the sweeps
-// are latency-bound unless many loads are in flight per wave: both sweeps
-// run in chunks of planes whose inputs (B in the forward sweep; the
-// forward-sweep result ud and every epilogue input in the backward sweep) are
-// loaded into registers one chunk ahead of their use.
+// are latency-bound unless many loads are in flight: each column is split
+// into segments solved by separate waves (below), and both sweeps run in
+// chunks of planes whose inputs (B in the forward sweep; the forward-sweep
+// result ud and the epilogue inputs in the backward sweep) are loaded into
+// registers one chunk ahead of their use.  The per-plane coefficient tables
+// are staged in LDS.
 // Included inside namespace sparc by kernels.h.
 #pragma once
 
@@ -25,13 +27,13 @@ struct EIn {
 };
 
 template <typename T, int EPI>
-__device__ __forceinline__ void load_epi(const Fld<T>& F, const T* __restrict__ ud, int c,
-                                         int k, EIn<T>& e) {
+__device__ __forceinline__ void load_epi(const Fld<T>& F, const T* __restrict__ ud, int c, int k,
+                                         bool own, EIn<T>& e) {
   const int n = F.nz;
-  if (k < 0) return;
   const long nxy = F.nxy;
   const long q = c + nxy * k;
-  if (k < n - 1) e.ud = ud[q];
+  e.ud = ud[q];
+  if (!own) return;
   const bool st = F.beta != (T)0;
   if (EPI == EPI_P3D || EPI == EPI_P2D) {
     e.w0 = F.W[0][q];
@@ -75,8 +77,8 @@ __device__ __forceinline__ T bg_or(const Fld<T>& F, int id, T v3d, int k) {
 }
 
 template <typename T, int EPI>
-__device__ __forceinline__ void epilogue(const Fld<T>& F, int c, int k, T val, const EIn<T>& e,
-                                         T* __restrict__ out) {
+__device__ __forceinline__ void epilogue(const Fld<T>& F, int c, int k, T val, T gk,
+                                         const EIn<T>& e, T* __restrict__ out) {
   const int n = F.nz;
   const long q = c + (long)F.nxy * k;
   const bool st = F.beta != (T)0;
@@ -120,7 +122,7 @@ __device__ __forceinline__ void epilogue(const Fld<T>& F, int c, int k, T val, c
       T rvx = -ri * e.w0;
       T rvy = -ri * e.w1;
       T rvz = -ri * gz;
-      rvz = rvz - rho * F.g[k] * ri;
+      rvz = rvz - rho * gk * ri;
       if (k == F.src_plane) rvz = rvz + F.forcing[c] / bgv(F, SPARC_BG_RHO0, c, k) * F.delta_width;
       if (k == 0 || k == n - 1) {
 #pragma unroll
@@ -141,7 +143,7 @@ __device__ __forceinline__ void epilogue(const Fld<T>& F, int c, int k, T val, c
       }
     } else {
       T rvx = -ri * e.w0;
-      T rvz = -ri * (gz + rho * F.g[k]);
+      T rvz = -ri * (gz + rho * gk);
       if (k == F.src_plane) rvz = rvz + F.forcing[c] * F.delta_width / bgv(F, SPARC_BG_RHO0, c, k);
       if (k == 0) {
         F.scr[SPARC_XI_X][q] = 0;
@@ -163,114 +165,190 @@ __device__ __forceinline__ void epilogue(const Fld<T>& F, int c, int k, T val, c
   }
 }
 
-template <typename T, int EPI>
-__global__ __launch_bounds__(64) void k_zsolve(Fld<T> F, TriCoef<T> tc, const T* __restrict__ lowr,
-                                               const T* __restrict__ invb,
-                                               const T* __restrict__ gam, int ibc,
-                                               const T* __restrict__ inA,
-                                               const T* __restrict__ inS, T beta_in,
-                                               T* __restrict__ ud, const T* __restrict__ prov,
-                                               T provscale, T* __restrict__ out) {
-  constexpr int UF = 16;  // forward chunk (planes)
-  constexpr int UB = 4;   // backward chunk (planes)
-  const int c = blockIdx.x * blockDim.x + threadIdx.x;
-  if (c >= F.nxy) return;
-  const long nxy = F.nxy;
+// Segmented solve: NSEG waves per 64 columns, wave s owning planes
+// [s*L, min(n, (s+1)*L)).  The forward recurrence u_k = (r_k - lowr_k u_{k-1})
+// / betad_k contracts by |lowr/betad| ~ 0.382 per row and the backward one
+// x_k = ud_k - gam_{k+1} x_{k+1} by gam ~ 0.382, so a wave starts each sweep
+// cold (u = 0, resp. x = ud) ZW planes outside its range; ZW is chosen so
+// 0.382^ZW is below the rounding of T (4e-9 for FP32, 3e-18 for FP64).  When
+// the warm-up would reach a boundary row, the sweep starts there exactly.
+template <typename T>
+struct ZWarm {
+  static constexpr int W = sizeof(T) == 4 ? 20 : 42;
+};
+
+template <typename T, int EPI, int NSEG>
+__global__ __launch_bounds__(64 * NSEG) void k_zsolve(Fld<T> F, TriCoef<T> tc,
+                                                      const T* __restrict__ lowr,
+                                                      const T* __restrict__ invb,
+                                                      const T* __restrict__ gam, int ibc,
+                                                      const T* __restrict__ inA,
+                                                      const T* __restrict__ inS, T beta_in,
+                                                      T* __restrict__ ud,
+                                                      const T* __restrict__ prov, T provscale,
+                                                      T* __restrict__ out) {
+  constexpr int UF = 8;  // forward chunk (planes loaded one chunk ahead)
+  constexpr int UB = 2;  // backward chunk
+  constexpr int ZW = ZWarm<T>::W;
+  extern __shared__ unsigned char zsm[];
   const int n = F.nz;
+  T* s_st = reinterpret_cast<T*>(zsm);  // stretch
+  T* s_gm = s_st + n;                   // gam
+  T* s_lw = s_gm + n;                   // lowr
+  T* s_ib = s_lw + n;                   // 1/betad
+  T* s_g = s_ib + n;                    // g
+  for (int k = threadIdx.x; k < n; k += blockDim.x) {
+    s_st[k] = F.stretch[k];
+    s_gm[k] = gam[k];
+    s_lw[k] = lowr[k];
+    s_ib[k] = invb[k];
+    s_g[k] = (EPI == EPI_RHS3D || EPI == EPI_RHS2D) ? F.g[k] : (T)0;
+  }
+  __syncthreads();
+  const int lane = threadIdx.x & 63;
+  const int seg = threadIdx.x >> 6;
+  const int c = blockIdx.x * 64 + lane;
+  const bool active = c < F.nxy;
+  const long nxy = F.nxy;
+  const int L = (n + NSEG - 1) / NSEG;
+  const int ks = seg * L;
+  const int ke = min(n, ks + L);
   auto B = [&](int k) -> T {
     const long q = c + nxy * k;
     T v = inA[q];
     if (inS) v = v + beta_in * inS[q];
     return v;
   };
-  // ---- forward sweep: rows 0 and 1
-  T wm1 = 0, w0 = B(0), w1 = B(1), w2 = B(2), w3 = B(3), w4 = B(4);
-  T r0;
-  if (ibc == 0)
-    r0 = 0;
-  else if (ibc == 1 || ibc == 3)
-    r0 = tc.b1 * w0 + tc.b2 * w1 + tc.b3 * w2 + tc.b4 * w3;
-  else
-    r0 = prov ? prov[c] * provscale : (T)0;
-  const T r1 = tc.c1 * w0 + tc.c2 * w1 + tc.c3 * w2 + tc.c4 * w3 + tc.c5 * w4;
-  if (ibc & 1) r0 = (tc.alpha2 * r0 - tc.beta1 * r1) * tc.cst;
-  T u = r0 * invb[0];
-  ud[c] = u;
-  u = (r1 - lowr[1] * u) * invb[1];
-  ud[c + nxy] = u;
-  // ---- interior rows 2..n-3; row k consumes B(k+3) after its update
-  T na[UF], ns[UF];
-  auto load_chunk = [&](int kb) {
+  // ---------------- forward sweep
+  if (active && ks < ke) {
+    // rows 2..min(ke, n-2)-1 use the interior formula; rows 0, 1, n-2, n-1 are special
+    const bool exact = ks - ZW <= 2;
+    int k0;  // first interior row of the loop
+    T u, wm1, w0, w1, w2, w3, w4;
+    if (exact) {
+      w0 = B(0);
+      w1 = B(1);
+      w2 = B(2);
+      w3 = B(3);
+      w4 = B(4);
+      T r0;
+      if (ibc == 0)
+        r0 = 0;
+      else if (ibc == 1 || ibc == 3)
+        r0 = tc.b1 * w0 + tc.b2 * w1 + tc.b3 * w2 + tc.b4 * w3;
+      else
+        r0 = prov ? prov[c] * provscale : (T)0;
+      const T r1 = tc.c1 * w0 + tc.c2 * w1 + tc.c3 * w2 + tc.c4 * w3 + tc.c5 * w4;
+      if (ibc & 1) r0 = (tc.alpha2 * r0 - tc.beta1 * r1) * tc.cst;
+      u = r0 * s_ib[0];
+      if (ks == 0) ud[c] = u;
+      u = (r1 - s_lw[1] * u) * s_ib[1];
+      if (ks <= 1 && ke > 1) ud[c + nxy] = u;
+      k0 = 2;
+      wm1 = 0;
+    } else {
+      k0 = ks - ZW;
+      u = 0;
+      wm1 = B(k0 - 3);
+      w0 = B(k0 - 2);
+      w1 = B(k0 - 1);
+      w2 = B(k0);
+      w3 = B(k0 + 1);
+      w4 = B(k0 + 2);
+    }
+    const int kend = min(ke, n - 2);  // interior rows k0..kend-1
+    T na[UF], ns[UF];
+    auto load_chunk = [&](int kb) {
 #pragma unroll
-    for (int j = 0; j < UF; ++j) {
-      const int kk = kb + j;
-      if (kk < n) {
-        const long q = c + nxy * kk;
-        na[j] = inA[q];
-        if (inS) ns[j] = inS[q];
+      for (int j = 0; j < UF; ++j) {
+        const int kk = kb + j;
+        if (kk < n) {
+          const long q = c + nxy * kk;
+          na[j] = inA[q];
+          if (inS) ns[j] = inS[q];
+        }
+      }
+    };
+    load_chunk(k0 + 3);
+    for (int kb = k0; kb < kend; kb += UF) {
+      T ca[UF], cs[UF];
+#pragma unroll
+      for (int j = 0; j < UF; ++j) {
+        ca[j] = na[j];
+        cs[j] = ns[j];
+      }
+      if (kb + UF < kend) load_chunk(kb + UF + 3);
+#pragma unroll
+      for (int j = 0; j < UF; ++j) {
+        const int k = kb + j;
+        if (k < kend) {
+          const T r = tc.ci1 * (w3 - w1) + tc.ci2 * (w4 - w0);
+          u = (r - s_lw[k] * u) * s_ib[k];
+          if (k >= ks) ud[c + nxy * k] = u;
+          wm1 = w0;
+          w0 = w1;
+          w1 = w2;
+          w2 = w3;
+          w3 = w4;
+          if (k + 3 < n) w4 = inS ? ca[j] + beta_in * cs[j] : ca[j];
+        }
       }
     }
-  };
-  load_chunk(5);
-  for (int k0 = 2; k0 <= n - 3; k0 += UF) {
-    T ca[UF], cs[UF];
-#pragma unroll
-    for (int j = 0; j < UF; ++j) {
-      ca[j] = na[j];
-      cs[j] = ns[j];
-    }
-    if (k0 + UF <= n - 3) load_chunk(k0 + UF + 3);
-#pragma unroll
-    for (int j = 0; j < UF; ++j) {
-      const int k = k0 + j;
-      if (k <= n - 3) {
-        const T r = tc.ci1 * (w3 - w1) + tc.ci2 * (w4 - w0);
-        u = (r - lowr[k] * u) * invb[k];
-        ud[c + nxy * k] = u;
-        wm1 = w0;
-        w0 = w1;
-        w1 = w2;
-        w2 = w3;
-        w3 = w4;
-        if (k + 3 < n) w4 = inS ? ca[j] + beta_in * cs[j] : ca[j];
+    if (ke == n) {
+      // window holds B(n-5..n-1) in wm1..w3
+      const T rn2 = -(tc.c1 * w3 + tc.c2 * w2 + tc.c3 * w1 + tc.c4 * w0 + tc.c5 * wm1);
+      u = (rn2 - s_lw[n - 2] * u) * s_ib[n - 2];
+      ud[c + nxy * (n - 2)] = u;
+      T rn1;
+      if (ibc == 0)
+        rn1 = 0;
+      else if (ibc == 1 || ibc == 2) {
+        rn1 = -(tc.b1 * w3 + tc.b2 * w2 + tc.b3 * w1 + tc.b4 * w0);
+        rn1 = (tc.alpha2 * rn1 - tc.beta1 * rn2) * tc.cst;
+      } else {
+        rn1 = prov ? prov[c] * provscale : (T)0;
       }
+      u = (rn1 - s_lw[n - 1] * u) * s_ib[n - 1];
+      ud[c + nxy * (n - 1)] = u;  // x_{n-1}: read back by every wave that starts exactly
     }
   }
-  // window now holds B(n-5..n-1) in wm1..w3
-  const T rn2 = -(tc.c1 * w3 + tc.c2 * w2 + tc.c3 * w1 + tc.c4 * w0 + tc.c5 * wm1);
-  u = (rn2 - lowr[n - 2] * u) * invb[n - 2];
-  ud[c + nxy * (n - 2)] = u;
-  T rn1;
-  if (ibc == 0)
-    rn1 = 0;
-  else if (ibc == 1 || ibc == 2) {
-    rn1 = -(tc.b1 * w3 + tc.b2 * w2 + tc.b3 * w1 + tc.b4 * w0);
-    rn1 = (tc.alpha2 * rn1 - tc.beta1 * rn2) * tc.cst;
-  } else {
-    rn1 = prov ? prov[c] * provscale : (T)0;
-  }
-  u = (rn1 - lowr[n - 1] * u) * invb[n - 1];
-  // ---- backward sweep + epilogue, chunks of UB planes loaded one chunk ahead
-  EIn<T> nx[UB];
-#pragma unroll
-  for (int j = 0; j < UB; ++j) load_epi<T, EPI>(F, ud, c, n - 1 - j, nx[j]);
-  T x = u;
-  for (int kh = n - 1; kh >= 0; kh -= UB) {
-    EIn<T> cu[UB];
-#pragma unroll
-    for (int j = 0; j < UB; ++j) cu[j] = nx[j];
-    if (kh - UB >= 0) {
-#pragma unroll
-      for (int j = 0; j < UB; ++j) load_epi<T, EPI>(F, ud, c, kh - UB - j, nx[j]);
-    }
+  __syncthreads();
+  // ---------------- backward sweep + epilogue
+  if (active && ks < ke) {
+    const bool exact = ke - 1 + ZW >= n - 1;
+    const int kstart = exact ? n - 1 : ke - 1 + ZW;
+    EIn<T> nx[UB];
 #pragma unroll
     for (int j = 0; j < UB; ++j) {
-      const int k = kh - j;
-      if (k >= 0) {
-        if (k < n - 1) x = cu[j].ud - gam[k + 1] * x;
-        epilogue<T, EPI>(F, c, k, x * F.stretch[k], cu[j], out);
+      const int k = kstart - j;
+      if (k >= ks) load_epi<T, EPI>(F, ud, c, k, k < ke, nx[j]);
+    }
+    T x = 0;
+    bool first = true;
+    for (int kh = kstart; kh >= ks; kh -= UB) {
+      EIn<T> cu[UB];
+#pragma unroll
+      for (int j = 0; j < UB; ++j) cu[j] = nx[j];
+      if (kh - UB >= ks) {
+#pragma unroll
+        for (int j = 0; j < UB; ++j) {
+          const int k = kh - UB - j;
+          if (k >= ks) load_epi<T, EPI>(F, ud, c, k, k < ke, nx[j]);
+        }
+      }
+#pragma unroll
+      for (int j = 0; j < UB; ++j) {
+        const int k = kh - j;
+        if (k >= ks) {
+          if (first) {
+            x = cu[j].ud;  // exact x_{n-1} = u_{n-1}, or the cold start
+            first = false;
+          } else {
+            x = cu[j].ud - s_gm[k + 1] * x;
+          }
+          if (k < ke) epilogue<T, EPI>(F, c, k, x * s_st[k], s_g[k], cu[j], out);
+        }
       }
     }
   }
 }
-

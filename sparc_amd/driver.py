@@ -90,6 +90,7 @@ def workload(name: str, precision: int | None = None) -> Workload:
     box2s  128 x 128 x 150, 100 Mm (the small grid of the serving mix)
     box2l  384 x 384 x 300, 300 Mm (the large grid of the serving mix)
     tiny   32 x 32 x 40, 25 Mm (parity tests)
+    tall   32 x 32 x 200, 25 Mm (parity tests of the segmented z solve)
     two_d  256 x 1 x 300, 400 Mm, option 17 (the reference's wired 2-D case)
     """
     if name == "box1":
@@ -107,6 +108,9 @@ def workload(name: str, precision: int | None = None) -> Workload:
     elif name == "tiny":
         nx = ny = 32
         nz, L, nsteps, prec = 40, 25.0, 20, abi.FP64
+    elif name == "tall":
+        nx = ny = 32
+        nz, L, nsteps, prec = 200, 25.0, 10, abi.FP64
     elif name == "two_d":
         tab = bgm.polytrope_table(300, 0.5, -12.0)
         P = Params(nx=256, ny=1, nz=300, xlength=400.0 * M_CM, ylength=400.0 * M_CM,

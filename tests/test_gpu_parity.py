@@ -25,6 +25,49 @@ def tiny():
 
 
 @pytest.fixture(scope="module")
+def tall():
+    W = D.workload("tall")
+    S = D.setup_for(W)
+    return W, S
+
+
+@pytest.mark.parametrize("prec,tol", [(abi.FP64, TOL64), (abi.FP32, TOL32_OP)])
+def test_segmented_zsolve_matches_oracle(tall, prec, tol):
+    """nz = 200 runs the z solve in column segments with cold-start warm-up;
+    every ibc must still match DBYD2 to rounding."""
+    W, S = tall
+    eng, cfg = D.make_engine(W, S, precision=prec)
+    try:
+        st = S.derived["stretch"]
+        for seed in (11, 12):
+            f = _field(eng.shape, seed)
+            for ibc in (0, 1, 2, 3):
+                preset = _field(eng.shape, 20 + ibc)
+                want = O.ddz(f, ibc, st, preset=preset if ibc in (2, 3) else None)
+                got = eng.apply_op(abi.OP_DDZ, f, ibc=ibc, out=preset if ibc in (2, 3) else None)
+                assert rel_err(got, want) < tol * 10, (seed, ibc)
+    finally:
+        eng.close()
+
+
+@pytest.mark.parametrize("prec", [abi.FP64, abi.FP32])
+def test_step_tall_matches_oracle(tall, prec):
+    W, S = tall
+    eng, o = _run_pair(W, S, 4, prec)
+    try:
+        for f in range(6):
+            a, b = eng.get(abi.ARR_A, f), o.get(abi.ARR_A, f)
+            assert np.linalg.norm(b) > 0, f
+            if prec == abi.FP64:
+                assert rel_err(a, b) < TOL64, f
+            else:
+                assert rel_l2(a, b) < TOL32_L2, f
+    finally:
+        eng.close()
+        o.close()
+
+
+@pytest.fixture(scope="module")
 def two_d():
     W = D.workload("two_d")
     W.params.nx = 64
