@@ -204,11 +204,14 @@ __device__ __forceinline__ void rfft(C2<T>* buf, const C2<T>* __restrict__ tw) {
   if constexpr (PL::P == 3) rpass<T, N, R2, R0 * R1, DIR, LPB, NT>(buf, tw);
 }
 
-// lines per block: 32 KiB of complex values per block
+// lines per block: 16 KiB of complex values per block and 256 threads, i.e.
+// ~8 (FP32) / 4 (FP64) values per thread in the load and store phases
+// (measured on MI355X: 8 lines x 256 threads moves 4.5-5.1 TB/s at N = 256,
+// 16 lines x 256 threads 2.9-3.9 TB/s)
 template <typename T, int N>
 struct XCfg {
-  static constexpr int LPB = (32768 / (int)sizeof(C2<T>)) / N > 0
-                                 ? (32768 / (int)sizeof(C2<T>)) / N
+  static constexpr int LPB = (16384 / (int)sizeof(C2<T>)) / N > 0
+                                 ? (16384 / (int)sizeof(C2<T>)) / N
                                  : 1;
   static constexpr int NT = 256;
   static constexpr int LS = LineStride<N>::LS;
@@ -224,13 +227,12 @@ struct XArgs {
   C2<T>* spec[3];
 };
 
-template <typename T, int N, int MODE>
-__global__ __launch_bounds__(256) void k_xlines_n(int nrows, int nlines, XArgs<T> X, T beta,
-                                                  const T* __restrict__ mult,
-                                                  const C2<T>* __restrict__ twg, int ny, int nz,
-                                                  long nxy) {
-  using CF = XCfg<T, N>;
-  constexpr int LPB = CF::LPB, NT = CF::NT, LS = CF::LS;
+template <typename T, int N, int MODE, int LPB = XCfg<T, N>::LPB, int NT = XCfg<T, N>::NT>
+__global__ __launch_bounds__(NT) void k_xlines_n(int nrows, int nlines, XArgs<T> X, T beta,
+                                                 const T* __restrict__ mult,
+                                                 const C2<T>* __restrict__ twg, int ny, int nz,
+                                                 long nxy) {
+  constexpr int LS = LineStride<N>::LS;
   __shared__ C2<T> b0[LPB * LS];
   __shared__ C2<T> tw[N];
   const int fld = blockIdx.y;
@@ -241,7 +243,7 @@ __global__ __launch_bounds__(256) void k_xlines_n(int nrows, int nlines, XArgs<T
   for (int m = threadIdx.x; m < N; m += NT) tw[m] = twg[m];
   const int L0 = blockIdx.x * LPB;
   const int nl = min(LPB, nlines - L0);
-#pragma unroll 4
+#pragma unroll
   for (int e = threadIdx.x; e < LPB * N; e += NT) {
     const int l = e / N, x = e - l * N;
     C2<T> v = {0, 0};
@@ -289,7 +291,7 @@ __global__ __launch_bounds__(256) void k_xlines_n(int nrows, int nlines, XArgs<T
     return;
   }
   if (MODE == XM_DERIV || MODE == XM_FILTER) {
-#pragma unroll 4
+#pragma unroll
     for (int e = threadIdx.x; e < LPB * N; e += NT) {
       const int l = e / N, x = e - l * N;
       const T c = mult[x];
@@ -302,7 +304,7 @@ __global__ __launch_bounds__(256) void k_xlines_n(int nrows, int nlines, XArgs<T
     }
     __syncthreads();
     rfft<T, N, +1, LPB, NT>(b0, tw);
-#pragma unroll 4
+#pragma unroll
     for (int e = threadIdx.x; e < nl * N; e += NT) {
       const int l = e / N, x = e - l * N;
       const long L = L0 + l;
@@ -312,7 +314,7 @@ __global__ __launch_bounds__(256) void k_xlines_n(int nrows, int nlines, XArgs<T
       if (r1 < nrows) out[r1 * N + x] = v.y;
     }
   } else {
-#pragma unroll 4
+#pragma unroll
     for (int e = threadIdx.x; e < nl * N; e += NT) {
       const int l = e / N, x = e - l * N;
       const long L = L0 + l;
@@ -333,20 +335,23 @@ __host__ __device__ constexpr int pow2_floor(int v) { return v >= 2 ? 2 * pow2_f
 
 template <typename T, int NY, int MODE>
 struct YCfg {
-  // 32 KiB of complex values per block; row segments of 128 bytes at most
+  // 32 KiB of complex values per block; row segments of 128 bytes at most;
+  // derivative/filter tiles use 512 threads (measured 4.7 vs 3.7 TB/s with 256)
   static constexpr int LINES = pow2_floor((32768 / (int)sizeof(C2<T>)) / NY);
   static constexpr int CAP = 128 / (int)sizeof(C2<T>);  // complex lines of 128 B
   static constexpr int NL = LINES < CAP ? LINES : CAP;
   static constexpr int TX = (MODE == YM_DAMP) ? NL : 2 * NL;
+  static constexpr int NT = (MODE == YM_DAMP || NL * NY < 4096) ? 256 : 512;
 };
 
-template <typename T, int NY, int MODE>
-__global__ __launch_bounds__(256) void k_ylines_n(int nx, int nz, XArgs<T> X, T beta,
-                                                  const T* __restrict__ mult,
-                                                  const C2<T>* __restrict__ twg,
-                                                  const T* __restrict__ rates) {
-  using CF = YCfg<T, NY, MODE>;
-  constexpr int TX = CF::TX, NL = CF::NL, NT = 256, LS = LineStride<NY>::LS;
+template <typename T, int NY, int MODE, int NL = YCfg<T, NY, MODE>::NL,
+          int NT = YCfg<T, NY, MODE>::NT>
+__global__ __launch_bounds__(NT) void k_ylines_n(int nx, int nz, XArgs<T> X, T beta,
+                                                 const T* __restrict__ mult,
+                                                 const C2<T>* __restrict__ twg,
+                                                 const T* __restrict__ rates) {
+  constexpr int TX = (MODE == YM_DAMP) ? NL : 2 * NL;
+  constexpr int LS = LineStride<NY>::LS;
   __shared__ C2<T> b0[NL * LS];
   __shared__ C2<T> tw[NY];
   const int fld = blockIdx.z;
@@ -360,7 +365,7 @@ __global__ __launch_bounds__(256) void k_ylines_n(int nx, int nz, XArgs<T> X, T 
   const long k = blockIdx.y;
   if (MODE == YM_DERIV || MODE == YM_FILTER) {
     T* b0r = reinterpret_cast<T*>(b0);
-#pragma unroll 4
+#pragma unroll
     for (int e = threadIdx.x; e < TX * NY; e += NT) {
       const int y = e / TX, xx = e - y * TX;
       const long o = (i0 + xx) + (long)nx * y + nxy * k;
@@ -371,7 +376,7 @@ __global__ __launch_bounds__(256) void k_ylines_n(int nx, int nz, XArgs<T> X, T 
     __syncthreads();
     rfft<T, NY, -1, NL, NT>(b0, tw);
     C2<T>* r = b0;
-#pragma unroll 4
+#pragma unroll
     for (int e = threadIdx.x; e < NL * NY; e += NT) {
       const int l = e / NY, y = e - l * NY;
       const T c = mult[y];
@@ -385,7 +390,7 @@ __global__ __launch_bounds__(256) void k_ylines_n(int nx, int nz, XArgs<T> X, T 
     __syncthreads();
     rfft<T, NY, +1, NL, NT>(b0, tw);
     const T* rr = reinterpret_cast<const T*>(b0);
-#pragma unroll 4
+#pragma unroll
     for (int e = threadIdx.x; e < TX * NY; e += NT) {
       const int y = e / TX, xx = e - y * TX;
       const long o = (i0 + xx) + (long)nx * y + nxy * k;

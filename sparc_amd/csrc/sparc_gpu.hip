@@ -634,7 +634,7 @@ template <typename T, int MODE, int N>
 static void launch_xn(sparc_ctx* c, int nf, const XArgs<T>& X, T beta, const T* mult, int nrows,
                       int nlines) {
   constexpr int LPB = XCfg<T, N>::LPB;
-  hipLaunchKernelGGL((k_xlines_n<T, N, MODE>), dim3(cdiv(nlines, LPB), nf), dim3(256), 0,
+  hipLaunchKernelGGL((k_xlines_n<T, N, MODE>), dim3(cdiv(nlines, LPB), nf), dim3(XCfg<T, N>::NT), 0,
                      c->stream, nrows, nlines, X, beta, mult, (const C2<T>*)c->twx, c->cfg.ny,
                      c->cfg.nz, c->nxy);
 }
@@ -698,7 +698,8 @@ template <typename T, int MODE, int NY>
 static void launch_yn(sparc_ctx* c, int nf, const XArgs<T>& X, T beta, const T* mult) {
   constexpr int TX = YCfg<T, NY, MODE>::TX;
   const int planes = (MODE == YM_DAMP) ? (c->cfg.nz + 1) / 2 : c->cfg.nz;
-  hipLaunchKernelGGL((k_ylines_n<T, NY, MODE>), dim3(c->cfg.nx / TX, planes, nf), dim3(256), 0,
+  hipLaunchKernelGGL((k_ylines_n<T, NY, MODE>), dim3(c->cfg.nx / TX, planes, nf),
+                     dim3(YCfg<T, NY, MODE>::NT), 0,
                      c->stream, c->cfg.nx, c->cfg.nz, X, beta, mult, (const C2<T>*)c->twy,
                      (const T*)c->rates);
 }
