@@ -63,6 +63,9 @@ extern "C" {
 #define SPARC_ARR_SCR 1      /* scr(:,:,:,f): last stage RHS ("acc" for f=3..5) */
 #define SPARC_ARR_PMLVARS 2  /* pmlvars(:,:,:,f), f=0 psivz, f=1 psip           */
 #define SPARC_ARR_SCRPML 3   /* scrpml(:,:,:,f)                                 */
+#define SPARC_ARR_ALO 4      /* low-order part of a(:,:,:,f) when the FP32 state is
+                                compensated (cfg.compensated); get/put of
+                                SPARC_ARR_A then move hi+lo as one FP64 value    */
 
 /* 3-D background fields (sparc_set_background).  n = nx*ny*nz for a full
  * 3-D field or n = nz for a horizontally uniform z-profile. */
@@ -133,7 +136,10 @@ typedef struct sparc_config {
   int32_t cadforcing_step;/* floor(cadforcing/timestep) (initialize.f90:977)    */
   int32_t forcing_length; /* source samples (initialize.f90:981)                */
   int32_t num_steps;      /* Lagrange end guard (driver.f90:280-281)            */
-  int32_t reserved_i[8];
+  int32_t compensated;    /* FP32 only: hold the state as hi+lo FP32 pairs and run
+                             a += deltat*scr and the z filter in FP64 (the
+                             reference state is FP64); 0 = plain FP32 state     */
+  int32_t reserved_i[7];
   double xlength, ylength;/* box sides, cm (params.i:27)                        */
   double timestep;        /* s (params.i:27)                                    */
   double deltat;          /* timestep*dimc/diml (initialize.f90:472)            */
@@ -219,6 +225,20 @@ int sparc_snapshots_alloc(sparc_ctx *ctx, int nslots);
 int sparc_snapshot_store(sparc_ctx *ctx, int slot);
 /* Copies the snapshot bank of src into dst (dst must have room). */
 int sparc_snapshots_copy(sparc_ctx *dst, const sparc_ctx *src);
+/* Like sparc_kernel_accumulate, with the forward snapshot read from the bank
+ * of another context on the same device (src may equal ctx).  Waits for
+ * src's queued work first. */
+int sparc_kernel_accumulate_from(sparc_ctx *ctx, const sparc_ctx *src, int slot);
+
+/* ---- forward-wavefield checkpoints (restart state of driver.f90:489-524,
+ * WRITE_OUT_FULL_STATE all_modules.f90:623-694, kept in device memory) ---- */
+
+/* Reserves nslots restart states: a(1..6) and pmlvars(1..2).  The next step
+ * needs nothing else (the first LDDRK stage uses a alone, step.f90:52-56). */
+int sparc_checkpoints_alloc(sparc_ctx *ctx, int nslots);
+int sparc_checkpoint_save(sparc_ctx *ctx, int slot);
+int sparc_checkpoint_load(sparc_ctx *ctx, int slot);
+
 /* Zeroes the FP64 kernel accumulators; sets the gravity profile grav(nz)
  * used by the density kernel (unscaled by the PML, kernels.f90:298-301). */
 int sparc_kernel_begin(sparc_ctx *ctx, const double *grav, int nz);

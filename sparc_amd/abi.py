@@ -20,7 +20,7 @@ FP64, FP32 = 0, 1
 OPT_3D, OPT_2D = 4, 17
 FORWARD, ADJOINT = 0, 1
 XI_X, XI_Y, XI_Z, V_X, V_Y, V_Z = range(6)
-ARR_A, ARR_SCR, ARR_PMLVARS, ARR_SCRPML = range(4)
+ARR_A, ARR_SCR, ARR_PMLVARS, ARR_SCRPML, ARR_ALO = range(5)
 (BG_RHO0, BG_RHOINV, BG_C2RHO0, BG_GRADP0_Z, BG_GRADRHO0_Z, BG_AZ, BG_BZPML,
  BG_C2) = range(8)
 PROF_STRETCH, PROF_UNSTRETCH, PROF_G, PROF_Z = range(4)
@@ -38,7 +38,8 @@ EXPORTS = (
     "sparc_get_state", "sparc_put_state", "sparc_get_plane", "sparc_norm", "sparc_sample",
     "sparc_snapshots_alloc", "sparc_snapshot_store", "sparc_snapshots_copy",
     "sparc_kernel_begin", "sparc_kernel_accumulate", "sparc_kernel_get", "sparc_apply_op",
-    "sparc_profile_enable", "sparc_profile_read",
+    "sparc_profile_enable", "sparc_profile_read", "sparc_kernel_accumulate_from",
+    "sparc_checkpoints_alloc", "sparc_checkpoint_save", "sparc_checkpoint_load",
 )
 
 
@@ -60,7 +61,8 @@ class SparcConfig(C.Structure):
         ("cadforcing_step", C.c_int32),
         ("forcing_length", C.c_int32),
         ("num_steps", C.c_int32),
-        ("reserved_i", C.c_int32 * 8),
+        ("compensated", C.c_int32),
+        ("reserved_i", C.c_int32 * 7),
         ("xlength", C.c_double), ("ylength", C.c_double),
         ("timestep", C.c_double),
         ("deltat", C.c_double),
@@ -123,6 +125,10 @@ def lib() -> C.CDLL:
     L.sparc_kernel_accumulate.argtypes = [vp, C.c_int]
     L.sparc_kernel_get.argtypes = [vp, C.c_int, DP]
     L.sparc_apply_op.argtypes = [vp, C.c_int, C.c_int, DP, DP]
+    L.sparc_kernel_accumulate_from.argtypes = [vp, vp, C.c_int]
+    L.sparc_checkpoints_alloc.argtypes = [vp, C.c_int]
+    L.sparc_checkpoint_save.argtypes = [vp, C.c_int]
+    L.sparc_checkpoint_load.argtypes = [vp, C.c_int]
     L.sparc_profile_enable.argtypes = [vp, C.c_int]
     L.sparc_profile_read.argtypes = [vp, C.c_int, DP, DP, C.POINTER(C.c_int64)]
     _LIB = L
@@ -221,10 +227,10 @@ class Engine:
         self._chk(self.L.sparc_reset(self.h), "sparc_reset")
 
     def get(self, arr: int, field: int) -> np.ndarray:
-        n = self.N if arr in (ARR_A, ARR_SCR) else self.NP
-        out = np.empty(n)
+        full = arr in (ARR_A, ARR_SCR, ARR_ALO)
+        out = np.empty(self.N if full else self.NP)
         self._chk(self.L.sparc_get_state(self.h, arr, field, _dp(out)), "get_state")
-        return out.reshape(self.shape, order="F") if arr in (ARR_A, ARR_SCR) else out
+        return out.reshape(self.shape, order="F") if full else out
 
     def put(self, arr: int, field: int, data):
         a = np.asfortranarray(data, dtype=np.float64).ravel(order="F")
@@ -272,6 +278,18 @@ class Engine:
              else np.array(out, dtype=np.float64, order="F").ravel(order="F"))
         self._chk(self.L.sparc_apply_op(self.h, op, ibc, _dp(a), _dp(o)), "apply_op")
         return o.reshape(self.shape, order="F")
+
+    def kernel_accumulate_from(self, src: "Engine", slot: int):
+        self._chk(self.L.sparc_kernel_accumulate_from(self.h, src.h, slot), "kernel_accumulate_from")
+
+    def checkpoints_alloc(self, n: int):
+        self._chk(self.L.sparc_checkpoints_alloc(self.h, n), "checkpoints_alloc")
+
+    def checkpoint_save(self, slot: int):
+        self._chk(self.L.sparc_checkpoint_save(self.h, slot), "checkpoint_save")
+
+    def checkpoint_load(self, slot: int):
+        self._chk(self.L.sparc_checkpoint_load(self.h, slot), "checkpoint_load")
 
     def profile(self, on: bool):
         self._chk(self.L.sparc_profile_enable(self.h, int(on)), "profile_enable")

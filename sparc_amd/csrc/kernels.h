@@ -305,6 +305,79 @@ __global__ __launch_bounds__(64) void k_update_filter(T* a, const T* __restrict_
   }
 }
 
+// Compensated form for an FP32 state held as hi + lo: the update and the
+// filter run in FP64 on hi + lo + dt*scr and the result is split back into
+// hi = fl(v), lo = fl(v - hi), so the state carries ~48 significant bits and
+// the per-step rounding of the sum does not accumulate over a solve.
+__global__ __launch_bounds__(64) void k_update_filter_c(float* a, float* lo,
+                                                        const float* __restrict__ scr, double dt,
+                                                        int nxy, int nz, int kst) {
+  constexpr int U = 16;
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= nxy) return;
+  const double c0 = 0.75390625, h1 = 0.5 * 0.41015625, h2 = 0.5 * -0.234375,
+               h3 = 0.5 * 0.087890625, h4 = 0.5 * -0.01953125, h5 = 0.5 * 0.001953125;
+  auto V = [&](float x, float l, float s) -> double {
+    return (double)x + (double)l + dt * (double)s;
+  };
+  double w[11];
+#pragma unroll
+  for (int i = 0; i < 5; ++i) w[i] = 0.0;
+#pragma unroll
+  for (int i = 5; i < 11; ++i) {
+    const int k = i - 5;
+    const long q = c + (long)nxy * k;
+    w[i] = (k < nz) ? V(a[q], lo[q], scr[q]) : 0.0;
+  }
+  float na[U], nl[U], ns[U];
+  auto load_chunk = [&](int kb) {
+#pragma unroll
+    for (int j = 0; j < U; ++j) {
+      const int kk = kb + j;
+      if (kk < nz) {
+        const long q = c + (long)nxy * kk;
+        na[j] = a[q];
+        nl[j] = lo[q];
+        ns[j] = scr[q];
+      }
+    }
+  };
+  load_chunk(6);
+  for (int k0 = 0; k0 < nz; k0 += U) {
+    float ca[U], cl[U], cs[U];
+#pragma unroll
+    for (int j = 0; j < U; ++j) {
+      ca[j] = na[j];
+      cl[j] = nl[j];
+      cs[j] = ns[j];
+    }
+    if (k0 + U + 6 < nz) load_chunk(k0 + U + 6);
+#pragma unroll
+    for (int j = 0; j < U; ++j) {
+      const int k = k0 + j;
+      if (k < nz) {
+        double o;
+        if (k == 0 || k == nz - 1)
+          o = w[5];
+        else if (k <= 4)
+          o = (kst == 2) ? 0.5 * (w[6] + w[4]) : w[5];
+        else if (k >= nz - 5)
+          o = 0.5 * (w[6] + w[4]);
+        else
+          o = c0 * w[5] + h1 * (w[4] + w[6]) + h2 * (w[3] + w[7]) + h3 * (w[2] + w[8]) +
+              h4 * (w[1] + w[9]) + h5 * (w[0] + w[10]);
+        const long q = c + (long)nxy * k;
+        const float hi = (float)o;
+        a[q] = hi;
+        lo[q] = (float)(o - (double)hi);
+#pragma unroll
+        for (int i = 0; i < 10; ++i) w[i] = w[i + 1];
+        w[10] = (k + 6 < nz) ? V(ca[j], cl[j], cs[j]) : 0.0;
+      }
+    }
+  }
+}
+
 template <typename T>
 __global__ void k_axpy(T* y, const T* __restrict__ x, T alpha, long n) {
   long i = blockIdx.x * (long)blockDim.x + threadIdx.x;

@@ -23,11 +23,14 @@ struct sparc_ctx {
   int fp32 = 0;
   size_t ts = 8;
   hipStream_t stream = nullptr;
+  hipEvent_t xev = nullptr;  // orders this stream against another context's (cross-context calls)
   int device = 0;
   long N = 0, nxy = 0, NP = 0;
   int nzpml = 0;
   void* a[SPARC_NFIELDS] = {};
   void* scr[SPARC_NFIELDS] = {};
+  void* alo[SPARC_NFIELDS] = {};  // low-order state words (compensated FP32 state)
+  bool comp = false;
   void* pmlvars[2] = {};
   void* scrpml[2] = {};
   void* bg[SPARC_BG_COUNT] = {};
@@ -64,6 +67,8 @@ struct sparc_ctx {
   double snap_scale = 1.0;
   double* kern[SPARC_KERN_COUNT] = {};
   double* grav = nullptr;
+  int nckpt = 0;
+  void* ckpt = nullptr;
   std::vector<void*> allocs;
   size_t bytes = 0;
   std::string err;
@@ -262,6 +267,7 @@ static size_t footprint_bytes(const sparc_config* cfg) {
   size_t NP = nxy * (size_t)std::max(cfg->npmlbot + cfg->npmltop, 1);
   size_t b = 0;
   b += 12 * N * ts;                         // a, scr
+  if (cfg->precision == SPARC_FP32 && cfg->compensated) b += 6 * N * ts;  // alo
   b += 4 * NP * ts;                         // pmlvars, scrpml
   b += SPARC_BG_COUNT * N * ts;             // backgrounds (upper bound: all 3-D)
   b += 8 * N * ts;                          // work
@@ -328,11 +334,14 @@ extern "C" int sparc_create(const sparc_config* cfg, sparc_ctx** out) {
   c->NP = c->nxy * std::max(c->nzpml, 1);
   HIPCHK(c, hipGetDevice(&c->device));
   HIPCHK(c, hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+  HIPCHK(c, hipEventCreateWithFlags(&c->xev, hipEventDisableTiming));
   const size_t ts = c->ts, N = c->N;
   int rc;
+  c->comp = c->fp32 && cfg->compensated;
   for (int f = 0; f < SPARC_NFIELDS; ++f) {
     if ((rc = dalloc(c, &c->a[f], N * ts))) return rc;
     if ((rc = dalloc(c, &c->scr[f], N * ts))) return rc;
+    if (c->comp && (rc = dalloc(c, &c->alo[f], N * ts))) return rc;
   }
   for (int f = 0; f < 2; ++f) {
     if ((rc = dalloc(c, &c->pmlvars[f], c->NP * ts))) return rc;
@@ -452,6 +461,7 @@ extern "C" void sparc_destroy(sparc_ctx* c) {
   for (auto e : c->pool) (void)hipEventDestroy(e);
   for (void* p : c->allocs)
     if (p) (void)hipFree(p);
+  if (c->xev) (void)hipEventDestroy(c->xev);
   if (c->stream) (void)hipStreamDestroy(c->stream);
   delete c;
 }
@@ -471,6 +481,7 @@ extern "C" int sparc_reset(sparc_ctx* c) {
   for (int f = 0; f < SPARC_NFIELDS; ++f) {
     HIPCHK(c, hipMemsetAsync(c->a[f], 0, c->N * c->ts, c->stream));
     HIPCHK(c, hipMemsetAsync(c->scr[f], 0, c->N * c->ts, c->stream));
+    if (c->comp) HIPCHK(c, hipMemsetAsync(c->alo[f], 0, c->N * c->ts, c->stream));
   }
   for (int f = 0; f < 2; ++f) {
     HIPCHK(c, hipMemsetAsync(c->pmlvars[f], 0, c->NP * c->ts, c->stream));
@@ -905,9 +916,15 @@ static int step_impl(sparc_ctx* c, int t0, int nsteps) {
     // a += deltat*scr with the z filter (step.f90:114-118)
     const T dt = (T)c->cfg.deltat;
     for (int f = 0; f < SPARC_NFIELDS; ++f) {
-      ProfScope ps(c, SPARC_KC_UPDATE, 3.0 * (double)c->N * sizeof(T));
-      hipLaunchKernelGGL((k_update_filter<T>), dim3(cdiv(nxy, 64)), dim3(64), 0, c->stream, F.a[f],
-                         (const T*)F.scr[f], dt, nxy, c->cfg.nz, is2d ? 6 : 2);
+      ProfScope ps(c, SPARC_KC_UPDATE, (c->comp ? 5.0 : 3.0) * (double)c->N * sizeof(T));
+      if (c->comp) {
+        hipLaunchKernelGGL(k_update_filter_c, dim3(cdiv(nxy, 64)), dim3(64), 0, c->stream,
+                           (float*)F.a[f], (float*)c->alo[f], (const float*)F.scr[f],
+                           c->cfg.deltat, nxy, c->cfg.nz, is2d ? 6 : 2);
+      } else {
+        hipLaunchKernelGGL((k_update_filter<T>), dim3(cdiv(nxy, 64)), dim3(64), 0, c->stream,
+                           F.a[f], (const T*)F.scr[f], dt, nxy, c->cfg.nz, is2d ? 6 : 2);
+      }
       LAUNCHCHK(c);
     }
     if (c->nzpml > 0)
@@ -976,6 +993,10 @@ static void* arr_ptr(sparc_ctx* c, int arr, int field, size_t* n) {
     *n = c->NP;
     return arr == SPARC_ARR_PMLVARS ? c->pmlvars[field] : c->scrpml[field];
   }
+  if (arr == SPARC_ARR_ALO && c->comp && field >= 0 && field < SPARC_NFIELDS) {
+    *n = c->N;
+    return c->alo[field];
+  }
   return nullptr;
 }
 
@@ -984,7 +1005,12 @@ extern "C" int sparc_get_state(sparc_ctx* c, int arr, int field, double* h) {
   size_t n;
   void* p = arr_ptr(c, arr, field, &n);
   if (!p) return fail(c, SPARC_EINVAL, "bad array %d / field %d", arr, field);
-  return download(c, h, p, n, 1.0 / c->cfg.field_scale);
+  int rc = download(c, h, p, n, 1.0 / c->cfg.field_scale);
+  if (rc || !(c->comp && arr == SPARC_ARR_A)) return rc;
+  std::vector<double> lo(n);
+  if ((rc = download(c, lo.data(), c->alo[field], n, 1.0 / c->cfg.field_scale))) return rc;
+  for (size_t i = 0; i < n; ++i) h[i] += lo[i];
+  return SPARC_OK;
 }
 
 extern "C" int sparc_put_state(sparc_ctx* c, int arr, int field, const double* h) {
@@ -996,7 +1022,16 @@ extern "C" int sparc_put_state(sparc_ctx* c, int arr, int field, const double* h
   std::vector<double> t(h, h + n);
   if (c->cfg.field_scale != 1.0)
     for (auto& v : t) v *= c->cfg.field_scale;
-  return upload(c, p, t.data(), n);
+  if (!(c->comp && arr == SPARC_ARR_A)) return upload(c, p, t.data(), n);
+  // split each FP64 value into hi = fl(v) and lo = fl(v - hi)
+  std::vector<double> lo(n);
+  for (size_t i = 0; i < n; ++i) {
+    const double hi = (double)(float)t[i];
+    lo[i] = t[i] - hi;
+    t[i] = hi;
+  }
+  int rc = upload(c, p, t.data(), n);
+  return rc ? rc : upload(c, c->alo[field], lo.data(), n);
 }
 
 extern "C" int sparc_get_plane(sparc_ctx* c, int field, int k, double* h) {
@@ -1057,6 +1092,21 @@ extern "C" int sparc_sample(sparc_ctx* c, int field, int npts, const int32_t* ij
 // ---------------------------------------------------------------------------
 // kernels
 // ---------------------------------------------------------------------------
+// Cross-context ordering without blocking the host: `consumer` waits for the
+// work already queued on `producer`, and afterwards `producer` waits for what
+// `consumer` has queued that reads the producer's memory, so a later store
+// into the producer's snapshot bank cannot overtake that read.
+static int order_before(sparc_ctx* consumer, const sparc_ctx* producer) {
+  HIPCHK(consumer, hipEventRecord(producer->xev, producer->stream));
+  HIPCHK(consumer, hipStreamWaitEvent(consumer->stream, producer->xev, 0));
+  return SPARC_OK;
+}
+static int order_after(sparc_ctx* consumer, const sparc_ctx* producer) {
+  HIPCHK(consumer, hipEventRecord(consumer->xev, consumer->stream));
+  HIPCHK(consumer, hipStreamWaitEvent(producer->stream, consumer->xev, 0));
+  return SPARC_OK;
+}
+
 extern "C" int sparc_snapshots_alloc(sparc_ctx* c, int nslots) {
   if (!c || nslots < 0) return SPARC_EINVAL;
   if (nslots <= c->nslots) return SPARC_OK;
@@ -1088,11 +1138,11 @@ extern "C" int sparc_snapshots_copy(sparc_ctx* dst, const sparc_ctx* src) {
     return fail(dst, SPARC_EINVAL, "snapshot banks differ in shape or precision");
   int rc = sparc_snapshots_alloc(dst, src->nslots);
   if (rc) return rc;
-  HIPCHK(dst, hipStreamSynchronize(src->stream));
+  if ((rc = order_before(dst, src))) return rc;
   HIPCHK(dst, hipMemcpyAsync(dst->snap, src->snap, (size_t)src->nslots * 6 * src->N * src->ts,
                              hipMemcpyDeviceToDevice, dst->stream));
   dst->snap_scale = src->snap_scale;
-  return SPARC_OK;
+  return order_after(dst, src);
 }
 
 extern "C" int sparc_kernel_begin(sparc_ctx* c, const double* grav, int nz) {
@@ -1111,11 +1161,11 @@ extern "C" int sparc_kernel_begin(sparc_ctx* c, const double* grav, int nz) {
 }
 
 template <typename T>
-static int kernel_acc_impl(sparc_ctx* c, int slot) {
+static int kernel_acc_impl(sparc_ctx* c, const sparc_ctx* src, int slot) {
   const long N = c->N, nxy = c->nxy;
   const int nz = c->cfg.nz;
   Fld<T> F = make_fld<T>(c);
-  T* s = (T*)c->snap + (size_t)slot * 6 * N;
+  T* s = (T*)src->snap + (size_t)slot * 6 * N;
   const T *fxx = s, *fxy = s + N, *fxz = s + 2 * N, *fax = s + 3 * N, *fay = s + 4 * N,
           *faz = s + 5 * N;
   const T *avx = F.a[3], *avy = F.a[4], *avz = F.a[5];
@@ -1151,7 +1201,7 @@ static int kernel_acc_impl(sparc_ctx* c, int slot) {
   hipLaunchKernelGGL((k_update_filter<T>), dim3(cdiv(nxy, 64)), dim3(64), 0, c->stream, F.W[2],
                      (const T*)nullptr, (T)0, (int)nxy, nz, 2);
   LAUNCHCHK(c);
-  const double sff = 1.0 / c->snap_scale, sfa = 1.0 / c->cfg.field_scale;
+  const double sff = 1.0 / src->snap_scale, sfa = 1.0 / c->cfg.field_scale;
   hipLaunchKernelGGL((k_kern_accum<T>), dim3(cdiv(N, 256)), dim3(256), 0, c->stream,
                      c->kern[SPARC_KERN_HESSIAN], c->kern[SPARC_KERN_C2], c->kern[SPARC_KERN_RHO],
                      fxx, fxy, fxz, fax, fay, faz, avx, avy, avz, (const T*)F.W[5],
@@ -1162,12 +1212,58 @@ static int kernel_acc_impl(sparc_ctx* c, int slot) {
   return SPARC_OK;
 }
 
-extern "C" int sparc_kernel_accumulate(sparc_ctx* c, int slot) {
-  if (!c || slot < 0 || slot >= c->nslots) return fail(c, SPARC_EINVAL, "bad slot %d", slot);
+extern "C" int sparc_kernel_accumulate_from(sparc_ctx* c, const sparc_ctx* src, int slot) {
+  if (!c || !src) return SPARC_EINVAL;
+  if (slot < 0 || slot >= src->nslots) return fail(c, SPARC_EINVAL, "bad slot %d", slot);
+  if (src->N != c->N || src->fp32 != c->fp32)
+    return fail(c, SPARC_EINVAL, "snapshot bank differs in shape or precision");
   if (!c->kern[0]) return fail(c, SPARC_ESTATE, "sparc_kernel_begin not called");
   if (!c->bgset[SPARC_BG_C2]) return fail(c, SPARC_ESTATE, "background C2 not set");
-  return c->fp32 ? kernel_acc_impl<float>(c, slot) : kernel_acc_impl<double>(c, slot);
+  int rc;
+  if (src != c && (rc = order_before(c, src))) return rc;
+  rc = c->fp32 ? kernel_acc_impl<float>(c, src, slot) : kernel_acc_impl<double>(c, src, slot);
+  if (rc || src == c) return rc;
+  return order_after(c, src);
 }
+
+extern "C" int sparc_kernel_accumulate(sparc_ctx* c, int slot) {
+  return sparc_kernel_accumulate_from(c, c, slot);
+}
+
+extern "C" int sparc_checkpoints_alloc(sparc_ctx* c, int nslots) {
+  if (!c || nslots < 0) return SPARC_EINVAL;
+  if (nslots <= c->nckpt) return SPARC_OK;
+  void* p;
+  const size_t per = ((c->comp ? 12 : 6) * (size_t)c->N + 2 * (size_t)c->NP) * c->ts;
+  int rc = dalloc(c, &p, (size_t)nslots * per);
+  if (rc) return rc;
+  c->ckpt = p;
+  c->nckpt = nslots;
+  return SPARC_OK;
+}
+
+static int ckpt_copy(sparc_ctx* c, int slot, bool save) {
+  if (!c || slot < 0 || slot >= c->nckpt) return fail(c, SPARC_EINVAL, "bad checkpoint %d", slot);
+  const size_t fb = c->N * c->ts, pb = c->NP * c->ts;
+  const int nf = c->comp ? 2 * SPARC_NFIELDS : SPARC_NFIELDS;
+  char* base = (char*)c->ckpt + (size_t)slot * (nf * fb + 2 * pb);
+  for (int f = 0; f < nf; ++f) {
+    void* dev = f < SPARC_NFIELDS ? c->a[f] : c->alo[f - SPARC_NFIELDS];
+    void* d = save ? (void*)(base + f * fb) : dev;
+    const void* s = save ? (const void*)dev : (const void*)(base + f * fb);
+    HIPCHK(c, hipMemcpyAsync(d, s, fb, hipMemcpyDeviceToDevice, c->stream));
+  }
+  for (int f = 0; f < 2; ++f) {
+    char* pp = base + nf * fb + f * pb;
+    void* d = save ? (void*)pp : c->pmlvars[f];
+    const void* s = save ? (const void*)c->pmlvars[f] : (const void*)pp;
+    HIPCHK(c, hipMemcpyAsync(d, s, pb, hipMemcpyDeviceToDevice, c->stream));
+  }
+  return SPARC_OK;
+}
+
+extern "C" int sparc_checkpoint_save(sparc_ctx* c, int slot) { return ckpt_copy(c, slot, true); }
+extern "C" int sparc_checkpoint_load(sparc_ctx* c, int slot) { return ckpt_copy(c, slot, false); }
 
 extern "C" int sparc_kernel_get(sparc_ctx* c, int which, double* h) {
   if (!c || !h || which < 0 || which >= SPARC_KERN_COUNT || !c->kern[0])

@@ -36,6 +36,7 @@ class Params:
     obsheight: float = 0.0         # cm above the photosphere
     precision: int = abi.FP32
     field_scale: float = 1.0
+    compensated: bool = True       # FP32: hi+lo state updated in FP64 arithmetic
 
     @property
     def is2d(self) -> bool:
@@ -91,6 +92,8 @@ def workload(name: str, precision: int | None = None) -> Workload:
     box2l  384 x 384 x 300, 300 Mm (the large grid of the serving mix)
     tiny   32 x 32 x 40, 25 Mm (parity tests)
     tall   32 x 32 x 200, 25 Mm (parity tests of the segmented z solve)
+    column 32 x 32 x 100, 25 Mm, the vertical grid of box1 (adjoint/gradient tests:
+           receivers and source lie below the top PML and filter layers)
     two_d  256 x 1 x 300, 400 Mm, option 17 (the reference's wired 2-D case)
     """
     if name == "box1":
@@ -111,6 +114,9 @@ def workload(name: str, precision: int | None = None) -> Workload:
     elif name == "tall":
         nx = ny = 32
         nz, L, nsteps, prec = 200, 25.0, 10, abi.FP64
+    elif name == "column":
+        nx = ny = 32
+        nz, L, nsteps, prec = 100, 25.0, 400, abi.FP64
     elif name == "two_d":
         tab = bgm.polytrope_table(300, 0.5, -12.0)
         P = Params(nx=256, ny=1, nz=300, xlength=400.0 * M_CM, ylength=400.0 * M_CM,
@@ -159,6 +165,7 @@ def make_config(P: Params, S: bgm.Setup, mode: int, nsteps_total: int) -> abi.Sp
     c.delta_width = 2.0 / (S.z[k + 1] - S.z[k - 1])  # DETERMINE_STATUS (driver.f90:644,690)
     c.cadforcing = P.cadforcing
     c.field_scale = P.field_scale if P.precision == abi.FP32 else 1.0
+    c.compensated = int(P.compensated and P.precision == abi.FP32)
     return c
 
 
@@ -197,8 +204,10 @@ def forward_source(W: Workload, nt: int):
 
 
 def make_engine(W: Workload, S: bgm.Setup, mode: int = abi.FORWARD, nsteps: int | None = None,
-                precision: int | None = None):
+                precision: int | None = None, compensated: bool | None = None):
     P = W.params if precision is None else replace(W.params, precision=precision)
+    if compensated is not None:
+        P = replace(P, compensated=compensated)
     cfg = make_config(P, S, mode, nsteps if nsteps is not None else W.nsteps)
     eng = abi.Engine(cfg)
     load_setup(eng, S)

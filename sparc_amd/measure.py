@@ -14,10 +14,7 @@ import math
 import numpy as np
 
 
-def compute_tt(u0: np.ndarray, u: np.ndarray, dt: float) -> float:
-    """Travel-time shift of u relative to u0 (COMPUTE_TT): cc(lag) =
-    sum u0(t) u(t+lag) dt over all lags, parabola through the peak and its
-    neighbours, tau = -p1/(2 p2).  Positive tau: u arrives later."""
+def _tt_parts(u0: np.ndarray, u: np.ndarray, dt: float):
     nt = u0.size
     # cc(i) for i = -(nt-1)..nt-1 with cc(i) = sum_q u0(q) u(q+i)
     cc = np.correlate(u, u0, mode="full") * dt
@@ -29,7 +26,31 @@ def compute_tt(u0: np.ndarray, u: np.ndarray, dt: float) -> float:
     inv = np.linalg.inv(mat)
     p1 = inv[1] @ cc[loc - 1:loc + 2]
     p2 = inv[2] @ cc[loc - 1:loc + 2]
+    return loc - (nt - 1), inv, p1, p2
+
+
+def compute_tt(u0: np.ndarray, u: np.ndarray, dt: float) -> float:
+    """Travel-time shift of u relative to u0 (COMPUTE_TT): cc(lag) =
+    sum u0(t) u(t+lag) dt over all lags, parabola through the peak and its
+    neighbours, tau = -p1/(2 p2).  Positive tau: u arrives later."""
+    _, _, p1, p2 = _tt_parts(u0, u, dt)
     return float(-p1 * 0.5 / p2)
+
+
+def compute_tt_grad(u0: np.ndarray, u: np.ndarray, dt: float) -> np.ndarray:
+    """Exact derivative of compute_tt with respect to the samples of u
+    (valid away from ties of the correlation peak): d tau/d cc_i through
+    the parabola vertex, d cc(lag)/d u(m) = u0(m - lag) dt."""
+    lag0, inv, p1, p2 = _tt_parts(u0, u, dt)
+    dtau_dcc = -0.5 * inv[1] / p2 + 0.5 * p1 * inv[2] / p2 ** 2
+    nt = u0.size
+    g = np.zeros(nt)
+    for i, lag in enumerate((lag0 - 1, lag0, lag0 + 1)):
+        m = np.arange(nt)
+        q = m - lag
+        ok = (q >= 0) & (q < nt)
+        g[m[ok]] += dtau_dcc[i] * u0[q[ok]] * dt
+    return g
 
 
 def gaussian_window(nt: int, dt: float, center: float, width: float) -> np.ndarray:
@@ -75,16 +96,29 @@ class RingMeasurement:
         """0.5 * sum tau^2 (driver.f90:908, MISFIT_ALL)."""
         return 0.5 * float(np.sum(tau ** 2))
 
-    def adjoint_source(self, syn: np.ndarray, tau: np.ndarray) -> np.ndarray:
-        """d(misfit)/d(u_i(t_m)) per receiver (ADJOINT_SOURCE_COMP,
-        driver.f90:870-877): tau * W(t), W = -w du/dt / (sum (w du/dt)^2 dt)
-        with w the window; returned in forward time order (nrec, nt)."""
+    def adjoint_source(self, syn: np.ndarray, tau: np.ndarray, obs: np.ndarray | None = None,
+                       exact: bool = False) -> np.ndarray:
+        """d(misfit)/d(u_i(t)) per receiver as a density in time (the misfit
+        changes by sum_m a(t_m) du(t_m) dt), in forward time order.
+
+        Default: the reference's linearised form (ADJOINT_SOURCE_COMP,
+        driver.f90:870-877): tau * W(t), W = -w d(wu)/dt / (sum (d(wu)/dt)^2 dt)
+        (with the reference's box window w in {0,1} this is its
+        -ccdot/(sum ccdot^2 dt) over the window).
+        exact=True: the derivative of the discrete estimator itself
+        (compute_tt_grad on the windowed traces; needs obs), which the
+        parabola fit through cadence-spaced correlation samples makes differ
+        from the linearised form by the fit's bias."""
         out = np.zeros_like(syn)
         for r, (w, s) in enumerate(zip(self.windows, syn)):
-            cd = w * time_derivative(w * s, self.dt)
-            den = float(np.sum(cd ** 2) * self.dt)
+            if exact:
+                g = compute_tt_grad(w * obs[r], w * s, self.dt)
+                out[r] = tau[r] * w * g / self.dt
+                continue
+            ds = time_derivative(w * s, self.dt)
+            den = float(np.sum(ds ** 2) * self.dt)
             if den > 0.0:
-                out[r] = -tau[r] / den * cd
+                out[r] = -tau[r] / den * w * ds
         return out
 
 

@@ -65,23 +65,34 @@ def sample(eng, field: int, ijk: np.ndarray) -> np.ndarray:
     return a[ijk[:, 0], ijk[:, 1], ijk[:, 2] - 1]
 
 
-def forward_solve(eng, spec: SolveSpec, store: bool = True) -> np.ndarray:
+def _run_to(eng, t: int, i_end: int) -> int:
+    """Steps time indices t..i_end; returns the next time index."""
+    if i_end >= t:
+        eng.step(t, i_end - t + 1)
+    return i_end + 1
+
+
+def forward_solve(eng, spec: SolveSpec, store: bool = True, kstride: int | None = None) -> np.ndarray:
     """Runs the forward loop on an engine whose source is set; returns the
-    receiver traces (nrec, nt).  With store, forward snapshot k (stamp
-    k*steps) is kept in slot k."""
+    receiver traces (nrec, nt) sampled after steps m*steps.  With store, the
+    partial state after step j*kstride is kept in slot j (kstride defaults to
+    the output cadence, the reference's snapshot stamps)."""
     steps = spec.steps
+    ks = kstride or steps
     ijk = _ijk(spec, eng.cfg.o_rad)
+    nslots = spec.maxtime // ks + 1
     if store:
-        eng.snapshots_alloc(spec.nt)
+        eng.snapshots_alloc(nslots)
     traces = np.zeros((len(spec.receivers), spec.nt))
+    events = sorted(set(range(0, spec.maxtime + 1, steps)) | (set(range(0, spec.maxtime + 1, ks))
+                                                            if store else set()))
     t = 0
-    for m in range(spec.nt):
-        stamp = m * steps
-        eng.step(t, stamp - t + 1)
-        t = stamp + 1
-        traces[:, m] = sample(eng, abi.V_Z, ijk)
-        if store:
-            eng.snapshot_store(m)
+    for i in events:
+        t = _run_to(eng, t, i)
+        if i % steps == 0:
+            traces[:, i // steps] = sample(eng, abi.V_Z, ijk)
+        if store and i % ks == 0:
+            eng.snapshot_store(i // ks)
     return traces
 
 
@@ -101,19 +112,31 @@ def adjoint_vr(spec: SolveSpec, adj: np.ndarray, S: bgm.Setup) -> np.ndarray:
     return vr
 
 
-def adjoint_solve(eng, spec: SolveSpec, vr: np.ndarray, grav: np.ndarray) -> None:
-    """Adjoint loop with the Lagrange source; accumulates the kernel sums
-    against the forward snapshots held by ``eng`` (slot nt-1-m at adjoint
-    stamp m*steps)."""
-    steps = spec.steps
+def adjoint_solve(eng, spec: SolveSpec, vr: np.ndarray, grav: np.ndarray,
+                  kstride: int | None = None, aligned: bool = False) -> None:
+    """Adjoint loop with the Lagrange source, accumulating the kernel sums
+    against the forward snapshots held by ``eng``.
+
+    Reference pairing (aligned=False): forward stamp i_f with adjoint stamp
+    maxtime - i_f (kernels.f90:203,241).  The time-reversed source maps
+    adjoint time t to forward time T' - t with T' = (maxtime + 1) dt
+    (samples stamped after their step, driver.f90:455); the states after
+    steps i_f and i_a are paired exactly when i_a = maxtime - 1 - i_f
+    (aligned=True), one step earlier than the reference's pairing."""
+    ks = kstride or spec.steps
     eng.set_source(abi.SRC_LAGRANGE, vr, None, 10 ** 9)
     eng.kernel_begin(grav)
+    nslots = spec.maxtime // ks + 1
+    pairs = []
+    for j in range(nslots):
+        ia = spec.maxtime - j * ks - (1 if aligned else 0)
+        if ia >= 0:
+            pairs.append((ia, j))
+    pairs.sort()
     t = 0
-    for m in range(spec.nt):
-        stamp = m * steps
-        eng.step(t, stamp - t + 1)
-        t = stamp + 1
-        eng.kernel_accumulate(spec.nt - 1 - m)
+    for ia, j in pairs:
+        t = _run_to(eng, t, ia)
+        eng.kernel_accumulate(j)
 
 
 def cell_volume(S: bgm.Setup) -> np.ndarray:
@@ -124,10 +147,11 @@ def cell_volume(S: bgm.Setup) -> np.ndarray:
     return dxn * dyn * dz
 
 
-def kernel_lnc(raw_c2: np.ndarray, S: bgm.Setup, spec: SolveSpec) -> np.ndarray:
+def kernel_lnc(raw_c2: np.ndarray, S: bgm.Setup, spec: SolveSpec,
+               kstride: int | None = None) -> np.ndarray:
     """Sound-speed kernel K_lnc (per unit nondimensional volume) from the
     accumulated sum (kernels.f90:537: -2 rho0 c^2 sum * stepskern*timestep)."""
-    dtn = spec.steps * S.deltat
+    dtn = (kstride or spec.steps) * S.deltat
     return -2.0 * S.rho0 * S.c2 * raw_c2 * dtn
 
 
@@ -142,6 +166,9 @@ def make_engines(spec: SolveSpec, S: bgm.Setup, eng_factory=None):
         c.precision = spec.precision
         if spec.precision == abi.FP64:
             c.field_scale = 1.0
+            c.compensated = 0
+        else:
+            c.compensated = int(P.compensated)
     if eng_factory is None:
         fe, ae = abi.Engine(fcfg), abi.Engine(acfg)
     else:
@@ -174,20 +201,23 @@ class SolveResult:
 
 
 def full_solve(spec: SolveSpec, S: bgm.Setup, obs: np.ndarray, meas: RingMeasurement | None = None,
-               eng_factory=None) -> tuple[SolveResult, RingMeasurement]:
-    """forward -> travel times vs obs -> adjoint source -> adjoint + kernels."""
+               eng_factory=None, kstride: int | None = None, aligned: bool = False,
+               exact_adjoint: bool = False) -> tuple[SolveResult, RingMeasurement]:
+    """forward -> travel times vs obs -> adjoint source -> adjoint + kernels.
+    Defaults follow the reference (kernel sums at the output cadence, the
+    reference's snapshot pairing); kstride/aligned refine the quadrature."""
     fe, ae = make_engines(spec, S, eng_factory)
     try:
-        syn = forward_solve(fe, spec, store=True)
+        syn = forward_solve(fe, spec, store=True, kstride=kstride)
         if meas is None:
             meas = RingMeasurement(syn, spec.cad, spec.window_width)
         tau = meas.travel_times(obs, syn)
-        adj = meas.adjoint_source(syn, tau)
+        adj = meas.adjoint_source(syn, tau, obs=obs, exact=exact_adjoint)
         vr = adjoint_vr(spec, adj, S)
-        copy_snapshots(ae, fe, spec.nt)
-        adjoint_solve(ae, spec, vr, S.g_raw)
+        copy_snapshots(ae, fe, spec.maxtime // (kstride or spec.steps) + 1)
+        adjoint_solve(ae, spec, vr, S.g_raw, kstride=kstride, aligned=aligned)
         raw = {k: ae.kernel_get(k) for k in (abi.KERN_C2, abi.KERN_RHO, abi.KERN_HESSIAN)}
-        K = kernel_lnc(raw[abi.KERN_C2], S, spec)
+        K = kernel_lnc(raw[abi.KERN_C2], S, spec, kstride=kstride)
         res = SolveResult(traces=syn, tau=tau, misfit=meas.misfit(tau), adj=adj, kernel_raw=raw,
                           K_lnc=K)
         return res, meas
@@ -217,3 +247,91 @@ def bump_c2(S0: bgm.Setup, W: D.Workload, amp: float = 0.02, radius_mm: float = 
     r2 = ((x[:, None, None] - x0_mm) ** 2 + (y[None, :, None] - y0_mm) ** 2
           + (zmm[None, None, :] + depth_mm) ** 2)
     return S0.c2 * (1.0 + amp * np.exp(-r2 / radius_mm ** 2)) ** 2
+
+
+# ---------------------------------------------------------------------------
+# checkpointed forward + recomputing adjoint
+# ---------------------------------------------------------------------------
+def forward_checkpointed(eng, spec: SolveSpec, every: int) -> np.ndarray:
+    """Forward loop keeping only restart states: checkpoint g = the state
+    before step g*every (device memory).  Returns the receiver traces."""
+    steps = spec.steps
+    nck = spec.maxtime // every + 1
+    eng.checkpoints_alloc(nck)
+    ijk = _ijk(spec, eng.cfg.o_rad)
+    traces = np.zeros((len(spec.receivers), spec.nt))
+    events = sorted(set(range(0, spec.maxtime + 1, steps)) |
+                    set(g * every - 1 for g in range(1, nck)) | {-1})
+    t = 0
+    for i in events:
+        t = _run_to(eng, t, i)
+        if i >= 0 and i % steps == 0:
+            traces[:, i // steps] = sample(eng, abi.V_Z, ijk)
+        if (i + 1) % every == 0 and (i + 1) // every < nck:
+            eng.checkpoint_save((i + 1) // every)
+    return traces
+
+
+def adjoint_recompute(fe, ae, spec: SolveSpec, vr: np.ndarray, grav: np.ndarray, every: int,
+                      kstride: int | None = None, aligned: bool = False) -> None:
+    """Adjoint loop that regenerates the forward snapshots segment by segment
+    from the checkpoints of ``fe`` (last segment first, as the adjoint
+    consumes forward time backwards) and accumulates against them."""
+    ks = kstride or spec.steps
+    nslots = spec.maxtime // ks + 1
+    pairs = []
+    for j in range(nslots):
+        ia = spec.maxtime - j * ks - (1 if aligned else 0)
+        if ia >= 0:
+            pairs.append((ia, j))
+    pairs.sort()  # adjoint time ascending = forward slot descending
+    seg_of = lambda j: (j * ks) // every
+    per_seg = every // ks + 2
+    fe.snapshots_alloc(per_seg)
+    ae.set_source(abi.SRC_LAGRANGE, vr, None, 10 ** 9)
+    ae.kernel_begin(grav)
+    t_adj = 0
+    q = 0
+    while q < len(pairs):
+        g = seg_of(pairs[q][1])
+        group = []
+        while q < len(pairs) and seg_of(pairs[q][1]) == g:
+            group.append(pairs[q])
+            q += 1
+        # regenerate the forward snapshots of segment g
+        fe.checkpoint_load(g)
+        slots = sorted(j for _, j in group)
+        local = {j: n for n, j in enumerate(slots)}
+        t = g * every
+        for j in slots:
+            t = _run_to(fe, t, j * ks)
+            fe.snapshot_store(local[j])
+        for ia, j in group:
+            t_adj = _run_to(ae, t_adj, ia)
+            ae.kernel_accumulate_from(fe, local[j])
+
+
+def full_solve_checkpointed(spec: SolveSpec, S: bgm.Setup, obs: np.ndarray,
+                            meas: RingMeasurement | None, every: int,
+                            kstride: int | None = None, aligned: bool = False,
+                            exact_adjoint: bool = False):
+    """full_solve with checkpoints every ``every`` steps instead of a
+    snapshot bank for the whole run; returns (SolveResult, meas, peak device
+    bytes of the two contexts)."""
+    fe, ae = make_engines(spec, S)
+    try:
+        syn = forward_checkpointed(fe, spec, every)
+        if meas is None:
+            meas = RingMeasurement(syn, spec.cad, spec.window_width)
+        tau = meas.travel_times(obs, syn)
+        adj = meas.adjoint_source(syn, tau, obs=obs, exact=exact_adjoint)
+        vr = adjoint_vr(spec, adj, S)
+        adjoint_recompute(fe, ae, spec, vr, S.g_raw, every, kstride=kstride, aligned=aligned)
+        raw = {k: ae.kernel_get(k) for k in (abi.KERN_C2, abi.KERN_RHO, abi.KERN_HESSIAN)}
+        K = kernel_lnc(raw[abi.KERN_C2], S, spec, kstride=kstride)
+        peak = fe.device_bytes + ae.device_bytes
+        return (SolveResult(traces=syn, tau=tau, misfit=meas.misfit(tau), adj=adj,
+                            kernel_raw=raw, K_lnc=K), meas, peak)
+    finally:
+        fe.close()
+        ae.close()

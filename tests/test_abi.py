@@ -28,7 +28,7 @@ def test_library_exports_every_declared_symbol():
 
 
 def test_config_struct_layout():
-    # 16 int32 + 8 reserved int32, then 9 doubles + 7 reserved doubles
+    # 17 int32 + 7 reserved int32, then 9 doubles + 7 reserved doubles
     assert C.sizeof(abi.SparcConfig) == 24 * 4 + 16 * 8
 
 

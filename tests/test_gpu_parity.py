@@ -113,8 +113,8 @@ def test_operators_match_oracle(tiny, prec, tol):
         eng.close()
 
 
-def _run_pair(W, S, nsteps, prec, t0=0):
-    eng, cfg = D.make_engine(W, S, precision=prec, nsteps=nsteps)
+def _run_pair(W, S, nsteps, prec, t0=0, compensated=None):
+    eng, cfg = D.make_engine(W, S, precision=prec, nsteps=nsteps, compensated=compensated)
     o, _ = make_oracle(W, S, nsteps=nsteps)
     mask, series = D.forward_source(W, nsteps + 8)
     for x in (eng, o):
@@ -124,10 +124,10 @@ def _run_pair(W, S, nsteps, prec, t0=0):
     return eng, o
 
 
-@pytest.mark.parametrize("prec", [abi.FP64, abi.FP32])
-def test_step_3d_matches_oracle(tiny, prec):
+@pytest.mark.parametrize("prec,comp", [(abi.FP64, None), (abi.FP32, True), (abi.FP32, False)])
+def test_step_3d_matches_oracle(tiny, prec, comp):
     W, S = tiny
-    eng, o = _run_pair(W, S, 14, prec)  # freq_filtering=12: spectral filter at time 0 and 12
+    eng, o = _run_pair(W, S, 14, prec, compensated=comp)  # spectral filter at time 0 and 12
     try:
         for f in range(6):
             a, b = eng.get(abi.ARR_A, f), o.get(abi.ARR_A, f)
@@ -233,13 +233,21 @@ def test_sample_plane_norm(tiny):
         o.close()
 
 
-def test_put_get_roundtrip_and_reset(tiny):
+@pytest.mark.parametrize("comp", [True, False])
+def test_put_get_roundtrip_and_reset(tiny, comp):
     W, S = tiny
-    eng, _ = D.make_engine(W, S, precision=abi.FP32)
+    eng, _ = D.make_engine(W, S, precision=abi.FP32, compensated=comp)
     try:
         f = _field(eng.shape, 3)
         eng.put(abi.ARR_A, abi.XI_Z, f)
-        assert rel_err(eng.get(abi.ARR_A, abi.XI_Z), f) < 1e-7
+        # the compensated state keeps hi+lo: ~48 bits of each FP64 value
+        assert rel_err(eng.get(abi.ARR_A, abi.XI_Z), f) < (1e-13 if comp else 1e-7)
+        if comp:
+            hi = eng.get(abi.ARR_A, abi.XI_Z) - eng.get(abi.ARR_ALO, abi.XI_Z)
+            assert np.array_equal(hi, f.astype(np.float32).astype(np.float64))
+        else:
+            with pytest.raises(abi.SparcError):
+                eng.get(abi.ARR_ALO, abi.XI_Z)
         eng.reset()
         assert not np.any(eng.get(abi.ARR_A, abi.XI_Z))
     finally:

@@ -1,0 +1,144 @@
+"""One source's forward -> travel times -> adjoint -> kernel pipeline on the
+GPU engine (through the C ABI):
+
+* the FP64 engine against the FP64 oracle driven by the same host pipeline
+  (traces, travel times and kernels to FFT/tridiagonal rounding);
+* the FP32 engine against the FP64 engine on the same adjoint source
+  (sensitivity kernel within 1e-4 relative L2 over the model layers);
+* the checkpointed backward (forward restarts every 100 steps, snapshots
+  regenerated per segment) against the stored-snapshot backward (identical);
+* the torch.autograd gradient against central finite differences of the
+  misfit along 16 random Gaussian-bump directions (numpy default_rng(4)).
+"""
+import numpy as np
+import pytest
+import torch
+
+from oracle import oracle as O
+from sparc_amd import abi
+from sparc_amd import background as B
+from sparc_amd import driver as D
+from sparc_amd import pipeline as PL
+from sparc_amd.autograd import Problem, misfit
+from tests.helpers import rel_l2
+
+pytestmark = pytest.mark.gpu
+
+
+def _coords(W, S):
+    P = W.params
+    x = ((np.arange(P.nx) / P.nx) - 0.5) * P.xlength / 1e8
+    y = ((np.arange(P.ny) / P.ny) - 0.5) * P.ylength / 1e8
+    zmm = (S.z - 1.0) * B.RSUN_R / 1e8
+    return x, y, zmm
+
+
+def _bump(W, S, cx, cy, cz, r):
+    x, y, zmm = _coords(W, S)
+    return np.exp(-((x[:, None, None] - cx) ** 2 + (y[None, :, None] - cy) ** 2
+                    + (zmm[None, None, :] - cz) ** 2) / r ** 2)
+
+
+@pytest.fixture(scope="module")
+def column():
+    """32 x 32 x 100 over 25 Mm (the vertical grid of the 64^3 box), 601
+    steps, four receivers 3 Mm from the source, observed data from a +2 %
+    sound-speed bump 4 Mm deep."""
+    W = D.workload("column")
+    S0 = D.setup_for(W)
+    rec = np.array([[20, 16], [16, 20], [12, 16], [16, 12]])
+    maxtime = W.params.steps * 40
+    true = 0.02 * _bump(W, S0, 1.0, 0.0, -4.0, 2.0)
+    spec = PL.SolveSpec(W=W, receivers=rec, maxtime=maxtime, precision=abi.FP64)
+    obs = PL.observe(spec, D.setup_for(W, c2_3d=S0.c2 * np.exp(2 * true)))
+    return W, S0, rec, maxtime, obs
+
+
+def test_pipeline_fp64_engine_matches_oracle():
+    W = D.workload("tiny")
+    S0 = D.setup_for(W)
+    rec = np.array([[16, 16], [18, 16], [16, 13]])
+    spec = PL.SolveSpec(W=W, receivers=rec, maxtime=W.params.steps * 20, precision=abi.FP64)
+    true = PL.bump_c2(S0, W, amp=0.05, radius_mm=3.0, depth_mm=2.0)
+    obs = PL.observe(spec, D.setup_for(W, c2_3d=true))
+    ours, meas = PL.full_solve(spec, S0, obs, kstride=1, aligned=True)
+    ref, _ = PL.full_solve(spec, S0, obs, meas=meas, eng_factory=O.Oracle, kstride=1,
+                           aligned=True)
+    assert np.max(np.abs(ref.traces)) > 0
+    assert rel_l2(ours.traces, ref.traces) < 1e-10
+    assert np.max(np.abs(ours.tau - ref.tau)) < 1e-8
+    assert abs(ours.misfit - ref.misfit) <= 1e-8 * ref.misfit
+    for k in (abi.KERN_C2, abi.KERN_RHO, abi.KERN_HESSIAN):
+        assert np.linalg.norm(ref.kernel_raw[k]) > 0
+        assert rel_l2(ours.kernel_raw[k], ref.kernel_raw[k]) < 1e-9, k
+
+
+def test_fp32_kernel_matches_fp64(column):
+    """Same adjoint source into both precisions (the FP64 solve's); the FP32
+    state is the compensated hi+lo form.  The PML layers are excluded from
+    the kernel norm: the model is held fixed there."""
+    W, S0, rec, maxtime, obs = column
+    P = W.params
+    r64, meas = PL.full_solve(PL.SolveSpec(W, rec, maxtime, abi.FP64), S0, obs)
+    sp = PL.SolveSpec(W, rec, maxtime, abi.FP32)
+    fe, ae = PL.make_engines(sp, S0)
+    try:
+        syn = PL.forward_solve(fe, sp, store=True)
+        assert rel_l2(syn, r64.traces) < 1e-5
+        PL.copy_snapshots(ae, fe, sp.maxtime // sp.steps + 1)
+        PL.adjoint_solve(ae, sp, PL.adjoint_vr(sp, r64.adj, S0), S0.g_raw)
+        K = PL.kernel_lnc(ae.kernel_get(abi.KERN_C2), S0, sp)
+    finally:
+        fe.close()
+        ae.close()
+    inner = slice(P.npmlbot, P.nz - P.npmltop)
+    err = rel_l2(K[:, :, inner], r64.K_lnc[:, :, inner])
+    print(f"FP32 kernel rel L2 (model layers) {err:.3e}, whole box "
+          f"{rel_l2(K, r64.K_lnc):.3e}")
+    assert err < 1e-4
+
+
+def test_checkpointed_backward_equals_stored(column):
+    W, S0, rec, maxtime, obs = column
+    grads = {}
+    peaks = {}
+    for every in (None, 100):
+        pr = Problem.build(W, rec, maxtime, obs, precision=abi.FP32, checkpoint_every=every)
+        chi, saved = pr.forward(np.zeros((W.params.nx, W.params.ny, W.params.nz)))
+        grads[every] = pr.backward(saved)
+        peaks[every] = pr.last_peak_bytes
+    assert np.any(grads[None])
+    assert np.array_equal(grads[100], grads[None])
+    print(f"peak device bytes: stored snapshots {peaks[None]}, checkpoints {peaks[100]}")
+    assert peaks[100] < peaks[None]
+
+
+def test_autograd_gradient_matches_finite_differences(column):
+    """d misfit along 16 random Gaussian bumps in the model layers: the
+    adjoint gradient (kernel sums every step, exact time-reversal pairing)
+    against central differences with step 1e-4.  The kernel formula is the
+    adjoint of the continuous equations, so the two agree to the
+    discretisation error, not to rounding: the vector of 16 directional
+    derivatives within 10 % relative L2, every sizeable one of the same sign."""
+    W, S0, rec, maxtime, obs = column
+    pr = Problem.build(W, rec, maxtime, obs, precision=abi.FP64, kstride=1)
+    dlnc = torch.zeros((W.params.nx, W.params.ny, W.params.nz), dtype=torch.float64,
+                       requires_grad=True)
+    chi = misfit(dlnc, pr)
+    chi.backward()
+    g = dlnc.grad.numpy()
+    chi = float(chi.detach())
+    assert abs(chi - pr.evaluate(np.zeros(g.shape))) <= 1e-12 * chi
+    rng = np.random.default_rng(4)
+    eps = 1e-4
+    fd, ad = [], []
+    for _ in range(16):
+        h = _bump(W, S0, rng.uniform(-6, 6), rng.uniform(-6, 6), rng.uniform(-9, -3),
+                  rng.uniform(1.0, 1.5))
+        fd.append((pr.evaluate(eps * h) - pr.evaluate(-eps * h)) / (2 * eps))
+        ad.append(float(np.sum(g * h)))
+    fd, ad = np.array(fd), np.array(ad)
+    print("adjoint / FD per direction:", np.round(ad / fd, 3))
+    assert np.linalg.norm(ad - fd) < 0.1 * np.linalg.norm(fd)
+    big = np.abs(fd) > 0.05 * np.max(np.abs(fd))
+    assert np.all(np.sign(ad[big]) == np.sign(fd[big]))
