@@ -27,6 +27,17 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+# the kernel behind each timed class in the forward step of the 3-D box
+KERNEL_SYMBOLS = {
+    "xlines": "sparc::k_xlines_n<float, 256, 0 (XM_DERIV), 8, 256> -- x derivative",
+    "ylines": "sparc::k_ylines_n<float, 256, 0 (YM_DERIV), 16, 512> -- y derivative",
+    "damp": "sparc::k_xlines_n<float, 256, 2 (XM_DAMP_FWD), 8, 256> -- damping, forward x",
+    "damp_y": "sparc::k_ylines_n<float, 256, 2 (YM_DAMP), 16, 256> -- damping, y pass",
+    "damp_xi": "sparc::k_xlines_n<float, 256, 3 (XM_DAMP_INV), 8, 256> -- damping, inverse x",
+    "zsolve": "sparc::k_zsolve<float, 2 (EPI_RHS3D), 4> -- z solve + momentum/continuity RHS",
+    "zsolve_p": "sparc::k_zsolve<float, 1 (EPI_P3D), 4> -- z solve + pressure/density",
+    "update": "sparc::k_update_filter_c -- a += dt*scr with the z filter (hi+lo state)",
+}
 
 
 def main() -> int:
@@ -39,7 +50,12 @@ def main() -> int:
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--traffic-json", default=os.path.join(ROOT, "profiles", "traffic.json"),
                     help="HBM bytes per launch measured by rocprofv3 --pmc (optional)")
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
+                    help="only run 5 device-to-device copies of MIB MiB (calibrates the "
+                         "FETCH_SIZE/WRITE_SIZE counters against known bytes) and exit")
     args = ap.parse_args()
+    if args.copy_probe:
+        return copy_probe(args.copy_probe)
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (this build has no multi-GPU path); "
               f"--gpus {args.gpus} is not supported", file=sys.stderr)
@@ -96,11 +112,14 @@ def main() -> int:
             traffic = ent["hbm_bytes_per_launch"]
     except (OSError, ValueError):
         pass
-    roof = {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1),
+    roof = {"bound": "hbm", "kernel": dom, "kernel_symbol": KERNEL_SYMBOLS.get(dom, dom),
+            "achieved": round(achieved, 1),
             "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
             "traffic": traffic, "algorithmic_bytes_per_launch": by / nl,
             "us_per_launch": 1e3 * ms / nl, "kernels": kernels}
-    step_algo = (18 * 6 + 5 * 5) * (4 if prec == abi.FP32 else 8)  # bytes/pt-step, 5 3-D bg
+    # bytes per point and step: 18 field arrays moved per field and step, 5
+    # 3-D background arrays, 2 more per field for the compensated state
+    step_algo = (18 * 6 + (12 if cfg.compensated else 0) + 5 * 5) * (4 if prec == abi.FP32 else 8)
     eng.close()
 
     cpu = None
@@ -125,8 +144,9 @@ def main() -> int:
                                f"filters, dt={P.timestep:.4f}s",
                    "nx": P.nx, "ny": P.ny, "nz": P.nz, "parallelism": "single GPU",
                    "finite": finite,
-                   "step_roofline_frac_532B": round(mpts * 1e6 * step_algo / 1e9 / HBM_PEAK_GBS,
-                                                    4)},
+                   "compensated_fp32_state": bool(cfg.compensated),
+                   "step_bytes_per_point": step_algo,
+                   "step_roofline_frac": round(mpts * 1e6 * step_algo / 1e9 / HBM_PEAK_GBS, 4)},
         "roofline": roof,
         "cpu_baseline": cpu,
     }
@@ -134,9 +154,23 @@ def main() -> int:
     return 0
 
 
-def cpu_baseline(W):
+def copy_probe(mib: int) -> int:
+    import torch
+    n = mib * (1 << 20) // 4
+    x = torch.ones(n, dtype=torch.float32, device="cuda")
+    y = torch.empty_like(x)
+    for _ in range(5):
+        y.copy_(x)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_probe_bytes_read": 4 * n, "copy_probe_bytes_written": 4 * n,
+                      "copies": 5}))
+    return 0
+
+
+def cpu_baseline(W, min_seconds: float = 10.0, max_steps: int = 8):
     """FP64 CPU oracle (C restatement of the reference step, OpenMP) timed on
-    one RK step of the same workload."""
+    RK steps of the same workload: steps until min_seconds have passed (at
+    most max_steps), after one untimed step."""
     try:
         from oracle import oracle as O
         from sparc_amd import abi
@@ -150,14 +184,20 @@ def cpu_baseline(W):
         o.set_source(abi.SRC_SERIES, mask, series, 10 ** 9)
         o.step(0, 1)  # warm (time 0 also runs the spectral filter)
         t0 = time.perf_counter()
-        o.step(1, 1)
+        n = 0
+        while n < max_steps:
+            o.step(1 + n, 1)
+            n += 1
+            if time.perf_counter() - t0 >= min_seconds:
+                break
         dt = time.perf_counter() - t0
         o.close()
         P = W.params
-        threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
-        return {"value": round(P.nx * P.ny * P.nz / dt / 1e6, 4), "unit": "Mpt/s",
+        threads = O.omp_threads()
+        return {"value": round(P.nx * P.ny * P.nz * n / dt / 1e6, 4), "unit": "Mpt/s",
                 "cores": threads, "kind": "port",
-                "sample": f"1 RK step of {P.nx}x{P.ny}x{P.nz} (FP64, {dt:.1f} s)"}
+                "sample": f"{n} RK steps (time 1..{n}) of {P.nx}x{P.ny}x{P.nz}, FP64, "
+                          f"{dt:.1f} s on {threads} OpenMP threads"}
     except Exception as e:  # the baseline is reported, never required
         return {"value": None, "unit": "Mpt/s", "cores": None, "kind": "port",
                 "sample": f"unavailable: {e}"}

@@ -106,14 +106,21 @@ extern "C" {
 #define SPARC_OP_LOWPASS_X 5 /* Fermi low-pass along x (physics.f90:796-802)     */
 #define SPARC_OP_LOWPASS_Y 6 /* Fermi low-pass along y (physics.f90:810-828)     */
 
-/* kernel classes timed by sparc_profile_read */
-#define SPARC_KC_XLINES 0   /* x-line FFT kernels (derivative, filter)          */
-#define SPARC_KC_YLINES 1   /* y-line FFT kernels (derivative, filter)          */
-#define SPARC_KC_ZSOLVE 2   /* compact-FD z solve with fused RHS epilogue       */
-#define SPARC_KC_DAMP 3     /* 2-D damping transform passes                     */
+/* kernel classes timed by sparc_profile_read (in the time step each class is
+ * one kernel) */
+#define SPARC_KC_XLINES 0   /* x-line FFT derivative                            */
+#define SPARC_KC_YLINES 1   /* y-line FFT derivative                            */
+#define SPARC_KC_ZSOLVE 2   /* compact-FD z solve with the fused momentum and
+                               continuity RHS epilogue (and plain stores)      */
+#define SPARC_KC_DAMP 3     /* damping: forward x transform pass                */
 #define SPARC_KC_UPDATE 4   /* a += dt*scr with the z filter                    */
 #define SPARC_KC_OTHER 5    /* sources, PML update, kernel accumulation         */
-#define SPARC_KC_COUNT 6
+#define SPARC_KC_ZSOLVE_P 6 /* compact-FD z solve with the fused pressure and
+                               density epilogue                                 */
+#define SPARC_KC_DAMP_Y 7   /* damping: y transform, rates, inverse y           */
+#define SPARC_KC_DAMP_XI 8  /* damping: inverse x transform and RHS update      */
+#define SPARC_KC_FILTER 9   /* spectral low-pass along x or y (every 60 s)      */
+#define SPARC_KC_COUNT 10
 
 /* kernels returned by sparc_kernel_get (kernels.f90:276-313,530-537) */
 #define SPARC_KERN_C2 0       /* sum div(xi_f) div(v_a)                      */

@@ -32,6 +32,7 @@ def lib():
             raise RuntimeError(f"oracle not built: {path} (run make -C oracle)")
         L = C.CDLL(path)
         L.orc_create.restype = C.c_void_p
+        L.orc_threads.restype = C.c_int
         L.orc_create.argtypes = [C.c_void_p]
         L.orc_destroy.argtypes = [C.c_void_p]
         L.orc_beta.restype = C.c_double
@@ -197,6 +198,11 @@ def init_rhs(nx, ny, nz, is2d, npmlbot, npmltop, diml, dimc, z, c2, rho0, p0, g)
     out["stretch"] = st
     assert N == c2.size
     return out
+
+
+def omp_threads() -> int:
+    """OpenMP threads the oracle's loops use."""
+    return int(lib().orc_threads())
 
 
 class Oracle:

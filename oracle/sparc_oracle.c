@@ -25,6 +25,7 @@
  *                   GPU engine at FP64.
  */
 #include <complex.h>
+#include <omp.h>
 #include <math.h>
 #include <stdint.h>
 #include <stdlib.h>
@@ -475,6 +476,17 @@ void orc_destroy(orc *o) {
 }
 
 double orc_beta(const orc *o, int s) { return o->betas[s]; }
+
+/* threads the OpenMP loops of the oracle run on */
+int orc_threads(void) {
+  int n = 1;
+#pragma omp parallel
+  {
+#pragma omp single
+    n = omp_get_num_threads();
+  }
+  return n;
+}
 
 int orc_set_background(orc *o, int id, const double *h, long n) {
   if (id < 0 || id >= SPARC_BG_COUNT) return SPARC_EINVAL;

@@ -26,8 +26,10 @@ ARR_A, ARR_SCR, ARR_PMLVARS, ARR_SCRPML, ARR_ALO = range(5)
 PROF_STRETCH, PROF_UNSTRETCH, PROF_G, PROF_Z = range(4)
 SRC_NONE, SRC_SERIES, SRC_LAGRANGE = range(3)
 KERN_C2, KERN_RHO, KERN_HESSIAN = range(3)
-KC_XLINES, KC_YLINES, KC_ZSOLVE, KC_DAMP, KC_UPDATE, KC_OTHER = range(6)
-KC_NAMES = ("xlines", "ylines", "zsolve", "damp", "update", "other")
+(KC_XLINES, KC_YLINES, KC_ZSOLVE, KC_DAMP, KC_UPDATE, KC_OTHER, KC_ZSOLVE_P, KC_DAMP_Y,
+ KC_DAMP_XI, KC_FILTER) = range(10)
+KC_NAMES = ("xlines", "ylines", "zsolve", "damp", "update", "other", "zsolve_p", "damp_y",
+            "damp_xi", "filter")
 OP_DDX, OP_DDY, OP_DDZ, OP_ZFILTER, OP_DAMP, OP_LOWPASS_X, OP_LOWPASS_Y = range(7)
 
 # every entry point declared in include/sparc_gpu.h

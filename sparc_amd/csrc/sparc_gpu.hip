@@ -667,8 +667,11 @@ static int launch_xm(sparc_ctx* c, int nf, const XArgs<T>& X, T beta, const T* m
     nb = (X.s[0] ? 2 : 1) + 1;
   else
     nb = 3;
-  ProfScope ps(c, (MODE == XM_DAMP_FWD || MODE == XM_DAMP_INV) ? SPARC_KC_DAMP : SPARC_KC_XLINES,
-               nf * nb * fb);
+  const int cls = MODE == XM_DAMP_FWD   ? SPARC_KC_DAMP
+                  : MODE == XM_DAMP_INV ? SPARC_KC_DAMP_XI
+                  : MODE == XM_FILTER   ? SPARC_KC_FILTER
+                                        : SPARC_KC_XLINES;
+  ProfScope ps(c, cls, nf * nb * fb);
   if (fast_size(n)) {
     switch (n) {
       case 32: launch_xn<T, MODE, 32>(c, nf, X, beta, mult, (int)nrows, (int)nlines); break;
@@ -724,7 +727,7 @@ template <typename T, int MODE>
 static int launch_ym(sparc_ctx* c, int nf, const XArgs<T>& X, T beta, const T* mult) {
   const int nx = c->cfg.nx, ny = c->cfg.ny;
   const double fb = (double)c->N * sizeof(T);
-  ProfScope ps(c, MODE == YM_DAMP ? SPARC_KC_DAMP : SPARC_KC_YLINES,
+  ProfScope ps(c, MODE == YM_DAMP ? SPARC_KC_DAMP_Y : (MODE == YM_FILTER ? SPARC_KC_FILTER : SPARC_KC_YLINES),
                nf * (MODE == YM_DAMP ? 2.0 : (X.s[0] ? 3.0 : 2.0)) * fb);
   bool done = true;
   switch (ny) {
@@ -788,7 +791,8 @@ static int launch_z(sparc_ctx* c, const Fld<T>& F, int ibc, const T* inA, const 
     nb = in + 3 + (F.beta != (T)0 ? 6 : 3) + nbg({SPARC_BG_RHOINV}) + 6;
   else
     nb = in + 2 + (F.beta != (T)0 ? 4 : 2) + nbg({SPARC_BG_RHOINV}) + 4;
-  ProfScope ps(c, SPARC_KC_ZSOLVE, nb * (double)c->N * sizeof(T));
+  ProfScope ps(c, (EPI == EPI_P3D || EPI == EPI_P2D) ? SPARC_KC_ZSOLVE_P : SPARC_KC_ZSOLVE,
+               nb * (double)c->N * sizeof(T));
   // segments per column: L = nz/NSEG planes each, at least ~48 (>= 2x warm-up)
   const int nseg = nz >= 192 ? 4 : (nz >= 96 ? 2 : 1);
   const size_t lds = 5 * (size_t)nz * sizeof(T);
