@@ -230,11 +230,14 @@ int sparc_snapshots_alloc(sparc_ctx *ctx, int nslots);
 /* Stores the current forward snapshot (after a step: a(1..3), scr(4..6),
  * WRITE_OUT_PARTIAL_STATE) into slot. */
 int sparc_snapshot_store(sparc_ctx *ctx, int slot);
-/* Copies the snapshot bank of src into dst (dst must have room). */
+/* Copies the snapshot bank of src into dst (allocating room in dst), ordered
+ * against both streams like sparc_kernel_accumulate_from. */
 int sparc_snapshots_copy(sparc_ctx *dst, const sparc_ctx *src);
 /* Like sparc_kernel_accumulate, with the forward snapshot read from the bank
- * of another context on the same device (src may equal ctx).  Waits for
- * src's queued work first. */
+ * of another context on the same device (src may equal ctx).  Stream-ordered
+ * both ways without blocking the host: the read waits for src's queued work,
+ * and src's later work (a snapshot store into the same slot) waits for the
+ * read. */
 int sparc_kernel_accumulate_from(sparc_ctx *ctx, const sparc_ctx *src, int slot);
 
 /* ---- forward-wavefield checkpoints (restart state of driver.f90:489-524,

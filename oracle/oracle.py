@@ -53,6 +53,7 @@ def lib():
         L.orc_ddy.argtypes = [DP, DP, C.c_int, C.c_int, C.c_int, C.c_double]
         L.orc_ddz.argtypes = [DP, DP, C.c_int, C.c_int, C.c_int, C.c_int, DP]
         L.orc_zfilter.argtypes = [DP, C.c_int, C.c_int, C.c_int, C.c_int]
+        L.orc_lowpass.argtypes = [DP, C.c_int, C.c_int, C.c_int, C.c_int, DP]
         L.orc_lagrange_eval.argtypes = [DP, C.c_long, C.c_int, C.c_int, C.c_double,
                                         C.c_double, C.c_int, DP]
         L.orc_compute_tt.restype = C.c_double
@@ -145,6 +146,16 @@ def zfilter(f: np.ndarray, kst: int) -> np.ndarray:
     f = np.array(f, dtype=np.float64, order="F")
     nx, ny, nz = f.shape
     lib().orc_zfilter(_dp(f), nx, ny, nz, kst)
+    return f
+
+
+def lowpass(f: np.ndarray, axis: int, decay: np.ndarray) -> np.ndarray:
+    """Fermi low-pass along x (axis 0) or y (axis 1) with a decay table of
+    n/2+1 entries (normx folded)."""
+    f = np.array(f, dtype=np.float64, order="F")
+    nx, ny, nz = f.shape
+    d = np.ascontiguousarray(decay, dtype=np.float64)
+    lib().orc_lowpass(_dp(f), nx, ny, nz, axis, _dp(d))
     return f
 
 

@@ -364,6 +364,12 @@ static void lowpass_axis(double *f, int nx, int ny, int nz, int axis, const doub
   free(mult);
 }
 
+/* The Fermi low-pass of FILTER_VARS_3D along one axis (physics.f90:796-828),
+ * exported for its closed-form check. */
+void orc_lowpass(double *f, int nx, int ny, int nz, int axis, const double *decay) {
+  lowpass_axis(f, nx, ny, nz, axis, decay);
+}
+
 /* ======================================================================
  * The solver context (module globals of initialize.f90:24-157)
  * ====================================================================== */

@@ -17,7 +17,8 @@ HIPCC = shutil.which("hipcc") or "/opt/rocm/bin/hipcc"
 HIPFLAGS = ["--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-shared",
             "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
 SOURCES = [os.path.join(HERE, "csrc", "sparc_gpu.hip")]
-DEPS = SOURCES + [os.path.join(HERE, "csrc", f) for f in ("kernels.h", "fft_lds.h", "zsolve.h")] + [
+DEPS = SOURCES + [os.path.join(HERE, "csrc", f)
+                  for f in ("kernels.h", "fft_lds.h", "fft_reg.h", "zsolve.h")] + [
     os.path.join(ROOT, "include", "sparc_gpu.h")]
 
 
@@ -45,8 +46,32 @@ def build_oracle() -> None:
         raise RuntimeError("oracle build failed")
 
 
+FC = shutil.which("amdflang") or "/opt/rocm/bin/amdflang"
+FORTRAN_EXE = os.path.join(HERE, "fortran", "sparc_timestepping")
+
+
+def build_fortran() -> str | None:
+    """The Fortran host (include/sparc_gpu_mod.f90 + fortran/timestepping.f90)
+    linked against libsparc_gpu.so with an $ORIGIN rpath; skipped when no
+    Fortran compiler is present."""
+    if not os.path.exists(FC):
+        return None
+    moddir = os.path.join(HERE, "fortran", "_mod")
+    os.makedirs(moddir, exist_ok=True)
+    cmd = [FC, "-O2", "-J", moddir, "-o", FORTRAN_EXE,
+           os.path.join(ROOT, "include", "sparc_gpu_mod.f90"),
+           os.path.join(HERE, "fortran", "timestepping.f90"),
+           "-L" + HERE, "-lsparc_gpu", "-Wl,-rpath,$ORIGIN/.."]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        sys.stderr.write(r.stdout + r.stderr)
+        raise RuntimeError("amdflang failed building the Fortran host")
+    return FORTRAN_EXE
+
+
 def build_all() -> None:
     build_engine()
+    build_fortran()
     build_oracle()
 
 

@@ -203,6 +203,48 @@ def forward_source(W: Workload, nt: int):
     return mask, series
 
 
+def write_bundle(path: str, cfg: abi.SparcConfig, S: bgm.Setup, kind: int, mask: np.ndarray,
+                 series: np.ndarray | None, active_until: int, maxtime: int, steps: int) -> None:
+    """Writes the set-up bundle the Fortran host (sparc_amd/fortran/
+    timestepping.f90) reads: the configuration, every array load_setup hands
+    to an engine, the source, and the loop bounds."""
+    import ctypes as C
+    import struct
+
+    def arr(a):
+        return np.asfortranarray(a, dtype="<f8").ravel(order="F").tobytes()
+
+    with open(path, "wb") as f:
+        f.write(b"SPARCBN1")
+        f.write(C.string_at(C.addressof(cfg), C.sizeof(cfg)))
+        f.write(struct.pack("<ii", maxtime, steps))
+        bgs = bgm.background_arrays(S)
+        f.write(struct.pack("<i", len(bgs)))
+        for bid, a in bgs.items():
+            a = np.asarray(a)
+            f.write(struct.pack("<iq", bid, a.size))
+            f.write(arr(a))
+        d = S.derived
+        profs = [(abi.PROF_STRETCH, d["stretch"]), (abi.PROF_UNSTRETCH, d["unstretch"]),
+                 (abi.PROF_G, d["g"]), (abi.PROF_Z, S.z)]
+        f.write(struct.pack("<i", len(profs)))
+        for pid, a in profs:
+            f.write(struct.pack("<ii", pid, len(a)))
+            f.write(arr(a))
+        rates = np.asarray(S.rates) if S.rates is not None else np.zeros(0)
+        f.write(struct.pack("<q", rates.size))
+        f.write(arr(rates))
+        for dtab in (S.decay_x, S.decay_y):
+            f.write(struct.pack("<q", len(dtab)))
+            f.write(arr(dtab))
+        f.write(struct.pack("<iq", kind, np.asarray(mask).size))
+        f.write(arr(mask))
+        ser = np.zeros(0) if series is None else np.asarray(series)
+        f.write(struct.pack("<i", ser.size))
+        f.write(arr(ser))
+        f.write(struct.pack("<i", active_until))
+
+
 def make_engine(W: Workload, S: bgm.Setup, mode: int = abi.FORWARD, nsteps: int | None = None,
                 precision: int | None = None, compensated: bool | None = None):
     P = W.params if precision is None else replace(W.params, precision=precision)

@@ -105,6 +105,8 @@ def test_operators_match_oracle(tiny, prec, tol):
             got = eng.apply_op(abi.OP_DDZ, f, ibc=ibc, out=preset if ibc in (2, 3) else None)
             assert rel_err(got, want) < tol * 10, ibc
         assert rel_err(eng.apply_op(abi.OP_ZFILTER, f), O.zfilter(f, 2)) < tol
+        assert rel_err(eng.apply_op(abi.OP_LOWPASS_X, f), O.lowpass(f, 0, S.decay_x)) < tol
+        assert rel_err(eng.apply_op(abi.OP_LOWPASS_Y, f), O.lowpass(f, 1, S.decay_y)) < tol
         o, _ = make_oracle(W, S)
         want = o.damp_field(f, np.zeros_like(f))
         got = eng.apply_op(abi.OP_DAMP, f)

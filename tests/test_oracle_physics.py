@@ -1,0 +1,110 @@
+"""Closed forms and physical invariants that pin oracle functions the other
+oracle tests leave open: the periodic compact scheme (DBYD2 ibc 5 through
+MTRIDAG's cyclic solve), the Fermi low-pass, and the PML (an outgoing pulse
+is absorbed instead of reflected)."""
+import numpy as np
+
+from oracle import oracle as O
+from sparc_amd import abi
+from sparc_amd import background as B
+
+
+def test_dbyd2_periodic_modified_wavenumber():
+    """ibc 5 (periodic, dbyd2.f90 with ispecial=1 -> mtridag.f90 cyclic solve)
+    applied to sin(theta j) gives K cos(theta j) with the compact scheme's
+    modified wavenumber K = (a sin t + b/2 sin 2t) / (h (1 + 2 alpha cos t)),
+    a = 14/9, b = 1/9, alpha = 1/3 (single-rounded), h = 1/n2."""
+    n2 = 48
+    j = np.arange(n2)
+    alpha = float(np.float32(1.0 / 3.0))
+    for m in (1, 3, 7, 13):
+        th = 2 * np.pi * m / n2
+        f = np.sin(th * j)[None, :].repeat(3, axis=0)
+        got = O.dbyd2(np.zeros_like(f), f, 5)
+        K = (14.0 / 9.0 * np.sin(th) + 1.0 / 18.0 * np.sin(2 * th)) * n2 / (1 + 2 * alpha * np.cos(th))
+        want = K * np.cos(th * j)[None, :]
+        assert np.max(np.abs(got - want)) < 1e-7 * max(1.0, abs(K)), m
+
+
+def test_lowpass_single_mode():
+    """FILTER_VARS_3D's x/y low-pass (physics.f90:796-828) multiplies Fourier
+    mode m by the Fermi factor 1/(1 + exp(m + 1 - n/3)) (decay_coeff with
+    normx folded, initialize.f90:215-221)."""
+    nx, ny, nz = 48, 12, 3
+    dx = B.decay_coeff(nx)
+    dy = B.decay_coeff(ny)
+    x = np.arange(nx) / nx
+    y = np.arange(ny) / ny
+    for m in (0, 2, 9, 16, 20, 24):
+        f = np.broadcast_to(np.cos(2 * np.pi * m * x + 0.3)[:, None, None], (nx, ny, nz))
+        want = f / (1.0 + np.exp(m + 1 - nx / 3.0))
+        assert np.max(np.abs(O.lowpass(f, 0, dx) - want)) < 1e-6, m
+    for m in (1, 4, 6):
+        f = np.broadcast_to(np.sin(2 * np.pi * m * y)[None, :, None], (nx, ny, nz))
+        want = f / (1.0 + np.exp(m + 1 - ny / 3.0))
+        assert np.max(np.abs(O.lowpass(f, 1, dy) - want)) < 1e-6, m
+
+
+def _pulse_run(npml: int) -> float:
+    """Upgoing acoustic pulse in a uniform, unstratified 10 Mm column; returns
+    the interior energy after the pulse has left through the top, relative to
+    its initial energy."""
+    nx, ny, nz = 32, 4, 120
+    cfg = abi.SparcConfig()
+    cfg.abi_version = abi.ABI_VERSION
+    cfg.nx, cfg.ny, cfg.nz = nx, ny, nz
+    cfg.option = abi.OPT_3D
+    cfg.precision = abi.FP64
+    cfg.npmlbot = cfg.npmltop = npml
+    cfg.e_rad, cfg.o_rad = 60, 61
+    cfg.stretchx = cfg.stretchy = 1.0
+    Lz = 10e8 / B.RSUN_R  # 10 Mm in solar radii
+    z = np.linspace(0.0, Lz, nz)
+    dz = z[1] - z[0]
+    cfg.deltat = 0.4 * dz  # c = 1 (nondimensional)
+    o = O.Oracle(cfg)
+    shape = (nx, ny, nz)
+    one = np.ones(shape)
+    der = B.init_rhs(z, one, one, one, np.zeros(nz), npml, npml, False)
+    o.set_background(abi.BG_RHO0, one)
+    o.set_background(abi.BG_RHOINV, one)
+    o.set_background(abi.BG_C2RHO0, one)
+    o.set_background(abi.BG_GRADP0_Z, 0 * one)
+    o.set_background(abi.BG_GRADRHO0_Z, 0 * one)
+    o.set_background(abi.BG_AZ, der["az"])
+    o.set_background(abi.BG_BZPML, der["bzpml"])
+    o.set_profile(abi.PROF_STRETCH, der["stretch"])
+    o.set_profile(abi.PROF_G, np.zeros(nz))
+    sig = 6 * dz
+    zc = z[nz // 2]
+    f = np.exp(-0.5 * ((z - zc) / sig) ** 2)
+    fp = -(z - zc) / sig ** 2 * f
+    o.put(abi.ARR_A, abi.XI_Z, np.broadcast_to(f, shape))
+    o.put(abi.ARR_A, abi.V_Z, np.broadcast_to(-fp, shape))  # xi(z - c t): v = -c f'
+    inner = slice(15, nz - 15)
+
+    def energy():
+        xi = o.get(abi.ARR_A, abi.XI_Z)[0, 0]
+        v = o.get(abi.ARR_A, abi.V_Z)[0, 0]
+        dxi = np.gradient(xi, dz)
+        return float(np.sum(v[inner] ** 2 + dxi[inner] ** 2))
+
+    e0 = energy()
+    # until the pulse has crossed the top and a reflection would be 100 points
+    # back inside
+    nsteps = int(round((z[-1] - zc + 8 * sig + 100 * dz) / cfg.deltat))
+    o.step(1, nsteps)
+    e1 = energy()
+    o.close()
+    return e1 / e0
+
+
+def test_pml_absorbs_outgoing_pulse():
+    """PML_BOUNDARY_ACCOUNTING_3D with the profiles of mp_Initialize_RHS
+    (pml.f90:213-327, physics.f90:255-303): an upgoing pulse that has crossed
+    the top boundary leaves almost nothing behind, while the same run without
+    PML reflects it back into the interior."""
+    with_pml = _pulse_run(10)
+    without = _pulse_run(0)
+    assert without > 0.3, without
+    assert with_pml < 0.02, with_pml
