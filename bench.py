@@ -53,9 +53,18 @@ def main() -> int:
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run 5 device-to-device copies of MIB MiB (calibrates the "
                          "FETCH_SIZE/WRITE_SIZE counters against known bytes) and exit")
+    ap.add_argument("--serving", type=int, default=0, metavar="N",
+                    help="serving benchmark instead: the first N requests of the arrival "
+                         "stream through the FP32 serving loop, then through the plain FP64 path")
+    ap.add_argument("--serving-rate", type=float, default=1.0, help="arrivals per second")
     args = ap.parse_args()
     if args.copy_probe:
         return copy_probe(args.copy_probe)
+    if args.serving:
+        if args.gpus != 1:
+            print("bench.py measures one GPU only", file=sys.stderr)
+            return 2
+        return serving_bench(args.serving, args.serving_rate)
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (this build has no multi-GPU path); "
               f"--gpus {args.gpus} is not supported", file=sys.stderr)
@@ -149,6 +158,58 @@ def main() -> int:
                    "step_roofline_frac": round(mpts * 1e6 * step_algo / 1e9 / HBM_PEAK_GBS, 4)},
         "roofline": roof,
         "cpu_baseline": cpu,
+    }
+    print(json.dumps(line))
+    return 0
+
+
+def serving_bench(n: int, rate: float) -> int:
+    """Solves/hour of the FP32 serving loop against the plain FP64 path on the
+    same n requests (forward + adjoint + kernel each)."""
+    import torch
+
+    from sparc_amd import serve as SV
+
+    reqs = SV.c5_requests(n, rate_per_s=rate)
+    cache = SV.ModelCache()
+    t0 = time.perf_counter()
+    for r in reqs:  # the models of an inversion step exist before its solves
+        cache.setup(r.workload, r.alpha)
+    setup_s = time.perf_counter() - t0
+    free, total = torch.cuda.mem_get_info()
+    budget = int(0.9 * free)
+    print(f"[serving] {n} requests, set-up {setup_s:.1f}s, budget {budget / 2**30:.0f} GiB",
+          file=sys.stderr, flush=True)
+    srv = SV.Server(budget, max_inflight=8, cache=cache)
+    res = srv.run(reqs, log=lambda m: print(m, file=sys.stderr, flush=True))
+    if srv.errors:
+        print(f"[serving] errors: {srv.errors}", file=sys.stderr, flush=True)
+    fast = SV.summarize(res)
+    print(f"[serving] FP32 loop: {fast}", file=sys.stderr, flush=True)
+    plain = SV.plain_fp64(reqs, cache)
+    print(f"[serving] plain FP64: {plain}", file=sys.stderr, flush=True)
+    line = {
+        "metric": "forward+adjoint+kernel solves/hour (FP32 serving loop)",
+        "value": round(fast.get("solves_per_hour", 0.0), 2),
+        "unit": "solves/h",
+        "n_gpus": 1,
+        "higher_is_better": True,
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (polytrope models, observations = synthetics delayed by 2 s)",
+        "config": {"workload": f"first {n} requests of the arrival stream (Poisson {rate}/s, "
+                               "default_rng(5); grid mix 40/50/10 % of 128x128x150 / "
+                               "256x256x300 / 384x384x300; 1000-3000 steps; 8 ring receivers)",
+                   "requests": [(r.workload, r.nsteps) for r in reqs],
+                   "max_inflight": 8, "hbm_budget_gib": round(budget / 2**30, 1)},
+        "serving": {k: v for k, v in fast.items()},
+        "peak_inflight": srv.peak_inflight,
+        "peak_reserved_gib": round(srv.peak_used / 2**30, 2),
+        "plain_fp64": {"solves_per_hour": round(plain["solves_per_hour"], 2),
+                       "wall_s": round(plain["wall_s"], 2),
+                       "per_solve_s": [round(x, 2) for x in plain["per_solve_s"]]},
+        "speedup_vs_plain_fp64": round(fast.get("solves_per_hour", 0.0)
+                                       / plain["solves_per_hour"], 3),
     }
     print(json.dumps(line))
     return 0

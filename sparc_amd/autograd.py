@@ -97,9 +97,8 @@ class Problem:
                 PL.adjoint_recompute(fe, ae, self.spec, vr, S.g_raw, self.checkpoint_every,
                                      kstride=self.kstride, aligned=self.aligned)
             else:
-                PL.copy_snapshots(ae, fe, self.spec.maxtime // (self.kstride or self.spec.steps) + 1)
                 PL.adjoint_solve(ae, self.spec, vr, S.g_raw, kstride=self.kstride,
-                                 aligned=self.aligned)
+                                 aligned=self.aligned, src=fe)
             self.last_peak_bytes = fe.device_bytes + ae.device_bytes
             K = PL.kernel_lnc(ae.kernel_get(abi.KERN_C2), S, self.spec, kstride=self.kstride)
             return K * (PL.cell_volume(S) * self.model_mask())[None, None, :]

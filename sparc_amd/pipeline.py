@@ -113,7 +113,7 @@ def adjoint_vr(spec: SolveSpec, adj: np.ndarray, S: bgm.Setup) -> np.ndarray:
 
 
 def adjoint_solve(eng, spec: SolveSpec, vr: np.ndarray, grav: np.ndarray,
-                  kstride: int | None = None, aligned: bool = False) -> None:
+                  kstride: int | None = None, aligned: bool = False, src=None) -> None:
     """Adjoint loop with the Lagrange source, accumulating the kernel sums
     against the forward snapshots held by ``eng``.
 
@@ -136,7 +136,10 @@ def adjoint_solve(eng, spec: SolveSpec, vr: np.ndarray, grav: np.ndarray,
     t = 0
     for ia, j in pairs:
         t = _run_to(eng, t, ia)
-        eng.kernel_accumulate(j)
+        if src is None:
+            eng.kernel_accumulate(j)
+        else:
+            eng.kernel_accumulate_from(src, j)
 
 
 def cell_volume(S: bgm.Setup) -> np.ndarray:
@@ -214,8 +217,11 @@ def full_solve(spec: SolveSpec, S: bgm.Setup, obs: np.ndarray, meas: RingMeasure
         tau = meas.travel_times(obs, syn)
         adj = meas.adjoint_source(syn, tau, obs=obs, exact=exact_adjoint)
         vr = adjoint_vr(spec, adj, S)
-        copy_snapshots(ae, fe, spec.maxtime // (kstride or spec.steps) + 1)
-        adjoint_solve(ae, spec, vr, S.g_raw, kstride=kstride, aligned=aligned)
+        if hasattr(ae, "kernel_accumulate_from"):  # read the forward bank in place
+            adjoint_solve(ae, spec, vr, S.g_raw, kstride=kstride, aligned=aligned, src=fe)
+        else:
+            copy_snapshots(ae, fe, spec.maxtime // (kstride or spec.steps) + 1)
+            adjoint_solve(ae, spec, vr, S.g_raw, kstride=kstride, aligned=aligned)
         raw = {k: ae.kernel_get(k) for k in (abi.KERN_C2, abi.KERN_RHO, abi.KERN_HESSIAN)}
         K = kernel_lnc(raw[abi.KERN_C2], S, spec, kstride=kstride)
         res = SolveResult(traces=syn, tau=tau, misfit=meas.misfit(tau), adj=adj, kernel_raw=raw,
