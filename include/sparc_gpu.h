@@ -42,6 +42,7 @@ extern "C" {
 #define SPARC_FP32 1
 
 /* physics option, numbering of PARSE_OPTIONS (initialize.f90:778-824) */
+#define SPARC_OPT_QUIET_PML_3D 2         /* (rho,v,p) form, pml.f90:12-76 */
 #define SPARC_OPT_QUIET_PML_3D_DISPL 4   /* displacement.f90:482-658 */
 #define SPARC_OPT_QUIET_PML_2D_DISPL 17  /* physics2d.f90:948-1105   */
 
@@ -57,6 +58,10 @@ extern "C" {
 #define SPARC_V_Y 4
 #define SPARC_V_Z 5
 #define SPARC_NFIELDS 6
+/* option 2 keeps (rho, v, p) in the first five slots (initialize.f90:498-502);
+ * it is a forward-only formulation (the kernels need the displacement) */
+#define SPARC_RHO 0
+#define SPARC_P 4
 
 /* which array sparc_get_state / sparc_put_state touch */
 #define SPARC_ARR_A 0        /* a(:,:,:,f): the state                          */
@@ -84,7 +89,9 @@ extern "C" {
 #define SPARC_PROF_UNSTRETCH 1 /* dz/dk                                        */
 #define SPARC_PROF_G 2         /* gravity, PML-decayed (physics.f90:273)      */
 #define SPARC_PROF_Z 3         /* grid z (units of Rsun)                      */
-#define SPARC_PROF_COUNT 4
+#define SPARC_PROF_SOURCE_DEP 4 /* source_dep(k) = exp(-4e7 (z(k)-z(e_rad))^2)
+                                  (physics.f90:109): option 2's source shape */
+#define SPARC_PROF_COUNT 5
 
 /* source kinds (sparc_set_source) */
 #define SPARC_SRC_NONE 0

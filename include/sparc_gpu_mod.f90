@@ -18,7 +18,9 @@ module sparc_gpu
   integer(c_int), parameter :: SPARC_OK = 0, SPARC_EINVAL = -1, SPARC_EHIP = -2, &
                                SPARC_ENOMEM = -3, SPARC_ESTATE = -4
   integer(c_int), parameter :: SPARC_FP64 = 0, SPARC_FP32 = 1
-  integer(c_int), parameter :: SPARC_OPT_QUIET_PML_3D_DISPL = 4, SPARC_OPT_QUIET_PML_2D_DISPL = 17
+  integer(c_int), parameter :: SPARC_OPT_QUIET_PML_3D = 2, SPARC_OPT_QUIET_PML_3D_DISPL = 4, &
+                               SPARC_OPT_QUIET_PML_2D_DISPL = 17
+  integer(c_int), parameter :: SPARC_RHO = 0, SPARC_P = 4
   integer(c_int), parameter :: SPARC_FORWARD = 0, SPARC_ADJOINT = 1
   integer(c_int), parameter :: SPARC_XI_X = 0, SPARC_XI_Y = 1, SPARC_XI_Z = 2, &
                                SPARC_V_X = 3, SPARC_V_Y = 4, SPARC_V_Z = 5, SPARC_NFIELDS = 6
@@ -28,7 +30,8 @@ module sparc_gpu
                                SPARC_BG_GRADP0_Z = 3, SPARC_BG_GRADRHO0_Z = 4, SPARC_BG_AZ = 5, &
                                SPARC_BG_BZPML = 6, SPARC_BG_C2 = 7, SPARC_BG_COUNT = 8
   integer(c_int), parameter :: SPARC_PROF_STRETCH = 0, SPARC_PROF_UNSTRETCH = 1, &
-                               SPARC_PROF_G = 2, SPARC_PROF_Z = 3, SPARC_PROF_COUNT = 4
+                               SPARC_PROF_G = 2, SPARC_PROF_Z = 3, SPARC_PROF_SOURCE_DEP = 4, &
+                               SPARC_PROF_COUNT = 5
   integer(c_int), parameter :: SPARC_SRC_NONE = 0, SPARC_SRC_SERIES = 1, SPARC_SRC_LAGRANGE = 2
   integer(c_int), parameter :: SPARC_KERN_C2 = 0, SPARC_KERN_RHO = 1, SPARC_KERN_HESSIAN = 2
 

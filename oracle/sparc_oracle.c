@@ -746,6 +746,56 @@ static void rhs_option4(orc *o) {
     }
 }
 
+/* MP_QUIET_PML_3D, option 2 ((rho, v, p) form, pml.f90:12-76), no
+ * horizontal PML.  Slots: rho = 0, v = 1..3, p = 4. */
+static void rhs_option2(orc *o) {
+  int nx = o->nx, ny = o->ny, nz = o->nz;
+  long N = o->N, nxy = (long)nx * ny;
+  double *rho = o->temp, *v_x = o->temp + N, *v_y = o->temp + 2 * N, *v_z = o->temp + 3 * N;
+  double *p = o->temp + 4 * N;
+  double **bg = o->bg;
+  /* ddxyz(v_x, dvxdx, v_y, dvydy, v_z, dvzdz, 1); ddxyz(p, gradp_x, p, gradp_y, p, gradp_z, 0) */
+  orc_ddx(v_x, o->dxdx, nx, ny, nz, o->cfg.stretchx);
+  orc_ddy(v_y, o->dydy, nx, ny, nz, o->cfg.stretchy);
+  orc_ddz(v_z, o->dzdz, nx, ny, nz, 1, o->prof[SPARC_PROF_STRETCH]);
+  orc_ddx(p, o->gpx, nx, ny, nz, o->cfg.stretchx);
+  orc_ddy(p, o->gpy, nx, ny, nz, o->cfg.stretchy);
+  orc_ddz(p, o->gpz, nx, ny, nz, 0, o->prof[SPARC_PROF_STRETCH]);
+  /* PML_BOUNDARY_ACCOUNTING_3D: psip on gradp_z, psivz on dvzdz (pml.f90:222-228) */
+  pml_account(o, o->gpz, 1);
+  pml_account(o, o->dzdz, 0);
+  for (long c = 0; c < N; c++) o->div[c] = o->dxdx[c] + o->dydy[c] + o->dzdz[c];
+  double *RHScont = o->scr, *RHSv_x = o->scr + N, *RHSv_y = o->scr + 2 * N;
+  double *RHSv_z = o->scr + 3 * N, *RHSp = o->scr + 4 * N;
+  const double *rhoinv = bg[SPARC_BG_RHOINV], *g = o->prof[SPARC_PROF_G];
+  const double *sdep = o->prof[SPARC_PROF_SOURCE_DEP];
+  for (long c = 0; c < N; c++) {
+    RHScont[c] = -bg[SPARC_BG_GRADRHO0_Z][c] * v_z[c] - bg[SPARC_BG_RHO0][c] * o->div[c];
+    RHSv_x[c] = -rhoinv[c] * o->gpx[c];
+    RHSv_y[c] = -rhoinv[c] * o->gpy[c];
+  }
+  int src = o->src_kind != SPARC_SRC_NONE && o->time <= o->src_active_until;
+  for (int k = 0; k < nz; k++)
+    for (long c = 0; c < nxy; c++) {
+      long q = c + nxy * k;
+      RHSv_z[q] = -rhoinv[q] * (o->gpz[q] + rho[q] * g[k]);
+      if (src && abs(k - (o->cfg.e_rad - 1)) <= 4) RHSv_z[q] = RHSv_z[q] + o->forcing[c] * sdep[k];
+    }
+  for (long c = 0; c < N; c++)
+    RHSp[c] = -bg[SPARC_BG_C2RHO0][c] * o->div[c] - v_z[c] * bg[SPARC_BG_GRADP0_Z][c];
+  /* zero planes 1 and nz of the five fields, then DAMP_VELOCITY on every plane */
+  for (int f = 0; f < 5; f++)
+    for (long c = 0; c < nxy; c++) {
+      o->scr[f * N + nxy * (nz - 1) + c] = 0.0;
+      o->scr[f * N + c] = 0.0;
+    }
+  if (o->cfg.damp_waves) {
+    damp_field(o, v_x, RHSv_x);
+    damp_field(o, v_y, RHSv_y);
+    damp_field(o, v_z, RHSv_z);
+  }
+}
+
 /* MP_QUIET_PML_2D_DISPL, option 17 (physics2d.f90:948-1105), ny = 1. */
 static void rhs_option17(orc *o) {
   int nx = o->nx, ny = o->ny, nz = o->nz;
@@ -874,6 +924,8 @@ static void step(orc *o) {
     }
     if (o->cfg.option == SPARC_OPT_QUIET_PML_3D_DISPL)
       rhs_option4(o);
+    else if (o->cfg.option == SPARC_OPT_QUIET_PML_3D)
+      rhs_option2(o);
     else
       rhs_option17(o);
   }

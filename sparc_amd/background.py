@@ -205,6 +205,13 @@ def init_rhs(z, c2, rho0, p0, g, npmlbot: int, npmltop: int, is2d: bool) -> dict
                 gradrho0_z=gradrho0_z, gradp0_z=gradp0_z, g=g, az=az, bzpml=bzpml)
 
 
+def source_dep(z: np.ndarray, e_rad: int) -> np.ndarray:
+    """source_dep(k) = exp(-10**7 * 4 (z(k) - z(e_rad))^2) (physics.f90:109),
+    the vertical shape of option 2's source; e_rad is 1-based."""
+    z = np.asarray(z, dtype=np.float64)
+    return np.exp(-sp(1e7) * 4.0 * (z - z[e_rad - 1]) ** 2.0)
+
+
 def damping_rates(nx: int, ny: int, xlength: float, ylength: float, is2d: bool) -> np.ndarray:
     """INIT_DAMPING (damping.f90:10-65) / INIT_DAMPING_2D (71-120):
     rates((nx/2+1), ny) with normx*normy folded."""

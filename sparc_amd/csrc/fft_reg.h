@@ -225,6 +225,7 @@ struct XArgs {
   const T* s[3];
   T* out[3];
   C2<T>* spec[3];
+  int edges;  // damping inverse also on planes 1 and nz (option 2 zeroes before damping)
 };
 
 template <typename T, int N, int MODE, int LPB = XCfg<T, N>::LPB, int NT = XCfg<T, N>::NT>
@@ -322,8 +323,8 @@ __global__ __launch_bounds__(NT) void k_xlines_n(int nrows, int nlines, XArgs<T>
       const long k0 = 2 * q, k1 = k0 + 1;
       const long o0 = x + (long)N * j + nxy * k0;
       const C2<T> v = r[l * LS + lpad(x)];
-      if (k0 != 0 && k0 != nz - 1) out[o0] = out[o0] - v.x;
-      if (k1 < nz - 1) out[o0 + nxy] = out[o0 + nxy] - v.y;
+      if (X.edges || (k0 != 0 && k0 != nz - 1)) out[o0] = out[o0] - v.x;
+      if (k1 < nz - 1 || (X.edges && k1 < nz)) out[o0 + nxy] = out[o0 + nxy] - v.y;
     }
   }
 }

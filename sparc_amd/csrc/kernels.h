@@ -12,7 +12,7 @@
 
 namespace sparc {
 
-enum { EPI_STORE = 0, EPI_P3D = 1, EPI_RHS3D = 2, EPI_P2D = 3, EPI_RHS2D = 4 };
+enum { EPI_STORE = 0, EPI_P3D = 1, EPI_RHS3D = 2, EPI_P2D = 3, EPI_RHS2D = 4, EPI_C2 = 5, EPI_V2 = 6 };
 enum { XM_DERIV = 0, XM_FILTER = 1, XM_DAMP_FWD = 2, XM_DAMP_INV = 3 };
 enum { YM_DERIV = 0, YM_FILTER = 1, YM_DAMP = 2 };
 
@@ -33,6 +33,7 @@ struct Fld {
   T beta;
   int src_plane;  // 0-based plane receiving the source this step, -1 if none
   T delta_width;
+  const T* srcdep;  // option 2: source_dep(k) over planes |k - src_plane| <= 4
 };
 
 // Coefficients of DBYD2's right-hand side (dbyd2.f90:88-196), cast to T.

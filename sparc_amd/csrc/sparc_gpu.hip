@@ -209,8 +209,9 @@ static int check_config(const sparc_config* cfg, std::string* why) {
     *why = b;
     return SPARC_EINVAL;
   }
-  if (cfg->option != SPARC_OPT_QUIET_PML_3D_DISPL && cfg->option != SPARC_OPT_QUIET_PML_2D_DISPL) {
-    snprintf(b, sizeof b, "option %d not supported (4 or 17)", cfg->option);
+  if (cfg->option != SPARC_OPT_QUIET_PML_3D_DISPL && cfg->option != SPARC_OPT_QUIET_PML_2D_DISPL &&
+      cfg->option != SPARC_OPT_QUIET_PML_3D) {
+    snprintf(b, sizeof b, "option %d not supported (2, 4 or 17)", cfg->option);
     *why = b;
     return SPARC_EINVAL;
   }
@@ -231,13 +232,17 @@ static int check_config(const sparc_config* cfg, std::string* why) {
     *why = "option 17 needs ny == 1";
     return SPARC_EINVAL;
   }
-  if (cfg->option == SPARC_OPT_QUIET_PML_3D_DISPL) {
+  if (cfg->option == SPARC_OPT_QUIET_PML_3D && cfg->mode != SPARC_FORWARD) {
+    *why = "option 2 is forward-only (the kernels need the displacement formulation)";
+    return SPARC_EINVAL;
+  }
+  if (cfg->option != SPARC_OPT_QUIET_PML_2D_DISPL) {
     if (!make_plan(cfg->ny, &p) || cfg->ny < 2) {
-      *why = "ny must be 2^a*3^b >= 2 for option 4";
+      *why = "ny must be 2^a*3^b >= 2 for the 3-D options";
       return SPARC_EINVAL;
     }
     if (cfg->nx % 32 != 0) {
-      *why = "option 4 needs nx % 32 == 0";
+      *why = "the 3-D options need nx % 32 == 0";
       return SPARC_EINVAL;
     }
   }
@@ -271,7 +276,7 @@ static size_t footprint_bytes(const sparc_config* cfg) {
   b += 4 * NP * ts;                         // pmlvars, scrpml
   b += SPARC_BG_COUNT * N * ts;             // backgrounds (upper bound: all 3-D)
   b += 8 * N * ts;                          // work
-  if (cfg->option == SPARC_OPT_QUIET_PML_3D_DISPL && cfg->damp_waves)
+  if (cfg->option != SPARC_OPT_QUIET_PML_2D_DISPL && cfg->damp_waves)
     b += 3 * spec_elems(cfg) * 2 * ts;      // damping spectra
   b += (size_t)(cfg->nx + cfg->ny) * 6 * ts + 4 * 3 * cfg->nz * ts + SPARC_PROF_COUNT * cfg->nz * ts;
   b += (size_t)(cfg->nx / 2 + 1) * cfg->ny * ts + 2 * nxy * ts;
@@ -349,7 +354,7 @@ extern "C" int sparc_create(const sparc_config* cfg, sparc_ctx** out) {
   }
   for (int w = 0; w < 8; ++w)
     if ((rc = dalloc(c, &c->W[w], N * ts))) return rc;
-  if (cfg->option == SPARC_OPT_QUIET_PML_3D_DISPL && cfg->damp_waves)
+  if (cfg->option != SPARC_OPT_QUIET_PML_2D_DISPL && cfg->damp_waves)
     for (int f = 0; f < 3; ++f)
       if ((rc = dalloc(c, &c->spec[f], spec_elems(cfg) * 2 * ts))) return rc;
   for (int p = 0; p < SPARC_PROF_COUNT; ++p)
@@ -605,6 +610,7 @@ static Fld<T> make_fld(sparc_ctx* c) {
   F.npmltop = c->cfg.npmltop;
   F.src_plane = -1;
   F.delta_width = (T)c->cfg.delta_width;
+  F.srcdep = (const T*)c->prof[SPARC_PROF_SOURCE_DEP];
   return F;
 }
 
@@ -789,9 +795,14 @@ static int launch_z(sparc_ctx* c, const Fld<T>& F, int ibc, const T* inA, const 
     nb = in + 1 + nbg({SPARC_BG_C2RHO0, SPARC_BG_GRADP0_Z, SPARC_BG_RHO0, SPARC_BG_GRADRHO0_Z}) + 2;
   else if (EPI == EPI_RHS3D)
     nb = in + 3 + (F.beta != (T)0 ? 6 : 3) + nbg({SPARC_BG_RHOINV}) + 6;
+  else if (EPI == EPI_C2)  // v_z again (own plane), dvxdx, dvydy -> W4, W5
+    nb = in + 2 + nbg({SPARC_BG_C2RHO0, SPARC_BG_GRADP0_Z, SPARC_BG_RHO0, SPARC_BG_GRADRHO0_Z}) + 2;
+  else if (EPI == EPI_V2)  // gradp_x, gradp_y, W4, W5, rho temp -> scr of 5 fields
+    nb = in + 4 + (F.beta != (T)0 ? 2 : 1) + nbg({SPARC_BG_RHOINV}) + 5;
   else
     nb = in + 2 + (F.beta != (T)0 ? 4 : 2) + nbg({SPARC_BG_RHOINV}) + 4;
-  ProfScope ps(c, (EPI == EPI_P3D || EPI == EPI_P2D) ? SPARC_KC_ZSOLVE_P : SPARC_KC_ZSOLVE,
+  ProfScope ps(c, (EPI == EPI_P3D || EPI == EPI_P2D || EPI == EPI_C2) ? SPARC_KC_ZSOLVE_P
+                                                                      : SPARC_KC_ZSOLVE,
                nb * (double)c->N * sizeof(T));
   // segments per column: L = nz/NSEG planes each, at least ~48 (>= 2x warm-up)
   const int nseg = nz >= 192 ? 4 : (nz >= 96 ? 2 : 1);
@@ -876,6 +887,31 @@ static int run_stage(sparc_ctx* c, Fld<T>& F, int s) {
     }
     if ((rc = launch_z<T, EPI_RHS3D>(c, F, 0, F.W[3], nullptr, 0, nullptr, 0, nullptr))) return rc;
     if (damp && (rc = launch_xm<T, XM_DAMP_INV>(c, 3, DX, 0, nullptr))) return rc;
+  } else if (c->cfg.option == SPARC_OPT_QUIET_PML_3D) {
+    // ddxyz(v_x, v_y, v_z, bc=1) (pml.f90:27); the z solve's epilogue forms
+    // div v, RHScont and RHSp into W4, W5 (pml.f90:36-53)
+    if ((rc = launch_x<T, XM_DERIV>(c, F.a[1], S(1), beta, F.W[0], dmx, nullptr))) return rc;
+    if ((rc = launch_y<T, YM_DERIV>(c, F.a[2], S(2), beta, F.W[1], dmy, nullptr))) return rc;
+    if ((rc = launch_z<T, EPI_C2>(c, F, 1, F.a[3], S(3), beta, nullptr, 0, nullptr))) return rc;
+    // ddxyz(p, bc=0) (pml.f90:29)
+    if ((rc = launch_x<T, XM_DERIV>(c, F.a[4], S(4), beta, F.W[0], dmx, nullptr))) return rc;
+    if ((rc = launch_y<T, YM_DERIV>(c, F.a[4], S(4), beta, F.W[1], dmy, nullptr))) return rc;
+    const bool damp = c->cfg.damp_waves && c->damping_set;
+    XArgs<T> DX{};
+    DX.edges = 1;  // zeroing of planes 1, nz precedes DAMP_VELOCITY here (pml.f90:56-74)
+    for (int f = 0; f < 3; ++f) {
+      DX.a[f] = F.a[1 + f];
+      DX.s[f] = S(1 + f);
+      DX.out[f] = F.scr[1 + f];
+      DX.spec[f] = (C2<T>*)c->spec[f];
+    }
+    if (damp) {
+      if ((rc = launch_xm<T, XM_DAMP_FWD>(c, 3, DX, beta, nullptr))) return rc;
+      if ((rc = launch_ym<T, YM_DAMP>(c, 3, DX, 0, nullptr))) return rc;
+    }
+    // gradp_z + RHSv; RHScont, RHSp moved from W4, W5 into scr
+    if ((rc = launch_z<T, EPI_V2>(c, F, 0, F.a[4], S(4), beta, nullptr, 0, nullptr))) return rc;
+    if (damp && (rc = launch_xm<T, XM_DAMP_INV>(c, 3, DX, 0, nullptr))) return rc;
   } else {
     // option 17 (physics2d.f90:959-1069)
     if ((rc = launch_x<T, XM_DERIV>(c, F.a[0], S(0), beta, F.W[0], dmx, nullptr))) return rc;
@@ -919,7 +955,8 @@ static int step_impl(sparc_ctx* c, int t0, int nsteps) {
       if ((rc = run_stage<T>(c, F, s))) return rc;
     // a += deltat*scr with the z filter (step.f90:114-118)
     const T dt = (T)c->cfg.deltat;
-    for (int f = 0; f < SPARC_NFIELDS; ++f) {
+    const int nfld = c->cfg.option == SPARC_OPT_QUIET_PML_3D ? 5 : SPARC_NFIELDS;  // dimfive
+    for (int f = 0; f < nfld; ++f) {
       ProfScope ps(c, SPARC_KC_UPDATE, (c->comp ? 5.0 : 3.0) * (double)c->N * sizeof(T));
       if (c->comp) {
         hipLaunchKernelGGL(k_update_filter_c, dim3(cdiv(nxy, 64)), dim3(64), 0, c->stream,
@@ -972,10 +1009,13 @@ extern "C" int sparc_step(sparc_ctx* c, int t0, int nsteps) {
   if (!c->profset[SPARC_PROF_STRETCH] || !c->profset[SPARC_PROF_G] ||
       (c->cfg.option == SPARC_OPT_QUIET_PML_2D_DISPL && !c->profset[SPARC_PROF_UNSTRETCH]))
     return fail(c, SPARC_ESTATE, "stretch/g/unstretch profiles not set");
-  if (c->cfg.option == SPARC_OPT_QUIET_PML_3D_DISPL && c->cfg.damp_waves && !c->damping_set)
+  if (c->cfg.option != SPARC_OPT_QUIET_PML_2D_DISPL && c->cfg.damp_waves && !c->damping_set)
     return fail(c, SPARC_ESTATE, "damp_waves set but damping rates not given");
   if (c->cfg.freq_filtering > 0 && !c->filter_set)
     return fail(c, SPARC_ESTATE, "freq_filtering > 0 but filter not given");
+  if (c->cfg.option == SPARC_OPT_QUIET_PML_3D && c->src_kind != SPARC_SRC_NONE &&
+      !c->profset[SPARC_PROF_SOURCE_DEP])
+    return fail(c, SPARC_ESTATE, "option 2 with a source needs the source_dep profile");
   return c->fp32 ? step_impl<float>(c, t0, nsteps) : step_impl<double>(c, t0, nsteps);
 }
 
@@ -1125,6 +1165,8 @@ extern "C" int sparc_snapshots_alloc(sparc_ctx* c, int nslots) {
 
 extern "C" int sparc_snapshot_store(sparc_ctx* c, int slot) {
   if (!c || slot < 0 || slot >= c->nslots) return fail(c, SPARC_EINVAL, "bad slot %d", slot);
+  if (c->cfg.option == SPARC_OPT_QUIET_PML_3D)
+    return fail(c, SPARC_ESTATE, "snapshots hold xi and acc: displacement options (4, 17) only");
   char* s = (char*)c->snap + (size_t)slot * 6 * c->N * c->ts;
   const size_t fb = c->N * c->ts;
   for (int f = 0; f < 3; ++f) {
@@ -1151,6 +1193,8 @@ extern "C" int sparc_snapshots_copy(sparc_ctx* dst, const sparc_ctx* src) {
 
 extern "C" int sparc_kernel_begin(sparc_ctx* c, const double* grav, int nz) {
   if (!c || !grav || nz != c->cfg.nz) return fail(c, SPARC_EINVAL, "grav needs nz values");
+  if (c->cfg.option == SPARC_OPT_QUIET_PML_3D)
+    return fail(c, SPARC_ESTATE, "kernels need the displacement formulation (options 4, 17)");
   int rc;
   if (!c->kern[0]) {
     for (int q = 0; q < SPARC_KERN_COUNT; ++q)

@@ -57,14 +57,36 @@ __device__ __forceinline__ void load_epi(const Fld<T>& F, const T* __restrict__ 
       if (EPI == EPI_RHS3D) e.s1 = F.scr[SPARC_V_Y][q];
     }
     if (F.bg3d[SPARC_BG_RHOINV]) e.g0 = F.bg[SPARC_BG_RHOINV][q];
+  } else if (EPI == EPI_C2) {  // div v, RHScont, RHSp of option 2
+    e.w0 = F.W[0][q];
+    e.w1 = F.W[1][q];
+    e.a2 = F.a[SPARC_V_Z - 2][q];  // v_z in slot 3
+    if (st) e.s2 = F.scr[SPARC_V_Z - 2][q];
+    if (F.bg3d[SPARC_BG_C2RHO0]) e.g0 = F.bg[SPARC_BG_C2RHO0][q];
+    if (F.bg3d[SPARC_BG_GRADP0_Z]) e.g1 = F.bg[SPARC_BG_GRADP0_Z][q];
+    if (F.bg3d[SPARC_BG_RHO0]) e.g2 = F.bg[SPARC_BG_RHO0][q];
+    if (F.bg3d[SPARC_BG_GRADRHO0_Z]) e.g3 = F.bg[SPARC_BG_GRADRHO0_Z][q];
+  } else if (EPI == EPI_V2) {  // RHSv of option 2; moves RHScont, RHSp into scr
+    e.w0 = F.W[0][q];
+    e.w1 = F.W[1][q];
+    e.w4 = F.W[4][q];
+    e.ps = F.W[5][q];  // RHSp (ps is free: no PML psi in this epilogue's scratch slot)
+    e.a0 = F.a[SPARC_RHO][q];
+    if (st) e.s0 = F.scr[SPARC_RHO][q];
+    if (F.bg3d[SPARC_BG_RHOINV]) e.g0 = F.bg[SPARC_BG_RHOINV][q];
   }
   if (EPI != EPI_STORE) {
     const int pi = pml_index(k, n, F.npmlbot, F.npmltop);
     if (pi >= 0) {
-      const int w = (EPI == EPI_P3D || EPI == EPI_P2D) ? 0 : 1;
+      const int w = (EPI == EPI_P3D || EPI == EPI_P2D || EPI == EPI_C2) ? 0 : 1;
       const long pq = c + nxy * pi;
       e.pv = F.pmlvars[w][pq];
-      if (st) e.ps = F.scrpml[w][pq];
+      if (st) {
+        if (EPI == EPI_V2)
+          e.g3 = F.scrpml[w][pq];
+        else
+          e.ps = F.scrpml[w][pq];
+      }
       e.paz = bgv(F, SPARC_BG_AZ, c, k);
       e.pbz = bgv(F, SPARC_BG_BZPML, c, k);
     }
@@ -87,6 +109,49 @@ __device__ __forceinline__ void epilogue(const Fld<T>& F, int c, int k, T val, T
     return;
   }
   const int pi = pml_index(k, n, F.npmlbot, F.npmltop);
+  if (EPI == EPI_C2) {
+    // MP_QUIET_PML_3D (pml.f90:36-53): dvzdz with psivz, div, RHScont, RHSp
+    T dz = val;
+    if (pi >= 0) {
+      T psi = e.pv;
+      if (st) psi = psi + F.beta * e.ps;
+      F.scrpml[0][c + (long)F.nxy * pi] = e.paz * dz + e.pbz * psi;
+      dz = dz + psi;
+    }
+    const T div = e.w0 + e.w1 + dz;
+    T vz = e.a2;
+    if (st) vz = vz + F.beta * e.s2;
+    const T c2r = bg_or(F, SPARC_BG_C2RHO0, e.g0, k);
+    const T gp = bg_or(F, SPARC_BG_GRADP0_Z, e.g1, k);
+    const T r0 = bg_or(F, SPARC_BG_RHO0, e.g2, k);
+    const T gr = bg_or(F, SPARC_BG_GRADRHO0_Z, e.g3, k);
+    F.W[4][q] = -gr * vz - r0 * div;   // RHScont
+    F.W[5][q] = -c2r * div - vz * gp;  // RHSp
+    return;
+  }
+  if (EPI == EPI_V2) {
+    // gradp_z with psip, RHSv (pml.f90:41-50); planes 1, nz zeroed (:56-59)
+    T gz = val;
+    if (pi >= 0) {
+      T psi = e.pv;
+      if (st) psi = psi + F.beta * e.g3;
+      F.scrpml[1][c + (long)F.nxy * pi] = e.paz * gz + e.pbz * psi;
+      gz = gz + psi;
+    }
+    const T ri = bg_or(F, SPARC_BG_RHOINV, e.g0, k);
+    T rho = e.a0;
+    if (st) rho = rho + F.beta * e.s0;
+    T rvz = -ri * (gz + rho * gk);
+    if (F.src_plane >= 0 && k - F.src_plane <= 4 && F.src_plane - k <= 4)
+      rvz = rvz + F.forcing[c] * F.srcdep[k];
+    const bool edge = (k == 0 || k == n - 1);
+    F.scr[SPARC_RHO][q] = edge ? (T)0 : e.w4;
+    F.scr[1][q] = edge ? (T)0 : -ri * e.w0;
+    F.scr[2][q] = edge ? (T)0 : -ri * e.w1;
+    F.scr[3][q] = edge ? (T)0 : rvz;
+    F.scr[SPARC_P][q] = edge ? (T)0 : e.ps;
+    return;
+  }
   if (EPI == EPI_P3D || EPI == EPI_P2D) {
     T dz = val;
     if (pi >= 0) {
@@ -202,7 +267,7 @@ __global__ __launch_bounds__(64 * NSEG) void k_zsolve(Fld<T> F, TriCoef<T> tc,
     s_gm[k] = gam[k];
     s_lw[k] = lowr[k];
     s_ib[k] = invb[k];
-    s_g[k] = (EPI == EPI_RHS3D || EPI == EPI_RHS2D) ? F.g[k] : (T)0;
+    s_g[k] = (EPI == EPI_RHS3D || EPI == EPI_RHS2D || EPI == EPI_V2) ? F.g[k] : (T)0;
   }
   __syncthreads();
   const int lane = threadIdx.x & 63;

@@ -254,3 +254,57 @@ def test_put_get_roundtrip_and_reset(tiny, comp):
         assert not np.any(eng.get(abi.ARR_A, abi.XI_Z))
     finally:
         eng.close()
+
+
+@pytest.fixture(scope="module")
+def tiny_rhovp():
+    W = D.workload("tiny")
+    W.params.option = abi.OPT_RHOVP
+    S = D.setup_for(W)
+    return W, S
+
+
+@pytest.mark.parametrize("prec,comp", [(abi.FP64, None), (abi.FP32, True)])
+def test_step_option2_matches_oracle(tiny_rhovp, prec, comp):
+    """Option 2 ((rho, v, p) form, pml.f90:12-76): source over 9 planes with
+    source_dep, PML on dvz/dz and dp/dz, damping after the boundary zeroing."""
+    W, S = tiny_rhovp
+    eng, o = _run_pair(W, S, 14, prec, compensated=comp)
+    try:
+        for f in range(5):
+            a, b = eng.get(abi.ARR_A, f), o.get(abi.ARR_A, f)
+            assert np.linalg.norm(b) > 0, f
+            if prec == abi.FP64:
+                assert rel_err(a, b) < TOL64, f
+            else:
+                assert rel_l2(a, b) < TOL32_L2, f
+        assert not np.any(eng.get(abi.ARR_A, 5))
+        for f in range(2):
+            a, b = eng.get(abi.ARR_PMLVARS, f), o.get(abi.ARR_PMLVARS, f)
+            assert (rel_err(a, b) < TOL64) if prec == abi.FP64 else (rel_l2(a, b) < 1e-4)
+    finally:
+        eng.close()
+        o.close()
+
+
+def test_option2_random_state_matches_oracle(tiny_rhovp):
+    """A random state (boundary planes included) stepped 3 times: exercises
+    the damping of planes 1 and nz that option 2 applies after zeroing."""
+    W, S = tiny_rhovp
+    eng, _ = D.make_engine(W, S, precision=abi.FP64, nsteps=3)
+    o, _ = make_oracle(W, S, nsteps=3)
+    try:
+        for f in range(5):
+            x = _field(eng.shape, 40 + f) * 1e-3
+            eng.put(abi.ARR_A, f, x)
+            o.put(abi.ARR_A, f, x)
+        eng.step(1, 3)
+        o.step(1, 3)
+        for f in range(5):
+            assert rel_err(eng.get(abi.ARR_A, f), o.get(abi.ARR_A, f)) < TOL64, f
+        with pytest.raises(abi.SparcError):
+            eng.snapshots_alloc(1)
+            eng.snapshot_store(0)
+    finally:
+        eng.close()
+        o.close()

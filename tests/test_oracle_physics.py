@@ -108,3 +108,50 @@ def test_pml_absorbs_outgoing_pulse():
     without = _pulse_run(0)
     assert without > 0.3, without
     assert with_pml < 0.02, with_pml
+
+
+def test_option2_plane_wave_amplification():
+    """Option 2 ((rho, v, p) form, pml.f90:12-76) in a uniform, unstratified
+    box: a horizontal mode v_x = V cos(kx), p = P sin(kx), rho = R sin(kx)
+    obeys V' = -k P, P' = c^2 k V, R' = k V, so one LDDRK step is the
+    stability polynomial R(L dt) of that 3x3 system (step.f90:24-118)."""
+    nx, ny, nz = 32, 4, 40
+    cfg = abi.SparcConfig()
+    cfg.abi_version = abi.ABI_VERSION
+    cfg.nx, cfg.ny, cfg.nz = nx, ny, nz
+    cfg.option = abi.OPT_RHOVP
+    cfg.deltat = 2e-3
+    cfg.stretchx = cfg.stretchy = 1.0
+    cfg.e_rad, cfg.o_rad = 5, 6
+    o = O.Oracle(cfg)
+    c2 = 1.7
+    ones = np.ones(nz)
+    o.set_background(abi.BG_RHO0, ones)
+    o.set_background(abi.BG_RHOINV, ones)
+    o.set_background(abi.BG_C2RHO0, c2 * ones)
+    for b in (abi.BG_GRADP0_Z, abi.BG_GRADRHO0_Z, abi.BG_AZ, abi.BG_BZPML):
+        o.set_background(b, 0 * ones)
+    o.set_profile(abi.PROF_STRETCH, ones)
+    o.set_profile(abi.PROF_G, 0 * ones)
+    m = 3
+    x = np.arange(nx) / nx
+    k = 2 * B.PI_R * m
+    cosm = np.broadcast_to(np.cos(2 * np.pi * m * x)[:, None, None], (nx, ny, nz)).copy()
+    sinm = np.broadcast_to(np.sin(2 * np.pi * m * x)[:, None, None], (nx, ny, nz)).copy()
+    o.put(abi.ARR_A, 1, cosm)  # v_x
+    o.step(1, 1)  # time 1: no spectral filter (freq_filtering = 0)
+    got = {f: o.get(abi.ARR_A, f) for f in range(5)}
+    o.close()
+    opt = [1.0, 1.0, 0.5, float(np.float32(0.1665579725791151184)),
+           float(np.float32(0.03950410250586729987)), float(np.float32(0.007810706393337838236))]
+    L = np.array([[0.0, -k, 0.0], [c2 * k, 0.0, 0.0], [k, 0.0, 0.0]]) * cfg.deltat
+    R = sum(opt[j] * np.linalg.matrix_power(L, j) for j in range(6))
+    mid = slice(15, 25)
+    assert np.max(np.abs(got[1][:, :, mid] - R[0, 0] * cosm[:, :, mid])) < 1e-9
+    assert np.max(np.abs(got[abi.P][:, :, mid] - R[1, 0] * sinm[:, :, mid])) < 1e-9 * c2 * k
+    assert np.max(np.abs(got[abi.RHO][:, :, mid] - R[2, 0] * sinm[:, :, mid])) < 1e-9 * k
+    assert np.max(np.abs(R[1, 0])) > 1e-3  # the step moved energy into p
+    assert np.max(np.abs(got[2][:, :, mid])) < 1e-12  # v_y
+    # v_z: only the leak of the zeroed boundary planes through the compact
+    # solve (decaying ~0.27 per plane, 15 planes away)
+    assert np.max(np.abs(got[3][:, :, mid])) < 1e-7
