@@ -133,7 +133,10 @@ extern "C" {
 #define SPARC_KERN_C2 0       /* sum div(xi_f) div(v_a)                      */
 #define SPARC_KERN_RHO 1      /* density-kernel integrand sum                */
 #define SPARC_KERN_HESSIAN 2  /* sum |acc_f|^2                               */
-#define SPARC_KERN_COUNT 3
+#define SPARC_KERN_VX 3       /* flow kernels (cfg.flow_kernels):            */
+#define SPARC_KERN_VY 4       /*   sum_j d_i v_f,j * v_a,j, i = x, y, z      */
+#define SPARC_KERN_VZ 5       /*   (kernels.f90:317-332)                     */
+#define SPARC_KERN_COUNT 6
 
 typedef struct sparc_config {
   int32_t abi_version;    /* must be SPARC_ABI_VERSION                          */
@@ -153,7 +156,9 @@ typedef struct sparc_config {
   int32_t compensated;    /* FP32 only: hold the state as hi+lo FP32 pairs and run
                              a += deltat*scr and the z filter in FP64 (the
                              reference state is FP64); 0 = plain FP32 state     */
-  int32_t reserved_i[7];
+  int32_t flow_kernels;   /* snapshots also keep v (9 fields) and the kernel sums
+                             add the flow kernels SPARC_KERN_VX..VZ            */
+  int32_t reserved_i[6];
   double xlength, ylength;/* box sides, cm (params.i:27)                        */
   double timestep;        /* s (params.i:27)                                    */
   double deltat;          /* timestep*dimc/diml (initialize.f90:472)            */

@@ -33,7 +33,8 @@ module sparc_gpu
                                SPARC_PROF_G = 2, SPARC_PROF_Z = 3, SPARC_PROF_SOURCE_DEP = 4, &
                                SPARC_PROF_COUNT = 5
   integer(c_int), parameter :: SPARC_SRC_NONE = 0, SPARC_SRC_SERIES = 1, SPARC_SRC_LAGRANGE = 2
-  integer(c_int), parameter :: SPARC_KERN_C2 = 0, SPARC_KERN_RHO = 1, SPARC_KERN_HESSIAN = 2
+  integer(c_int), parameter :: SPARC_KERN_C2 = 0, SPARC_KERN_RHO = 1, SPARC_KERN_HESSIAN = 2, &
+                               SPARC_KERN_VX = 3, SPARC_KERN_VY = 4, SPARC_KERN_VZ = 5
 
   ! sparc_config of sparc_gpu.h: 24 int32 then 16 double, no padding
   type, bind(C) :: sparc_config
@@ -50,7 +51,8 @@ module sparc_gpu
     integer(c_int32_t) :: forcing_length
     integer(c_int32_t) :: num_steps
     integer(c_int32_t) :: compensated
-    integer(c_int32_t) :: reserved_i(7)
+    integer(c_int32_t) :: flow_kernels
+    integer(c_int32_t) :: reserved_i(6)
     real(c_double) :: xlength, ylength
     real(c_double) :: timestep
     real(c_double) :: deltat

@@ -998,31 +998,36 @@ double orc_norm(orc *o, int field) {
 /* ======================================================================
  * Kernels (kernels.f90:197-313): snapshot pairs accumulated in FP64.
  * ====================================================================== */
+/* fields per snapshot: xi, acc (+ v with the flow kernels) */
+static int snap_fields(const orc *o) { return o->cfg.flow_kernels ? 9 : 6; }
+
 int orc_snapshots_alloc(orc *o, int nslots) {
   free(o->snap);
   o->nslots = nslots;
-  o->snap = zalloc(sizeof(double) * 6 * o->N * (size_t)nslots);
+  o->snap = zalloc(sizeof(double) * snap_fields(o) * o->N * (size_t)nslots);
   return SPARC_OK;
 }
 
-/* WRITE_OUT_PARTIAL_STATE (all_modules.f90:760-776): xi = a(1..3), acc = scr(4..6) */
+/* WRITE_OUT_PARTIAL_STATE (all_modules.f90:760-776): xi = a(1..3), acc = scr(4..6),
+ * v = a(4..6) */
 int orc_snapshot_store(orc *o, int slot) {
   if (slot < 0 || slot >= o->nslots) return SPARC_EINVAL;
-  double *s = o->snap + (size_t)slot * 6 * o->N;
+  double *s = o->snap + (size_t)slot * snap_fields(o) * o->N;
   memcpy(s, o->a, sizeof(double) * 3 * o->N);
   memcpy(s + 3 * o->N, o->scr + 3 * o->N, sizeof(double) * 3 * o->N);
+  if (o->cfg.flow_kernels) memcpy(s + 6 * o->N, o->a + 3 * o->N, sizeof(double) * 3 * o->N);
   return SPARC_OK;
 }
 
 int orc_snapshot_get(orc *o, int slot, int f, double *h) {
-  if (slot < 0 || slot >= o->nslots || f < 0 || f >= 6) return SPARC_EINVAL;
-  memcpy(h, o->snap + ((size_t)slot * 6 + f) * o->N, sizeof(double) * o->N);
+  if (slot < 0 || slot >= o->nslots || f < 0 || f >= snap_fields(o)) return SPARC_EINVAL;
+  memcpy(h, o->snap + ((size_t)slot * snap_fields(o) + f) * o->N, sizeof(double) * o->N);
   return SPARC_OK;
 }
 
 int orc_snapshot_put(orc *o, int slot, int f, const double *h) {
-  if (slot < 0 || slot >= o->nslots || f < 0 || f >= 6) return SPARC_EINVAL;
-  memcpy(o->snap + ((size_t)slot * 6 + f) * o->N, h, sizeof(double) * o->N);
+  if (slot < 0 || slot >= o->nslots || f < 0 || f >= snap_fields(o)) return SPARC_EINVAL;
+  memcpy(o->snap + ((size_t)slot * snap_fields(o) + f) * o->N, h, sizeof(double) * o->N);
   return SPARC_OK;
 }
 
@@ -1055,7 +1060,7 @@ static void compute_div(orc *o, const double *fx, const double *fy, const double
 int orc_kernel_accumulate(orc *o, int slot) {
   if (slot < 0 || slot >= o->nslots) return SPARC_EINVAL;
   long N = o->N, nxy = (long)o->nx * o->ny;
-  const double *s = o->snap + (size_t)slot * 6 * N;
+  const double *s = o->snap + (size_t)slot * snap_fields(o) * N;
   const double *fxx = s, *fxy = s + N, *fxz = s + 2 * N;
   const double *fax = s + 3 * N, *fay = s + 4 * N, *faz = s + 5 * N;
   const double *avx = o->a + 3 * N, *avy = o->a + 4 * N, *avz = o->a + 5 * N;
@@ -1087,6 +1092,24 @@ int orc_kernel_accumulate(orc *o, int slot) {
       krho[q] = krho[q] - o->grav[k] * fxz[q] * temp3[q] +
                 (fxx[q] * d31[q] + fxy[q] * d32[q] + fxz[q] * d33[q]);
     }
+  if (o->cfg.flow_kernels) {
+    /* COMPUTE_DEL (kernels.f90:821-841): deriv(i,j) = d_i f_vel_j; kernelv(i) +=
+     * sum_j deriv(i,j) * a_vel_j (kernels.f90:317-332); no y part when ny <= 4 */
+    const double *av[3] = {avx, avy, avz};
+    double *kv[3] = {o->kern[SPARC_KERN_VX], o->kern[SPARC_KERN_VY], o->kern[SPARC_KERN_VZ]};
+    for (int j = 0; j < 3; j++) {
+      const double *fv = s + (size_t)(6 + j) * N;
+      orc_ddx(fv, d31, o->nx, o->ny, o->nz, o->cfg.stretchx);
+      if (o->ny > 4) orc_ddy(fv, d32, o->nx, o->ny, o->nz, o->cfg.stretchy);
+      else memset(d32, 0, sizeof(double) * N);
+      ddzkern(o, fv, d33, 1, t2);
+      for (long c = 0; c < N; c++) {
+        kv[0][c] = kv[0][c] + d31[c] * av[j][c];
+        kv[1][c] = kv[1][c] + d32[c] * av[j][c];
+        kv[2][c] = kv[2][c] + d33[c] * av[j][c];
+      }
+    }
+  }
   free(temp1);
   free(temp2);
   free(temp3);

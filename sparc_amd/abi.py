@@ -27,7 +27,7 @@ ARR_A, ARR_SCR, ARR_PMLVARS, ARR_SCRPML, ARR_ALO = range(5)
  BG_C2) = range(8)
 PROF_STRETCH, PROF_UNSTRETCH, PROF_G, PROF_Z, PROF_SOURCE_DEP = range(5)
 SRC_NONE, SRC_SERIES, SRC_LAGRANGE = range(3)
-KERN_C2, KERN_RHO, KERN_HESSIAN = range(3)
+KERN_C2, KERN_RHO, KERN_HESSIAN, KERN_VX, KERN_VY, KERN_VZ = range(6)
 (KC_XLINES, KC_YLINES, KC_ZSOLVE, KC_DAMP, KC_UPDATE, KC_OTHER, KC_ZSOLVE_P, KC_DAMP_Y,
  KC_DAMP_XI, KC_FILTER) = range(10)
 KC_NAMES = ("xlines", "ylines", "zsolve", "damp", "update", "other", "zsolve_p", "damp_y",
@@ -66,7 +66,8 @@ class SparcConfig(C.Structure):
         ("forcing_length", C.c_int32),
         ("num_steps", C.c_int32),
         ("compensated", C.c_int32),
-        ("reserved_i", C.c_int32 * 7),
+        ("flow_kernels", C.c_int32),
+        ("reserved_i", C.c_int32 * 6),
         ("xlength", C.c_double), ("ylength", C.c_double),
         ("timestep", C.c_double),
         ("deltat", C.c_double),

@@ -458,4 +458,19 @@ __global__ void k_kern_accum(double* __restrict__ hess, double* __restrict__ kc2
   krho[q] = kr;
 }
 
+// Flow-kernel sums for one forward velocity component j (kernels.f90:317-332):
+// K_v(i) += d_i v_f,j * v_a,j with d_x, d_y, d_z v_f,j given (dy == nullptr: 0).
+template <typename T>
+__global__ void k_kern_flow(double* __restrict__ kx, double* __restrict__ ky,
+                            double* __restrict__ kz, const T* __restrict__ dx,
+                            const T* __restrict__ dy, const T* __restrict__ dz,
+                            const T* __restrict__ av, double sff, double sfa, long n) {
+  long q = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (q >= n) return;
+  const double a = (double)av[q] * sfa;
+  kx[q] = kx[q] + (double)dx[q] * sff * a;
+  if (dy) ky[q] = ky[q] + (double)dy[q] * sff * a;
+  kz[q] = kz[q] + (double)dz[q] * sff * a;
+}
+
 }  // namespace sparc

@@ -1151,11 +1151,14 @@ static int order_after(sparc_ctx* consumer, const sparc_ctx* producer) {
   return SPARC_OK;
 }
 
+// fields per snapshot slot: xi, acc (+ v when the flow kernels are on)
+static int snap_fields(const sparc_ctx* c) { return c->cfg.flow_kernels ? 9 : 6; }
+
 extern "C" int sparc_snapshots_alloc(sparc_ctx* c, int nslots) {
   if (!c || nslots < 0) return SPARC_EINVAL;
   if (nslots <= c->nslots) return SPARC_OK;
   void* p;
-  int rc = dalloc(c, &p, (size_t)nslots * 6 * c->N * c->ts);
+  int rc = dalloc(c, &p, (size_t)nslots * snap_fields(c) * c->N * c->ts);
   if (rc) return rc;
   c->snap = p;
   c->nslots = nslots;
@@ -1167,12 +1170,15 @@ extern "C" int sparc_snapshot_store(sparc_ctx* c, int slot) {
   if (!c || slot < 0 || slot >= c->nslots) return fail(c, SPARC_EINVAL, "bad slot %d", slot);
   if (c->cfg.option == SPARC_OPT_QUIET_PML_3D)
     return fail(c, SPARC_ESTATE, "snapshots hold xi and acc: displacement options (4, 17) only");
-  char* s = (char*)c->snap + (size_t)slot * 6 * c->N * c->ts;
+  char* s = (char*)c->snap + (size_t)slot * snap_fields(c) * c->N * c->ts;
   const size_t fb = c->N * c->ts;
   for (int f = 0; f < 3; ++f) {
     HIPCHK(c, hipMemcpyAsync(s + f * fb, c->a[f], fb, hipMemcpyDeviceToDevice, c->stream));
     HIPCHK(c, hipMemcpyAsync(s + (3 + f) * fb, c->scr[3 + f], fb, hipMemcpyDeviceToDevice,
                              c->stream));
+    if (c->cfg.flow_kernels)
+      HIPCHK(c, hipMemcpyAsync(s + (6 + f) * fb, c->a[3 + f], fb, hipMemcpyDeviceToDevice,
+                               c->stream));
   }
   c->snap_scale = c->cfg.field_scale;
   return SPARC_OK;
@@ -1180,12 +1186,13 @@ extern "C" int sparc_snapshot_store(sparc_ctx* c, int slot) {
 
 extern "C" int sparc_snapshots_copy(sparc_ctx* dst, const sparc_ctx* src) {
   if (!dst || !src) return SPARC_EINVAL;
-  if (dst->N != src->N || dst->fp32 != src->fp32)
-    return fail(dst, SPARC_EINVAL, "snapshot banks differ in shape or precision");
+  if (dst->N != src->N || dst->fp32 != src->fp32 || snap_fields(dst) != snap_fields(src))
+    return fail(dst, SPARC_EINVAL, "snapshot banks differ in shape, precision or fields");
   int rc = sparc_snapshots_alloc(dst, src->nslots);
   if (rc) return rc;
   if ((rc = order_before(dst, src))) return rc;
-  HIPCHK(dst, hipMemcpyAsync(dst->snap, src->snap, (size_t)src->nslots * 6 * src->N * src->ts,
+  HIPCHK(dst, hipMemcpyAsync(dst->snap, src->snap,
+                             (size_t)src->nslots * snap_fields(src) * src->N * src->ts,
                              hipMemcpyDeviceToDevice, dst->stream));
   dst->snap_scale = src->snap_scale;
   return order_after(dst, src);
@@ -1196,12 +1203,13 @@ extern "C" int sparc_kernel_begin(sparc_ctx* c, const double* grav, int nz) {
   if (c->cfg.option == SPARC_OPT_QUIET_PML_3D)
     return fail(c, SPARC_ESTATE, "kernels need the displacement formulation (options 4, 17)");
   int rc;
+  const int nk = c->cfg.flow_kernels ? SPARC_KERN_COUNT : SPARC_KERN_VX;
   if (!c->kern[0]) {
-    for (int q = 0; q < SPARC_KERN_COUNT; ++q)
+    for (int q = 0; q < nk; ++q)
       if ((rc = dalloc(c, (void**)&c->kern[q], c->N * 8))) return rc;
     if ((rc = dalloc(c, (void**)&c->grav, nz * 8))) return rc;
   }
-  for (int q = 0; q < SPARC_KERN_COUNT; ++q)
+  for (int q = 0; q < nk; ++q)
     HIPCHK(c, hipMemsetAsync(c->kern[q], 0, c->N * 8, c->stream));
   HIPCHK(c, hipMemcpyAsync(c->grav, grav, nz * 8, hipMemcpyHostToDevice, c->stream));
   HIPCHK(c, hipStreamSynchronize(c->stream));
@@ -1213,7 +1221,7 @@ static int kernel_acc_impl(sparc_ctx* c, const sparc_ctx* src, int slot) {
   const long N = c->N, nxy = c->nxy;
   const int nz = c->cfg.nz;
   Fld<T> F = make_fld<T>(c);
-  T* s = (T*)src->snap + (size_t)slot * 6 * N;
+  T* s = (T*)src->snap + (size_t)slot * snap_fields(src) * N;
   const T *fxx = s, *fxy = s + N, *fxz = s + 2 * N, *fax = s + 3 * N, *fay = s + 4 * N,
           *faz = s + 5 * N;
   const T *avx = F.a[3], *avy = F.a[4], *avz = F.a[5];
@@ -1257,6 +1265,26 @@ static int kernel_acc_impl(sparc_ctx* c, const sparc_ctx* src, int slot) {
                      (const T*)F.W[2], (const T*)c->bg[SPARC_BG_C2], c->bg3d[SPARC_BG_C2],
                      (const double*)c->grav, sff, sfa, nxy, N);
   LAUNCHCHK(c);
+  if (c->cfg.flow_kernels) {
+    // COMPUTE_DEL + kernelv (kernels.f90:317-332, 821-841): for each forward
+    // velocity component j, K_v(i) += d_i v_f,j * v_a,j (no y part when ny <= 4)
+    const T* av[3] = {avx, avy, avz};
+    for (int j = 0; j < 3; ++j) {
+      const T* fv = s + (size_t)(6 + j) * N;
+      if ((rc = launch_x<T, XM_DERIV>(c, fv, nullptr, 0, F.W[0], dmx, nullptr))) return rc;
+      if (has_y && (rc = launch_y<T, YM_DERIV>(c, fv, nullptr, 0, F.W[1], dmy, nullptr)))
+        return rc;
+      if ((rc = launch_z<T, EPI_STORE>(c, F, 1, fv, nullptr, 0, nullptr, 0, F.W[2]))) return rc;
+      hipLaunchKernelGGL((k_update_filter<T>), dim3(cdiv(nxy, 64)), dim3(64), 0, c->stream,
+                         F.W[2], (const T*)nullptr, (T)0, (int)nxy, nz, 2);
+      LAUNCHCHK(c);
+      hipLaunchKernelGGL((k_kern_flow<T>), dim3(cdiv(N, 256)), dim3(256), 0, c->stream,
+                         c->kern[SPARC_KERN_VX], c->kern[SPARC_KERN_VY], c->kern[SPARC_KERN_VZ],
+                         (const T*)F.W[0], has_y ? (const T*)F.W[1] : nullptr,
+                         (const T*)F.W[2], av[j], sff, sfa, N);
+      LAUNCHCHK(c);
+    }
+  }
   return SPARC_OK;
 }
 
@@ -1265,6 +1293,8 @@ extern "C" int sparc_kernel_accumulate_from(sparc_ctx* c, const sparc_ctx* src, 
   if (slot < 0 || slot >= src->nslots) return fail(c, SPARC_EINVAL, "bad slot %d", slot);
   if (src->N != c->N || src->fp32 != c->fp32)
     return fail(c, SPARC_EINVAL, "snapshot bank differs in shape or precision");
+  if (c->cfg.flow_kernels && !src->cfg.flow_kernels)
+    return fail(c, SPARC_EINVAL, "flow kernels need snapshots that keep v (flow_kernels)");
   if (!c->kern[0]) return fail(c, SPARC_ESTATE, "sparc_kernel_begin not called");
   if (!c->bgset[SPARC_BG_C2]) return fail(c, SPARC_ESTATE, "background C2 not set");
   int rc;
@@ -1314,8 +1344,8 @@ extern "C" int sparc_checkpoint_save(sparc_ctx* c, int slot) { return ckpt_copy(
 extern "C" int sparc_checkpoint_load(sparc_ctx* c, int slot) { return ckpt_copy(c, slot, false); }
 
 extern "C" int sparc_kernel_get(sparc_ctx* c, int which, double* h) {
-  if (!c || !h || which < 0 || which >= SPARC_KERN_COUNT || !c->kern[0])
-    return fail(c, SPARC_EINVAL, "bad kernel id or kernels not begun");
+  if (!c || !h || which < 0 || which >= SPARC_KERN_COUNT || !c->kern[0] || !c->kern[which])
+    return fail(c, SPARC_EINVAL, "bad kernel id, kernels not begun, or flow kernels off");
   HIPCHK(c, hipStreamSynchronize(c->stream));
   HIPCHK(c, hipMemcpy(h, c->kern[which], c->N * 8, hipMemcpyDeviceToHost));
   return SPARC_OK;

@@ -37,6 +37,7 @@ class Params:
     precision: int = abi.FP32
     field_scale: float = 1.0
     compensated: bool = True       # FP32: hi+lo state updated in FP64 arithmetic
+    flow_kernels: bool = False     # snapshots keep v; kernel sums add K_v (kernels.f90:317-332)
 
     @property
     def is2d(self) -> bool:
@@ -166,6 +167,7 @@ def make_config(P: Params, S: bgm.Setup, mode: int, nsteps_total: int) -> abi.Sp
     c.cadforcing = P.cadforcing
     c.field_scale = P.field_scale if P.precision == abi.FP32 else 1.0
     c.compensated = int(P.compensated and P.precision == abi.FP32)
+    c.flow_kernels = int(P.flow_kernels)
     return c
 
 

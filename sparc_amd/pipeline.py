@@ -158,6 +158,14 @@ def kernel_lnc(raw_c2: np.ndarray, S: bgm.Setup, spec: SolveSpec,
     return -2.0 * S.rho0 * S.c2 * raw_c2 * dtn
 
 
+def kernel_flow(raw_v: np.ndarray, S: bgm.Setup, spec: SolveSpec,
+                kstride: int | None = None) -> np.ndarray:
+    """Flow kernel component (per unit nondimensional volume and velocity)
+    from the accumulated sum (kernels.f90:485: -2 rho0 sum * stepskern*timestep)."""
+    dtn = (kstride or spec.steps) * S.deltat
+    return -2.0 * S.rho0 * raw_v * dtn
+
+
 def make_engines(spec: SolveSpec, S: bgm.Setup, eng_factory=None):
     """Forward and adjoint engines (or oracles via eng_factory(cfg, S))."""
     W = spec.W
@@ -188,8 +196,9 @@ def copy_snapshots(dst, src, nslots: int) -> None:
         dst.snapshots_copy_from(src)
         return
     dst.snapshots_alloc(nslots)
+    nf = 9 if getattr(src.cfg, "flow_kernels", 0) else 6
     for s in range(nslots):
-        for f in range(6):
+        for f in range(nf):
             dst.snapshot_put(s, f, src.snapshot_get(s, f))
 
 

@@ -169,10 +169,14 @@ def test_step_2d_option17_matches_oracle(two_d, prec):
         o.close()
 
 
-def test_adjoint_lagrange_source_and_kernels_match_oracle(tiny):
+@pytest.mark.parametrize("flow", [False, True])
+def test_adjoint_lagrange_source_and_kernels_match_oracle(tiny, flow):
     """Forward run storing snapshots at the output cadence, adjoint run driven
-    by a 7-point-Lagrange source at o_rad, kernel accumulation in FP64."""
-    W, S = tiny
+    by a 7-point-Lagrange source at o_rad, kernel accumulation in FP64 (with
+    the flow kernels, snapshots also keep v)."""
+    from dataclasses import replace
+    W0, S = tiny
+    W = D.Workload(W0.name, replace(W0.params, flow_kernels=flow), W0.tab, W0.nsteps)
     nsteps = 18
     steps = W.params.steps
     nslots = nsteps // steps
@@ -196,7 +200,7 @@ def test_adjoint_lagrange_source_and_kernels_match_oracle(tiny):
     ae.snapshots_copy_from(fe)
     ao.snapshots_alloc(nslots)
     for s in range(nslots):
-        for f in range(6):
+        for f in range(9 if flow else 6):
             ao.snapshot_put(s, f, fo.snapshot_get(s, f))
     grav = S.g_raw
     for x in (ae, ao):
@@ -209,7 +213,13 @@ def test_adjoint_lagrange_source_and_kernels_match_oracle(tiny):
     try:
         for f in range(3, 6):
             assert rel_err(ae.get(abi.ARR_A, f), ao.get(abi.ARR_A, f)) < TOL64
-        for k in (abi.KERN_C2, abi.KERN_RHO, abi.KERN_HESSIAN):
+        kerns = (abi.KERN_C2, abi.KERN_RHO, abi.KERN_HESSIAN)
+        if flow:
+            kerns += (abi.KERN_VX, abi.KERN_VY, abi.KERN_VZ)
+        else:
+            with pytest.raises(abi.SparcError):
+                ae.kernel_get(abi.KERN_VX)
+        for k in kerns:
             a, b = ae.kernel_get(k), ao.kernel_get(k)
             assert np.linalg.norm(b) > 0
             assert rel_err(a, b) < TOL64, k

@@ -155,3 +155,46 @@ def test_option2_plane_wave_amplification():
     # v_z: only the leak of the zeroed boundary planes through the compact
     # solve (decaying ~0.27 per plane, 15 planes away)
     assert np.max(np.abs(got[3][:, :, mid])) < 1e-7
+
+
+def test_flow_kernel_closed_form():
+    """kernelv(i) += sum_j d_i v_f,j v_a,j (kernels.f90:317-332, COMPUTE_DEL
+    :821-841): with v_f = (sin(kx), 0, cos(ky)) and v_a = (1, 0, 2) uniform,
+    one accumulation gives K_x = k cos(kx), K_y = -2 k sin(ky), K_z = 0 in
+    the interior (the z derivative of z-uniform fields)."""
+    nx, ny, nz = 32, 16, 30
+    cfg = abi.SparcConfig()
+    cfg.abi_version = abi.ABI_VERSION
+    cfg.nx, cfg.ny, cfg.nz = nx, ny, nz
+    cfg.option = abi.OPT_3D
+    cfg.deltat = 1e-3
+    cfg.stretchx = cfg.stretchy = 1.0
+    cfg.e_rad, cfg.o_rad = 5, 6
+    cfg.flow_kernels = 1
+    o = O.Oracle(cfg)
+    o.set_profile(abi.PROF_STRETCH, np.ones(nz))
+    o.set_background(abi.BG_C2, np.ones(nz))
+    shape = (nx, ny, nz)
+    x = np.arange(nx) / nx
+    y = np.arange(ny) / ny
+    m = 2
+    k = 2 * B.PI_R * m
+    sinx = np.broadcast_to(np.sin(2 * np.pi * m * x)[:, None, None], shape)
+    cosx = np.broadcast_to(np.cos(2 * np.pi * m * x)[:, None, None], shape)
+    siny = np.broadcast_to(np.sin(2 * np.pi * m * y)[None, :, None], shape)
+    cosy = np.broadcast_to(np.cos(2 * np.pi * m * y)[None, :, None], shape)
+    o.snapshots_alloc(1)
+    o.snapshot_put(0, 6, sinx)
+    o.snapshot_put(0, 8, cosy)
+    o.put(abi.ARR_A, abi.V_X, np.ones(shape))
+    o.put(abi.ARR_A, abi.V_Z, 2 * np.ones(shape))
+    o.kernel_begin(np.zeros(nz))
+    o.kernel_accumulate(0)
+    kx, ky, kz = (o.kernel_get(q) for q in (abi.KERN_VX, abi.KERN_VY, abi.KERN_VZ))
+    o.close()
+    mid = slice(8, nz - 8)
+    assert np.max(np.abs(kx[:, :, mid] - k * cosx[:, :, mid])) < 1e-9 * k
+    assert np.max(np.abs(ky[:, :, mid] + 2 * k * siny[:, :, mid])) < 1e-9 * k
+    # d/dz of a z-uniform field: zero up to the single-rounded boundary-row
+    # constants of DBYD2 (dbyd2.f90:122-125) leaking 8 planes inwards
+    assert np.max(np.abs(kz[:, :, mid])) < 1e-6 * k
