@@ -44,6 +44,7 @@ extern "C" {
 /* physics option, numbering of PARSE_OPTIONS (initialize.f90:778-824) */
 #define SPARC_OPT_QUIET_PML_3D 2         /* (rho,v,p) form, pml.f90:12-76 */
 #define SPARC_OPT_QUIET_PML_3D_DISPL 4   /* displacement.f90:482-658 */
+#define SPARC_OPT_MHD_PML_3D_DISPL 10    /* displacement.f90:663-849 */
 #define SPARC_OPT_QUIET_PML_2D_DISPL 17  /* physics2d.f90:948-1105   */
 
 /* solve direction, compute_forward / compute_adjoint (initialize.f90:1006-1045) */
@@ -66,7 +67,9 @@ extern "C" {
 /* which array sparc_get_state / sparc_put_state touch */
 #define SPARC_ARR_A 0        /* a(:,:,:,f): the state                          */
 #define SPARC_ARR_SCR 1      /* scr(:,:,:,f): last stage RHS ("acc" for f=3..5) */
-#define SPARC_ARR_PMLVARS 2  /* pmlvars(:,:,:,f), f=0 psivz, f=1 psip           */
+#define SPARC_ARR_PMLVARS 2  /* pmlvars(:,:,:,f), f=0 psivz, f=1 psip; option 10
+                                adds 2 psiinductionbx, 3 psiinductionby,
+                                4 psidzbx, 5 psidzby (initialize.f90:480-495) */
 #define SPARC_ARR_SCRPML 3   /* scrpml(:,:,:,f)                                 */
 #define SPARC_ARR_ALO 4      /* low-order part of a(:,:,:,f) when the FP32 state is
                                 compensated (cfg.compensated); get/put of
@@ -82,7 +85,19 @@ extern "C" {
 #define SPARC_BG_AZ 5          /* az    = -dampz       (physics.f90:291)      */
 #define SPARC_BG_BZPML 6       /* bzpml = -alphaz-dampz (physics.f90:290)      */
 #define SPARC_BG_C2 7          /* c2, used by the sound-speed/density kernels */
-#define SPARC_BG_COUNT 8
+/* option 10: background field B0 = (box, boy, boz), its curl and the
+ * horizontal background gradients (physics.f90:117-190) */
+#define SPARC_BG_BX 8
+#define SPARC_BG_BY 9
+#define SPARC_BG_BZ 10
+#define SPARC_BG_CURLBX 11
+#define SPARC_BG_CURLBY 12
+#define SPARC_BG_CURLBZ 13
+#define SPARC_BG_GRADP0_X 14
+#define SPARC_BG_GRADP0_Y 15
+#define SPARC_BG_GRADRHO0_X 16
+#define SPARC_BG_GRADRHO0_Y 17
+#define SPARC_BG_COUNT 18
 
 /* z-profiles (sparc_set_profile), length nz */
 #define SPARC_PROF_STRETCH 0   /* 1/(dz/dk) (initialize.f90:464-468)          */
@@ -91,7 +106,9 @@ extern "C" {
 #define SPARC_PROF_Z 3         /* grid z (units of Rsun)                      */
 #define SPARC_PROF_SOURCE_DEP 4 /* source_dep(k) = exp(-4e7 (z(k)-z(e_rad))^2)
                                   (physics.f90:109): option 2's source shape */
-#define SPARC_PROF_COUNT 5
+#define SPARC_PROF_SPONGE 5    /* spongez(k), planes 1..90 zeroed (physics.f90:103-109,
+                                  293); option 10 applies it for k >= nz-35     */
+#define SPARC_PROF_COUNT 6
 
 /* source kinds (sparc_set_source) */
 #define SPARC_SRC_NONE 0

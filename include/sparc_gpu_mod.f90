@@ -19,7 +19,7 @@ module sparc_gpu
                                SPARC_ENOMEM = -3, SPARC_ESTATE = -4
   integer(c_int), parameter :: SPARC_FP64 = 0, SPARC_FP32 = 1
   integer(c_int), parameter :: SPARC_OPT_QUIET_PML_3D = 2, SPARC_OPT_QUIET_PML_3D_DISPL = 4, &
-                               SPARC_OPT_QUIET_PML_2D_DISPL = 17
+                               SPARC_OPT_MHD_PML_3D_DISPL = 10, SPARC_OPT_QUIET_PML_2D_DISPL = 17
   integer(c_int), parameter :: SPARC_RHO = 0, SPARC_P = 4
   integer(c_int), parameter :: SPARC_FORWARD = 0, SPARC_ADJOINT = 1
   integer(c_int), parameter :: SPARC_XI_X = 0, SPARC_XI_Y = 1, SPARC_XI_Z = 2, &
@@ -28,10 +28,14 @@ module sparc_gpu
                                SPARC_ARR_SCRPML = 3, SPARC_ARR_ALO = 4
   integer(c_int), parameter :: SPARC_BG_RHO0 = 0, SPARC_BG_RHOINV = 1, SPARC_BG_C2RHO0 = 2, &
                                SPARC_BG_GRADP0_Z = 3, SPARC_BG_GRADRHO0_Z = 4, SPARC_BG_AZ = 5, &
-                               SPARC_BG_BZPML = 6, SPARC_BG_C2 = 7, SPARC_BG_COUNT = 8
+                               SPARC_BG_BZPML = 6, SPARC_BG_C2 = 7, SPARC_BG_BX = 8, &
+                               SPARC_BG_BY = 9, SPARC_BG_BZ = 10, SPARC_BG_CURLBX = 11, &
+                               SPARC_BG_CURLBY = 12, SPARC_BG_CURLBZ = 13, SPARC_BG_GRADP0_X = 14, &
+                               SPARC_BG_GRADP0_Y = 15, SPARC_BG_GRADRHO0_X = 16, &
+                               SPARC_BG_GRADRHO0_Y = 17, SPARC_BG_COUNT = 18
   integer(c_int), parameter :: SPARC_PROF_STRETCH = 0, SPARC_PROF_UNSTRETCH = 1, &
                                SPARC_PROF_G = 2, SPARC_PROF_Z = 3, SPARC_PROF_SOURCE_DEP = 4, &
-                               SPARC_PROF_COUNT = 5
+                               SPARC_PROF_SPONGE = 5, SPARC_PROF_COUNT = 6
   integer(c_int), parameter :: SPARC_SRC_NONE = 0, SPARC_SRC_SERIES = 1, SPARC_SRC_LAGRANGE = 2
   integer(c_int), parameter :: SPARC_KERN_C2 = 0, SPARC_KERN_RHO = 1, SPARC_KERN_HESSIAN = 2, &
                                SPARC_KERN_VX = 3, SPARC_KERN_VY = 4, SPARC_KERN_VZ = 5

@@ -378,7 +378,9 @@ typedef struct orc {
   int nx, ny, nz, nzpml;
   long N, NP;
   double *a, *temp, *scr;             /* 6*N each */
-  double *pmlvars, *psipml, *scrpml;  /* 2*NP each */
+  double *pmlvars, *psipml, *scrpml;  /* npv*NP each */
+  int npv;                            /* psi fields: 2, or 6 for option 10 */
+  double *mw[12];                     /* option 10 work arrays */
   double *bg[SPARC_BG_COUNT];         /* N each (z-profiles expanded) */
   double *prof[SPARC_PROF_COUNT];     /* nz each */
   double *damping_rates;              /* (nx/2+1)*ny */
@@ -417,9 +419,12 @@ orc *orc_create(const sparc_config *cfg) {
   o->a = zalloc(sizeof(double) * 6 * N);
   o->temp = zalloc(sizeof(double) * 6 * N);
   o->scr = zalloc(sizeof(double) * 6 * N);
-  o->pmlvars = zalloc(sizeof(double) * 2 * o->NP);
-  o->psipml = zalloc(sizeof(double) * 2 * o->NP);
-  o->scrpml = zalloc(sizeof(double) * 2 * o->NP);
+  o->npv = (cfg->option == SPARC_OPT_MHD_PML_3D_DISPL) ? 6 : 2;
+  o->pmlvars = zalloc(sizeof(double) * o->npv * o->NP);
+  o->psipml = zalloc(sizeof(double) * o->npv * o->NP);
+  o->scrpml = zalloc(sizeof(double) * o->npv * o->NP);
+  for (int q = 0; q < 12; q++)
+    o->mw[q] = (cfg->option == SPARC_OPT_MHD_PML_3D_DISPL) ? zalloc(sizeof(double) * N) : NULL;
   for (int b = 0; b < SPARC_BG_COUNT; b++) o->bg[b] = zalloc(sizeof(double) * N);
   for (int b = 0; b < SPARC_PROF_COUNT; b++) o->prof[b] = zalloc(sizeof(double) * o->nz);
   o->damping_rates = zalloc(sizeof(double) * (o->nx / 2 + 1) * o->ny);
@@ -456,6 +461,7 @@ void orc_destroy(orc *o) {
   free(o->pmlvars);
   free(o->psipml);
   free(o->scrpml);
+  for (int q = 0; q < 12; q++) free(o->mw[q]);
   for (int b = 0; b < SPARC_BG_COUNT; b++) free(o->bg[b]);
   for (int b = 0; b < SPARC_PROF_COUNT; b++) free(o->prof[b]);
   free(o->damping_rates);
@@ -796,6 +802,133 @@ static void rhs_option2(orc *o) {
   }
 }
 
+/* The PML of MP_MHD_PML_3D_DISPL: bottom layers only (displacement.f90:681-692) */
+static void pml_account_bottom(orc *o, double *d, int which) {
+  long nxy = (long)o->nx * o->ny;
+  for (int k = 1; k <= o->cfg.npmlbot; k++) {
+    double *psi = o->psipml + which * o->NP + nxy * (k - 1);
+    double *rhs = o->scrpml + which * o->NP + nxy * (k - 1);
+    long off = nxy * (k - 1);
+    for (long c = 0; c < nxy; c++) {
+      rhs[c] = o->bg[SPARC_BG_AZ][off + c] * d[off + c] + o->bg[SPARC_BG_BZPML][off + c] * psi[c];
+      d[off + c] = d[off + c] + psi[c];
+    }
+  }
+}
+
+/* MP_MHD_PML_3D_DISPL, option 10 (displacement.f90:663-849), kernel mode,
+ * sources at the planes e_rad/o_rad (erad_2d = e_rad everywhere).
+ * ddxyz(f1,d1,f2,d2,f3,d3,bc): d1 = d/dx f1, d2 = d/dy f2, d3 = d/dz f3. */
+static void rhs_option10(orc *o) {
+  int nx = o->nx, ny = o->ny, nz = o->nz;
+  long N = o->N, nxy = (long)nx * ny;
+  double *xi_x = o->temp, *xi_y = o->temp + N, *xi_z = o->temp + 2 * N;
+  double *v_x = o->temp + 3 * N, *v_y = o->temp + 4 * N, *v_z = o->temp + 5 * N;
+  double **bg = o->bg, **m = o->mw;
+  const double *bx0 = bg[SPARC_BG_BX], *by0 = bg[SPARC_BG_BY], *bz0 = bg[SPARC_BG_BZ];
+  const double *st = o->prof[SPARC_PROF_STRETCH];
+  double sx = o->cfg.stretchx, sy = o->cfg.stretchy;
+  double *f1 = m[0], *f2 = m[1], *f3 = m[2];
+  /* flux = -(xi x B0) (:673-675) */
+  for (long c = 0; c < N; c++) {
+    f1[c] = -(xi_z[c] * by0[c] - xi_y[c] * bz0[c]);
+    f2[c] = -(xi_x[c] * bz0[c] - xi_z[c] * bx0[c]);
+    f3[c] = -(xi_y[c] * bx0[c] - xi_x[c] * by0[c]);
+  }
+  /* div xi with psivz, p, rho (:677-700) */
+  orc_ddx(xi_x, o->dxdx, nx, ny, nz, sx);
+  orc_ddy(xi_y, o->dydy, nx, ny, nz, sy);
+  orc_ddz(xi_z, o->dzdz, nx, ny, nz, 1, st);
+  pml_account_bottom(o, o->dzdz, 0);
+  for (long c = 0; c < N; c++) {
+    double div = o->dxdx[c] + o->dydy[c] + o->dzdz[c];
+    o->p[c] = -bg[SPARC_BG_C2RHO0][c] * div - xi_z[c] * bg[SPARC_BG_GRADP0_Z][c] -
+              xi_x[c] * bg[SPARC_BG_GRADP0_X][c] - xi_y[c] * bg[SPARC_BG_GRADP0_Y][c];
+    o->rho[c] = -bg[SPARC_BG_RHO0][c] * div - xi_z[c] * bg[SPARC_BG_GRADRHO0_Z][c] -
+                xi_x[c] * bg[SPARC_BG_GRADRHO0_X][c] - xi_y[c] * bg[SPARC_BG_GRADRHO0_Y][c];
+  }
+  /* grad p with psip (:702-717) */
+  orc_ddx(o->p, o->gpx, nx, ny, nz, sx);
+  orc_ddy(o->p, o->gpy, nx, ny, nz, sy);
+  orc_ddz(o->p, o->gpz, nx, ny, nz, 0, st);
+  pml_account_bottom(o, o->gpz, 1);
+  /* b = curl(flux) (:719-753) */
+  double *dxf3 = m[3], *dyf3 = m[4], *dzf2 = m[5], *dxf2 = m[6], *dyf1 = m[7], *dzf1 = m[8];
+  orc_ddx(f3, dxf3, nx, ny, nz, sx);
+  orc_ddy(f3, dyf3, nx, ny, nz, sy);
+  orc_ddz(f2, dzf2, nx, ny, nz, 1, st);
+  orc_ddx(f2, dxf2, nx, ny, nz, sx);
+  orc_ddy(f1, dyf1, nx, ny, nz, sy);
+  orc_ddz(f1, dzf1, nx, ny, nz, 1, st);
+  pml_account_bottom(o, dzf2, 2);
+  pml_account_bottom(o, dzf1, 3);
+  double *bx = f1, *by = f2, *bz = f3; /* fluxes are consumed */
+  for (long c = 0; c < N; c++) {
+    double b1 = dyf3[c] - dzf2[c], b2 = -dxf3[c] + dzf1[c], b3 = dxf2[c] - dyf1[c];
+    bx[c] = b1;
+    by[c] = b2;
+    bz[c] = b3;
+  }
+  /* J = curl(b) (:755-788) */
+  double *dxby = m[3], *dybz = m[4], *dzbx = m[5], *dxbz = m[6], *dybx = m[7], *dzby = m[8];
+  orc_ddx(by, dxby, nx, ny, nz, sx);
+  orc_ddy(bz, dybz, nx, ny, nz, sy);
+  orc_ddz(bx, dzbx, nx, ny, nz, 1, st);
+  orc_ddx(bz, dxbz, nx, ny, nz, sx);
+  orc_ddy(bx, dybx, nx, ny, nz, sy);
+  orc_ddz(by, dzby, nx, ny, nz, 1, st);
+  pml_account_bottom(o, dzbx, 4);
+  pml_account_bottom(o, dzby, 5);
+  double *RHSxi_x = o->scr, *RHSxi_y = o->scr + N, *RHSxi_z = o->scr + 2 * N;
+  double *RHSv_x = o->scr + 3 * N, *RHSv_y = o->scr + 4 * N, *RHSv_z = o->scr + 5 * N;
+  const double *rhoinv = bg[SPARC_BG_RHOINV], *g = o->prof[SPARC_PROF_G];
+  const double *sponge = o->prof[SPARC_PROF_SPONGE];
+  const double *jx0 = bg[SPARC_BG_CURLBX], *jy0 = bg[SPARC_BG_CURLBY], *jz0 = bg[SPARC_BG_CURLBZ];
+  for (int k = 0; k < nz; k++)
+    for (long c = 0; c < nxy; c++) {
+      long q = c + nxy * k;
+      double jx = dybz[q] - dzby[q], jy = -dxbz[q] + dzbx[q], jz = dxby[q] - dybx[q];
+      /* cross(J, B0) and cross(J0, b) (:790-801) */
+      double l1 = jy * bz0[q] - jz * by0[q], l2 = jz * bx0[q] - jx * bz0[q];
+      double l3 = jx * by0[q] - jy * bx0[q];
+      double m1 = jy0[q] * bz[q] - jz0[q] * by[q], m2 = jz0[q] * bx[q] - jx0[q] * bz[q];
+      double m3 = jx0[q] * by[q] - jy0[q] * bx[q];
+      double rx = rhoinv[q] * (-o->gpx[q] + l1) + rhoinv[q] * m1;
+      double ry = rhoinv[q] * (-o->gpy[q] + l2) + rhoinv[q] * m2;
+      double rz = rhoinv[q] * (-o->gpz[q] + l3) + rhoinv[q] * m3;
+      if (k + 1 >= nz - 35) { /* top sponge (:803-807) */
+        rx = rx - sponge[k] * v_x[q];
+        ry = ry - sponge[k] * v_y[q];
+        rz = rz - sponge[k] * v_z[q];
+      }
+      rz = rz - rhoinv[q] * o->rho[q] * g[k];
+      RHSv_x[q] = rx;
+      RHSv_y[q] = ry;
+      RHSv_z[q] = rz;
+      RHSxi_x[q] = v_x[q];
+      RHSxi_y[q] = v_y[q];
+      RHSxi_z[q] = v_z[q];
+    }
+  if (o->cfg.damp_waves) {
+    damp_field(o, v_x, RHSv_x);
+    damp_field(o, v_y, RHSv_y);
+    damp_field(o, v_z, RHSv_z);
+  }
+  /* kernel-mode source at orad_2d / erad_2d (:813-836) */
+  if (o->src_kind != SPARC_SRC_NONE && o->time <= o->src_active_until) {
+    int kk = (o->cfg.mode == SPARC_ADJOINT) ? o->cfg.o_rad - 1 : o->cfg.e_rad - 1;
+    for (long c = 0; c < nxy; c++) {
+      long q = c + nxy * kk;
+      RHSv_z[q] = RHSv_z[q] + o->forcing[c] / bg[SPARC_BG_RHO0][q] * o->cfg.delta_width;
+    }
+  }
+  for (int f = 0; f < 6; f++)
+    for (long c = 0; c < nxy; c++) {
+      o->scr[f * N + nxy * (nz - 1) + c] = 0.0;
+      o->scr[f * N + c] = 0.0;
+    }
+}
+
 /* MP_QUIET_PML_2D_DISPL, option 17 (physics2d.f90:948-1105), ny = 1. */
 static void rhs_option17(orc *o) {
   int nx = o->nx, ny = o->ny, nz = o->nz;
@@ -910,7 +1043,7 @@ static void filter_vars(orc *o) {
 
 /* STEP (step.f90:43-128) */
 static void step(orc *o) {
-  long N6 = 6 * o->N, NP2 = 2 * o->NP;
+  long N6 = 6 * o->N, NP2 = (long)o->npv * o->NP;
   int npml = o->nzpml > 0;
   for (int s = 0; s < 5; s++) {
     if (s == 0) {
@@ -926,6 +1059,8 @@ static void step(orc *o) {
       rhs_option4(o);
     else if (o->cfg.option == SPARC_OPT_QUIET_PML_3D)
       rhs_option2(o);
+    else if (o->cfg.option == SPARC_OPT_MHD_PML_3D_DISPL)
+      rhs_option10(o);
     else
       rhs_option17(o);
   }
@@ -960,11 +1095,11 @@ static double *arr_ptr(orc *o, int arr, int field, long *n) {
     *n = o->N;
     return o->scr + field * o->N;
   }
-  if (arr == SPARC_ARR_PMLVARS && field >= 0 && field < 2) {
+  if (arr == SPARC_ARR_PMLVARS && field >= 0 && field < o->npv) {
     *n = o->NP;
     return o->pmlvars + field * o->NP;
   }
-  if (arr == SPARC_ARR_SCRPML && field >= 0 && field < 2) {
+  if (arr == SPARC_ARR_SCRPML && field >= 0 && field < o->npv) {
     *n = o->NP;
     return o->scrpml + field * o->NP;
   }

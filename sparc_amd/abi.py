@@ -19,13 +19,15 @@ OK, EINVAL, EHIP, ENOMEM, ESTATE = 0, -1, -2, -3, -4
 FP64, FP32 = 0, 1
 OPT_3D, OPT_2D = 4, 17
 OPT_RHOVP = 2          # (rho, v, p) form, forward only
+OPT_MHD = 10           # linear MHD, displacement form
 RHO, P = 0, 4          # option 2 slots of rho and p (v in 1..3)
 FORWARD, ADJOINT = 0, 1
 XI_X, XI_Y, XI_Z, V_X, V_Y, V_Z = range(6)
 ARR_A, ARR_SCR, ARR_PMLVARS, ARR_SCRPML, ARR_ALO = range(5)
 (BG_RHO0, BG_RHOINV, BG_C2RHO0, BG_GRADP0_Z, BG_GRADRHO0_Z, BG_AZ, BG_BZPML,
- BG_C2) = range(8)
-PROF_STRETCH, PROF_UNSTRETCH, PROF_G, PROF_Z, PROF_SOURCE_DEP = range(5)
+ BG_C2, BG_BX, BG_BY, BG_BZ, BG_CURLBX, BG_CURLBY, BG_CURLBZ, BG_GRADP0_X, BG_GRADP0_Y,
+ BG_GRADRHO0_X, BG_GRADRHO0_Y) = range(18)
+PROF_STRETCH, PROF_UNSTRETCH, PROF_G, PROF_Z, PROF_SOURCE_DEP, PROF_SPONGE = range(6)
 SRC_NONE, SRC_SERIES, SRC_LAGRANGE = range(3)
 KERN_C2, KERN_RHO, KERN_HESSIAN, KERN_VX, KERN_VY, KERN_VZ = range(6)
 (KC_XLINES, KC_YLINES, KC_ZSOLVE, KC_DAMP, KC_UPDATE, KC_OTHER, KC_ZSOLVE_P, KC_DAMP_Y,

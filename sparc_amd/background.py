@@ -205,6 +205,18 @@ def init_rhs(z, c2, rho0, p0, g, npmlbot: int, npmltop: int, is2d: bool) -> dict
                 gradrho0_z=gradrho0_z, gradp0_z=gradp0_z, g=g, az=az, bzpml=bzpml)
 
 
+def sponge_z(z: np.ndarray, npml_zeroed: int = 90) -> np.ndarray:
+    """spongez(k) of mp_Initialize_RHS (physics.f90:103-107) with the planes
+    1..90 zeroed as the PML branch does (physics.f90:293); option 10 applies
+    it to v for k >= nz-35.  Literals are single-rounded as in the reference."""
+    z = np.asarray(z, dtype=np.float64)
+    with np.errstate(over="ignore"):
+        s = (200.0 / (1.0 + np.exp((sp(1.002) - z) / sp(0.0001)))
+             + 300.0 / (1.0 + np.exp((z - (z[0] + sp(0.005))) / sp(0.0005))))
+    s[:min(npml_zeroed, z.size)] = 0.0
+    return s
+
+
 def source_dep(z: np.ndarray, e_rad: int) -> np.ndarray:
     """source_dep(k) = exp(-10**7 * 4 (z(k) - z(e_rad))^2) (physics.f90:109),
     the vertical shape of option 2's source; e_rad is 1-based."""

@@ -20,8 +20,9 @@ template <typename T>
 struct Fld {
   T* a[SPARC_NFIELDS];
   T* scr[SPARC_NFIELDS];
-  T* pmlvars[2];
-  T* scrpml[2];
+  T* pmlvars[6];
+  T* scrpml[6];
+  T* M[12];  // option 10 work arrays
   const T* bg[SPARC_BG_COUNT];
   int bg3d[SPARC_BG_COUNT];
   const T* stretch;
@@ -456,6 +457,115 @@ __global__ void k_kern_accum(double* __restrict__ hess, double* __restrict__ kc2
   const double D31 = d31[q] * sfa, D32 = d32 ? d32[q] * sfa : 0.0, D33 = d33[q] * sfa;
   kr = kr - grav[k] * Fxz * Diva + (Fxx * D31 + Fxy * D32 + Fxz * D33);
   krho[q] = kr;
+}
+
+// ---------------------------------------------------------------------------
+// Option 10 (MP_MHD_PML_3D_DISPL, displacement.f90:663-849): pointwise parts
+// between the derivative passes.  Stage temp = a + beta*scr is formed on the
+// fly; M[] are the context's work arrays.
+// ---------------------------------------------------------------------------
+template <typename T>
+__device__ __forceinline__ T tmp_of(const Fld<T>& F, int f, long q) {
+  T v = F.a[f][q];
+  if (F.beta != (T)0) v = v + F.beta * F.scr[f][q];
+  return v;
+}
+
+// flux = xi x B0 (displacement.f90:673-675: flux1 = -(xi_z boy - xi_y boz), ...)
+template <typename T>
+__global__ void k_mhd_flux(Fld<T> F, long n) {
+  const long q = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (q >= n) return;
+  const int k = (int)(q / F.nxy), c = (int)(q - (long)k * F.nxy);
+  const T xx = tmp_of(F, SPARC_XI_X, q), xy = tmp_of(F, SPARC_XI_Y, q), xz = tmp_of(F, SPARC_XI_Z, q);
+  const T bx = bgv(F, SPARC_BG_BX, c, k), by = bgv(F, SPARC_BG_BY, c, k), bz = bgv(F, SPARC_BG_BZ, c, k);
+  F.M[0][q] = -(xz * by - xy * bz);
+  F.M[1][q] = -(xx * bz - xz * bx);
+  F.M[2][q] = -(xy * bx - xx * by);
+}
+
+// div xi -> p, rho with the horizontal background gradients (:694-700):
+// W0 + W1 + W2 = div, out p -> W3, rho -> W4
+template <typename T>
+__global__ void k_mhd_pr(Fld<T> F, long n) {
+  const long q = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (q >= n) return;
+  const int k = (int)(q / F.nxy), c = (int)(q - (long)k * F.nxy);
+  const T xx = tmp_of(F, SPARC_XI_X, q), xy = tmp_of(F, SPARC_XI_Y, q), xz = tmp_of(F, SPARC_XI_Z, q);
+  const T div = F.W[0][q] + F.W[1][q] + F.W[2][q];
+  F.W[3][q] = -bgv(F, SPARC_BG_C2RHO0, c, k) * div - xz * bgv(F, SPARC_BG_GRADP0_Z, c, k) -
+              xx * bgv(F, SPARC_BG_GRADP0_X, c, k) - xy * bgv(F, SPARC_BG_GRADP0_Y, c, k);
+  F.W[4][q] = -bgv(F, SPARC_BG_RHO0, c, k) * div - xz * bgv(F, SPARC_BG_GRADRHO0_Z, c, k) -
+              xx * bgv(F, SPARC_BG_GRADRHO0_X, c, k) - xy * bgv(F, SPARC_BG_GRADRHO0_Y, c, k);
+}
+
+// PML of option 10, bottom layers only (:684-692 and the like): psi field w
+// on derivative d over planes 0..npmlbot-1
+template <typename T>
+__global__ void k_pml_bottom(Fld<T> F, T* __restrict__ d, int w, long n) {
+  const long q = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (q >= n) return;
+  const int k = (int)(q / F.nxy), c = (int)(q - (long)k * F.nxy);
+  T psi = F.pmlvars[w][q];
+  if (F.beta != (T)0) psi = psi + F.beta * F.scrpml[w][q];
+  const T dv = d[q];
+  F.scrpml[w][q] = bgv(F, SPARC_BG_AZ, c, k) * dv + bgv(F, SPARC_BG_BZPML, c, k) * psi;
+  d[q] = dv + psi;
+}
+
+// b = curl(flux) from its pieces (:744-746): M0 = dy f3 - dz f2,
+// M1 = -dx f3 + dz f1, M2 = dx f2 - dy f1 with the pieces in M6..M11
+template <typename T>
+__global__ void k_mhd_b(Fld<T> F, long n) {
+  const long q = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (q >= n) return;
+  F.M[0][q] = F.M[7][q] - F.M[8][q];
+  F.M[1][q] = -F.M[6][q] + F.M[11][q];
+  F.M[2][q] = F.M[9][q] - F.M[10][q];
+}
+
+// J = curl(b), the Lorentz force J x B0 + J0 x b, grad p, sponge, gravity,
+// RHSxi = v, source, zeroed planes 1 and nz (:776-846).  gradp in M3..M5,
+// b in M0..M2, curl pieces dx by, dy bz, dz bx, dx bz, dy bx, dz by in M6..M11,
+// rho in W4.  The damping of v is subtracted afterwards.
+template <typename T>
+__global__ void k_mhd_rhs(Fld<T> F, const T* __restrict__ sponge, long n) {
+  const long q = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (q >= n) return;
+  const int nz = F.nz;
+  const int k = (int)(q / F.nxy), c = (int)(q - (long)k * F.nxy);
+  const T vx = tmp_of(F, SPARC_V_X, q), vy = tmp_of(F, SPARC_V_Y, q), vz = tmp_of(F, SPARC_V_Z, q);
+  if (k == 0 || k == nz - 1) {
+#pragma unroll
+    for (int f = 0; f < SPARC_NFIELDS; ++f) F.scr[f][q] = 0;
+    return;
+  }
+  const T jx = F.M[7][q] - F.M[11][q];
+  const T jy = -F.M[9][q] + F.M[8][q];
+  const T jz = F.M[6][q] - F.M[10][q];
+  const T bx = F.M[0][q], by = F.M[1][q], bz = F.M[2][q];
+  const T b0x = bgv(F, SPARC_BG_BX, c, k), b0y = bgv(F, SPARC_BG_BY, c, k),
+          b0z = bgv(F, SPARC_BG_BZ, c, k);
+  const T j0x = bgv(F, SPARC_BG_CURLBX, c, k), j0y = bgv(F, SPARC_BG_CURLBY, c, k),
+          j0z = bgv(F, SPARC_BG_CURLBZ, c, k);
+  const T ri = bgv(F, SPARC_BG_RHOINV, c, k);
+  T rx = ri * (-F.M[3][q] + (jy * b0z - jz * b0y)) + ri * (j0y * bz - j0z * by);
+  T ry = ri * (-F.M[4][q] + (b0x * jz - jx * b0z)) + ri * (bx * j0z - j0x * bz);
+  T rz = ri * (-F.M[5][q] + (jx * b0y - jy * b0x)) + ri * (j0x * by - j0y * bx);
+  if (k + 1 >= nz - 35) {
+    const T sp = sponge[k];
+    rx = rx - sp * vx;
+    ry = ry - sp * vy;
+    rz = rz - sp * vz;
+  }
+  rz = rz - ri * F.W[4][q] * F.g[k];
+  if (k == F.src_plane) rz = rz + F.forcing[c] / bgv(F, SPARC_BG_RHO0, c, k) * F.delta_width;
+  F.scr[SPARC_XI_X][q] = vx;
+  F.scr[SPARC_XI_Y][q] = vy;
+  F.scr[SPARC_XI_Z][q] = vz;
+  F.scr[SPARC_V_X][q] = rx;
+  F.scr[SPARC_V_Y][q] = ry;
+  F.scr[SPARC_V_Z][q] = rz;
 }
 
 // Flow-kernel sums for one forward velocity component j (kernels.f90:317-332):

@@ -31,8 +31,10 @@ struct sparc_ctx {
   void* scr[SPARC_NFIELDS] = {};
   void* alo[SPARC_NFIELDS] = {};  // low-order state words (compensated FP32 state)
   bool comp = false;
-  void* pmlvars[2] = {};
-  void* scrpml[2] = {};
+  void* pmlvars[6] = {};
+  void* scrpml[6] = {};
+  int npv = 2;         // psi fields: 2, or 6 for option 10
+  void* mw[12] = {};   // option 10 work arrays (fluxes, b, derivative pieces)
   void* bg[SPARC_BG_COUNT] = {};
   int bg3d[SPARC_BG_COUNT] = {};
   bool bgset[SPARC_BG_COUNT] = {};
@@ -210,8 +212,8 @@ static int check_config(const sparc_config* cfg, std::string* why) {
     return SPARC_EINVAL;
   }
   if (cfg->option != SPARC_OPT_QUIET_PML_3D_DISPL && cfg->option != SPARC_OPT_QUIET_PML_2D_DISPL &&
-      cfg->option != SPARC_OPT_QUIET_PML_3D) {
-    snprintf(b, sizeof b, "option %d not supported (2, 4 or 17)", cfg->option);
+      cfg->option != SPARC_OPT_QUIET_PML_3D && cfg->option != SPARC_OPT_MHD_PML_3D_DISPL) {
+    snprintf(b, sizeof b, "option %d not supported (2, 4, 10 or 17)", cfg->option);
     *why = b;
     return SPARC_EINVAL;
   }
@@ -273,7 +275,9 @@ static size_t footprint_bytes(const sparc_config* cfg) {
   size_t b = 0;
   b += 12 * N * ts;                         // a, scr
   if (cfg->precision == SPARC_FP32 && cfg->compensated) b += 6 * N * ts;  // alo
-  b += 4 * NP * ts;                         // pmlvars, scrpml
+  const size_t npv = cfg->option == SPARC_OPT_MHD_PML_3D_DISPL ? 6 : 2;
+  b += 2 * npv * NP * ts;                   // pmlvars, scrpml
+  if (cfg->option == SPARC_OPT_MHD_PML_3D_DISPL) b += 12 * N * ts;  // MHD work
   b += SPARC_BG_COUNT * N * ts;             // backgrounds (upper bound: all 3-D)
   b += 8 * N * ts;                          // work
   if (cfg->option != SPARC_OPT_QUIET_PML_2D_DISPL && cfg->damp_waves)
@@ -348,10 +352,14 @@ extern "C" int sparc_create(const sparc_config* cfg, sparc_ctx** out) {
     if ((rc = dalloc(c, &c->scr[f], N * ts))) return rc;
     if (c->comp && (rc = dalloc(c, &c->alo[f], N * ts))) return rc;
   }
-  for (int f = 0; f < 2; ++f) {
+  c->npv = cfg->option == SPARC_OPT_MHD_PML_3D_DISPL ? 6 : 2;
+  for (int f = 0; f < c->npv; ++f) {
     if ((rc = dalloc(c, &c->pmlvars[f], c->NP * ts))) return rc;
     if ((rc = dalloc(c, &c->scrpml[f], c->NP * ts))) return rc;
   }
+  if (cfg->option == SPARC_OPT_MHD_PML_3D_DISPL)
+    for (int w = 0; w < 12; ++w)
+      if ((rc = dalloc(c, &c->mw[w], N * ts))) return rc;
   for (int w = 0; w < 8; ++w)
     if ((rc = dalloc(c, &c->W[w], N * ts))) return rc;
   if (cfg->option != SPARC_OPT_QUIET_PML_2D_DISPL && cfg->damp_waves)
@@ -488,7 +496,7 @@ extern "C" int sparc_reset(sparc_ctx* c) {
     HIPCHK(c, hipMemsetAsync(c->scr[f], 0, c->N * c->ts, c->stream));
     if (c->comp) HIPCHK(c, hipMemsetAsync(c->alo[f], 0, c->N * c->ts, c->stream));
   }
-  for (int f = 0; f < 2; ++f) {
+  for (int f = 0; f < c->npv; ++f) {
     HIPCHK(c, hipMemsetAsync(c->pmlvars[f], 0, c->NP * c->ts, c->stream));
     HIPCHK(c, hipMemsetAsync(c->scrpml[f], 0, c->NP * c->ts, c->stream));
   }
@@ -590,10 +598,11 @@ static Fld<T> make_fld(sparc_ctx* c) {
     F.a[f] = (T*)c->a[f];
     F.scr[f] = (T*)c->scr[f];
   }
-  for (int f = 0; f < 2; ++f) {
+  for (int f = 0; f < c->npv; ++f) {
     F.pmlvars[f] = (T*)c->pmlvars[f];
     F.scrpml[f] = (T*)c->scrpml[f];
   }
+  for (int w = 0; w < 12; ++w) F.M[w] = (T*)c->mw[w];
   for (int b = 0; b < SPARC_BG_COUNT; ++b) {
     F.bg[b] = (const T*)c->bg[b];
     F.bg3d[b] = c->bg3d[b];
@@ -887,6 +896,85 @@ static int run_stage(sparc_ctx* c, Fld<T>& F, int s) {
     }
     if ((rc = launch_z<T, EPI_RHS3D>(c, F, 0, F.W[3], nullptr, 0, nullptr, 0, nullptr))) return rc;
     if (damp && (rc = launch_xm<T, XM_DAMP_INV>(c, 3, DX, 0, nullptr))) return rc;
+  } else if (c->cfg.option == SPARC_OPT_MHD_PML_3D_DISPL) {
+    // MP_MHD_PML_3D_DISPL (displacement.f90:663-849): 18 derivative passes
+    // (ddx, ddy, compact ddz) joined by pointwise kernels
+    const long N = c->N;
+    const dim3 gN(cdiv(N, 256)), b256(256);
+    const long NB = c->nxy * (long)c->cfg.npmlbot;
+    auto pmlb = [&](T* d, int w) -> int {
+      if (NB <= 0) return SPARC_OK;
+      ProfScope ps(c, SPARC_KC_OTHER, 5.0 * (double)NB * sizeof(T));
+      hipLaunchKernelGGL((k_pml_bottom<T>), dim3(cdiv(NB, 256)), b256, 0, c->stream, F, d, w, NB);
+      LAUNCHCHK(c);
+      return SPARC_OK;
+    };
+    auto dz = [&](const T* in, const T* s, T b, int ibc, T* out) -> int {
+      return launch_z<T, EPI_STORE>(c, F, ibc, in, s, b, nullptr, 0, out);
+    };
+    T** M = F.M;
+    {
+      ProfScope ps(c, SPARC_KC_OTHER, 9.0 * (double)N * sizeof(T));
+      hipLaunchKernelGGL((k_mhd_flux<T>), gN, b256, 0, c->stream, F, N);
+      LAUNCHCHK(c);
+    }
+    // div xi (bc=1) with psivz; p, rho
+    if ((rc = launch_x<T, XM_DERIV>(c, F.a[0], S(0), beta, F.W[0], dmx, nullptr))) return rc;
+    if ((rc = launch_y<T, YM_DERIV>(c, F.a[1], S(1), beta, F.W[1], dmy, nullptr))) return rc;
+    if ((rc = dz(F.a[2], S(2), beta, 1, F.W[2]))) return rc;
+    if ((rc = pmlb(F.W[2], 0))) return rc;
+    {
+      ProfScope ps(c, SPARC_KC_OTHER, 11.0 * (double)N * sizeof(T));
+      hipLaunchKernelGGL((k_mhd_pr<T>), gN, b256, 0, c->stream, F, N);
+      LAUNCHCHK(c);
+    }
+    // grad p (bc=0) with psip
+    if ((rc = launch_x<T, XM_DERIV>(c, F.W[3], nullptr, 0, M[3], dmx, nullptr))) return rc;
+    if ((rc = launch_y<T, YM_DERIV>(c, F.W[3], nullptr, 0, M[4], dmy, nullptr))) return rc;
+    if ((rc = dz(F.W[3], nullptr, 0, 0, M[5]))) return rc;
+    if ((rc = pmlb(M[5], 1))) return rc;
+    // b = curl(flux): dx f3, dy f3, dz f2 (psiinductionbx), dx f2, dy f1, dz f1 (psiinductionby)
+    if ((rc = launch_x<T, XM_DERIV>(c, M[2], nullptr, 0, M[6], dmx, nullptr))) return rc;
+    if ((rc = launch_y<T, YM_DERIV>(c, M[2], nullptr, 0, M[7], dmy, nullptr))) return rc;
+    if ((rc = dz(M[1], nullptr, 0, 1, M[8]))) return rc;
+    if ((rc = pmlb(M[8], 2))) return rc;
+    if ((rc = launch_x<T, XM_DERIV>(c, M[1], nullptr, 0, M[9], dmx, nullptr))) return rc;
+    if ((rc = launch_y<T, YM_DERIV>(c, M[0], nullptr, 0, M[10], dmy, nullptr))) return rc;
+    if ((rc = dz(M[0], nullptr, 0, 1, M[11]))) return rc;
+    if ((rc = pmlb(M[11], 3))) return rc;
+    {
+      ProfScope ps(c, SPARC_KC_OTHER, 9.0 * (double)N * sizeof(T));
+      hipLaunchKernelGGL((k_mhd_b<T>), gN, b256, 0, c->stream, F, N);
+      LAUNCHCHK(c);
+    }
+    // J = curl(b): dx by, dy bz, dz bx (psidzbx), dx bz, dy bx, dz by (psidzby)
+    if ((rc = launch_x<T, XM_DERIV>(c, M[1], nullptr, 0, M[6], dmx, nullptr))) return rc;
+    if ((rc = launch_y<T, YM_DERIV>(c, M[2], nullptr, 0, M[7], dmy, nullptr))) return rc;
+    if ((rc = dz(M[0], nullptr, 0, 1, M[8]))) return rc;
+    if ((rc = pmlb(M[8], 4))) return rc;
+    if ((rc = launch_x<T, XM_DERIV>(c, M[2], nullptr, 0, M[9], dmx, nullptr))) return rc;
+    if ((rc = launch_y<T, YM_DERIV>(c, M[0], nullptr, 0, M[10], dmy, nullptr))) return rc;
+    if ((rc = dz(M[1], nullptr, 0, 1, M[11]))) return rc;
+    if ((rc = pmlb(M[11], 5))) return rc;
+    const bool damp = c->cfg.damp_waves && c->damping_set;
+    XArgs<T> DX{};
+    for (int f = 0; f < 3; ++f) {
+      DX.a[f] = F.a[3 + f];
+      DX.s[f] = S(3 + f);
+      DX.out[f] = F.scr[3 + f];
+      DX.spec[f] = (C2<T>*)c->spec[f];
+    }
+    if (damp) {
+      if ((rc = launch_xm<T, XM_DAMP_FWD>(c, 3, DX, beta, nullptr))) return rc;
+      if ((rc = launch_ym<T, YM_DAMP>(c, 3, DX, 0, nullptr))) return rc;
+    }
+    {
+      ProfScope ps(c, SPARC_KC_OTHER, 25.0 * (double)N * sizeof(T));
+      hipLaunchKernelGGL((k_mhd_rhs<T>), gN, b256, 0, c->stream, F,
+                         (const T*)c->prof[SPARC_PROF_SPONGE], N);
+      LAUNCHCHK(c);
+    }
+    if (damp && (rc = launch_xm<T, XM_DAMP_INV>(c, 3, DX, 0, nullptr))) return rc;
   } else if (c->cfg.option == SPARC_OPT_QUIET_PML_3D) {
     // ddxyz(v_x, v_y, v_z, bc=1) (pml.f90:27); the z solve's epilogue forms
     // div v, RHScont and RHSp into W4, W5 (pml.f90:36-53)
@@ -969,7 +1057,7 @@ static int step_impl(sparc_ctx* c, int t0, int nsteps) {
       LAUNCHCHK(c);
     }
     if (c->nzpml > 0)
-      for (int f = 0; f < 2; ++f) {
+      for (int f = 0; f < c->npv; ++f) {
         ProfScope ps(c, SPARC_KC_OTHER, 3.0 * (double)c->NP * sizeof(T));
         hipLaunchKernelGGL((k_axpy<T>), dim3(cdiv(c->NP, 256)), dim3(256), 0, c->stream,
                            F.pmlvars[f], (const T*)F.scrpml[f], dt, c->NP);
@@ -1016,6 +1104,12 @@ extern "C" int sparc_step(sparc_ctx* c, int t0, int nsteps) {
   if (c->cfg.option == SPARC_OPT_QUIET_PML_3D && c->src_kind != SPARC_SRC_NONE &&
       !c->profset[SPARC_PROF_SOURCE_DEP])
     return fail(c, SPARC_ESTATE, "option 2 with a source needs the source_dep profile");
+  if (c->cfg.option == SPARC_OPT_MHD_PML_3D_DISPL) {
+    for (int id = SPARC_BG_BX; id <= SPARC_BG_GRADRHO0_Y; ++id)
+      if (!c->bgset[id]) return fail(c, SPARC_ESTATE, "option 10 needs background %d", id);
+    if (!c->profset[SPARC_PROF_SPONGE])
+      return fail(c, SPARC_ESTATE, "option 10 needs the sponge profile");
+  }
   return c->fp32 ? step_impl<float>(c, t0, nsteps) : step_impl<double>(c, t0, nsteps);
 }
 
@@ -1033,7 +1127,7 @@ static void* arr_ptr(sparc_ctx* c, int arr, int field, size_t* n) {
     *n = c->N;
     return arr == SPARC_ARR_A ? c->a[field] : c->scr[field];
   }
-  if ((arr == SPARC_ARR_PMLVARS || arr == SPARC_ARR_SCRPML) && field >= 0 && field < 2) {
+  if ((arr == SPARC_ARR_PMLVARS || arr == SPARC_ARR_SCRPML) && field >= 0 && field < c->npv) {
     *n = c->NP;
     return arr == SPARC_ARR_PMLVARS ? c->pmlvars[field] : c->scrpml[field];
   }
@@ -1312,7 +1406,7 @@ extern "C" int sparc_checkpoints_alloc(sparc_ctx* c, int nslots) {
   if (!c || nslots < 0) return SPARC_EINVAL;
   if (nslots <= c->nckpt) return SPARC_OK;
   void* p;
-  const size_t per = ((c->comp ? 12 : 6) * (size_t)c->N + 2 * (size_t)c->NP) * c->ts;
+  const size_t per = ((c->comp ? 12 : 6) * (size_t)c->N + c->npv * (size_t)c->NP) * c->ts;
   int rc = dalloc(c, &p, (size_t)nslots * per);
   if (rc) return rc;
   c->ckpt = p;
@@ -1324,14 +1418,14 @@ static int ckpt_copy(sparc_ctx* c, int slot, bool save) {
   if (!c || slot < 0 || slot >= c->nckpt) return fail(c, SPARC_EINVAL, "bad checkpoint %d", slot);
   const size_t fb = c->N * c->ts, pb = c->NP * c->ts;
   const int nf = c->comp ? 2 * SPARC_NFIELDS : SPARC_NFIELDS;
-  char* base = (char*)c->ckpt + (size_t)slot * (nf * fb + 2 * pb);
+  char* base = (char*)c->ckpt + (size_t)slot * (nf * fb + c->npv * pb);
   for (int f = 0; f < nf; ++f) {
     void* dev = f < SPARC_NFIELDS ? c->a[f] : c->alo[f - SPARC_NFIELDS];
     void* d = save ? (void*)(base + f * fb) : dev;
     const void* s = save ? (const void*)dev : (const void*)(base + f * fb);
     HIPCHK(c, hipMemcpyAsync(d, s, fb, hipMemcpyDeviceToDevice, c->stream));
   }
-  for (int f = 0; f < 2; ++f) {
+  for (int f = 0; f < c->npv; ++f) {
     char* pp = base + nf * fb + f * pb;
     void* d = save ? (void*)pp : c->pmlvars[f];
     const void* s = save ? (const void*)c->pmlvars[f] : (const void*)pp;

@@ -318,3 +318,86 @@ def test_option2_random_state_matches_oracle(tiny_rhovp):
     finally:
         eng.close()
         o.close()
+
+
+def _mhd_case(prec=abi.FP64, nsteps=6):
+    """The tiny polytrope box with option 10: bottom PML only, a smooth 3-D
+    field B0 with v_A ~ 0.3 c, its curl, horizontal background gradients and a
+    top sponge ramp (the reference's sponge is zero on boxes of <= 90 planes,
+    so a ramp exercises the term here)."""
+    from dataclasses import replace
+    W0 = D.workload("tiny")
+    P = replace(W0.params, option=abi.OPT_MHD, npmltop=0, precision=prec)
+    W = D.Workload("tiny", P, W0.tab, W0.nsteps)
+    S = D.setup_for(W)
+    eng, cfg = D.make_engine(W, S, precision=prec, nsteps=nsteps)
+    o, _ = make_oracle(W, S, nsteps=nsteps)
+    nx, ny, nz = eng.shape
+    x = np.arange(nx)[:, None, None] / nx
+    y = np.arange(ny)[None, :, None] / ny
+    zz = np.arange(nz)[None, None, :] / nz
+    scale = 0.3 * np.sqrt(S.c2 * S.rho0)
+    bgs = {
+        abi.BG_BX: scale * (0.5 + 0.3 * np.cos(2 * np.pi * y)),
+        abi.BG_BY: scale * 0.4 * np.sin(2 * np.pi * x) * np.cos(np.pi * zz),
+        abi.BG_BZ: scale * (0.8 + 0.2 * np.cos(2 * np.pi * (x + y))),
+        abi.BG_CURLBX: scale * 0.1 * np.sin(2 * np.pi * (x - y)),
+        abi.BG_CURLBY: scale * 0.1 * np.cos(2 * np.pi * x),
+        abi.BG_CURLBZ: scale * 0.1 * np.sin(2 * np.pi * y),
+    }
+    gp = np.broadcast_to(S.derived["gradp0_z"], (nx, ny, nz))
+    gr = np.broadcast_to(S.derived["gradrho0_z"], (nx, ny, nz))
+    bgs[abi.BG_GRADP0_X] = 0.05 * gp * np.sin(2 * np.pi * x)
+    bgs[abi.BG_GRADP0_Y] = 0.05 * gp * np.cos(2 * np.pi * y)
+    bgs[abi.BG_GRADRHO0_X] = 0.05 * gr * np.cos(2 * np.pi * x)
+    bgs[abi.BG_GRADRHO0_Y] = 0.05 * gr * np.sin(2 * np.pi * y)
+    sponge = np.zeros(nz)
+    sponge[nz - 12:] = np.linspace(0.0, 50.0, 12)
+    for e in (eng, o):
+        for bid, arr in bgs.items():
+            e.set_background(bid, np.broadcast_to(arr, (nx, ny, nz)))
+        e.set_profile(abi.PROF_SPONGE, sponge)
+    return W, S, eng, o
+
+
+@pytest.mark.parametrize("prec", [abi.FP64, abi.FP32])
+def test_step_option10_matches_oracle(prec):
+    """Option 10 (linear MHD, displacement.f90:663-849): flux, curl, Lorentz
+    force, bottom-only PML on six psi fields, top sponge, damping, source."""
+    W, S, eng, o = _mhd_case(prec, nsteps=14)
+    try:
+        mask, series = D.forward_source(W, 22)
+        for x in (eng, o):
+            x.set_source(abi.SRC_SERIES, mask, series, 10 ** 9)
+            x.step(0, 14)
+        for f in range(6):
+            a, b = eng.get(abi.ARR_A, f), o.get(abi.ARR_A, f)
+            assert np.linalg.norm(b) > 0, f
+            if prec == abi.FP64:
+                assert rel_err(a, b) < TOL64, f
+            else:
+                assert rel_l2(a, b) < TOL32_L2, f
+        for f in range(6):
+            a, b = eng.get(abi.ARR_PMLVARS, f), o.get(abi.ARR_PMLVARS, f)
+            assert np.linalg.norm(b) > 0, f
+            assert (rel_err(a, b) < TOL64) if prec == abi.FP64 else (rel_l2(a, b) < 1e-4)
+    finally:
+        eng.close()
+        o.close()
+
+
+def test_option10_random_state_matches_oracle():
+    """A random state everywhere (sponge and PML planes included), 3 steps."""
+    W, S, eng, o = _mhd_case(abi.FP64, nsteps=3)
+    try:
+        for f in range(6):
+            xr = _field(eng.shape, 60 + f) * 1e-4
+            eng.put(abi.ARR_A, f, xr)
+            o.put(abi.ARR_A, f, xr)
+        eng.step(1, 3)
+        o.step(1, 3)
+        for f in range(6):
+            assert rel_err(eng.get(abi.ARR_A, f), o.get(abi.ARR_A, f)) < TOL64, f
+    finally:
+        eng.close()
+        o.close()
