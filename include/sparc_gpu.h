@@ -46,6 +46,7 @@ extern "C" {
 #define SPARC_OPT_QUIET_PML_3D_DISPL 4   /* displacement.f90:482-658 */
 #define SPARC_OPT_MHD_PML_3D_DISPL 10    /* displacement.f90:663-849 */
 #define SPARC_OPT_QUIET_PML_2D_DISPL 17  /* physics2d.f90:948-1105   */
+#define SPARC_OPT_MHD_PML_2D_DISPL 18    /* physics2d.f90:724-943    */
 
 /* solve direction, compute_forward / compute_adjoint (initialize.f90:1006-1045) */
 #define SPARC_FORWARD 0

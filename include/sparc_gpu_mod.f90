@@ -19,7 +19,8 @@ module sparc_gpu
                                SPARC_ENOMEM = -3, SPARC_ESTATE = -4
   integer(c_int), parameter :: SPARC_FP64 = 0, SPARC_FP32 = 1
   integer(c_int), parameter :: SPARC_OPT_QUIET_PML_3D = 2, SPARC_OPT_QUIET_PML_3D_DISPL = 4, &
-                               SPARC_OPT_MHD_PML_3D_DISPL = 10, SPARC_OPT_QUIET_PML_2D_DISPL = 17
+                               SPARC_OPT_MHD_PML_3D_DISPL = 10, SPARC_OPT_QUIET_PML_2D_DISPL = 17, &
+                               SPARC_OPT_MHD_PML_2D_DISPL = 18
   integer(c_int), parameter :: SPARC_RHO = 0, SPARC_P = 4
   integer(c_int), parameter :: SPARC_FORWARD = 0, SPARC_ADJOINT = 1
   integer(c_int), parameter :: SPARC_XI_X = 0, SPARC_XI_Y = 1, SPARC_XI_Z = 2, &

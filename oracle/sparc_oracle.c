@@ -419,12 +419,13 @@ orc *orc_create(const sparc_config *cfg) {
   o->a = zalloc(sizeof(double) * 6 * N);
   o->temp = zalloc(sizeof(double) * 6 * N);
   o->scr = zalloc(sizeof(double) * 6 * N);
-  o->npv = (cfg->option == SPARC_OPT_MHD_PML_3D_DISPL) ? 6 : 2;
+  int mhd = cfg->option == SPARC_OPT_MHD_PML_3D_DISPL || cfg->option == SPARC_OPT_MHD_PML_2D_DISPL;
+  o->npv = mhd ? 6 : 2;
   o->pmlvars = zalloc(sizeof(double) * o->npv * o->NP);
   o->psipml = zalloc(sizeof(double) * o->npv * o->NP);
   o->scrpml = zalloc(sizeof(double) * o->npv * o->NP);
   for (int q = 0; q < 12; q++)
-    o->mw[q] = (cfg->option == SPARC_OPT_MHD_PML_3D_DISPL) ? zalloc(sizeof(double) * N) : NULL;
+    o->mw[q] = mhd ? zalloc(sizeof(double) * N) : NULL;
   for (int b = 0; b < SPARC_BG_COUNT; b++) o->bg[b] = zalloc(sizeof(double) * N);
   for (int b = 0; b < SPARC_PROF_COUNT; b++) o->prof[b] = zalloc(sizeof(double) * o->nz);
   o->damping_rates = zalloc(sizeof(double) * (o->nx / 2 + 1) * o->ny);
@@ -929,6 +930,83 @@ static void rhs_option10(orc *o) {
     }
 }
 
+/* MP_MHD_PML_2D_DISPL, option 18 (physics2d.f90:724-943), ny = 1, kernel mode,
+ * sources at the planes e_rad/o_rad (erad_2d uniform). */
+static void rhs_option18(orc *o) {
+  int nx = o->nx, ny = o->ny, nz = o->nz;
+  long N = o->N, nxy = (long)nx * ny;
+  double *xi_x = o->temp, *xi_z = o->temp + 2 * N;
+  double *v_x = o->temp + 3 * N, *v_z = o->temp + 5 * N;
+  double **bg = o->bg, **m = o->mw;
+  const double *st = o->prof[SPARC_PROF_STRETCH], *unstretch = o->prof[SPARC_PROF_UNSTRETCH];
+  double sx = o->cfg.stretchx;
+  /* ddx(xi_x); dxizdz(nz) preset; ddz(xi_z, 3); psivz (:735-752) */
+  orc_ddx(xi_x, o->dxdx, nx, ny, nz, sx);
+  for (long c = 0; c < nxy; c++)
+    o->dzdz[c + nxy * (nz - 1)] = -o->dxdx[c + nxy * (nz - 1)] * unstretch[nz - 1];
+  orc_ddz(xi_z, o->dzdz, nx, ny, nz, 3, st);
+  pml_account(o, o->dzdz, 0);
+  for (long c = 0; c < N; c++) {
+    double div = o->dxdx[c] + o->dzdz[c];
+    o->p[c] = -bg[SPARC_BG_C2RHO0][c] * div - xi_z[c] * bg[SPARC_BG_GRADP0_Z][c] -
+              xi_x[c] * bg[SPARC_BG_GRADP0_X][c];
+    o->rho[c] = -bg[SPARC_BG_RHO0][c] * div - xi_z[c] * bg[SPARC_BG_GRADRHO0_Z][c] -
+                xi_x[c] * bg[SPARC_BG_GRADRHO0_X][c];
+  }
+  /* ddx(p); ddz(p, 2) with gradp_z(:,:,1) the caller's (zero) value; psip (:775-791) */
+  orc_ddx(o->p, o->gpx, nx, ny, nz, sx);
+  for (long c = 0; c < nxy; c++) o->gpz[c] = 0.0;
+  orc_ddz(o->p, o->gpz, nx, ny, nz, 2, st);
+  pml_account(o, o->gpz, 1);
+  /* flux2, bx = -(ddz(flux2, 0) + psiinductionbx), bz = ddx(flux2) (:815-845) */
+  double *f2 = m[0], *bx = m[1], *bz = m[2], *dzbx = m[3], *cby = m[4];
+  for (long c = 0; c < N; c++)
+    f2[c] = -(xi_x[c] * bg[SPARC_BG_BZ][c] - xi_z[c] * bg[SPARC_BG_BX][c]);
+  orc_ddz(f2, bx, nx, ny, nz, 0, st);
+  orc_ddx(f2, bz, nx, ny, nz, sx);
+  pml_account(o, bx, 2);
+  for (long c = 0; c < N; c++) bx[c] = -bx[c];
+  /* dzbx = ddz(bx, 0) + psidzbx, curlby = -ddx(bz) + dzbx (:852-873) */
+  orc_ddz(bx, dzbx, nx, ny, nz, 0, st);
+  orc_ddx(bz, cby, nx, ny, nz, sx);
+  pml_account(o, dzbx, 4);
+  double *RHSxi_x = o->scr, *RHSxi_z = o->scr + 2 * N;
+  double *RHSv_x = o->scr + 3 * N, *RHSv_z = o->scr + 5 * N;
+  const double *rhoinv = bg[SPARC_BG_RHOINV], *g = o->prof[SPARC_PROF_G];
+  for (int k = 0; k < nz; k++)
+    for (long c = 0; c < nxy; c++) {
+      long q = c + nxy * k;
+      double curlby = -cby[q] + dzbx[q];
+      double fl1 = curlby * bg[SPARC_BG_BZ][q], fl3 = -curlby * bg[SPARC_BG_BX][q];
+      double rx = rhoinv[q] * (-o->gpx[q] + fl1);
+      double rz = rhoinv[q] * (-o->gpz[q] + fl3);
+      fl1 = bg[SPARC_BG_CURLBY][q] * bz[q];
+      fl3 = -bg[SPARC_BG_CURLBY][q] * bx[q];
+      rx = rx + rhoinv[q] * fl1;
+      rz = rz + rhoinv[q] * fl3;
+      rz = rz - rhoinv[q] * o->rho[q] * g[k];
+      RHSv_x[q] = rx;
+      RHSv_z[q] = rz;
+      RHSxi_x[q] = v_x[q];
+      RHSxi_z[q] = v_z[q];
+    }
+  /* DAMP_VELOCITY, 2-D branch: v_x and v_z (damping.f90:162-178) */
+  if (o->cfg.damp_waves) {
+    damp_field(o, v_x, RHSv_x);
+    damp_field(o, v_z, RHSv_z);
+  }
+  if (o->src_kind != SPARC_SRC_NONE && o->time <= o->src_active_until) {
+    int kk = (o->cfg.mode == SPARC_ADJOINT) ? o->cfg.o_rad - 1 : o->cfg.e_rad - 1;
+    for (long c = 0; c < nxy; c++) {
+      long q = c + nxy * kk;
+      RHSv_z[q] = RHSv_z[q] + o->forcing[c] * o->cfg.delta_width / bg[SPARC_BG_RHO0][q];
+    }
+  }
+  /* only plane 1 is zeroed (:930-933) */
+  for (int f = 0; f < 6; f++)
+    for (long c = 0; c < nxy; c++) o->scr[f * N + c] = 0.0;
+}
+
 /* MP_QUIET_PML_2D_DISPL, option 17 (physics2d.f90:948-1105), ny = 1. */
 static void rhs_option17(orc *o) {
   int nx = o->nx, ny = o->ny, nz = o->nz;
@@ -1026,7 +1104,8 @@ void orc_zfilter(double *f, int nx, int ny, int nz, int kst) {
 static void filter_vars(orc *o) {
   int nx = o->nx, ny = o->ny, nz = o->nz;
   long N = o->N;
-  int is2d = (o->cfg.option == SPARC_OPT_QUIET_PML_2D_DISPL);
+  int is2d = (o->cfg.option == SPARC_OPT_QUIET_PML_2D_DISPL ||
+              o->cfg.option == SPARC_OPT_MHD_PML_2D_DISPL);
   for (int l = 0; l < 6; l++) zfilter_field(o->a + l * N, nx, ny, nz, is2d ? 6 : 2, o->term);
   int ff = o->cfg.freq_filtering;
   if (ff <= 0) return;
@@ -1061,6 +1140,8 @@ static void step(orc *o) {
       rhs_option2(o);
     else if (o->cfg.option == SPARC_OPT_MHD_PML_3D_DISPL)
       rhs_option10(o);
+    else if (o->cfg.option == SPARC_OPT_MHD_PML_2D_DISPL)
+      rhs_option18(o);
     else
       rhs_option17(o);
   }

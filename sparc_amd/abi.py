@@ -20,6 +20,7 @@ FP64, FP32 = 0, 1
 OPT_3D, OPT_2D = 4, 17
 OPT_RHOVP = 2          # (rho, v, p) form, forward only
 OPT_MHD = 10           # linear MHD, displacement form
+OPT_MHD2D = 18         # linear MHD, displacement form, ny = 1
 RHO, P = 0, 4          # option 2 slots of rho and p (v in 1..3)
 FORWARD, ADJOINT = 0, 1
 XI_X, XI_Y, XI_Z, V_X, V_Y, V_Z = range(6)

@@ -568,6 +568,82 @@ __global__ void k_mhd_rhs(Fld<T> F, const T* __restrict__ sponge, long n) {
   F.scr[SPARC_V_Z][q] = rz;
 }
 
+// Option 18 (MP_MHD_PML_2D_DISPL, physics2d.f90:724-943), ny = 1.
+// PML on every PML plane (top and bottom): psi field w on derivative d.
+template <typename T>
+__global__ void k_pml_planes(Fld<T> F, T* __restrict__ d, int w, long n) {
+  const long q = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (q >= n) return;
+  const int k = (int)(q / F.nxy), c = (int)(q - (long)k * F.nxy);
+  const int pi = pml_index(k, F.nz, F.npmlbot, F.npmltop);
+  if (pi < 0) return;
+  const long pq = c + (long)F.nxy * pi;
+  T psi = F.pmlvars[w][pq];
+  if (F.beta != (T)0) psi = psi + F.beta * F.scrpml[w][pq];
+  const T dv = d[q];
+  F.scrpml[w][pq] = bgv(F, SPARC_BG_AZ, c, k) * dv + bgv(F, SPARC_BG_BZPML, c, k) * psi;
+  d[q] = dv + psi;
+}
+
+// div xi = W0 + W2 -> p (W3), rho (W4); flux2 = -(xi_x boz - xi_z box) -> M0
+// (physics2d.f90:767-769,815)
+template <typename T>
+__global__ void k_mhd2_pr(Fld<T> F, long n) {
+  const long q = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (q >= n) return;
+  const int k = (int)(q / F.nxy), c = (int)(q - (long)k * F.nxy);
+  const T xx = tmp_of(F, SPARC_XI_X, q), xz = tmp_of(F, SPARC_XI_Z, q);
+  const T div = F.W[0][q] + F.W[2][q];
+  F.W[3][q] = -bgv(F, SPARC_BG_C2RHO0, c, k) * div - xz * bgv(F, SPARC_BG_GRADP0_Z, c, k) -
+              xx * bgv(F, SPARC_BG_GRADP0_X, c, k);
+  F.W[4][q] = -bgv(F, SPARC_BG_RHO0, c, k) * div - xz * bgv(F, SPARC_BG_GRADRHO0_Z, c, k) -
+              xx * bgv(F, SPARC_BG_GRADRHO0_X, c, k);
+  F.M[0][q] = -(xx * bgv(F, SPARC_BG_BZ, c, k) - xz * bgv(F, SPARC_BG_BX, c, k));
+}
+
+template <typename T>
+__global__ void k_negate(T* __restrict__ d, long n) {
+  const long q = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (q < n) d[q] = -d[q];
+}
+
+// curlby = -ddx(bz) + dzbx; RHSv = rhoinv (-grad p + curlby x B0 + curlB0 x b),
+// gravity, RHSxi = v, source; plane 1 zeroed (physics2d.f90:876-933).  gradp_x
+// M6, gradp_z M7, bx M1, bz M2, dzbx M3, ddx(bz) M4, rho W4.
+template <typename T>
+__global__ void k_mhd2_rhs(Fld<T> F, long n) {
+  const long q = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (q >= n) return;
+  const int k = (int)(q / F.nxy), c = (int)(q - (long)k * F.nxy);
+  const T vx = tmp_of(F, SPARC_V_X, q), vz = tmp_of(F, SPARC_V_Z, q);
+  if (k == 0) {
+#pragma unroll
+    for (int f = 0; f < SPARC_NFIELDS; ++f) F.scr[f][q] = 0;
+    return;
+  }
+  const T ri = bgv(F, SPARC_BG_RHOINV, c, k);
+  const T cby = -F.M[4][q] + F.M[3][q];
+  const T j0y = bgv(F, SPARC_BG_CURLBY, c, k);
+  T rx = ri * (-F.M[6][q] + cby * bgv(F, SPARC_BG_BZ, c, k));
+  T rz = ri * (-F.M[7][q] + (-cby * bgv(F, SPARC_BG_BX, c, k)));
+  rx = rx + ri * (j0y * F.M[2][q]);
+  rz = rz + ri * (-j0y * F.M[1][q]);
+  rz = rz - ri * F.W[4][q] * F.g[k];
+  if (k == F.src_plane) rz = rz + F.forcing[c] * F.delta_width / bgv(F, SPARC_BG_RHO0, c, k);
+  F.scr[SPARC_XI_X][q] = vx;
+  F.scr[SPARC_XI_Z][q] = vz;
+  F.scr[SPARC_V_X][q] = rx;
+  F.scr[SPARC_V_Z][q] = rz;
+}
+
+// out[q] -= term[q] for planes 2..nz (damping applied before plane 1 is zeroed)
+template <typename T>
+__global__ void k_sub_above_plane1(T* __restrict__ out, const T* __restrict__ term, long nxy,
+                                   long n) {
+  const long q = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (q >= nxy && q < n) out[q] = out[q] - term[q];
+}
+
 // Flow-kernel sums for one forward velocity component j (kernels.f90:317-332):
 // K_v(i) += d_i v_f,j * v_a,j with d_x, d_y, d_z v_f,j given (dy == nullptr: 0).
 template <typename T>

@@ -34,7 +34,8 @@ struct sparc_ctx {
   void* pmlvars[6] = {};
   void* scrpml[6] = {};
   int npv = 2;         // psi fields: 2, or 6 for option 10
-  void* mw[12] = {};   // option 10 work arrays (fluxes, b, derivative pieces)
+  void* mw[12] = {};   // option 10/18 work arrays (fluxes, b, derivative pieces)
+  void* dmul2 = nullptr;  // option 18: x-multipliers of the 2-D damping
   void* bg[SPARC_BG_COUNT] = {};
   int bg3d[SPARC_BG_COUNT] = {};
   bool bgset[SPARC_BG_COUNT] = {};
@@ -204,6 +205,13 @@ static size_t spec_elems(const sparc_config* cfg) {
   return (size_t)((cfg->nz + 1) / 2) * cfg->nx * cfg->ny;
 }
 
+static bool is_mhd(int option) {
+  return option == SPARC_OPT_MHD_PML_3D_DISPL || option == SPARC_OPT_MHD_PML_2D_DISPL;
+}
+static bool is_2d(int option) {
+  return option == SPARC_OPT_QUIET_PML_2D_DISPL || option == SPARC_OPT_MHD_PML_2D_DISPL;
+}
+
 static int check_config(const sparc_config* cfg, std::string* why) {
   char b[256];
   if (cfg->abi_version != SPARC_ABI_VERSION) {
@@ -212,8 +220,8 @@ static int check_config(const sparc_config* cfg, std::string* why) {
     return SPARC_EINVAL;
   }
   if (cfg->option != SPARC_OPT_QUIET_PML_3D_DISPL && cfg->option != SPARC_OPT_QUIET_PML_2D_DISPL &&
-      cfg->option != SPARC_OPT_QUIET_PML_3D && cfg->option != SPARC_OPT_MHD_PML_3D_DISPL) {
-    snprintf(b, sizeof b, "option %d not supported (2, 4, 10 or 17)", cfg->option);
+      cfg->option != SPARC_OPT_QUIET_PML_3D && !is_mhd(cfg->option)) {
+    snprintf(b, sizeof b, "option %d not supported (2, 4, 10, 17 or 18)", cfg->option);
     *why = b;
     return SPARC_EINVAL;
   }
@@ -230,15 +238,15 @@ static int check_config(const sparc_config* cfg, std::string* why) {
     *why = "nx must be 2^a*3^b";
     return SPARC_EINVAL;
   }
-  if (cfg->option == SPARC_OPT_QUIET_PML_2D_DISPL && cfg->ny != 1) {
-    *why = "option 17 needs ny == 1";
+  if (is_2d(cfg->option) && cfg->ny != 1) {
+    *why = "the 2-D options need ny == 1";
     return SPARC_EINVAL;
   }
   if (cfg->option == SPARC_OPT_QUIET_PML_3D && cfg->mode != SPARC_FORWARD) {
     *why = "option 2 is forward-only (the kernels need the displacement formulation)";
     return SPARC_EINVAL;
   }
-  if (cfg->option != SPARC_OPT_QUIET_PML_2D_DISPL) {
+  if (!is_2d(cfg->option)) {
     if (!make_plan(cfg->ny, &p) || cfg->ny < 2) {
       *why = "ny must be 2^a*3^b >= 2 for the 3-D options";
       return SPARC_EINVAL;
@@ -275,12 +283,12 @@ static size_t footprint_bytes(const sparc_config* cfg) {
   size_t b = 0;
   b += 12 * N * ts;                         // a, scr
   if (cfg->precision == SPARC_FP32 && cfg->compensated) b += 6 * N * ts;  // alo
-  const size_t npv = cfg->option == SPARC_OPT_MHD_PML_3D_DISPL ? 6 : 2;
+  const size_t npv = is_mhd(cfg->option) ? 6 : 2;
   b += 2 * npv * NP * ts;                   // pmlvars, scrpml
-  if (cfg->option == SPARC_OPT_MHD_PML_3D_DISPL) b += 12 * N * ts;  // MHD work
+  if (is_mhd(cfg->option)) b += 12 * N * ts + cfg->nx * ts;  // MHD work, 2-D damping
   b += SPARC_BG_COUNT * N * ts;             // backgrounds (upper bound: all 3-D)
   b += 8 * N * ts;                          // work
-  if (cfg->option != SPARC_OPT_QUIET_PML_2D_DISPL && cfg->damp_waves)
+  if (!is_2d(cfg->option) && cfg->damp_waves)
     b += 3 * spec_elems(cfg) * 2 * ts;      // damping spectra
   b += (size_t)(cfg->nx + cfg->ny) * 6 * ts + 4 * 3 * cfg->nz * ts + SPARC_PROF_COUNT * cfg->nz * ts;
   b += (size_t)(cfg->nx / 2 + 1) * cfg->ny * ts + 2 * nxy * ts;
@@ -352,17 +360,19 @@ extern "C" int sparc_create(const sparc_config* cfg, sparc_ctx** out) {
     if ((rc = dalloc(c, &c->scr[f], N * ts))) return rc;
     if (c->comp && (rc = dalloc(c, &c->alo[f], N * ts))) return rc;
   }
-  c->npv = cfg->option == SPARC_OPT_MHD_PML_3D_DISPL ? 6 : 2;
+  c->npv = is_mhd(cfg->option) ? 6 : 2;
   for (int f = 0; f < c->npv; ++f) {
     if ((rc = dalloc(c, &c->pmlvars[f], c->NP * ts))) return rc;
     if ((rc = dalloc(c, &c->scrpml[f], c->NP * ts))) return rc;
   }
-  if (cfg->option == SPARC_OPT_MHD_PML_3D_DISPL)
+  if (is_mhd(cfg->option)) {
     for (int w = 0; w < 12; ++w)
       if ((rc = dalloc(c, &c->mw[w], N * ts))) return rc;
+    if ((rc = dalloc(c, &c->dmul2, cfg->nx * ts))) return rc;
+  }
   for (int w = 0; w < 8; ++w)
     if ((rc = dalloc(c, &c->W[w], N * ts))) return rc;
-  if (cfg->option != SPARC_OPT_QUIET_PML_2D_DISPL && cfg->damp_waves)
+  if (!is_2d(cfg->option) && cfg->damp_waves)
     for (int f = 0; f < 3; ++f)
       if ((rc = dalloc(c, &c->spec[f], spec_elems(cfg) * 2 * ts))) return rc;
   for (int p = 0; p < SPARC_PROF_COUNT; ++p)
@@ -542,6 +552,13 @@ extern "C" int sparc_set_damping(sparc_ctx* c, const double* r, size_t n) {
   if (!c || !r || n != (size_t)(c->cfg.nx / 2 + 1) * c->cfg.ny)
     return fail(c, SPARC_EINVAL, "damping rates need (nx/2+1)*ny values");
   c->damping_set = true;
+  if (c->dmul2 && c->cfg.ny == 1) {  // DAMP_VELOCITY's 2-D transform is along x only
+    const int nx = c->cfg.nx;
+    std::vector<double> t(nx);
+    for (int m = 0; m < nx; ++m) t[m] = r[(m <= nx / 2) ? m : nx - m];
+    int rc = upload(c, c->dmul2, t.data(), nx);
+    if (rc) return rc;
+  }
   return upload(c, c->rates, r, n);
 }
 
@@ -975,6 +992,68 @@ static int run_stage(sparc_ctx* c, Fld<T>& F, int s) {
       LAUNCHCHK(c);
     }
     if (damp && (rc = launch_xm<T, XM_DAMP_INV>(c, 3, DX, 0, nullptr))) return rc;
+  } else if (c->cfg.option == SPARC_OPT_MHD_PML_2D_DISPL) {
+    // MP_MHD_PML_2D_DISPL (physics2d.f90:724-943), ny = 1
+    const long N = c->N;
+    const dim3 gN(cdiv(N, 256)), b256(256);
+    const int nz = c->cfg.nz;
+    T** M = F.M;
+    auto pmla = [&](T* d, int w) -> int {
+      if (c->nzpml <= 0) return SPARC_OK;
+      ProfScope ps(c, SPARC_KC_OTHER, 5.0 * (double)c->NP * sizeof(T));
+      hipLaunchKernelGGL((k_pml_planes<T>), gN, b256, 0, c->stream, F, d, w, N);
+      LAUNCHCHK(c);
+      return SPARC_OK;
+    };
+    auto dz = [&](const T* in, const T* s, T b, int ibc, const T* prov, T ps, T* out) -> int {
+      return launch_z<T, EPI_STORE>(c, F, ibc, in, s, b, prov, ps, out);
+    };
+    // ddx(xi_x); ddz(xi_z, 3) with dxizdz(nz) = -dxixdx(nz)*unstretch(nz); psivz
+    if ((rc = launch_x<T, XM_DERIV>(c, F.a[0], S(0), beta, F.W[0], dmx, nullptr))) return rc;
+    const T us = (T)(-c->unstretch_h[nz - 1]);
+    if ((rc = dz(F.a[2], S(2), beta, 3, F.W[0] + c->nxy * (nz - 1), us, F.W[2]))) return rc;
+    if ((rc = pmla(F.W[2], 0))) return rc;
+    {
+      ProfScope ps(c, SPARC_KC_OTHER, 10.0 * (double)N * sizeof(T));
+      hipLaunchKernelGGL((k_mhd2_pr<T>), gN, b256, 0, c->stream, F, N);
+      LAUNCHCHK(c);
+    }
+    // ddx(p); ddz(p, 2) with gradp_z(:,:,1) = 0; psip
+    if ((rc = launch_x<T, XM_DERIV>(c, F.W[3], nullptr, 0, M[6], dmx, nullptr))) return rc;
+    if ((rc = dz(F.W[3], nullptr, 0, 2, nullptr, 0, M[7]))) return rc;
+    if ((rc = pmla(M[7], 1))) return rc;
+    // bx = -(ddz(flux2, 0) + psiinductionbx), bz = ddx(flux2)
+    if ((rc = dz(M[0], nullptr, 0, 0, nullptr, 0, M[1]))) return rc;
+    if ((rc = launch_x<T, XM_DERIV>(c, M[0], nullptr, 0, M[2], dmx, nullptr))) return rc;
+    if ((rc = pmla(M[1], 2))) return rc;
+    hipLaunchKernelGGL((k_negate<T>), gN, b256, 0, c->stream, M[1], N);
+    LAUNCHCHK(c);
+    // dzbx = ddz(bx, 0) + psidzbx; ddx(bz)
+    if ((rc = dz(M[1], nullptr, 0, 0, nullptr, 0, M[3]))) return rc;
+    if ((rc = launch_x<T, XM_DERIV>(c, M[2], nullptr, 0, M[4], dmx, nullptr))) return rc;
+    if ((rc = pmla(M[3], 4))) return rc;
+    // DAMP_VELOCITY 2-D (v_x, v_z along x) on the stage's temp v, before RHSv is written
+    const bool damp = c->cfg.damp_waves && c->damping_set;
+    if (damp) {
+      if ((rc = launch_x<T, XM_FILTER>(c, F.a[3], S(3), beta, F.W[5], (const T*)c->dmul2,
+                                       nullptr)))
+        return rc;
+      if ((rc = launch_x<T, XM_FILTER>(c, F.a[5], S(5), beta, F.W[6], (const T*)c->dmul2,
+                                       nullptr)))
+        return rc;
+    }
+    {
+      ProfScope ps(c, SPARC_KC_OTHER, 16.0 * (double)N * sizeof(T));
+      hipLaunchKernelGGL((k_mhd2_rhs<T>), gN, b256, 0, c->stream, F, N);
+      LAUNCHCHK(c);
+    }
+    if (damp) {
+      hipLaunchKernelGGL((k_sub_above_plane1<T>), gN, b256, 0, c->stream, F.scr[3],
+                         (const T*)F.W[5], c->nxy, N);
+      hipLaunchKernelGGL((k_sub_above_plane1<T>), gN, b256, 0, c->stream, F.scr[5],
+                         (const T*)F.W[6], c->nxy, N);
+      LAUNCHCHK(c);
+    }
   } else if (c->cfg.option == SPARC_OPT_QUIET_PML_3D) {
     // ddxyz(v_x, v_y, v_z, bc=1) (pml.f90:27); the z solve's epilogue forms
     // div v, RHScont and RHSp into W4, W5 (pml.f90:36-53)
@@ -1019,7 +1098,7 @@ template <typename T>
 static int step_impl(sparc_ctx* c, int t0, int nsteps) {
   Fld<T> F = make_fld<T>(c);
   const int nxy = (int)c->nxy;
-  const bool is2d = c->cfg.option == SPARC_OPT_QUIET_PML_2D_DISPL;
+  const bool is2d = is_2d(c->cfg.option);
   const T scale = (T)c->cfg.field_scale;
   int rc;
   for (int i = t0; i < t0 + nsteps; ++i) {
@@ -1095,7 +1174,7 @@ extern "C" int sparc_step(sparc_ctx* c, int t0, int nsteps) {
   for (int id : need)
     if (!c->bgset[id]) return fail(c, SPARC_ESTATE, "background %d not set", id);
   if (!c->profset[SPARC_PROF_STRETCH] || !c->profset[SPARC_PROF_G] ||
-      (c->cfg.option == SPARC_OPT_QUIET_PML_2D_DISPL && !c->profset[SPARC_PROF_UNSTRETCH]))
+      (is_2d(c->cfg.option) && !c->profset[SPARC_PROF_UNSTRETCH]))
     return fail(c, SPARC_ESTATE, "stretch/g/unstretch profiles not set");
   if (c->cfg.option != SPARC_OPT_QUIET_PML_2D_DISPL && c->cfg.damp_waves && !c->damping_set)
     return fail(c, SPARC_ESTATE, "damp_waves set but damping rates not given");
@@ -1109,6 +1188,14 @@ extern "C" int sparc_step(sparc_ctx* c, int t0, int nsteps) {
       if (!c->bgset[id]) return fail(c, SPARC_ESTATE, "option 10 needs background %d", id);
     if (!c->profset[SPARC_PROF_SPONGE])
       return fail(c, SPARC_ESTATE, "option 10 needs the sponge profile");
+  }
+  if (c->cfg.option == SPARC_OPT_MHD_PML_2D_DISPL) {
+    static const int need2[] = {SPARC_BG_BX, SPARC_BG_BZ, SPARC_BG_CURLBY, SPARC_BG_GRADP0_X,
+                                SPARC_BG_GRADRHO0_X};
+    for (int id : need2)
+      if (!c->bgset[id]) return fail(c, SPARC_ESTATE, "option 18 needs background %d", id);
+    if (!c->profset[SPARC_PROF_UNSTRETCH])
+      return fail(c, SPARC_ESTATE, "option 18 needs the unstretch profile");
   }
   return c->fp32 ? step_impl<float>(c, t0, nsteps) : step_impl<double>(c, t0, nsteps);
 }

@@ -41,7 +41,7 @@ class Params:
 
     @property
     def is2d(self) -> bool:
-        return self.option == abi.OPT_2D
+        return self.option in (abi.OPT_2D, abi.OPT_MHD2D)
 
     @property
     def steps(self) -> int:
@@ -142,7 +142,8 @@ def make_config(P: Params, S: bgm.Setup, mode: int, nsteps_total: int) -> abi.Sp
     c.option = P.option
     c.precision = P.precision
     c.npmlbot, c.npmltop = P.npmlbot, P.npmltop
-    c.damp_waves = int(P.damp_waves and not P.is2d)
+    # option 17 never damps (physics2d.f90:1071); option 18 does (:900)
+    c.damp_waves = int(P.damp_waves and P.option != abi.OPT_2D)
     c.freq_filtering = P.freq_filtering
     if P.is2d:  # physics2d.f90:42,54 override the nearest-point search
         e_rad, o_rad = P.nz - 5, P.nz - 1

@@ -401,3 +401,60 @@ def test_option10_random_state_matches_oracle():
     finally:
         eng.close()
         o.close()
+
+
+def _mhd2d_case(prec=abi.FP64, nsteps=6):
+    """Option 18 on the 64 x 1 x 60 box of the option-17 tests: B0 = (box, boz)
+    with v_A ~ 0.3 c, curl boy, horizontal background gradients, 2-D damping."""
+    from dataclasses import replace
+    tab = __import__("sparc_amd.background", fromlist=["x"]).polytrope_table(60, 0.5, -12.0)
+    P = replace(D.workload("two_d").params, nx=64, nz=60, xlength=100e8, ylength=100e8,
+                option=abi.OPT_MHD2D, damp_waves=True, precision=prec)
+    W = D.Workload("two_d", P, tab, 400)
+    S = D.setup_for(W)
+    eng, cfg = D.make_engine(W, S, precision=prec, nsteps=nsteps)
+    o, _ = make_oracle(W, S, nsteps=nsteps)
+    nx, ny, nz = eng.shape
+    x = np.arange(nx)[:, None, None] / nx
+    zz = np.arange(nz)[None, None, :] / nz
+    scale = 0.3 * np.sqrt(S.c2 * S.rho0)
+    gp = np.broadcast_to(S.derived["gradp0_z"], (nx, ny, nz))
+    gr = np.broadcast_to(S.derived["gradrho0_z"], (nx, ny, nz))
+    bgs = {abi.BG_BX: scale * (0.5 + 0.3 * np.cos(2 * np.pi * x)),
+           abi.BG_BZ: scale * (0.8 + 0.2 * np.sin(2 * np.pi * x) * np.cos(np.pi * zz)),
+           abi.BG_CURLBY: scale * 0.1 * np.sin(2 * np.pi * x),
+           abi.BG_GRADP0_X: 0.05 * gp * np.sin(2 * np.pi * x),
+           abi.BG_GRADRHO0_X: 0.05 * gr * np.cos(2 * np.pi * x)}
+    for e in (eng, o):
+        for bid, arr in bgs.items():
+            e.set_background(bid, np.broadcast_to(arr, (nx, ny, nz)))
+    return W, S, eng, o
+
+
+@pytest.mark.parametrize("prec", [abi.FP64, abi.FP32])
+def test_step_option18_matches_oracle(prec):
+    """Option 18 (2-D linear MHD, physics2d.f90:724-943) through a (time+1)%ff
+    spectral-filter firing: ibc 3/2 presets, PML top and bottom on four psi
+    fields, 2-D damping of v_x and v_z, plane 1 zeroed."""
+    W, S, eng, o = _mhd2d_case(prec, nsteps=90)
+    try:
+        mask, series = D.forward_source(W, 98)
+        for x in (eng, o):
+            x.set_source(abi.SRC_SERIES, mask, series, 10 ** 9)
+            x.step(0, 90)
+        for f in (0, 2, 3, 5):
+            a, b = eng.get(abi.ARR_A, f), o.get(abi.ARR_A, f)
+            assert np.linalg.norm(b) > 0, f
+            if prec == abi.FP64:
+                assert rel_err(a, b) < TOL64, f
+            else:
+                assert rel_l2(a, b) < TOL32_L2, f
+        for f in (1, 4):
+            assert not np.any(eng.get(abi.ARR_A, f))
+        for f in (0, 1, 2, 4):
+            a, b = eng.get(abi.ARR_PMLVARS, f), o.get(abi.ARR_PMLVARS, f)
+            assert np.linalg.norm(b) > 0, f
+            assert (rel_err(a, b) < TOL64) if prec == abi.FP64 else (rel_l2(a, b) < 1e-4)
+    finally:
+        eng.close()
+        o.close()
