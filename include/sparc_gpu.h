@@ -43,6 +43,7 @@ extern "C" {
 
 /* physics option, numbering of PARSE_OPTIONS (initialize.f90:778-824) */
 #define SPARC_OPT_QUIET_PML_3D 2         /* (rho,v,p) form, pml.f90:12-76 */
+#define SPARC_OPT_MHD_PML_3D 8           /* (rho,v,p,b) form, pml.f90:80-208 */
 #define SPARC_OPT_QUIET_PML_3D_DISPL 4   /* displacement.f90:482-658 */
 #define SPARC_OPT_MHD_PML_3D_DISPL 10    /* displacement.f90:663-849 */
 #define SPARC_OPT_QUIET_PML_2D_DISPL 17  /* physics2d.f90:948-1105   */
@@ -64,13 +65,17 @@ extern "C" {
  * it is a forward-only formulation (the kernels need the displacement) */
 #define SPARC_RHO 0
 #define SPARC_P 4
+/* option 8 ((rho, v, p, b) form) adds b in slots 5..7 (initialize.f90:544-551) */
+#define SPARC_BX 5
+#define SPARC_MAXFIELDS 8
 
 /* which array sparc_get_state / sparc_put_state touch */
 #define SPARC_ARR_A 0        /* a(:,:,:,f): the state                          */
 #define SPARC_ARR_SCR 1      /* scr(:,:,:,f): last stage RHS ("acc" for f=3..5) */
-#define SPARC_ARR_PMLVARS 2  /* pmlvars(:,:,:,f), f=0 psivz, f=1 psip; option 10
-                                adds 2 psiinductionbx, 3 psiinductionby,
-                                4 psidzbx, 5 psidzby (initialize.f90:480-495) */
+#define SPARC_ARR_PMLVARS 2  /* pmlvars(:,:,:,f), f=0 psivz, f=1 psip; the MHD
+                                options add 2 psidzbx, 3 psidzby,
+                                4 psiinductionbx, 5 psiinductionby
+                                (initialize.f90:485-565)                        */
 #define SPARC_ARR_SCRPML 3   /* scrpml(:,:,:,f)                                 */
 #define SPARC_ARR_ALO 4      /* low-order part of a(:,:,:,f) when the FP32 state is
                                 compensated (cfg.compensated); get/put of

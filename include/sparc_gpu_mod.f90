@@ -20,7 +20,8 @@ module sparc_gpu
   integer(c_int), parameter :: SPARC_FP64 = 0, SPARC_FP32 = 1
   integer(c_int), parameter :: SPARC_OPT_QUIET_PML_3D = 2, SPARC_OPT_QUIET_PML_3D_DISPL = 4, &
                                SPARC_OPT_MHD_PML_3D_DISPL = 10, SPARC_OPT_QUIET_PML_2D_DISPL = 17, &
-                               SPARC_OPT_MHD_PML_2D_DISPL = 18
+                               SPARC_OPT_MHD_PML_2D_DISPL = 18, SPARC_OPT_MHD_PML_3D = 8
+  integer(c_int), parameter :: SPARC_BX = 5, SPARC_MAXFIELDS = 8
   integer(c_int), parameter :: SPARC_RHO = 0, SPARC_P = 4
   integer(c_int), parameter :: SPARC_FORWARD = 0, SPARC_ADJOINT = 1
   integer(c_int), parameter :: SPARC_XI_X = 0, SPARC_XI_Y = 1, SPARC_XI_Z = 2, &

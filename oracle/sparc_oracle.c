@@ -379,7 +379,8 @@ typedef struct orc {
   long N, NP;
   double *a, *temp, *scr;             /* 6*N each */
   double *pmlvars, *psipml, *scrpml;  /* npv*NP each */
-  int npv;                            /* psi fields: 2, or 6 for option 10 */
+  int npv;                            /* psi fields: 2, or 6 for the MHD options */
+  int nf;                             /* dimfive: 5, 6, or 8 for option 8 */
   double *mw[12];                     /* option 10 work arrays */
   double *bg[SPARC_BG_COUNT];         /* N each (z-profiles expanded) */
   double *prof[SPARC_PROF_COUNT];     /* nz each */
@@ -416,10 +417,12 @@ orc *orc_create(const sparc_config *cfg) {
   o->N = (long)o->nx * o->ny * o->nz;
   o->NP = (long)o->nx * o->ny * (o->nzpml > 0 ? o->nzpml : 1);
   long N = o->N;
-  o->a = zalloc(sizeof(double) * 6 * N);
-  o->temp = zalloc(sizeof(double) * 6 * N);
-  o->scr = zalloc(sizeof(double) * 6 * N);
-  int mhd = cfg->option == SPARC_OPT_MHD_PML_3D_DISPL || cfg->option == SPARC_OPT_MHD_PML_2D_DISPL;
+  o->nf = (cfg->option == SPARC_OPT_QUIET_PML_3D) ? 5 : (cfg->option == SPARC_OPT_MHD_PML_3D ? 8 : 6);
+  o->a = zalloc(sizeof(double) * SPARC_MAXFIELDS * N);
+  o->temp = zalloc(sizeof(double) * SPARC_MAXFIELDS * N);
+  o->scr = zalloc(sizeof(double) * SPARC_MAXFIELDS * N);
+  int mhd = cfg->option == SPARC_OPT_MHD_PML_3D_DISPL || cfg->option == SPARC_OPT_MHD_PML_2D_DISPL ||
+            cfg->option == SPARC_OPT_MHD_PML_3D;
   o->npv = mhd ? 6 : 2;
   o->pmlvars = zalloc(sizeof(double) * o->npv * o->NP);
   o->psipml = zalloc(sizeof(double) * o->npv * o->NP);
@@ -861,8 +864,8 @@ static void rhs_option10(orc *o) {
   orc_ddx(f2, dxf2, nx, ny, nz, sx);
   orc_ddy(f1, dyf1, nx, ny, nz, sy);
   orc_ddz(f1, dzf1, nx, ny, nz, 1, st);
-  pml_account_bottom(o, dzf2, 2);
-  pml_account_bottom(o, dzf1, 3);
+  pml_account_bottom(o, dzf2, 4); /* psiinductionbx */
+  pml_account_bottom(o, dzf1, 5); /* psiinductionby */
   double *bx = f1, *by = f2, *bz = f3; /* fluxes are consumed */
   for (long c = 0; c < N; c++) {
     double b1 = dyf3[c] - dzf2[c], b2 = -dxf3[c] + dzf1[c], b3 = dxf2[c] - dyf1[c];
@@ -878,8 +881,8 @@ static void rhs_option10(orc *o) {
   orc_ddx(bz, dxbz, nx, ny, nz, sx);
   orc_ddy(bx, dybx, nx, ny, nz, sy);
   orc_ddz(by, dzby, nx, ny, nz, 1, st);
-  pml_account_bottom(o, dzbx, 4);
-  pml_account_bottom(o, dzby, 5);
+  pml_account_bottom(o, dzbx, 2); /* psidzbx */
+  pml_account_bottom(o, dzby, 3); /* psidzby */
   double *RHSxi_x = o->scr, *RHSxi_y = o->scr + N, *RHSxi_z = o->scr + 2 * N;
   double *RHSv_x = o->scr + 3 * N, *RHSv_y = o->scr + 4 * N, *RHSv_z = o->scr + 5 * N;
   const double *rhoinv = bg[SPARC_BG_RHOINV], *g = o->prof[SPARC_PROF_G];
@@ -930,6 +933,101 @@ static void rhs_option10(orc *o) {
     }
 }
 
+/* MP_MHD_PML_3D, option 8 ((rho, v, p, b) form, pml.f90:80-208), no
+ * horizontal PML.  Slots: rho 0, v 1..3, p 4, b 5..7. */
+static void rhs_option8(orc *o) {
+  int nx = o->nx, ny = o->ny, nz = o->nz;
+  long N = o->N, nxy = (long)nx * ny;
+  double *rho = o->temp, *vx = o->temp + N, *vy = o->temp + 2 * N, *vz = o->temp + 3 * N;
+  double *p = o->temp + 4 * N, *bx = o->temp + 5 * N, *by = o->temp + 6 * N, *bz = o->temp + 7 * N;
+  double **bg = o->bg, **m = o->mw;
+  const double *st = o->prof[SPARC_PROF_STRETCH];
+  double sx = o->cfg.stretchx, sy = o->cfg.stretchy;
+  const double *b0x = bg[SPARC_BG_BX], *b0y = bg[SPARC_BG_BY], *b0z = bg[SPARC_BG_BZ];
+  double *f1 = m[0], *f2 = m[1], *f3 = m[2];
+  /* flux = v x B0 (:89-91: flux1 = -(v_z boy - v_y boz), ...) */
+  for (long c = 0; c < N; c++) {
+    f1[c] = -(vz[c] * b0y[c] - vy[c] * b0z[c]);
+    f2[c] = -(vx[c] * b0z[c] - vz[c] * b0x[c]);
+    f3[c] = -(vy[c] * b0x[c] - vx[c] * b0y[c]);
+  }
+  /* the six ddxyz calls (:104-115) */
+  double *dxf3 = m[3], *dzf1 = m[4], *dyf3 = m[5], *dzf2 = m[6], *dxf2 = m[7], *dybz = m[8];
+  double *dzbx = m[9], *dxby = m[10], *dybx = m[11], *dxbz = o->div, *dyf1 = o->p, *dzby = o->rho;
+  orc_ddx(vx, o->dxdx, nx, ny, nz, sx);
+  orc_ddy(vy, o->dydy, nx, ny, nz, sy);
+  orc_ddz(vz, o->dzdz, nx, ny, nz, 1, st);
+  orc_ddx(f3, dxf3, nx, ny, nz, sx);
+  orc_ddy(p, o->gpy, nx, ny, nz, sy);
+  orc_ddz(f1, dzf1, nx, ny, nz, 1, st);
+  orc_ddx(p, o->gpx, nx, ny, nz, sx);
+  orc_ddy(f3, dyf3, nx, ny, nz, sy);
+  orc_ddz(f2, dzf2, nx, ny, nz, 1, st);
+  orc_ddx(f2, dxf2, nx, ny, nz, sx);
+  orc_ddy(bz, dybz, nx, ny, nz, sy);
+  orc_ddz(bx, dzbx, nx, ny, nz, 1, st);
+  orc_ddx(by, dxby, nx, ny, nz, sx);
+  orc_ddy(bx, dybx, nx, ny, nz, sy);
+  orc_ddz(p, o->gpz, nx, ny, nz, 1, st);
+  orc_ddx(bz, dxbz, nx, ny, nz, sx);
+  orc_ddy(f1, dyf1, nx, ny, nz, sy);
+  orc_ddz(by, dzby, nx, ny, nz, 1, st);
+  /* PML_BOUNDARY_ACCOUNTING_3D with the magnetic terms (pml.f90:213-280) */
+  pml_account(o, o->gpz, 1);
+  pml_account(o, o->dzdz, 0);
+  pml_account(o, dzf2, 4);
+  pml_account(o, dzf1, 5);
+  pml_account(o, dzbx, 2);
+  pml_account(o, dzby, 3);
+  double *RHScont = o->scr, *RHSv_x = o->scr + N, *RHSv_y = o->scr + 2 * N, *RHSv_z = o->scr + 3 * N;
+  double *RHSp = o->scr + 4 * N, *RHSb_x = o->scr + 5 * N, *RHSb_y = o->scr + 6 * N;
+  double *RHSb_z = o->scr + 7 * N;
+  const double *rhoinv = bg[SPARC_BG_RHOINV], *g = o->prof[SPARC_PROF_G];
+  const double *sdep = o->prof[SPARC_PROF_SOURCE_DEP];
+  const double *j0x = bg[SPARC_BG_CURLBX], *j0y = bg[SPARC_BG_CURLBY], *j0z = bg[SPARC_BG_CURLBZ];
+  int src = o->src_kind != SPARC_SRC_NONE && o->time <= o->src_active_until;
+  for (int k = 0; k < nz; k++)
+    for (long c = 0; c < nxy; c++) {
+      long q = c + nxy * k;
+      double rbx = dyf3[q] - dzf2[q], rby = -dxf3[q] + dzf1[q], rbz = dxf2[q] - dyf1[q];
+      double div = o->dxdx[q] + o->dydy[q] + o->dzdz[q];
+      double jx = dybz[q] - dzby[q], jy = -dxbz[q] + dzbx[q], jz = dxby[q] - dybx[q];
+      double rc = -bg[SPARC_BG_GRADRHO0_X][q] * vx[q] - bg[SPARC_BG_GRADRHO0_Y][q] * vy[q] -
+                  bg[SPARC_BG_GRADRHO0_Z][q] * vz[q] - bg[SPARC_BG_RHO0][q] * div;
+      double rp = -bg[SPARC_BG_C2RHO0][q] * div - vx[q] * bg[SPARC_BG_GRADP0_X][q] -
+                  vy[q] * bg[SPARC_BG_GRADP0_Y][q] - vz[q] * bg[SPARC_BG_GRADP0_Z][q];
+      double l1 = jy * b0z[q] - jz * b0y[q], l2 = b0x[q] * jz - jx * b0z[q];
+      double l3 = jx * b0y[q] - jy * b0x[q];
+      double rx = -rhoinv[q] * (o->gpx[q] - l1), ry = -rhoinv[q] * (o->gpy[q] - l2);
+      double rz = -rhoinv[q] * (o->gpz[q] - l3);
+      double m1 = j0y[q] * bz[q] - j0z[q] * by[q], m2 = bx[q] * j0z[q] - j0x[q] * bz[q];
+      double m3 = j0x[q] * by[q] - j0y[q] * bx[q];
+      rx = rx + rhoinv[q] * m1;
+      ry = ry + rhoinv[q] * m2;
+      rz = rz + rhoinv[q] * m3;
+      rz = rz - rhoinv[q] * rho[q] * g[k];
+      if (src && abs(k - (o->cfg.e_rad - 1)) <= 4) rz = rz + o->forcing[c] * sdep[k];
+      RHScont[q] = rc;
+      RHSv_x[q] = rx;
+      RHSv_y[q] = ry;
+      RHSv_z[q] = rz;
+      RHSp[q] = rp;
+      RHSb_x[q] = rbx;
+      RHSb_y[q] = rby;
+      RHSb_z[q] = rbz;
+    }
+  for (int f = 0; f < 8; f++)
+    for (long c = 0; c < nxy; c++) {
+      o->scr[f * N + nxy * (nz - 1) + c] = 0.0;
+      o->scr[f * N + c] = 0.0;
+    }
+  if (o->cfg.damp_waves) {
+    damp_field(o, vx, RHSv_x);
+    damp_field(o, vy, RHSv_y);
+    damp_field(o, vz, RHSv_z);
+  }
+}
+
 /* MP_MHD_PML_2D_DISPL, option 18 (physics2d.f90:724-943), ny = 1, kernel mode,
  * sources at the planes e_rad/o_rad (erad_2d uniform). */
 static void rhs_option18(orc *o) {
@@ -964,12 +1062,12 @@ static void rhs_option18(orc *o) {
     f2[c] = -(xi_x[c] * bg[SPARC_BG_BZ][c] - xi_z[c] * bg[SPARC_BG_BX][c]);
   orc_ddz(f2, bx, nx, ny, nz, 0, st);
   orc_ddx(f2, bz, nx, ny, nz, sx);
-  pml_account(o, bx, 2);
+  pml_account(o, bx, 4); /* psiinductionbx */
   for (long c = 0; c < N; c++) bx[c] = -bx[c];
   /* dzbx = ddz(bx, 0) + psidzbx, curlby = -ddx(bz) + dzbx (:852-873) */
   orc_ddz(bx, dzbx, nx, ny, nz, 0, st);
   orc_ddx(bz, cby, nx, ny, nz, sx);
-  pml_account(o, dzbx, 4);
+  pml_account(o, dzbx, 2); /* psidzbx */
   double *RHSxi_x = o->scr, *RHSxi_z = o->scr + 2 * N;
   double *RHSv_x = o->scr + 3 * N, *RHSv_z = o->scr + 5 * N;
   const double *rhoinv = bg[SPARC_BG_RHOINV], *g = o->prof[SPARC_PROF_G];
@@ -1106,7 +1204,7 @@ static void filter_vars(orc *o) {
   long N = o->N;
   int is2d = (o->cfg.option == SPARC_OPT_QUIET_PML_2D_DISPL ||
               o->cfg.option == SPARC_OPT_MHD_PML_2D_DISPL);
-  for (int l = 0; l < 6; l++) zfilter_field(o->a + l * N, nx, ny, nz, is2d ? 6 : 2, o->term);
+  for (int l = 0; l < o->nf; l++) zfilter_field(o->a + l * N, nx, ny, nz, is2d ? 6 : 2, o->term);
   int ff = o->cfg.freq_filtering;
   if (ff <= 0) return;
   if (!is2d) {
@@ -1122,7 +1220,7 @@ static void filter_vars(orc *o) {
 
 /* STEP (step.f90:43-128) */
 static void step(orc *o) {
-  long N6 = 6 * o->N, NP2 = (long)o->npv * o->NP;
+  long N6 = (long)o->nf * o->N, NP2 = (long)o->npv * o->NP;
   int npml = o->nzpml > 0;
   for (int s = 0; s < 5; s++) {
     if (s == 0) {
@@ -1142,6 +1240,8 @@ static void step(orc *o) {
       rhs_option10(o);
     else if (o->cfg.option == SPARC_OPT_MHD_PML_2D_DISPL)
       rhs_option18(o);
+    else if (o->cfg.option == SPARC_OPT_MHD_PML_3D)
+      rhs_option8(o);
     else
       rhs_option17(o);
   }
@@ -1168,11 +1268,11 @@ int orc_step(orc *o, int t0, int nsteps) {
 }
 
 static double *arr_ptr(orc *o, int arr, int field, long *n) {
-  if (arr == SPARC_ARR_A && field >= 0 && field < 6) {
+  if (arr == SPARC_ARR_A && field >= 0 && field < o->nf) {
     *n = o->N;
     return o->a + field * o->N;
   }
-  if (arr == SPARC_ARR_SCR && field >= 0 && field < 6) {
+  if (arr == SPARC_ARR_SCR && field >= 0 && field < o->nf) {
     *n = o->N;
     return o->scr + field * o->N;
   }

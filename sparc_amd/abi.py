@@ -21,6 +21,8 @@ OPT_3D, OPT_2D = 4, 17
 OPT_RHOVP = 2          # (rho, v, p) form, forward only
 OPT_MHD = 10           # linear MHD, displacement form
 OPT_MHD2D = 18         # linear MHD, displacement form, ny = 1
+OPT_MHD8 = 8           # linear MHD, (rho, v, p, b) form, forward only
+BX = 5                 # option 8 slot of b_x (b in 5..7)
 RHO, P = 0, 4          # option 2 slots of rho and p (v in 1..3)
 FORWARD, ADJOINT = 0, 1
 XI_X, XI_Y, XI_Z, V_X, V_Y, V_Z = range(6)

@@ -18,8 +18,8 @@ enum { YM_DERIV = 0, YM_FILTER = 1, YM_DAMP = 2 };
 
 template <typename T>
 struct Fld {
-  T* a[SPARC_NFIELDS];
-  T* scr[SPARC_NFIELDS];
+  T* a[SPARC_MAXFIELDS];
+  T* scr[SPARC_MAXFIELDS];
   T* pmlvars[6];
   T* scrpml[6];
   T* M[12];  // option 10 work arrays
@@ -566,6 +566,73 @@ __global__ void k_mhd_rhs(Fld<T> F, const T* __restrict__ sponge, long n) {
   F.scr[SPARC_V_X][q] = rx;
   F.scr[SPARC_V_Y][q] = ry;
   F.scr[SPARC_V_Z][q] = rz;
+}
+
+// Option 8 (MP_MHD_PML_3D, pml.f90:80-208), slots rho 0, v 1..3, p 4, b 5..7.
+// flux = v x B0 (pml.f90:89-91) -> M0..M2
+template <typename T>
+__global__ void k_mhd8_flux(Fld<T> F, long n) {
+  const long q = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (q >= n) return;
+  const int k = (int)(q / F.nxy), c = (int)(q - (long)k * F.nxy);
+  const T vx = tmp_of(F, 1, q), vy = tmp_of(F, 2, q), vz = tmp_of(F, 3, q);
+  const T bx = bgv(F, SPARC_BG_BX, c, k), by = bgv(F, SPARC_BG_BY, c, k), bz = bgv(F, SPARC_BG_BZ, c, k);
+  F.M[0][q] = -(vz * by - vy * bz);
+  F.M[1][q] = -(vx * bz - vz * bx);
+  F.M[2][q] = -(vy * bx - vx * by);
+}
+
+// Continuity, momentum (J x B0 + J0 x b), pressure and induction RHS, source
+// over 9 planes, planes 1 and nz zeroed (pml.f90:119-190).  Derivatives:
+// div parts W0..W2, grad p (W4, W3, W5), d_x f3 M3, d_z f1 M4, d_y f3 M5,
+// d_z f2 M6, d_x f2 M7, d_y bz M8, d_z bx M9, d_x by M10, d_y bx M11,
+// d_x bz W6, d_y f1 M1, d_z by M2.  The damping of v is subtracted afterwards.
+template <typename T>
+__global__ void k_mhd8_rhs(Fld<T> F, long n) {
+  const long q = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (q >= n) return;
+  const int nz = F.nz;
+  const int k = (int)(q / F.nxy), c = (int)(q - (long)k * F.nxy);
+  const T rho = tmp_of(F, 0, q), vx = tmp_of(F, 1, q), vy = tmp_of(F, 2, q), vz = tmp_of(F, 3, q);
+  const T bx = tmp_of(F, 5, q), by = tmp_of(F, 6, q), bz = tmp_of(F, 7, q);
+  if (k == 0 || k == nz - 1) {
+#pragma unroll
+    for (int f = 0; f < 8; ++f) F.scr[f][q] = 0;
+    return;
+  }
+  const T rbx = F.M[5][q] - F.M[6][q];
+  const T rby = -F.M[3][q] + F.M[4][q];
+  const T rbz = F.M[7][q] - F.M[1][q];
+  const T div = F.W[0][q] + F.W[1][q] + F.W[2][q];
+  const T jx = F.M[8][q] - F.M[2][q];
+  const T jy = -F.W[6][q] + F.M[9][q];
+  const T jz = F.M[10][q] - F.M[11][q];
+  const T rc = -bgv(F, SPARC_BG_GRADRHO0_X, c, k) * vx - bgv(F, SPARC_BG_GRADRHO0_Y, c, k) * vy -
+               bgv(F, SPARC_BG_GRADRHO0_Z, c, k) * vz - bgv(F, SPARC_BG_RHO0, c, k) * div;
+  const T rp = -bgv(F, SPARC_BG_C2RHO0, c, k) * div - vx * bgv(F, SPARC_BG_GRADP0_X, c, k) -
+               vy * bgv(F, SPARC_BG_GRADP0_Y, c, k) - vz * bgv(F, SPARC_BG_GRADP0_Z, c, k);
+  const T b0x = bgv(F, SPARC_BG_BX, c, k), b0y = bgv(F, SPARC_BG_BY, c, k),
+          b0z = bgv(F, SPARC_BG_BZ, c, k);
+  const T j0x = bgv(F, SPARC_BG_CURLBX, c, k), j0y = bgv(F, SPARC_BG_CURLBY, c, k),
+          j0z = bgv(F, SPARC_BG_CURLBZ, c, k);
+  const T ri = bgv(F, SPARC_BG_RHOINV, c, k);
+  T rx = -ri * (F.W[4][q] - (jy * b0z - jz * b0y));
+  T ry = -ri * (F.W[3][q] - (b0x * jz - jx * b0z));
+  T rz = -ri * (F.W[5][q] - (jx * b0y - jy * b0x));
+  rx = rx + ri * (j0y * bz - j0z * by);
+  ry = ry + ri * (bx * j0z - j0x * bz);
+  rz = rz + ri * (j0x * by - j0y * bx);
+  rz = rz - ri * rho * F.g[k];
+  if (F.src_plane >= 0 && k - F.src_plane <= 4 && F.src_plane - k <= 4)
+    rz = rz + F.forcing[c] * F.srcdep[k];
+  F.scr[0][q] = rc;
+  F.scr[1][q] = rx;
+  F.scr[2][q] = ry;
+  F.scr[3][q] = rz;
+  F.scr[4][q] = rp;
+  F.scr[5][q] = rbx;
+  F.scr[6][q] = rby;
+  F.scr[7][q] = rbz;
 }
 
 // Option 18 (MP_MHD_PML_2D_DISPL, physics2d.f90:724-943), ny = 1.

@@ -27,9 +27,10 @@ struct sparc_ctx {
   int device = 0;
   long N = 0, nxy = 0, NP = 0;
   int nzpml = 0;
-  void* a[SPARC_NFIELDS] = {};
-  void* scr[SPARC_NFIELDS] = {};
-  void* alo[SPARC_NFIELDS] = {};  // low-order state words (compensated FP32 state)
+  void* a[SPARC_MAXFIELDS] = {};
+  void* scr[SPARC_MAXFIELDS] = {};
+  void* alo[SPARC_MAXFIELDS] = {};  // low-order state words (compensated FP32 state)
+  int nf = SPARC_NFIELDS;           // dimfive: 5 (option 2), 6 (displacement), 8 (option 8)
   bool comp = false;
   void* pmlvars[6] = {};
   void* scrpml[6] = {};
@@ -206,7 +207,12 @@ static size_t spec_elems(const sparc_config* cfg) {
 }
 
 static bool is_mhd(int option) {
-  return option == SPARC_OPT_MHD_PML_3D_DISPL || option == SPARC_OPT_MHD_PML_2D_DISPL;
+  return option == SPARC_OPT_MHD_PML_3D_DISPL || option == SPARC_OPT_MHD_PML_2D_DISPL ||
+         option == SPARC_OPT_MHD_PML_3D;
+}
+// dimfive (initialize.f90:310-312)
+static int nfields_of(int option) {
+  return option == SPARC_OPT_QUIET_PML_3D ? 5 : (option == SPARC_OPT_MHD_PML_3D ? 8 : 6);
 }
 static bool is_2d(int option) {
   return option == SPARC_OPT_QUIET_PML_2D_DISPL || option == SPARC_OPT_MHD_PML_2D_DISPL;
@@ -221,7 +227,7 @@ static int check_config(const sparc_config* cfg, std::string* why) {
   }
   if (cfg->option != SPARC_OPT_QUIET_PML_3D_DISPL && cfg->option != SPARC_OPT_QUIET_PML_2D_DISPL &&
       cfg->option != SPARC_OPT_QUIET_PML_3D && !is_mhd(cfg->option)) {
-    snprintf(b, sizeof b, "option %d not supported (2, 4, 10, 17 or 18)", cfg->option);
+    snprintf(b, sizeof b, "option %d not supported (2, 4, 8, 10, 17 or 18)", cfg->option);
     *why = b;
     return SPARC_EINVAL;
   }
@@ -242,8 +248,9 @@ static int check_config(const sparc_config* cfg, std::string* why) {
     *why = "the 2-D options need ny == 1";
     return SPARC_EINVAL;
   }
-  if (cfg->option == SPARC_OPT_QUIET_PML_3D && cfg->mode != SPARC_FORWARD) {
-    *why = "option 2 is forward-only (the kernels need the displacement formulation)";
+  if ((cfg->option == SPARC_OPT_QUIET_PML_3D || cfg->option == SPARC_OPT_MHD_PML_3D) &&
+      cfg->mode != SPARC_FORWARD) {
+    *why = "options 2 and 8 are forward-only (the kernels need the displacement formulation)";
     return SPARC_EINVAL;
   }
   if (!is_2d(cfg->option)) {
@@ -281,8 +288,9 @@ static size_t footprint_bytes(const sparc_config* cfg) {
   size_t nxy = (size_t)cfg->nx * cfg->ny;
   size_t NP = nxy * (size_t)std::max(cfg->npmlbot + cfg->npmltop, 1);
   size_t b = 0;
-  b += 12 * N * ts;                         // a, scr
-  if (cfg->precision == SPARC_FP32 && cfg->compensated) b += 6 * N * ts;  // alo
+  const size_t nf = nfields_of(cfg->option);
+  b += 2 * nf * N * ts;                     // a, scr
+  if (cfg->precision == SPARC_FP32 && cfg->compensated) b += nf * N * ts;  // alo
   const size_t npv = is_mhd(cfg->option) ? 6 : 2;
   b += 2 * npv * NP * ts;                   // pmlvars, scrpml
   if (is_mhd(cfg->option)) b += 12 * N * ts + cfg->nx * ts;  // MHD work, 2-D damping
@@ -355,7 +363,8 @@ extern "C" int sparc_create(const sparc_config* cfg, sparc_ctx** out) {
   const size_t ts = c->ts, N = c->N;
   int rc;
   c->comp = c->fp32 && cfg->compensated;
-  for (int f = 0; f < SPARC_NFIELDS; ++f) {
+  c->nf = nfields_of(cfg->option);
+  for (int f = 0; f < c->nf; ++f) {
     if ((rc = dalloc(c, &c->a[f], N * ts))) return rc;
     if ((rc = dalloc(c, &c->scr[f], N * ts))) return rc;
     if (c->comp && (rc = dalloc(c, &c->alo[f], N * ts))) return rc;
@@ -501,7 +510,7 @@ extern "C" int sparc_device_bytes(const sparc_ctx* c, size_t* bytes) {
 
 extern "C" int sparc_reset(sparc_ctx* c) {
   if (!c || !c->stream) return SPARC_EINVAL;
-  for (int f = 0; f < SPARC_NFIELDS; ++f) {
+  for (int f = 0; f < c->nf; ++f) {
     HIPCHK(c, hipMemsetAsync(c->a[f], 0, c->N * c->ts, c->stream));
     HIPCHK(c, hipMemsetAsync(c->scr[f], 0, c->N * c->ts, c->stream));
     if (c->comp) HIPCHK(c, hipMemsetAsync(c->alo[f], 0, c->N * c->ts, c->stream));
@@ -611,7 +620,7 @@ extern "C" int sparc_set_source(sparc_ctx* c, int kind, const double* mask, size
 template <typename T>
 static Fld<T> make_fld(sparc_ctx* c) {
   Fld<T> F{};
-  for (int f = 0; f < SPARC_NFIELDS; ++f) {
+  for (int f = 0; f < SPARC_MAXFIELDS; ++f) {
     F.a[f] = (T*)c->a[f];
     F.scr[f] = (T*)c->scr[f];
   }
@@ -954,11 +963,11 @@ static int run_stage(sparc_ctx* c, Fld<T>& F, int s) {
     if ((rc = launch_x<T, XM_DERIV>(c, M[2], nullptr, 0, M[6], dmx, nullptr))) return rc;
     if ((rc = launch_y<T, YM_DERIV>(c, M[2], nullptr, 0, M[7], dmy, nullptr))) return rc;
     if ((rc = dz(M[1], nullptr, 0, 1, M[8]))) return rc;
-    if ((rc = pmlb(M[8], 2))) return rc;
+    if ((rc = pmlb(M[8], 4))) return rc;  // psiinductionbx
     if ((rc = launch_x<T, XM_DERIV>(c, M[1], nullptr, 0, M[9], dmx, nullptr))) return rc;
     if ((rc = launch_y<T, YM_DERIV>(c, M[0], nullptr, 0, M[10], dmy, nullptr))) return rc;
     if ((rc = dz(M[0], nullptr, 0, 1, M[11]))) return rc;
-    if ((rc = pmlb(M[11], 3))) return rc;
+    if ((rc = pmlb(M[11], 5))) return rc;  // psiinductionby
     {
       ProfScope ps(c, SPARC_KC_OTHER, 9.0 * (double)N * sizeof(T));
       hipLaunchKernelGGL((k_mhd_b<T>), gN, b256, 0, c->stream, F, N);
@@ -968,11 +977,11 @@ static int run_stage(sparc_ctx* c, Fld<T>& F, int s) {
     if ((rc = launch_x<T, XM_DERIV>(c, M[1], nullptr, 0, M[6], dmx, nullptr))) return rc;
     if ((rc = launch_y<T, YM_DERIV>(c, M[2], nullptr, 0, M[7], dmy, nullptr))) return rc;
     if ((rc = dz(M[0], nullptr, 0, 1, M[8]))) return rc;
-    if ((rc = pmlb(M[8], 4))) return rc;
+    if ((rc = pmlb(M[8], 2))) return rc;  // psidzbx
     if ((rc = launch_x<T, XM_DERIV>(c, M[2], nullptr, 0, M[9], dmx, nullptr))) return rc;
     if ((rc = launch_y<T, YM_DERIV>(c, M[0], nullptr, 0, M[10], dmy, nullptr))) return rc;
     if ((rc = dz(M[1], nullptr, 0, 1, M[11]))) return rc;
-    if ((rc = pmlb(M[11], 5))) return rc;
+    if ((rc = pmlb(M[11], 3))) return rc;  // psidzby
     const bool damp = c->cfg.damp_waves && c->damping_set;
     XArgs<T> DX{};
     for (int f = 0; f < 3; ++f) {
@@ -989,6 +998,76 @@ static int run_stage(sparc_ctx* c, Fld<T>& F, int s) {
       ProfScope ps(c, SPARC_KC_OTHER, 25.0 * (double)N * sizeof(T));
       hipLaunchKernelGGL((k_mhd_rhs<T>), gN, b256, 0, c->stream, F,
                          (const T*)c->prof[SPARC_PROF_SPONGE], N);
+      LAUNCHCHK(c);
+    }
+    if (damp && (rc = launch_xm<T, XM_DAMP_INV>(c, 3, DX, 0, nullptr))) return rc;
+  } else if (c->cfg.option == SPARC_OPT_MHD_PML_3D) {
+    // MP_MHD_PML_3D (pml.f90:80-208): six ddxyz calls, PML top and bottom on
+    // six psi fields, then one pointwise pass for all eight right-hand sides
+    const long N = c->N;
+    const dim3 gN(cdiv(N, 256)), b256(256);
+    T** M = F.M;
+    auto pmla = [&](T* d, int w) -> int {
+      if (c->nzpml <= 0) return SPARC_OK;
+      ProfScope ps(c, SPARC_KC_OTHER, 5.0 * (double)c->NP * sizeof(T));
+      hipLaunchKernelGGL((k_pml_planes<T>), gN, b256, 0, c->stream, F, d, w, N);
+      LAUNCHCHK(c);
+      return SPARC_OK;
+    };
+    auto X = [&](const T* in, const T* s, T b, T* out) {
+      return launch_x<T, XM_DERIV>(c, in, s, b, out, dmx, nullptr);
+    };
+    auto Y = [&](const T* in, const T* s, T b, T* out) {
+      return launch_y<T, YM_DERIV>(c, in, s, b, out, dmy, nullptr);
+    };
+    auto Z = [&](const T* in, const T* s, T b, T* out) {
+      return launch_z<T, EPI_STORE>(c, F, 1, in, s, b, nullptr, 0, out);
+    };
+    {
+      ProfScope ps(c, SPARC_KC_OTHER, 12.0 * (double)N * sizeof(T));
+      hipLaunchKernelGGL((k_mhd8_flux<T>), gN, b256, 0, c->stream, F, N);
+      LAUNCHCHK(c);
+    }
+    // ddxyz(v_x, v_y, v_z); (flux3, p, flux1); (p, flux3, flux2); (flux2, bz, bx);
+    // (by, bx, p); (bz, flux1, by) -- all with bc = 1
+    if ((rc = X(F.a[1], S(1), beta, F.W[0])) || (rc = Y(F.a[2], S(2), beta, F.W[1])) ||
+        (rc = Z(F.a[3], S(3), beta, F.W[2])))
+      return rc;
+    if ((rc = X(M[2], nullptr, 0, M[3])) || (rc = Y(F.a[4], S(4), beta, F.W[3])) ||
+        (rc = Z(M[0], nullptr, 0, M[4])))
+      return rc;
+    if ((rc = X(F.a[4], S(4), beta, F.W[4])) || (rc = Y(M[2], nullptr, 0, M[5])) ||
+        (rc = Z(M[1], nullptr, 0, M[6])))
+      return rc;
+    if ((rc = X(M[1], nullptr, 0, M[7])) || (rc = Y(F.a[7], S(7), beta, M[8])) ||
+        (rc = Z(F.a[5], S(5), beta, M[9])))
+      return rc;
+    if ((rc = X(F.a[6], S(6), beta, M[10])) || (rc = Y(F.a[5], S(5), beta, M[11])) ||
+        (rc = Z(F.a[4], S(4), beta, F.W[5])))
+      return rc;
+    if ((rc = X(F.a[7], S(7), beta, F.W[6])) || (rc = Y(M[0], nullptr, 0, M[1])) ||
+        (rc = Z(F.a[6], S(6), beta, M[2])))
+      return rc;
+    // PML_BOUNDARY_ACCOUNTING_3D: psip, psivz, psiinductionbx, psiinductionby, psidzbx, psidzby
+    if ((rc = pmla(F.W[5], 1)) || (rc = pmla(F.W[2], 0)) || (rc = pmla(M[6], 4)) ||
+        (rc = pmla(M[4], 5)) || (rc = pmla(M[9], 2)) || (rc = pmla(M[2], 3)))
+      return rc;
+    const bool damp = c->cfg.damp_waves && c->damping_set;
+    XArgs<T> DX{};
+    DX.edges = 1;  // zeroing of planes 1, nz precedes DAMP_VELOCITY (pml.f90:193-206)
+    for (int f = 0; f < 3; ++f) {
+      DX.a[f] = F.a[1 + f];
+      DX.s[f] = S(1 + f);
+      DX.out[f] = F.scr[1 + f];
+      DX.spec[f] = (C2<T>*)c->spec[f];
+    }
+    if (damp) {
+      if ((rc = launch_xm<T, XM_DAMP_FWD>(c, 3, DX, beta, nullptr))) return rc;
+      if ((rc = launch_ym<T, YM_DAMP>(c, 3, DX, 0, nullptr))) return rc;
+    }
+    {
+      ProfScope ps(c, SPARC_KC_OTHER, 45.0 * (double)N * sizeof(T));
+      hipLaunchKernelGGL((k_mhd8_rhs<T>), gN, b256, 0, c->stream, F, N);
       LAUNCHCHK(c);
     }
     if (damp && (rc = launch_xm<T, XM_DAMP_INV>(c, 3, DX, 0, nullptr))) return rc;
@@ -1025,13 +1104,13 @@ static int run_stage(sparc_ctx* c, Fld<T>& F, int s) {
     // bx = -(ddz(flux2, 0) + psiinductionbx), bz = ddx(flux2)
     if ((rc = dz(M[0], nullptr, 0, 0, nullptr, 0, M[1]))) return rc;
     if ((rc = launch_x<T, XM_DERIV>(c, M[0], nullptr, 0, M[2], dmx, nullptr))) return rc;
-    if ((rc = pmla(M[1], 2))) return rc;
+    if ((rc = pmla(M[1], 4))) return rc;  // psiinductionbx
     hipLaunchKernelGGL((k_negate<T>), gN, b256, 0, c->stream, M[1], N);
     LAUNCHCHK(c);
     // dzbx = ddz(bx, 0) + psidzbx; ddx(bz)
     if ((rc = dz(M[1], nullptr, 0, 0, nullptr, 0, M[3]))) return rc;
     if ((rc = launch_x<T, XM_DERIV>(c, M[2], nullptr, 0, M[4], dmx, nullptr))) return rc;
-    if ((rc = pmla(M[3], 4))) return rc;
+    if ((rc = pmla(M[3], 2))) return rc;  // psidzbx
     // DAMP_VELOCITY 2-D (v_x, v_z along x) on the stage's temp v, before RHSv is written
     const bool damp = c->cfg.damp_waves && c->damping_set;
     if (damp) {
@@ -1122,7 +1201,7 @@ static int step_impl(sparc_ctx* c, int t0, int nsteps) {
       if ((rc = run_stage<T>(c, F, s))) return rc;
     // a += deltat*scr with the z filter (step.f90:114-118)
     const T dt = (T)c->cfg.deltat;
-    const int nfld = c->cfg.option == SPARC_OPT_QUIET_PML_3D ? 5 : SPARC_NFIELDS;  // dimfive
+    const int nfld = c->nf;  // dimfive
     for (int f = 0; f < nfld; ++f) {
       ProfScope ps(c, SPARC_KC_UPDATE, (c->comp ? 5.0 : 3.0) * (double)c->N * sizeof(T));
       if (c->comp) {
@@ -1180,9 +1259,12 @@ extern "C" int sparc_step(sparc_ctx* c, int t0, int nsteps) {
     return fail(c, SPARC_ESTATE, "damp_waves set but damping rates not given");
   if (c->cfg.freq_filtering > 0 && !c->filter_set)
     return fail(c, SPARC_ESTATE, "freq_filtering > 0 but filter not given");
-  if (c->cfg.option == SPARC_OPT_QUIET_PML_3D && c->src_kind != SPARC_SRC_NONE &&
-      !c->profset[SPARC_PROF_SOURCE_DEP])
-    return fail(c, SPARC_ESTATE, "option 2 with a source needs the source_dep profile");
+  if ((c->cfg.option == SPARC_OPT_QUIET_PML_3D || c->cfg.option == SPARC_OPT_MHD_PML_3D) &&
+      c->src_kind != SPARC_SRC_NONE && !c->profset[SPARC_PROF_SOURCE_DEP])
+    return fail(c, SPARC_ESTATE, "options 2 and 8 with a source need the source_dep profile");
+  if (c->cfg.option == SPARC_OPT_MHD_PML_3D)
+    for (int id = SPARC_BG_BX; id <= SPARC_BG_GRADRHO0_Y; ++id)
+      if (!c->bgset[id]) return fail(c, SPARC_ESTATE, "option 8 needs background %d", id);
   if (c->cfg.option == SPARC_OPT_MHD_PML_3D_DISPL) {
     for (int id = SPARC_BG_BX; id <= SPARC_BG_GRADRHO0_Y; ++id)
       if (!c->bgset[id]) return fail(c, SPARC_ESTATE, "option 10 needs background %d", id);
@@ -1210,7 +1292,7 @@ extern "C" int sparc_sync(sparc_ctx* c) {
 // state access
 // ---------------------------------------------------------------------------
 static void* arr_ptr(sparc_ctx* c, int arr, int field, size_t* n) {
-  if ((arr == SPARC_ARR_A || arr == SPARC_ARR_SCR) && field >= 0 && field < SPARC_NFIELDS) {
+  if ((arr == SPARC_ARR_A || arr == SPARC_ARR_SCR) && field >= 0 && field < c->nf) {
     *n = c->N;
     return arr == SPARC_ARR_A ? c->a[field] : c->scr[field];
   }
@@ -1218,7 +1300,7 @@ static void* arr_ptr(sparc_ctx* c, int arr, int field, size_t* n) {
     *n = c->NP;
     return arr == SPARC_ARR_PMLVARS ? c->pmlvars[field] : c->scrpml[field];
   }
-  if (arr == SPARC_ARR_ALO && c->comp && field >= 0 && field < SPARC_NFIELDS) {
+  if (arr == SPARC_ARR_ALO && c->comp && field >= 0 && field < c->nf) {
     *n = c->N;
     return c->alo[field];
   }
@@ -1260,14 +1342,14 @@ extern "C" int sparc_put_state(sparc_ctx* c, int arr, int field, const double* h
 }
 
 extern "C" int sparc_get_plane(sparc_ctx* c, int field, int k, double* h) {
-  if (!c || !h || field < 0 || field >= SPARC_NFIELDS || k < 1 || k > c->cfg.nz)
+  if (!c || !h || field < 0 || field >= c->nf || k < 1 || k > c->cfg.nz)
     return fail(c, SPARC_EINVAL, "bad field/plane");
   return download(c, h, (char*)c->a[field] + (size_t)(k - 1) * c->nxy * c->ts, c->nxy,
                   1.0 / c->cfg.field_scale);
 }
 
 extern "C" int sparc_norm(sparc_ctx* c, int field, double* out) {
-  if (!c || !out || field < 0 || field >= SPARC_NFIELDS) return SPARC_EINVAL;
+  if (!c || !out || field < 0 || field >= c->nf) return SPARC_EINVAL;
   std::vector<double> h(c->N);
   int rc = download(c, h.data(), c->a[field], c->N, 1.0 / c->cfg.field_scale);
   if (rc) return rc;
@@ -1287,7 +1369,7 @@ __global__ void k_gather(double* out, const T* __restrict__ f, const int32_t* __
 }
 
 extern "C" int sparc_sample(sparc_ctx* c, int field, int npts, const int32_t* ijk, double* h) {
-  if (!c || !h || !ijk || npts < 0 || field < 0 || field >= SPARC_NFIELDS)
+  if (!c || !h || !ijk || npts < 0 || field < 0 || field >= c->nf)
     return SPARC_EINVAL;
   if (npts == 0) return SPARC_OK;
   for (int p = 0; p < npts; ++p)
@@ -1349,8 +1431,8 @@ extern "C" int sparc_snapshots_alloc(sparc_ctx* c, int nslots) {
 
 extern "C" int sparc_snapshot_store(sparc_ctx* c, int slot) {
   if (!c || slot < 0 || slot >= c->nslots) return fail(c, SPARC_EINVAL, "bad slot %d", slot);
-  if (c->cfg.option == SPARC_OPT_QUIET_PML_3D)
-    return fail(c, SPARC_ESTATE, "snapshots hold xi and acc: displacement options (4, 17) only");
+  if (c->cfg.option == SPARC_OPT_QUIET_PML_3D || c->cfg.option == SPARC_OPT_MHD_PML_3D)
+    return fail(c, SPARC_ESTATE, "snapshots hold xi and acc: displacement options only");
   char* s = (char*)c->snap + (size_t)slot * snap_fields(c) * c->N * c->ts;
   const size_t fb = c->N * c->ts;
   for (int f = 0; f < 3; ++f) {
@@ -1381,8 +1463,8 @@ extern "C" int sparc_snapshots_copy(sparc_ctx* dst, const sparc_ctx* src) {
 
 extern "C" int sparc_kernel_begin(sparc_ctx* c, const double* grav, int nz) {
   if (!c || !grav || nz != c->cfg.nz) return fail(c, SPARC_EINVAL, "grav needs nz values");
-  if (c->cfg.option == SPARC_OPT_QUIET_PML_3D)
-    return fail(c, SPARC_ESTATE, "kernels need the displacement formulation (options 4, 17)");
+  if (c->cfg.option == SPARC_OPT_QUIET_PML_3D || c->cfg.option == SPARC_OPT_MHD_PML_3D)
+    return fail(c, SPARC_ESTATE, "kernels need the displacement formulation");
   int rc;
   const int nk = c->cfg.flow_kernels ? SPARC_KERN_COUNT : SPARC_KERN_VX;
   if (!c->kern[0]) {
@@ -1493,7 +1575,7 @@ extern "C" int sparc_checkpoints_alloc(sparc_ctx* c, int nslots) {
   if (!c || nslots < 0) return SPARC_EINVAL;
   if (nslots <= c->nckpt) return SPARC_OK;
   void* p;
-  const size_t per = ((c->comp ? 12 : 6) * (size_t)c->N + c->npv * (size_t)c->NP) * c->ts;
+  const size_t per = ((c->comp ? 2 : 1) * c->nf * (size_t)c->N + c->npv * (size_t)c->NP) * c->ts;
   int rc = dalloc(c, &p, (size_t)nslots * per);
   if (rc) return rc;
   c->ckpt = p;
@@ -1504,10 +1586,10 @@ extern "C" int sparc_checkpoints_alloc(sparc_ctx* c, int nslots) {
 static int ckpt_copy(sparc_ctx* c, int slot, bool save) {
   if (!c || slot < 0 || slot >= c->nckpt) return fail(c, SPARC_EINVAL, "bad checkpoint %d", slot);
   const size_t fb = c->N * c->ts, pb = c->NP * c->ts;
-  const int nf = c->comp ? 2 * SPARC_NFIELDS : SPARC_NFIELDS;
+  const int nf = c->comp ? 2 * c->nf : c->nf;
   char* base = (char*)c->ckpt + (size_t)slot * (nf * fb + c->npv * pb);
   for (int f = 0; f < nf; ++f) {
-    void* dev = f < SPARC_NFIELDS ? c->a[f] : c->alo[f - SPARC_NFIELDS];
+    void* dev = f < c->nf ? c->a[f] : c->alo[f - c->nf];
     void* d = save ? (void*)(base + f * fb) : dev;
     const void* s = save ? (const void*)dev : (const void*)(base + f * fb);
     HIPCHK(c, hipMemcpyAsync(d, s, fb, hipMemcpyDeviceToDevice, c->stream));

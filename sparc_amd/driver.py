@@ -181,7 +181,7 @@ def load_setup(eng, S: bgm.Setup) -> None:
     eng.set_profile(abi.PROF_UNSTRETCH, d["unstretch"])
     eng.set_profile(abi.PROF_G, d["g"])
     eng.set_profile(abi.PROF_Z, S.z)
-    if eng.cfg.option == abi.OPT_RHOVP:
+    if eng.cfg.option in (abi.OPT_RHOVP, abi.OPT_MHD8):
         eng.set_profile(abi.PROF_SOURCE_DEP, bgm.source_dep(S.z, eng.cfg.e_rad))
     eng.set_damping(S.rates)
     eng.set_filter(S.decay_x, S.decay_y)

@@ -288,7 +288,8 @@ def test_step_option2_matches_oracle(tiny_rhovp, prec, comp):
                 assert rel_err(a, b) < TOL64, f
             else:
                 assert rel_l2(a, b) < TOL32_L2, f
-        assert not np.any(eng.get(abi.ARR_A, 5))
+        with pytest.raises(abi.SparcError):  # option 2 has five fields (dimfive = 5)
+            eng.get(abi.ARR_A, 5)
         for f in range(2):
             a, b = eng.get(abi.ARR_PMLVARS, f), o.get(abi.ARR_PMLVARS, f)
             assert (rel_err(a, b) < TOL64) if prec == abi.FP64 else (rel_l2(a, b) < 1e-4)
@@ -320,14 +321,15 @@ def test_option2_random_state_matches_oracle(tiny_rhovp):
         o.close()
 
 
-def _mhd_case(prec=abi.FP64, nsteps=6):
+def _mhd_case(prec=abi.FP64, nsteps=6, option=abi.OPT_MHD):
     """The tiny polytrope box with option 10: bottom PML only, a smooth 3-D
     field B0 with v_A ~ 0.3 c, its curl, horizontal background gradients and a
     top sponge ramp (the reference's sponge is zero on boxes of <= 90 planes,
     so a ramp exercises the term here)."""
     from dataclasses import replace
     W0 = D.workload("tiny")
-    P = replace(W0.params, option=abi.OPT_MHD, npmltop=0, precision=prec)
+    P = replace(W0.params, option=option, npmltop=0 if option == abi.OPT_MHD else 10,
+                precision=prec)
     W = D.Workload("tiny", P, W0.tab, W0.nsteps)
     S = D.setup_for(W)
     eng, cfg = D.make_engine(W, S, precision=prec, nsteps=nsteps)
@@ -451,10 +453,53 @@ def test_step_option18_matches_oracle(prec):
                 assert rel_l2(a, b) < TOL32_L2, f
         for f in (1, 4):
             assert not np.any(eng.get(abi.ARR_A, f))
-        for f in (0, 1, 2, 4):
+        for f in (0, 1, 2, 4):  # psivz, psip, psidzbx, psiinductionbx
             a, b = eng.get(abi.ARR_PMLVARS, f), o.get(abi.ARR_PMLVARS, f)
             assert np.linalg.norm(b) > 0, f
             assert (rel_err(a, b) < TOL64) if prec == abi.FP64 else (rel_l2(a, b) < 1e-4)
+    finally:
+        eng.close()
+        o.close()
+
+
+@pytest.mark.parametrize("prec", [abi.FP64, abi.FP32])
+def test_step_option8_matches_oracle(prec):
+    """Option 8 ((rho, v, p, b) linear MHD, pml.f90:80-208): induction, Lorentz
+    force, PML top and bottom on six psi fields, source over 9 planes,
+    damping after the boundary zeroing."""
+    W, S, eng, o = _mhd_case(prec, nsteps=14, option=abi.OPT_MHD8)
+    try:
+        mask, series = D.forward_source(W, 22)
+        for x in (eng, o):
+            x.set_source(abi.SRC_SERIES, mask, series, 10 ** 9)
+            x.step(0, 14)
+        for f in range(8):
+            a, b = eng.get(abi.ARR_A, f), o.get(abi.ARR_A, f)
+            assert np.linalg.norm(b) > 0, f
+            if prec == abi.FP64:
+                assert rel_err(a, b) < TOL64, f
+            else:
+                assert rel_l2(a, b) < TOL32_L2, f
+        for f in range(6):
+            a, b = eng.get(abi.ARR_PMLVARS, f), o.get(abi.ARR_PMLVARS, f)
+            assert np.linalg.norm(b) > 0, f
+            assert (rel_err(a, b) < TOL64) if prec == abi.FP64 else (rel_l2(a, b) < 1e-4)
+    finally:
+        eng.close()
+        o.close()
+
+
+def test_option8_random_state_matches_oracle():
+    W, S, eng, o = _mhd_case(abi.FP64, nsteps=3, option=abi.OPT_MHD8)
+    try:
+        for f in range(8):
+            xr = _field(eng.shape, 70 + f) * 1e-4
+            eng.put(abi.ARR_A, f, xr)
+            o.put(abi.ARR_A, f, xr)
+        eng.step(1, 3)
+        o.step(1, 3)
+        for f in range(8):
+            assert rel_err(eng.get(abi.ARR_A, f), o.get(abi.ARR_A, f)) < TOL64, f
     finally:
         eng.close()
         o.close()

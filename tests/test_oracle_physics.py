@@ -292,3 +292,50 @@ def test_option18_alfven_wave_amplification():
     assert np.max(np.abs(got[abi.V_Z][:, :, mid] - R[1, 0] * cosm[:, :, mid])) < 1e-7 * B0 * k
     for f in (abi.XI_X, abi.V_X):
         assert np.max(np.abs(got[f][:, :, mid])) < 1e-7 * B0 * k, f
+
+
+def test_option8_alfven_wave_amplification():
+    """Option 8 ((rho, v, p, b) form, pml.f90:80-208), uniform box, B0 = B x:
+    v_y = V cos(kx), b_y = Y sin(kx) obey V' = kB Y, Y' = -kB V (induction
+    curl(v x B0), force curl(b) x B0 / rho0), so one LDDRK step is the
+    stability polynomial of that 2x2 system."""
+    nx, ny, nz = 32, 8, 40
+    cfg = abi.SparcConfig()
+    cfg.abi_version = abi.ABI_VERSION
+    cfg.nx, cfg.ny, cfg.nz = nx, ny, nz
+    cfg.option = abi.OPT_MHD8
+    cfg.deltat = 2e-3
+    cfg.stretchx = cfg.stretchy = 1.0
+    cfg.e_rad, cfg.o_rad = 5, 6
+    o = O.Oracle(cfg)
+    ones = np.ones(nz)
+    B0 = 1.3
+    o.set_background(abi.BG_RHO0, ones)
+    o.set_background(abi.BG_RHOINV, ones)
+    o.set_background(abi.BG_C2RHO0, 0.7 * ones)
+    o.set_background(abi.BG_BX, B0 * ones)
+    for b in (abi.BG_GRADP0_Z, abi.BG_GRADRHO0_Z, abi.BG_AZ, abi.BG_BZPML, abi.BG_BY, abi.BG_BZ,
+              abi.BG_CURLBX, abi.BG_CURLBY, abi.BG_CURLBZ, abi.BG_GRADP0_X, abi.BG_GRADP0_Y,
+              abi.BG_GRADRHO0_X, abi.BG_GRADRHO0_Y):
+        o.set_background(b, 0 * ones)
+    o.set_profile(abi.PROF_STRETCH, ones)
+    o.set_profile(abi.PROF_G, 0 * ones)
+    m = 3
+    x = np.arange(nx) / nx
+    k = 2 * B.PI_R * m
+    cosm = np.broadcast_to(np.cos(2 * np.pi * m * x)[:, None, None], (nx, ny, nz)).copy()
+    sinm = np.broadcast_to(np.sin(2 * np.pi * m * x)[:, None, None], (nx, ny, nz)).copy()
+    o.put(abi.ARR_A, 2, cosm)  # v_y
+    o.step(1, 1)
+    got = {f: o.get(abi.ARR_A, f) for f in range(8)}
+    o.close()
+    opt = [1.0, 1.0, 0.5, float(np.float32(0.1665579725791151184)),
+           float(np.float32(0.03950410250586729987)), float(np.float32(0.007810706393337838236))]
+    L = np.array([[0.0, k * B0], [-k * B0, 0.0]]) * cfg.deltat
+    R = sum(opt[j] * np.linalg.matrix_power(L, j) for j in range(6))
+    mid = slice(15, 25)
+    assert abs(R[1, 0]) > 1e-3
+    assert np.max(np.abs(got[2][:, :, mid] - R[0, 0] * cosm[:, :, mid])) < 1e-7
+    assert np.max(np.abs(got[6][:, :, mid] - R[1, 0] * sinm[:, :, mid])) < 1e-7 * B0 * k
+    for f in (0, 1, 3, 4, 5, 7):
+        assert np.max(np.abs(got[f][:, :, mid])) < 1e-7 * B0 * k, f
