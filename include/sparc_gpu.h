@@ -159,7 +159,10 @@ extern "C" {
 #define SPARC_KERN_VX 3       /* flow kernels (cfg.flow_kernels):            */
 #define SPARC_KERN_VY 4       /*   sum_j d_i v_f,j * v_a,j, i = x, y, z      */
 #define SPARC_KERN_VZ 5       /*   (kernels.f90:317-332)                     */
-#define SPARC_KERN_COUNT 6
+#define SPARC_KERN_BX 6       /* magnetic kernels of the MHD displacement    */
+#define SPARC_KERN_BY 7       /*   options 10/18 (kernels.f90:336-430),      */
+#define SPARC_KERN_BZ 8       /*   unscaled sums                             */
+#define SPARC_KERN_COUNT 9
 
 typedef struct sparc_config {
   int32_t abi_version;    /* must be SPARC_ABI_VERSION                          */

@@ -1373,6 +1373,108 @@ static void compute_div(orc *o, const double *fx, const double *fy, const double
   for (long c = 0; c < N; c++) dive[c] = dive[c] + t1[c];
 }
 
+/* CROSS_KERN (kernels.f90:757-766): c = a x b */
+static void cross3(const double *ax, const double *ay, const double *az, const double *bx,
+                   const double *by, const double *bz, double *cx, double *cy, double *cz,
+                   long N) {
+  for (long c = 0; c < N; c++) {
+    double x = ay[c] * bz[c] - az[c] * by[c], y = bx[c] * az[c] - ax[c] * bz[c];
+    double z = ax[c] * by[c] - ay[c] * bx[c];
+    cx[c] = x;
+    cy[c] = y;
+    cz[c] = z;
+  }
+}
+
+/* CURL_KERN (kernels.f90:770-793); its bcz is an unassigned module integer
+ * (initialize.f90:145), zero in static storage, so the z derivatives take
+ * ibc 0 (and FILTER_VAR_Z through ddzkern) */
+static void curl_kern(orc *o, const double *fx, const double *fy, const double *fz, double *cx,
+                      double *cy, double *cz, double *t, double *t2) {
+  long N = o->N;
+  orc_ddx(fy, cz, o->nx, o->ny, o->nz, o->cfg.stretchx);
+  orc_ddy(fx, t, o->nx, o->ny, o->nz, o->cfg.stretchy);
+  for (long c = 0; c < N; c++) cz[c] = cz[c] - t[c];
+  orc_ddy(fz, cx, o->nx, o->ny, o->nz, o->cfg.stretchy);
+  ddzkern(o, fy, t, 0, t2);
+  for (long c = 0; c < N; c++) cx[c] = cx[c] - t[c];
+  ddzkern(o, fx, cy, 0, t2);
+  orc_ddx(fz, t, o->nx, o->ny, o->nz, o->cfg.stretchx);
+  for (long c = 0; c < N; c++) cy[c] = cy[c] - t[c];
+}
+
+/* The MAGNETIC_KERNELS sums (kernels.f90:336-430) for one snapshot pair */
+static void magnetic_kernels(orc *o, const double *fx, const double *fy, const double *fz,
+                             const double *ax, const double *ay, const double *az) {
+  long N = o->N;
+  double **bg = o->bg;
+  const double *b0x = bg[SPARC_BG_BX], *b0y = bg[SPARC_BG_BY], *b0z = bg[SPARC_BG_BZ];
+  const double *j0x = bg[SPARC_BG_CURLBX], *j0y = bg[SPARC_BG_CURLBY], *j0z = bg[SPARC_BG_CURLBZ];
+  double *kx = o->kern[SPARC_KERN_BX], *ky = o->kern[SPARC_KERN_BY], *kz = o->kern[SPARC_KERN_BZ];
+  double **m = o->mw;
+  double *tx = m[0], *ty = m[1], *tz = m[2], *fbx = m[3], *fby = m[4], *fbz = m[5];
+  double *ux = m[6], *uy = m[7], *uz = m[8], *wx = m[9], *wy = m[10], *wz = m[11];
+  double *s1 = o->term, *s2 = o->div, *d = o->p;
+  /* v_a x curl(curl(xi x B0)) */
+  cross3(fx, fy, fz, b0x, b0y, b0z, tx, ty, tz, N);
+  curl_kern(o, tx, ty, tz, fbx, fby, fbz, s1, s2);
+  curl_kern(o, fbx, fby, fbz, ux, uy, uz, s1, s2);
+  cross3(ax, ay, az, ux, uy, uz, wx, wy, wz, N);
+  for (long c = 0; c < N; c++) {
+    kx[c] += wx[c];
+    ky[c] += wy[c];
+    kz[c] += wz[c];
+  }
+  /* curl(fb x v_a) */
+  cross3(fbx, fby, fbz, ax, ay, az, tx, ty, tz, N);
+  curl_kern(o, tx, ty, tz, wx, wy, wz, s1, s2);
+  for (long c = 0; c < N; c++) {
+    kx[c] += wx[c];
+    ky[c] += wy[c];
+    kz[c] += wz[c];
+  }
+  /* xi x curl(curl(v_a x B0)) */
+  cross3(ax, ay, az, b0x, b0y, b0z, tx, ty, tz, N);
+  curl_kern(o, tx, ty, tz, fbx, fby, fbz, s1, s2);
+  curl_kern(o, fbx, fby, fbz, ux, uy, uz, s1, s2);
+  cross3(fx, fy, fz, ux, uy, uz, wx, wy, wz, N);
+  for (long c = 0; c < N; c++) {
+    kx[c] += wx[c];
+    ky[c] += wy[c];
+    kz[c] += wz[c];
+  }
+  /* curl(v_a x J0) x xi */
+  cross3(ax, ay, az, j0x, j0y, j0z, wx, wy, wz, N);
+  curl_kern(o, wx, wy, wz, tx, ty, tz, s1, s2);
+  cross3(tx, ty, tz, fx, fy, fz, wx, wy, wz, N);
+  for (long c = 0; c < N; c++) {
+    kx[c] += wx[c];
+    ky[c] += wy[c];
+    kz[c] += wz[c];
+  }
+  /* (xi x J0) div(v_a) */
+  compute_div(o, ax, ay, az, d, s1, s2);
+  cross3(fx, fy, fz, j0x, j0y, j0z, tx, ty, tz, N);
+  for (long c = 0; c < N; c++) {
+    kx[c] += tx[c] * d[c];
+    ky[c] += ty[c] * d[c];
+    kz[c] += tz[c] * d[c];
+  }
+  /* curl(B0 x (xi div(v_a))) */
+  for (long c = 0; c < N; c++) {
+    tx[c] = fx[c] * d[c];
+    ty[c] = fy[c] * d[c];
+    tz[c] = fz[c] * d[c];
+  }
+  cross3(b0x, b0y, b0z, tx, ty, tz, wx, wy, wz, N);
+  curl_kern(o, wx, wy, wz, tx, ty, tz, s1, s2);
+  for (long c = 0; c < N; c++) {
+    kx[c] += tx[c];
+    ky[c] += ty[c];
+    kz[c] += tz[c];
+  }
+}
+
 int orc_kernel_accumulate(orc *o, int slot) {
   if (slot < 0 || slot >= o->nslots) return SPARC_EINVAL;
   long N = o->N, nxy = (long)o->nx * o->ny;
@@ -1408,6 +1510,8 @@ int orc_kernel_accumulate(orc *o, int slot) {
       krho[q] = krho[q] - o->grav[k] * fxz[q] * temp3[q] +
                 (fxx[q] * d31[q] + fxy[q] * d32[q] + fxz[q] * d33[q]);
     }
+  if (o->cfg.option == SPARC_OPT_MHD_PML_3D_DISPL || o->cfg.option == SPARC_OPT_MHD_PML_2D_DISPL)
+    magnetic_kernels(o, fxx, fxy, fxz, avx, avy, avz);
   if (o->cfg.flow_kernels) {
     /* COMPUTE_DEL (kernels.f90:821-841): deriv(i,j) = d_i f_vel_j; kernelv(i) +=
      * sum_j deriv(i,j) * a_vel_j (kernels.f90:317-332); no y part when ny <= 4 */

@@ -711,6 +711,79 @@ __global__ void k_sub_above_plane1(T* __restrict__ out, const T* __restrict__ te
   if (q >= nxy && q < n) out[q] = out[q] - term[q];
 }
 
+// ---------------------------------------------------------------------------
+// Magnetic kernels (kernels.f90:336-430): vector operands are three arrays,
+// or a background triple (B0 or curl B0) read through bgv.
+// ---------------------------------------------------------------------------
+template <typename T>
+struct Vec3 {
+  const T* p[3];  // arrays, or
+  int bg;         // first background id of a triple (SPARC_BG_BX, SPARC_BG_CURLBX), -1 if arrays
+  const T* s;     // optional scalar field multiplying the vector (nullptr: none)
+};
+
+template <typename T>
+__device__ __forceinline__ void vec_at(const Fld<T>& F, const Vec3<T>& v, long q, int c, int k,
+                                       T* o) {
+#pragma unroll
+  for (int i = 0; i < 3; ++i) o[i] = v.bg >= 0 ? bgv(F, v.bg + i, c, k) : v.p[i][q];
+  if (v.s) {
+    const T m = v.s[q];
+#pragma unroll
+    for (int i = 0; i < 3; ++i) o[i] = o[i] * m;
+  }
+}
+
+// c = a x b (CROSS_KERN, kernels.f90:757-766) into T arrays, or with ACC
+// added to the FP64 sums times `scale` (dst = the kernel sums).
+template <typename T, bool ACC>
+__global__ void k_cross3(Fld<T> F, Vec3<T> a, Vec3<T> b, T* cx, T* cy, T* cz, double* kx,
+                         double* ky, double* kz, double scale, long n) {
+  const long q = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (q >= n) return;
+  const int k = (int)(q / F.nxy), c = (int)(q - (long)k * F.nxy);
+  T u[3], v[3];
+  vec_at(F, a, q, c, k, u);
+  vec_at(F, b, q, c, k, v);
+  const T x = u[1] * v[2] - u[2] * v[1], y = v[0] * u[2] - u[0] * v[2], z = u[0] * v[1] - u[1] * v[0];
+  if (ACC) {
+    kx[q] = kx[q] + scale * (double)x;
+    ky[q] = ky[q] + scale * (double)y;
+    kz[q] = kz[q] + scale * (double)z;
+  } else {
+    cx[q] = x;
+    cy[q] = y;
+    cz[q] = z;
+  }
+}
+
+// k += scale * v (v optionally times a scalar field)
+template <typename T>
+__global__ void k_acc_vec(Fld<T> F, Vec3<T> v, double* kx, double* ky, double* kz, double scale,
+                          long n) {
+  const long q = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (q >= n) return;
+  const int k = (int)(q / F.nxy), c = (int)(q - (long)k * F.nxy);
+  T u[3];
+  vec_at(F, v, q, c, k, u);
+  kx[q] = kx[q] + scale * (double)u[0];
+  ky[q] = ky[q] + scale * (double)u[1];
+  kz[q] = kz[q] + scale * (double)u[2];
+}
+
+// curl from its six derivative pieces (CURL_KERN, kernels.f90:779-789):
+// c = (dy fz - dz fy, dz fx - dx fz, dx fy - dy fx); dyfx/dyfz may be null (ny = 1)
+template <typename T>
+__global__ void k_curl_combine(T* cx, T* cy, T* cz, const T* dxfy, const T* dyfx, const T* dyfz,
+                               const T* dzfy, const T* dzfx, const T* dxfz, long n) {
+  const long q = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (q >= n) return;
+  const T a = dyfz ? dyfz[q] : (T)0, b = dyfx ? dyfx[q] : (T)0;
+  cx[q] = a - dzfy[q];
+  cy[q] = dzfx[q] - dxfz[q];
+  cz[q] = dxfy[q] - b;
+}
+
 // Flow-kernel sums for one forward velocity component j (kernels.f90:317-332):
 // K_v(i) += d_i v_f,j * v_a,j with d_x, d_y, d_z v_f,j given (dy == nullptr: 0).
 template <typename T>

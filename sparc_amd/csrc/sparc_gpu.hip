@@ -1466,16 +1466,123 @@ extern "C" int sparc_kernel_begin(sparc_ctx* c, const double* grav, int nz) {
   if (c->cfg.option == SPARC_OPT_QUIET_PML_3D || c->cfg.option == SPARC_OPT_MHD_PML_3D)
     return fail(c, SPARC_ESTATE, "kernels need the displacement formulation");
   int rc;
-  const int nk = c->cfg.flow_kernels ? SPARC_KERN_COUNT : SPARC_KERN_VX;
+  const bool mag = c->cfg.option == SPARC_OPT_MHD_PML_3D_DISPL ||
+                   c->cfg.option == SPARC_OPT_MHD_PML_2D_DISPL;
+  auto needed = [&](int q) {
+    return q < SPARC_KERN_VX || (c->cfg.flow_kernels && q < SPARC_KERN_BX) ||
+           (mag && q >= SPARC_KERN_BX);
+  };
   if (!c->kern[0]) {
-    for (int q = 0; q < nk; ++q)
-      if ((rc = dalloc(c, (void**)&c->kern[q], c->N * 8))) return rc;
+    for (int q = 0; q < SPARC_KERN_COUNT; ++q)
+      if (needed(q) && (rc = dalloc(c, (void**)&c->kern[q], c->N * 8))) return rc;
     if ((rc = dalloc(c, (void**)&c->grav, nz * 8))) return rc;
   }
-  for (int q = 0; q < nk; ++q)
-    HIPCHK(c, hipMemsetAsync(c->kern[q], 0, c->N * 8, c->stream));
+  for (int q = 0; q < SPARC_KERN_COUNT; ++q)
+    if (c->kern[q]) HIPCHK(c, hipMemsetAsync(c->kern[q], 0, c->N * 8, c->stream));
   HIPCHK(c, hipMemcpyAsync(c->grav, grav, nz * 8, hipMemcpyHostToDevice, c->stream));
   HIPCHK(c, hipStreamSynchronize(c->stream));
+  return SPARC_OK;
+}
+
+// MAGNETIC_KERNELS (kernels.f90:336-430) for one snapshot pair: six bilinear
+// terms in xi_f and v_a built from cross products with B0, curl B0 and seven
+// CURL_KERNs (their z derivatives with ibc 0 + FILTER_VAR_Z: bcz is an
+// unassigned module integer, initialize.f90:145).  Work: W0..W6, M0..M11.
+template <typename T>
+static int magnetic_kernels(sparc_ctx* c, Fld<T>& F, const T* fx, const T* fy, const T* fz,
+                            const T* ax, const T* ay, const T* az, double scale) {
+  const long N = c->N, nxy = c->nxy;
+  const int nz = c->cfg.nz;
+  const dim3 gN(cdiv(N, 256)), b256(256);
+  const bool has_y = c->cfg.ny > 1;
+  const T* dmx = (const T*)c->dmulx;
+  const T* dmy = (const T*)c->dmuly;
+  double *kx = c->kern[SPARC_KERN_BX], *ky = c->kern[SPARC_KERN_BY], *kz = c->kern[SPARC_KERN_BZ];
+  T** M = F.M;
+  T** W = F.W;
+  int rc;
+  auto arr = [](const T* x, const T* y, const T* z, const T* s = nullptr) {
+    Vec3<T> v{};
+    v.p[0] = x;
+    v.p[1] = y;
+    v.p[2] = z;
+    v.bg = -1;
+    v.s = s;
+    return v;
+  };
+  auto bgt = [](int id) {
+    Vec3<T> v{};
+    v.bg = id;
+    return v;
+  };
+  auto cross = [&](const Vec3<T>& a, const Vec3<T>& b, T* ox, T* oy, T* oz) {
+    hipLaunchKernelGGL((k_cross3<T, false>), gN, b256, 0, c->stream, F, a, b, ox, oy, oz,
+                       (double*)nullptr, (double*)nullptr, (double*)nullptr, 0.0, N);
+  };
+  auto acc_cross = [&](const Vec3<T>& a, const Vec3<T>& b) {
+    hipLaunchKernelGGL((k_cross3<T, true>), gN, b256, 0, c->stream, F, a, b, (T*)nullptr,
+                       (T*)nullptr, (T*)nullptr, kx, ky, kz, scale, N);
+  };
+  auto acc = [&](const Vec3<T>& v) {
+    hipLaunchKernelGGL((k_acc_vec<T>), gN, b256, 0, c->stream, F, v, kx, ky, kz, scale, N);
+  };
+  auto dzk = [&](const T* in, T* out) -> int {  // ddzkern with ibc 0
+    int r = launch_z<T, EPI_STORE>(c, F, 0, in, nullptr, 0, nullptr, 0, out);
+    if (r) return r;
+    hipLaunchKernelGGL((k_update_filter<T>), dim3(cdiv(nxy, 64)), dim3(64), 0, c->stream, out,
+                       (const T*)nullptr, (T)0, (int)nxy, nz, 2);
+    return SPARC_OK;
+  };
+  auto curl = [&](const T* x, const T* y, const T* z, T* ox, T* oy, T* oz) -> int {
+    int r;
+    if ((r = launch_x<T, XM_DERIV>(c, y, nullptr, 0, W[0], dmx, nullptr))) return r;
+    if (has_y && (r = launch_y<T, YM_DERIV>(c, x, nullptr, 0, W[1], dmy, nullptr))) return r;
+    if (has_y && (r = launch_y<T, YM_DERIV>(c, z, nullptr, 0, W[2], dmy, nullptr))) return r;
+    if ((r = dzk(y, W[3]))) return r;
+    if ((r = dzk(x, W[4]))) return r;
+    if ((r = launch_x<T, XM_DERIV>(c, z, nullptr, 0, W[5], dmx, nullptr))) return r;
+    hipLaunchKernelGGL((k_curl_combine<T>), gN, b256, 0, c->stream, ox, oy, oz,
+                       (const T*)W[0], has_y ? (const T*)W[1] : nullptr,
+                       has_y ? (const T*)W[2] : nullptr, (const T*)W[3], (const T*)W[4],
+                       (const T*)W[5], N);
+    return SPARC_OK;
+  };
+  const Vec3<T> xi = arr(fx, fy, fz), va = arr(ax, ay, az);
+  const Vec3<T> B0 = bgt(SPARC_BG_BX), J0 = bgt(SPARC_BG_CURLBX);
+  // v_a x curl(curl(xi x B0)); fb = curl(xi x B0) kept in M3..M5
+  cross(xi, B0, M[0], M[1], M[2]);
+  if ((rc = curl(M[0], M[1], M[2], M[3], M[4], M[5]))) return rc;
+  if ((rc = curl(M[3], M[4], M[5], M[6], M[7], M[8]))) return rc;
+  acc_cross(va, arr(M[6], M[7], M[8]));
+  // curl(fb x v_a)
+  cross(arr(M[3], M[4], M[5]), va, M[0], M[1], M[2]);
+  if ((rc = curl(M[0], M[1], M[2], M[9], M[10], M[11]))) return rc;
+  acc(arr(M[9], M[10], M[11]));
+  // xi x curl(curl(v_a x B0))
+  cross(va, B0, M[0], M[1], M[2]);
+  if ((rc = curl(M[0], M[1], M[2], M[3], M[4], M[5]))) return rc;
+  if ((rc = curl(M[3], M[4], M[5], M[6], M[7], M[8]))) return rc;
+  acc_cross(xi, arr(M[6], M[7], M[8]));
+  // curl(v_a x J0) x xi
+  cross(va, J0, M[0], M[1], M[2]);
+  if ((rc = curl(M[0], M[1], M[2], M[3], M[4], M[5]))) return rc;
+  acc_cross(arr(M[3], M[4], M[5]), xi);
+  // (xi x J0) div(v_a), div as COMPUTE_DEL (no y part when ny <= 4)
+  if ((rc = launch_x<T, XM_DERIV>(c, ax, nullptr, 0, W[0], dmx, nullptr))) return rc;
+  const bool div_y = c->cfg.ny > 4;
+  if (div_y && (rc = launch_y<T, YM_DERIV>(c, ay, nullptr, 0, W[1], dmy, nullptr))) return rc;
+  if ((rc = launch_z<T, EPI_STORE>(c, F, 1, az, nullptr, 0, nullptr, 0, W[2]))) return rc;
+  hipLaunchKernelGGL((k_update_filter<T>), dim3(cdiv(nxy, 64)), dim3(64), 0, c->stream, W[2],
+                     (const T*)nullptr, (T)0, (int)nxy, nz, 2);
+  hipLaunchKernelGGL((k_sum3<T>), gN, b256, 0, c->stream, W[6], (const T*)W[0],
+                     div_y ? (const T*)W[1] : nullptr, (const T*)W[2], N);
+  cross(xi, J0, M[0], M[1], M[2]);
+  acc(arr(M[0], M[1], M[2], W[6]));
+  // curl(B0 x (xi div(v_a)))
+  cross(B0, arr(fx, fy, fz, W[6]), M[0], M[1], M[2]);
+  if ((rc = curl(M[0], M[1], M[2], M[3], M[4], M[5]))) return rc;
+  acc(arr(M[3], M[4], M[5]));
+  LAUNCHCHK(c);
   return SPARC_OK;
 }
 
@@ -1528,6 +1635,8 @@ static int kernel_acc_impl(sparc_ctx* c, const sparc_ctx* src, int slot) {
                      (const T*)F.W[2], (const T*)c->bg[SPARC_BG_C2], c->bg3d[SPARC_BG_C2],
                      (const double*)c->grav, sff, sfa, nxy, N);
   LAUNCHCHK(c);
+  if (c->cfg.option == SPARC_OPT_MHD_PML_3D_DISPL || c->cfg.option == SPARC_OPT_MHD_PML_2D_DISPL)
+    if ((rc = magnetic_kernels<T>(c, F, fxx, fxy, fxz, avx, avy, avz, sff * sfa))) return rc;
   if (c->cfg.flow_kernels) {
     // COMPUTE_DEL + kernelv (kernels.f90:317-332, 821-841): for each forward
     // velocity component j, K_v(i) += d_i v_f,j * v_a,j (no y part when ny <= 4)

@@ -166,6 +166,13 @@ def kernel_flow(raw_v: np.ndarray, S: bgm.Setup, spec: SolveSpec,
     return -2.0 * S.rho0 * raw_v * dtn
 
 
+def kernel_magnetic(raw_b: np.ndarray, S: bgm.Setup, spec: SolveSpec,
+                    kstride: int | None = None) -> np.ndarray:
+    """Magnetic kernel component from the accumulated sum (kernels.f90:504:
+    sum * stepskern*timestep, nondimensional units)."""
+    return raw_b * (kstride or spec.steps) * S.deltat
+
+
 def make_engines(spec: SolveSpec, S: bgm.Setup, eng_factory=None):
     """Forward and adjoint engines (or oracles via eng_factory(cfg, S))."""
     W = spec.W

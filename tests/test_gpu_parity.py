@@ -321,7 +321,7 @@ def test_option2_random_state_matches_oracle(tiny_rhovp):
         o.close()
 
 
-def _mhd_case(prec=abi.FP64, nsteps=6, option=abi.OPT_MHD):
+def _mhd_case(prec=abi.FP64, nsteps=6, option=abi.OPT_MHD, mode=abi.FORWARD):
     """The tiny polytrope box with option 10: bottom PML only, a smooth 3-D
     field B0 with v_A ~ 0.3 c, its curl, horizontal background gradients and a
     top sponge ramp (the reference's sponge is zero on boxes of <= 90 planes,
@@ -332,8 +332,8 @@ def _mhd_case(prec=abi.FP64, nsteps=6, option=abi.OPT_MHD):
                 precision=prec)
     W = D.Workload("tiny", P, W0.tab, W0.nsteps)
     S = D.setup_for(W)
-    eng, cfg = D.make_engine(W, S, precision=prec, nsteps=nsteps)
-    o, _ = make_oracle(W, S, nsteps=nsteps)
+    eng, cfg = D.make_engine(W, S, precision=prec, nsteps=nsteps, mode=mode)
+    o, _ = make_oracle(W, S, nsteps=nsteps, mode=mode)
     nx, ny, nz = eng.shape
     x = np.arange(nx)[:, None, None] / nx
     y = np.arange(ny)[None, :, None] / ny
@@ -503,3 +503,47 @@ def test_option8_random_state_matches_oracle():
     finally:
         eng.close()
         o.close()
+
+
+def test_mhd_adjoint_and_kernels_match_oracle():
+    """Option 10 forward with snapshots, adjoint with the Lagrange source, and
+    the kernel sums including the magnetic kernels K_B (kernels.f90:336-430)."""
+    nsteps = 18
+    fe, fo = _mhd_case(abi.FP64, nsteps=nsteps)[2:]
+    W, S, ae, ao = _mhd_case(abi.FP64, nsteps=nsteps, mode=abi.ADJOINT)
+    steps = W.params.steps
+    nslots = nsteps // steps
+    try:
+        mask, series = D.forward_source(W, nsteps + 8)
+        for x in (fe, fo):
+            x.set_source(abi.SRC_SERIES, mask, series, 10 ** 9)
+            x.snapshots_alloc(nslots)
+        for sl in range(nslots):
+            for x in (fe, fo):
+                x.step(sl * steps, steps)
+                x.snapshot_store(sl)
+        rng = np.random.default_rng(19)
+        nt = ae.cfg.forcing_length + 8
+        vr = np.zeros((W.params.nx, W.params.ny, nt))
+        vr[:, :, :ae.cfg.forcing_length] = rng.standard_normal(
+            (W.params.nx, W.params.ny, ae.cfg.forcing_length)) * 1e-6
+        ao.snapshots_alloc(nslots)
+        for sl in range(nslots):
+            for f in range(6):
+                ao.snapshot_put(sl, f, fo.snapshot_get(sl, f))
+        for x in (ae, ao):
+            x.set_source(abi.SRC_LAGRANGE, vr, None, 10 ** 9)
+            x.kernel_begin(S.g_raw)
+        for sl in range(nslots):
+            for x in (ae, ao):
+                x.step(sl * steps, steps)
+            ae.kernel_accumulate_from(fe, nslots - 1 - sl)
+            ao.kernel_accumulate(nslots - 1 - sl)
+        for k in (abi.KERN_C2, abi.KERN_RHO, abi.KERN_HESSIAN, abi.KERN_BX, abi.KERN_BY,
+                  abi.KERN_BZ):
+            a, b = ae.kernel_get(k), ao.kernel_get(k)
+            assert np.linalg.norm(b) > 0, k
+            assert rel_err(a, b) < TOL64, k
+    finally:
+        for x in (fe, fo, ae, ao):
+            x.close()

@@ -339,3 +339,42 @@ def test_option8_alfven_wave_amplification():
     assert np.max(np.abs(got[6][:, :, mid] - R[1, 0] * sinm[:, :, mid])) < 1e-7 * B0 * k
     for f in (0, 1, 3, 4, 5, 7):
         assert np.max(np.abs(got[f][:, :, mid])) < 1e-7 * B0 * k, f
+
+
+def test_magnetic_kernel_closed_form():
+    """MAGNETIC_KERNELS (kernels.f90:336-430) with B0 = B x uniform (curl B0 = 0),
+    xi_f = (0, sin kx, 0), v_a = (0, cos kx, 0): only v_a x curl curl(xi x B0)
+    and xi x curl curl(v_a x B0) survive, each -B k^2 sin(kx) cos(kx) x, so
+    K_B = (-B k^2 sin 2kx, 0, 0)."""
+    nx, ny, nz = 32, 8, 30
+    cfg = abi.SparcConfig()
+    cfg.abi_version = abi.ABI_VERSION
+    cfg.nx, cfg.ny, cfg.nz = nx, ny, nz
+    cfg.option = abi.OPT_MHD
+    cfg.deltat = 1e-3
+    cfg.stretchx = cfg.stretchy = 1.0
+    cfg.e_rad, cfg.o_rad = 5, 6
+    o = O.Oracle(cfg)
+    o.set_profile(abi.PROF_STRETCH, np.ones(nz))
+    B0 = 1.7
+    o.set_background(abi.BG_C2, np.ones(nz))
+    o.set_background(abi.BG_BX, B0 * np.ones(nz))
+    shape = (nx, ny, nz)
+    m = 2
+    k = 2 * B.PI_R * m
+    x = np.arange(nx) / nx
+    sinx = np.broadcast_to(np.sin(2 * np.pi * m * x)[:, None, None], shape)
+    cosx = np.broadcast_to(np.cos(2 * np.pi * m * x)[:, None, None], shape)
+    o.snapshots_alloc(1)
+    o.snapshot_put(0, abi.XI_Y, sinx)
+    o.put(abi.ARR_A, abi.V_Y, cosx)
+    o.kernel_begin(np.zeros(nz))
+    o.kernel_accumulate(0)
+    kb = [o.kernel_get(q) for q in (abi.KERN_BX, abi.KERN_BY, abi.KERN_BZ)]
+    o.close()
+    mid = slice(10, nz - 10)
+    want = -B0 * k * k * np.sin(2 * 2 * np.pi * m * x)[:, None, None]
+    scale = B0 * k * k
+    assert np.max(np.abs(kb[0][:, :, mid] - want)) < 1e-6 * scale
+    assert np.max(np.abs(kb[1][:, :, mid])) < 1e-6 * scale
+    assert np.max(np.abs(kb[2][:, :, mid])) < 1e-6 * scale
