@@ -230,6 +230,12 @@ int sparc_set_filter(sparc_ctx *ctx, const double *decay_x, const double *decay_
  *   zero padded), series unused; active while time <= active_until. */
 int sparc_set_source(sparc_ctx *ctx, int kind, const double *mask, size_t nmask,
                      const double *series, int nt, int active_until);
+/* Per-pixel source planes of the MHD displacement options: erad_2d (forward)
+ * or orad_2d (adjoint), 1-based, nx*ny values, from the Wilson-depression
+ * search of mp_Initialize_RHS (physics.f90:66-90); delta_width_2d(i,j) =
+ * 2/(z(k+1)-z(k-1)) is formed from the z profile (set it first).  n = 0
+ * returns to the uniform plane e_rad / o_rad. */
+int sparc_set_source_planes(sparc_ctx *ctx, const int32_t *k_xy, size_t n);
 
 /* ---- the hot path ------------------------------------------------------ */
 

@@ -134,6 +134,14 @@ module sparc_gpu
       integer(c_int), value :: nt, active_until
     end function
 
+    integer(c_int) function sparc_set_source_planes(ctx, k_xy, n) &
+        bind(C, name="sparc_set_source_planes")
+      import :: c_int, c_ptr, c_int32_t, c_size_t
+      type(c_ptr), value :: ctx
+      integer(c_int32_t), intent(in) :: k_xy(*)
+      integer(c_size_t), value :: n
+    end function
+
     ! the hot path: nsteps calls of step() from time index t0
     integer(c_int) function sparc_step(ctx, t0, nsteps) bind(C, name="sparc_step")
       import :: c_int, c_ptr

@@ -54,6 +54,7 @@ def lib():
         L.orc_ddz.argtypes = [DP, DP, C.c_int, C.c_int, C.c_int, C.c_int, DP]
         L.orc_zfilter.argtypes = [DP, C.c_int, C.c_int, C.c_int, C.c_int]
         L.orc_lowpass.argtypes = [DP, C.c_int, C.c_int, C.c_int, C.c_int, DP]
+        L.orc_set_source_planes.argtypes = [C.c_void_p, C.POINTER(C.c_int32), C.c_long]
         L.orc_lagrange_eval.argtypes = [DP, C.c_long, C.c_int, C.c_int, C.c_double,
                                         C.c_double, C.c_int, DP]
         L.orc_compute_tt.restype = C.c_double
@@ -266,6 +267,14 @@ class Oracle:
         s = np.ascontiguousarray(series if series is not None else np.zeros(1), dtype=np.float64)
         nt = s.size if kind == 1 else (m.size // (self.cfg.nx * self.cfg.ny))
         assert self.L.orc_set_source(self.h, kind, _dp(m), m.size, _dp(s), nt, active_until) == 0
+
+    def set_source_planes(self, planes):
+        if planes is None:
+            assert self.L.orc_set_source_planes(self.h, None, 0) == 0
+            return
+        p = np.asfortranarray(planes, dtype=np.int32).ravel(order="F")
+        assert self.L.orc_set_source_planes(self.h, p.ctypes.data_as(C.POINTER(C.c_int32)),
+                                            p.size) == 0
 
     def step(self, t0, n):
         assert self.L.orc_step(self.h, t0, n) == 0

@@ -391,6 +391,8 @@ typedef struct orc {
   double *dxdx, *dydy, *dzdz, *div, *p, *rho, *gpx, *gpy, *gpz, *term;
   /* source */
   int src_kind, src_nt, src_active_until;
+  int *src_planes;  /* per-pixel 0-based source planes (MHD), NULL = e_rad/o_rad */
+  double *src_dw;   /* delta_width_2d */
   double *src_mask, *src_series;
   double *forcing; /* nx*ny */
   int time_old;
@@ -462,6 +464,8 @@ void orc_destroy(orc *o) {
   free(o->a);
   free(o->temp);
   free(o->scr);
+  free(o->src_planes);
+  free(o->src_dw);
   free(o->pmlvars);
   free(o->psipml);
   free(o->scrpml);
@@ -533,6 +537,27 @@ int orc_set_damping(orc *o, const double *r, long n) {
 int orc_set_filter(orc *o, const double *dx, const double *dy) {
   memcpy(o->decay_x, dx, sizeof(double) * (o->nx / 2 + 1));
   memcpy(o->decay_y, dy, sizeof(double) * (o->ny / 2 + 1));
+  return SPARC_OK;
+}
+
+/* erad_2d / orad_2d and delta_width_2d (physics.f90:66-90) */
+int orc_set_source_planes(orc *o, const int32_t *k_xy, long n) {
+  free(o->src_planes);
+  free(o->src_dw);
+  o->src_planes = NULL;
+  o->src_dw = NULL;
+  if (n == 0) return SPARC_OK;
+  long nxy = (long)o->nx * o->ny;
+  if (n != nxy) return SPARC_EINVAL;
+  o->src_planes = malloc(sizeof(int) * nxy);
+  o->src_dw = malloc(sizeof(double) * nxy);
+  const double *z = o->prof[SPARC_PROF_Z];
+  for (long c = 0; c < nxy; c++) {
+    int k = k_xy[c];
+    if (k < 2 || k > o->nz - 1) return SPARC_EINVAL;
+    o->src_planes[c] = k - 1;
+    o->src_dw[c] = 2.0 / (z[k] - z[k - 2]);
+  }
   return SPARC_OK;
 }
 
@@ -922,8 +947,10 @@ static void rhs_option10(orc *o) {
   if (o->src_kind != SPARC_SRC_NONE && o->time <= o->src_active_until) {
     int kk = (o->cfg.mode == SPARC_ADJOINT) ? o->cfg.o_rad - 1 : o->cfg.e_rad - 1;
     for (long c = 0; c < nxy; c++) {
-      long q = c + nxy * kk;
-      RHSv_z[q] = RHSv_z[q] + o->forcing[c] / bg[SPARC_BG_RHO0][q] * o->cfg.delta_width;
+      int kc = o->src_planes ? o->src_planes[c] : kk;
+      double dw = o->src_planes ? o->src_dw[c] : o->cfg.delta_width;
+      long q = c + nxy * kc;
+      RHSv_z[q] = RHSv_z[q] + o->forcing[c] * dw / bg[SPARC_BG_RHO0][q];
     }
   }
   for (int f = 0; f < 6; f++)
@@ -1096,8 +1123,10 @@ static void rhs_option18(orc *o) {
   if (o->src_kind != SPARC_SRC_NONE && o->time <= o->src_active_until) {
     int kk = (o->cfg.mode == SPARC_ADJOINT) ? o->cfg.o_rad - 1 : o->cfg.e_rad - 1;
     for (long c = 0; c < nxy; c++) {
-      long q = c + nxy * kk;
-      RHSv_z[q] = RHSv_z[q] + o->forcing[c] * o->cfg.delta_width / bg[SPARC_BG_RHO0][q];
+      int kc = o->src_planes ? o->src_planes[c] : kk;
+      double dw = o->src_planes ? o->src_dw[c] : o->cfg.delta_width;
+      long q = c + nxy * kc;
+      RHSv_z[q] = RHSv_z[q] + o->forcing[c] * dw / bg[SPARC_BG_RHO0][q];
     }
   }
   /* only plane 1 is zeroed (:930-933) */

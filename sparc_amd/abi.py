@@ -43,7 +43,7 @@ OP_DDX, OP_DDY, OP_DDZ, OP_ZFILTER, OP_DAMP, OP_LOWPASS_X, OP_LOWPASS_Y = range(
 EXPORTS = (
     "sparc_create", "sparc_destroy", "sparc_last_error", "sparc_device_bytes",
     "sparc_footprint", "sparc_set_background", "sparc_set_profile", "sparc_set_damping",
-    "sparc_set_filter", "sparc_set_source", "sparc_step", "sparc_sync", "sparc_reset",
+    "sparc_set_filter", "sparc_set_source", "sparc_set_source_planes", "sparc_step", "sparc_sync", "sparc_reset",
     "sparc_get_state", "sparc_put_state", "sparc_get_plane", "sparc_norm", "sparc_sample",
     "sparc_snapshots_alloc", "sparc_snapshot_store", "sparc_snapshots_copy",
     "sparc_kernel_begin", "sparc_kernel_accumulate", "sparc_kernel_get", "sparc_apply_op",
@@ -120,6 +120,7 @@ def lib() -> C.CDLL:
     L.sparc_set_damping.argtypes = [vp, DP, C.c_size_t]
     L.sparc_set_filter.argtypes = [vp, DP, DP]
     L.sparc_set_source.argtypes = [vp, C.c_int, DP, C.c_size_t, DP, C.c_int, C.c_int]
+    L.sparc_set_source_planes.argtypes = [vp, C.POINTER(C.c_int32), C.c_size_t]
     L.sparc_step.argtypes = [vp, C.c_int, C.c_int]
     L.sparc_sync.argtypes = [vp]
     L.sparc_reset.argtypes = [vp]
@@ -226,6 +227,15 @@ class Engine:
             nt = m.size // (self.cfg.nx * self.cfg.ny)
         self._chk(self.L.sparc_set_source(self.h, kind, _dp(m), m.size, _dp(s), nt, active_until),
                   "set_source")
+
+    def set_source_planes(self, planes):
+        """1-based per-pixel source planes (nx, ny), or None for the uniform plane."""
+        if planes is None:
+            self._chk(self.L.sparc_set_source_planes(self.h, None, 0), "set_source_planes")
+            return
+        p = np.asfortranarray(planes, dtype=np.int32).ravel(order="F")
+        self._chk(self.L.sparc_set_source_planes(
+            self.h, p.ctypes.data_as(C.POINTER(C.c_int32)), p.size), "set_source_planes")
 
     def step(self, t0: int, n: int):
         self._chk(self.L.sparc_step(self.h, t0, n), "sparc_step")

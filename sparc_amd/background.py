@@ -205,6 +205,23 @@ def init_rhs(z, c2, rho0, p0, g, npmlbot: int, npmltop: int, is2d: bool) -> dict
                 gradrho0_z=gradrho0_z, gradp0_z=gradp0_z, g=g, az=az, bzpml=bzpml)
 
 
+def wilson_planes(rho0_3d: np.ndarray, rho_q: np.ndarray, k_ref: int) -> np.ndarray:
+    """erad_2d / orad_2d of mp_Initialize_RHS (physics.f90:66-90): per pixel the
+    1-based plane in [k_ref-50, k_ref+10] whose rho0 is closest to the quiet
+    profile's rho_q(k_ref), later planes winning ties (the <= of the search),
+    starting from a tolerance of 1.  Pixels with no plane within it get 0."""
+    nx, ny, nz = rho0_3d.shape
+    target = float(rho_q[k_ref - 1])
+    best = np.ones((nx, ny))
+    out = np.zeros((nx, ny), dtype=np.int32)
+    for k in range(max(1, k_ref - 50), min(nz, k_ref + 10) + 1):
+        d = np.abs(target - rho0_3d[:, :, k - 1])
+        take = d <= best
+        out[take] = k
+        best[take] = d[take]
+    return out
+
+
 def sponge_z(z: np.ndarray, npml_zeroed: int = 90) -> np.ndarray:
     """spongez(k) of mp_Initialize_RHS (physics.f90:103-107) with the planes
     1..90 zeroed as the PML branch does (physics.f90:293); option 10 applies

@@ -35,7 +35,10 @@ struct Fld {
   int src_plane;  // 0-based plane receiving the source this step, -1 if none
   T delta_width;
   const T* srcdep;  // option 2: source_dep(k) over planes |k - src_plane| <= 4
+  const int32_t* src_planes;  // MHD: per-pixel 0-based source planes (null: src_plane)
+  const T* src_dw;            // MHD: delta_width_2d per pixel
 };
+
 
 // Coefficients of DBYD2's right-hand side (dbyd2.f90:88-196), cast to T.
 template <typename T>
@@ -54,6 +57,20 @@ struct TriCoef {
 template <typename T>
 __device__ __forceinline__ T bgv(const Fld<T>& F, int id, int c, int k) {
   return F.bg3d[id] ? F.bg[id][c + (long)F.nxy * k] : F.bg[id][k];
+}
+
+// The kernel-mode source term of the MHD displacement options at plane k of
+// column c (displacement.f90:813-836, physics2d.f90:902-926): at the uniform
+// plane, or at the pixel's erad_2d/orad_2d with its delta_width_2d.
+template <typename T>
+__device__ __forceinline__ T mhd_source(const Fld<T>& F, int c, int k) {
+  if (F.src_plane < 0) return (T)0;
+  if (F.src_planes) {
+    if (k != F.src_planes[c]) return (T)0;
+    return F.forcing[c] * F.src_dw[c] / bgv(F, SPARC_BG_RHO0, c, k);
+  }
+  if (k != F.src_plane) return (T)0;
+  return F.forcing[c] * F.delta_width / bgv(F, SPARC_BG_RHO0, c, k);
 }
 
 __device__ __forceinline__ int pml_index(int k, int nz, int nbot, int ntop) {
@@ -559,7 +576,7 @@ __global__ void k_mhd_rhs(Fld<T> F, const T* __restrict__ sponge, long n) {
     rz = rz - sp * vz;
   }
   rz = rz - ri * F.W[4][q] * F.g[k];
-  if (k == F.src_plane) rz = rz + F.forcing[c] / bgv(F, SPARC_BG_RHO0, c, k) * F.delta_width;
+  rz = rz + mhd_source(F, c, k);
   F.scr[SPARC_XI_X][q] = vx;
   F.scr[SPARC_XI_Y][q] = vy;
   F.scr[SPARC_XI_Z][q] = vz;
@@ -696,7 +713,7 @@ __global__ void k_mhd2_rhs(Fld<T> F, long n) {
   rx = rx + ri * (j0y * F.M[2][q]);
   rz = rz + ri * (-j0y * F.M[1][q]);
   rz = rz - ri * F.W[4][q] * F.g[k];
-  if (k == F.src_plane) rz = rz + F.forcing[c] * F.delta_width / bgv(F, SPARC_BG_RHO0, c, k);
+  rz = rz + mhd_source(F, c, k);
   F.scr[SPARC_XI_X][q] = vx;
   F.scr[SPARC_XI_Z][q] = vz;
   F.scr[SPARC_V_X][q] = rx;

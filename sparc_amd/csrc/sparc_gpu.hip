@@ -42,7 +42,8 @@ struct sparc_ctx {
   bool bgset[SPARC_BG_COUNT] = {};
   void* prof[SPARC_PROF_COUNT] = {};
   bool profset[SPARC_PROF_COUNT] = {};
-  std::vector<double> unstretch_h;
+  std::vector<double> unstretch_h, z_h;
+  void* planes_buf = nullptr;  // device storage behind src_planes
   void* W[8] = {};
   void* spec[3] = {};
   void* twx = nullptr;
@@ -63,6 +64,8 @@ struct sparc_ctx {
   // source
   int src_kind = SPARC_SRC_NONE, src_nt = 0, src_active_until = -1;
   void* src_mask = nullptr;
+  int32_t* src_planes = nullptr;  // per-pixel 0-based source planes (MHD), null = uniform
+  void* src_dw = nullptr;         // delta_width_2d per pixel
   std::vector<double> src_series;
   void* forcing = nullptr;
   // kernels
@@ -554,6 +557,7 @@ extern "C" int sparc_set_profile(sparc_ctx* c, int id, const double* h, int n) {
     return fail(c, SPARC_EINVAL, "bad profile id %d or length %d", id, n);
   c->profset[id] = true;
   if (id == SPARC_PROF_UNSTRETCH) c->unstretch_h.assign(h, h + n);
+  if (id == SPARC_PROF_Z) c->z_h.assign(h, h + n);
   return upload(c, c->prof[id], h, n);
 }
 
@@ -584,6 +588,36 @@ extern "C" int sparc_set_filter(sparc_ctx* c, const double* dx, const double* dy
   rc = mk(c->cfg.ny, dy, c->fmuly);
   if (rc) return rc;
   c->filter_set = true;
+  return SPARC_OK;
+}
+
+extern "C" int sparc_set_source_planes(sparc_ctx* c, const int32_t* k_xy, size_t n) {
+  if (!c) return SPARC_EINVAL;
+  if (n == 0) {
+    c->src_planes = nullptr;  // the device buffers stay allocated for reuse
+    return SPARC_OK;
+  }
+  if (!k_xy || n != (size_t)c->nxy) return fail(c, SPARC_EINVAL, "source planes need nx*ny values");
+  if (!c->profset[SPARC_PROF_Z]) return fail(c, SPARC_ESTATE, "set the z profile first");
+  const int nz = c->cfg.nz;
+  std::vector<int32_t> k0(n);
+  std::vector<double> dw(n);
+  for (size_t i = 0; i < n; ++i) {
+    const int k = k_xy[i];
+    if (k < 2 || k > nz - 1) return fail(c, SPARC_EINVAL, "source plane %d outside 2..nz-1", k);
+    k0[i] = k - 1;
+    dw[i] = 2.0 / (c->z_h[k] - c->z_h[k - 2]);  // 2/(z(k+1)-z(k-1)) with 1-based k
+  }
+  int rc;
+  void* p = c->planes_buf;
+  if (!p) {
+    if ((rc = dalloc(c, &p, n * sizeof(int32_t)))) return rc;
+    c->planes_buf = p;
+    if ((rc = dalloc(c, &c->src_dw, n * c->ts))) return rc;
+  }
+  HIPCHK(c, hipMemcpy(p, k0.data(), n * sizeof(int32_t), hipMemcpyHostToDevice));
+  if ((rc = upload(c, c->src_dw, dw.data(), n))) return rc;
+  c->src_planes = (int32_t*)p;
   return SPARC_OK;
 }
 
@@ -646,6 +680,8 @@ static Fld<T> make_fld(sparc_ctx* c) {
   F.src_plane = -1;
   F.delta_width = (T)c->cfg.delta_width;
   F.srcdep = (const T*)c->prof[SPARC_PROF_SOURCE_DEP];
+  F.src_planes = c->src_planes;
+  F.src_dw = (const T*)c->src_dw;
   return F;
 }
 

@@ -547,3 +547,33 @@ def test_mhd_adjoint_and_kernels_match_oracle():
     finally:
         for x in (fe, fo, ae, ao):
             x.close()
+
+
+def test_option10_wilson_depression_planes_match_oracle():
+    """Option 10 with per-pixel source planes (erad_2d from the Wilson-
+    depression search, physics.f90:66-90): a density model whose layers sit
+    one to two planes lower under part of the box moves the source there."""
+    from sparc_amd import background as Bm
+    W, S, eng, o = _mhd_case(abi.FP64, nsteps=10)
+    try:
+        nx, ny, nz = eng.shape
+        e_rad = eng.cfg.e_rad
+        rho_q = S.rho0[0, 0, :] if S.rho0.ndim == 3 else S.rho0
+        x = np.arange(nx)[:, None, None] / nx
+        shift = np.where(np.cos(2 * np.pi * x) > 0.3, 2, 0) + np.zeros((nx, ny, 1), dtype=int)
+        k = np.arange(nz)[None, None, :]
+        rho_dep = np.take(rho_q, np.clip(k + shift, 0, nz - 1))  # depressed layers
+        planes = Bm.wilson_planes(rho_dep, rho_q, e_rad)
+        assert set(np.unique(planes)) == {e_rad, e_rad - 2}
+        mask, series = D.forward_source(W, 18)
+        for e in (eng, o):
+            e.set_source(abi.SRC_SERIES, mask, series, 10 ** 9)
+            e.set_source_planes(planes)
+            e.step(0, 10)
+        for f in range(6):
+            a, b = eng.get(abi.ARR_A, f), o.get(abi.ARR_A, f)
+            assert np.linalg.norm(b) > 0, f
+            assert rel_err(a, b) < TOL64, f
+    finally:
+        eng.close()
+        o.close()
