@@ -162,7 +162,10 @@ extern "C" {
 #define SPARC_KERN_BX 6       /* magnetic kernels of the MHD displacement    */
 #define SPARC_KERN_BY 7       /*   options 10/18 (kernels.f90:336-430),      */
 #define SPARC_KERN_BZ 8       /*   unscaled sums                             */
-#define SPARC_KERN_COUNT 9
+#define SPARC_KERN_PSI 9      /* stream-function kernel: y component of      */
+                              /*   CURL_KERN(K_Bx, 0, K_Bz), formed from the */
+                              /*   sums when read (kernels.f90:508-509)      */
+#define SPARC_KERN_COUNT 10
 
 typedef struct sparc_config {
   int32_t abi_version;    /* must be SPARC_ABI_VERSION                          */

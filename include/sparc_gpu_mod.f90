@@ -41,7 +41,8 @@ module sparc_gpu
   integer(c_int), parameter :: SPARC_SRC_NONE = 0, SPARC_SRC_SERIES = 1, SPARC_SRC_LAGRANGE = 2
   integer(c_int), parameter :: SPARC_KERN_C2 = 0, SPARC_KERN_RHO = 1, SPARC_KERN_HESSIAN = 2, &
                                SPARC_KERN_VX = 3, SPARC_KERN_VY = 4, SPARC_KERN_VZ = 5, &
-                               SPARC_KERN_BX = 6, SPARC_KERN_BY = 7, SPARC_KERN_BZ = 8
+                               SPARC_KERN_BX = 6, SPARC_KERN_BY = 7, SPARC_KERN_BZ = 8, &
+                               SPARC_KERN_PSI = 9
 
   ! sparc_config of sparc_gpu.h: 24 int32 then 16 double, no padding
   type, bind(C) :: sparc_config

@@ -67,6 +67,7 @@ def lib():
         L.orc_kernel_begin.argtypes = [C.c_void_p, DP, C.c_int]
         L.orc_kernel_accumulate.argtypes = [C.c_void_p, C.c_int]
         L.orc_kernel_get.argtypes = [C.c_void_p, C.c_int, DP]
+        L.orc_kernel_put.argtypes = [C.c_void_p, C.c_int, DP]
         L.orc_damp_field.argtypes = [C.c_void_p, DP, DP]
         L.orc_init_rhs.argtypes = ([C.c_int] * 6 + [C.c_double] * 2 + [DP] * 13)
         _LIB = L
@@ -326,3 +327,7 @@ class Oracle:
         out = np.zeros(self.N)
         assert self.L.orc_kernel_get(self.h, which, _dp(out)) == 0
         return out.reshape(self.shape, order="F")
+
+    def kernel_put(self, which, data):
+        d = np.asfortranarray(data, dtype=np.float64)
+        assert self.L.orc_kernel_put(self.h, which, _dp(d)) == 0

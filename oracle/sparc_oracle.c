@@ -1572,7 +1572,25 @@ int orc_kernel_accumulate(orc *o, int slot) {
 
 int orc_kernel_get(orc *o, int which, double *h) {
   if (which < 0 || which >= SPARC_KERN_COUNT) return SPARC_EINVAL;
+  if (which == SPARC_KERN_PSI) {
+    /* kernel_vectorpsi (kernels.f90:508-509): the y component of
+     * CURL_KERN(K_Bx, 0, K_Bz) = ddzkern(K_Bx, ibc 0) - ddx(K_Bz) */
+    long N = o->N;
+    double *t = zalloc(sizeof(double) * N), *t2 = zalloc(sizeof(double) * N);
+    ddzkern(o, o->kern[SPARC_KERN_BX], o->kern[SPARC_KERN_PSI], 0, t2);
+    orc_ddx(o->kern[SPARC_KERN_BZ], t, o->nx, o->ny, o->nz, o->cfg.stretchx);
+    for (long c = 0; c < N; c++) o->kern[SPARC_KERN_PSI][c] -= t[c];
+    free(t);
+    free(t2);
+  }
   memcpy(h, o->kern[which], sizeof(double) * o->N);
+  return SPARC_OK;
+}
+
+/* sets a kernel sum directly, so tests can pin what is formed from the sums */
+int orc_kernel_put(orc *o, int which, const double *h) {
+  if (which < 0 || which >= SPARC_KERN_COUNT) return SPARC_EINVAL;
+  memcpy(o->kern[which], h, sizeof(double) * o->N);
   return SPARC_OK;
 }
 

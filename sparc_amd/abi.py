@@ -32,7 +32,7 @@ ARR_A, ARR_SCR, ARR_PMLVARS, ARR_SCRPML, ARR_ALO = range(5)
  BG_GRADRHO0_X, BG_GRADRHO0_Y) = range(18)
 PROF_STRETCH, PROF_UNSTRETCH, PROF_G, PROF_Z, PROF_SOURCE_DEP, PROF_SPONGE = range(6)
 SRC_NONE, SRC_SERIES, SRC_LAGRANGE = range(3)
-KERN_C2, KERN_RHO, KERN_HESSIAN, KERN_VX, KERN_VY, KERN_VZ, KERN_BX, KERN_BY, KERN_BZ = range(9)
+KERN_C2, KERN_RHO, KERN_HESSIAN, KERN_VX, KERN_VY, KERN_VZ, KERN_BX, KERN_BY, KERN_BZ, KERN_PSI = range(10)
 (KC_XLINES, KC_YLINES, KC_ZSOLVE, KC_DAMP, KC_UPDATE, KC_OTHER, KC_ZSOLVE_P, KC_DAMP_Y,
  KC_DAMP_XI, KC_FILTER) = range(10)
 KC_NAMES = ("xlines", "ylines", "zsolve", "damp", "update", "other", "zsolve_p", "damp_y",

@@ -816,4 +816,18 @@ __global__ void k_kern_flow(double* __restrict__ kx, double* __restrict__ ky,
   kz[q] = kz[q] + (double)dz[q] * sff * a;
 }
 
+template <typename T>
+__global__ void k_from_double(T* __restrict__ out, const double* __restrict__ in, long n) {
+  long q = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (q < n) out[q] = (T)in[q];
+}
+
+// y component of CURL_KERN(fx, 0, fz) = d_z fx - d_x fz, into FP64
+template <typename T>
+__global__ void k_curl_y(double* __restrict__ out, const T* __restrict__ dzfx,
+                         const T* __restrict__ dxfz, long n) {
+  long q = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (q < n) out[q] = (double)dzfx[q] - (double)dxfz[q];
+}
+
 }  // namespace sparc

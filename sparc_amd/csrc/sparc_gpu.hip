@@ -1751,9 +1751,38 @@ static int ckpt_copy(sparc_ctx* c, int slot, bool save) {
 extern "C" int sparc_checkpoint_save(sparc_ctx* c, int slot) { return ckpt_copy(c, slot, true); }
 extern "C" int sparc_checkpoint_load(sparc_ctx* c, int slot) { return ckpt_copy(c, slot, false); }
 
+// kernel_vectorpsi (kernels.f90:508-509): CURL_KERN(K_Bx, 0, K_Bz), y component,
+// d_z with ibc 0 + FILTER_VAR_Z, at the state precision over the FP64 sums.
+template <typename T>
+static int psi_kernel(sparc_ctx* c) {
+  Fld<T> F = make_fld<T>(c);
+  const long N = c->N, nxy = c->nxy;
+  const int nz = c->cfg.nz;
+  const dim3 gN(cdiv(N, 256)), b256(256);
+  T** W = F.W;
+  int rc;
+  hipLaunchKernelGGL((k_from_double<T>), gN, b256, 0, c->stream, W[0],
+                     (const double*)c->kern[SPARC_KERN_BX], N);
+  hipLaunchKernelGGL((k_from_double<T>), gN, b256, 0, c->stream, W[1],
+                     (const double*)c->kern[SPARC_KERN_BZ], N);
+  if ((rc = launch_z<T, EPI_STORE>(c, F, 0, W[0], nullptr, 0, nullptr, 0, W[2]))) return rc;
+  hipLaunchKernelGGL((k_update_filter<T>), dim3(cdiv(nxy, 64)), dim3(64), 0, c->stream, W[2],
+                     (const T*)nullptr, (T)0, (int)nxy, nz, 2);
+  if ((rc = launch_x<T, XM_DERIV>(c, W[1], nullptr, 0, W[3], (const T*)c->dmulx, nullptr)))
+    return rc;
+  hipLaunchKernelGGL((k_curl_y<T>), gN, b256, 0, c->stream, c->kern[SPARC_KERN_PSI],
+                     (const T*)W[2], (const T*)W[3], N);
+  HIPCHK(c, hipGetLastError());
+  return SPARC_OK;
+}
+
 extern "C" int sparc_kernel_get(sparc_ctx* c, int which, double* h) {
   if (!c || !h || which < 0 || which >= SPARC_KERN_COUNT || !c->kern[0] || !c->kern[which])
     return fail(c, SPARC_EINVAL, "bad kernel id, kernels not begun, or flow kernels off");
+  if (which == SPARC_KERN_PSI) {
+    const int rc = c->fp32 ? psi_kernel<float>(c) : psi_kernel<double>(c);
+    if (rc) return rc;
+  }
   HIPCHK(c, hipStreamSynchronize(c->stream));
   HIPCHK(c, hipMemcpy(h, c->kern[which], c->N * 8, hipMemcpyDeviceToHost));
   return SPARC_OK;

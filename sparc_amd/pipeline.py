@@ -209,6 +209,19 @@ def copy_snapshots(dst, src, nslots: int) -> None:
             dst.snapshot_put(s, f, src.snapshot_get(s, f))
 
 
+def kernel_ids(cfg) -> list:
+    """Kernel sums a solve with this configuration produces: c2, rho and the
+    hessian always, the flow kernels when switched on, the magnetic kernels
+    and the stream-function kernel for the MHD displacement options
+    (kernels.f90:465-517)."""
+    ids = [abi.KERN_C2, abi.KERN_RHO, abi.KERN_HESSIAN]
+    if getattr(cfg, "flow_kernels", 0):
+        ids += [abi.KERN_VX, abi.KERN_VY, abi.KERN_VZ]
+    if cfg.option in (abi.OPT_MHD, abi.OPT_MHD2D):
+        ids += [abi.KERN_BX, abi.KERN_BY, abi.KERN_BZ, abi.KERN_PSI]
+    return ids
+
+
 @dataclass
 class SolveResult:
     traces: np.ndarray
@@ -238,7 +251,7 @@ def full_solve(spec: SolveSpec, S: bgm.Setup, obs: np.ndarray, meas: RingMeasure
         else:
             copy_snapshots(ae, fe, spec.maxtime // (kstride or spec.steps) + 1)
             adjoint_solve(ae, spec, vr, S.g_raw, kstride=kstride, aligned=aligned)
-        raw = {k: ae.kernel_get(k) for k in (abi.KERN_C2, abi.KERN_RHO, abi.KERN_HESSIAN)}
+        raw = {k: ae.kernel_get(k) for k in kernel_ids(ae.cfg)}
         K = kernel_lnc(raw[abi.KERN_C2], S, spec, kstride=kstride)
         res = SolveResult(traces=syn, tau=tau, misfit=meas.misfit(tau), adj=adj, kernel_raw=raw,
                           K_lnc=K)
@@ -349,7 +362,7 @@ def full_solve_checkpointed(spec: SolveSpec, S: bgm.Setup, obs: np.ndarray,
         adj = meas.adjoint_source(syn, tau, obs=obs, exact=exact_adjoint)
         vr = adjoint_vr(spec, adj, S)
         adjoint_recompute(fe, ae, spec, vr, S.g_raw, every, kstride=kstride, aligned=aligned)
-        raw = {k: ae.kernel_get(k) for k in (abi.KERN_C2, abi.KERN_RHO, abi.KERN_HESSIAN)}
+        raw = {k: ae.kernel_get(k) for k in kernel_ids(ae.cfg)}
         K = kernel_lnc(raw[abi.KERN_C2], S, spec, kstride=kstride)
         peak = fe.device_bytes + ae.device_bytes
         return (SolveResult(traces=syn, tau=tau, misfit=meas.misfit(tau), adj=adj,

@@ -540,7 +540,7 @@ def test_mhd_adjoint_and_kernels_match_oracle():
             ae.kernel_accumulate_from(fe, nslots - 1 - sl)
             ao.kernel_accumulate(nslots - 1 - sl)
         for k in (abi.KERN_C2, abi.KERN_RHO, abi.KERN_HESSIAN, abi.KERN_BX, abi.KERN_BY,
-                  abi.KERN_BZ):
+                  abi.KERN_BZ, abi.KERN_PSI):
             a, b = ae.kernel_get(k), ao.kernel_get(k)
             assert np.linalg.norm(b) > 0, k
             assert rel_err(a, b) < TOL64, k

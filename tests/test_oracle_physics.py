@@ -378,3 +378,35 @@ def test_magnetic_kernel_closed_form():
     assert np.max(np.abs(kb[0][:, :, mid] - want)) < 1e-6 * scale
     assert np.max(np.abs(kb[1][:, :, mid])) < 1e-6 * scale
     assert np.max(np.abs(kb[2][:, :, mid])) < 1e-6 * scale
+
+
+def test_psi_kernel_is_the_y_curl_of_the_magnetic_kernel():
+    """kernel_vectorpsi = (CURL_KERN(K_Bx, 0, K_Bz))_y = ddzkern K_Bx - ddx K_Bz
+    (kernels.f90:508-509).  The x and z operators act on different axes and
+    commute, so a gradient field K_B = (ddx phi, ddzkern phi) has zero curl to
+    rounding; a swapped component or sign leaves 2 ddx ddzkern phi."""
+    nx, ny, nz = 32, 1, 40
+    cfg = abi.SparcConfig()
+    cfg.abi_version = abi.ABI_VERSION
+    cfg.nx, cfg.ny, cfg.nz = nx, ny, nz
+    cfg.option = abi.OPT_MHD2D
+    cfg.deltat = 1e-3
+    cfg.stretchx = cfg.stretchy = 1.3
+    cfg.e_rad, cfg.o_rad = 5, 6
+    o = O.Oracle(cfg)
+    stretch = 1.0 + 0.2 * np.linspace(0.0, 1.0, nz)
+    o.set_profile(abi.PROF_STRETCH, stretch)
+    o.set_background(abi.BG_C2, np.ones(nz))
+    rng = np.random.default_rng(3)
+    phi = rng.standard_normal((nx, ny, nz))
+    kx = O.ddx(phi, cfg.stretchx)
+    kz = O.zfilter(O.ddz(phi, 0, stretch), 2)
+    cross = O.zfilter(O.ddz(kx, 0, stretch), 2)
+    o.kernel_begin(np.zeros(nz))
+    o.kernel_put(abi.KERN_BX, kx)
+    o.kernel_put(abi.KERN_BZ, kz)
+    psi = o.kernel_get(abi.KERN_PSI)
+    o.close()
+    scale = np.max(np.abs(cross))
+    assert scale > 1.0
+    assert np.max(np.abs(psi)) < 1e-12 * scale
