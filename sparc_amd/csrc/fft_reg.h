@@ -149,11 +149,17 @@ template <int N>
 struct LineStride {
   static constexpr int LS = lpad(N) + 8;
 };
+// y tiles are filled and drained with consecutive lanes on consecutive lines:
+// a stride of 1 (mod 32) complex values puts 16 such lanes on distinct banks
+template <int N>
+struct YLineStride {
+  static constexpr int LS = (lpad(N) + 31) / 32 * 32 + 1;
+};
 
 // One in-place Stockham pass of radix R over LPB lines of length N in LDS:
 // every thread loads its butterflies into registers, the block synchronizes,
 // then the results are written back.
-template <typename T, int N, int R, int NS, int DIR, int LPB, int NT>
+template <typename T, int N, int R, int NS, int DIR, int LPB, int NT, int LS>
 __device__ __forceinline__ void rpass(C2<T>* buf, const C2<T>* __restrict__ tw) {
   constexpr int NB = N / R;
   constexpr int TOT = LPB * NB;
@@ -166,7 +172,7 @@ __device__ __forceinline__ void rpass(C2<T>* buf, const C2<T>* __restrict__ tw) 
       const int line = w / NB;
       const int j = w - line * NB;
 #pragma unroll
-      for (int r = 0; r < R; ++r) v[it][r] = buf[line * LineStride<N>::LS + lpad(j + r * NB)];
+      for (int r = 0; r < R; ++r) v[it][r] = buf[line * LS + lpad(j + r * NB)];
     }
   }
   __syncthreads();
@@ -188,20 +194,20 @@ __device__ __forceinline__ void rpass(C2<T>* buf, const C2<T>* __restrict__ tw) 
       dft_reg<T, R, DIR>(v[it]);
       const int idxD = (j / NS) * NS * R + k;
 #pragma unroll
-      for (int r = 0; r < R; ++r) buf[line * LineStride<N>::LS + lpad(idxD + r * NS)] = v[it][r];
+      for (int r = 0; r < R; ++r) buf[line * LS + lpad(idxD + r * NS)] = v[it][r];
     }
   }
   __syncthreads();
 }
 
 // Full in-place FFT of LPB lines in buf.
-template <typename T, int N, int DIR, int LPB, int NT>
+template <typename T, int N, int DIR, int LPB, int NT, int LS = LineStride<N>::LS>
 __device__ __forceinline__ void rfft(C2<T>* buf, const C2<T>* __restrict__ tw) {
   using PL = RPlan<N>;
   constexpr int R0 = PL::R[0], R1 = PL::R[1], R2 = PL::R[2];
-  rpass<T, N, R0, 1, DIR, LPB, NT>(buf, tw);
-  rpass<T, N, R1, R0, DIR, LPB, NT>(buf, tw);
-  if constexpr (PL::P == 3) rpass<T, N, R2, R0 * R1, DIR, LPB, NT>(buf, tw);
+  rpass<T, N, R0, 1, DIR, LPB, NT, LS>(buf, tw);
+  rpass<T, N, R1, R0, DIR, LPB, NT, LS>(buf, tw);
+  if constexpr (PL::P == 3) rpass<T, N, R2, R0 * R1, DIR, LPB, NT, LS>(buf, tw);
 }
 
 // lines per block: 16 KiB of complex values per block and 256 threads, i.e.
@@ -244,8 +250,13 @@ __global__ __launch_bounds__(NT) void k_xlines_n(int nrows, int nlines, XArgs<T>
   for (int m = threadIdx.x; m < N; m += NT) tw[m] = twg[m];
   const int L0 = blockIdx.x * LPB;
   const int nl = min(LPB, nlines - L0);
+  // damping inverse: the RHS values it updates, loaded with the spectrum
+  constexpr int IT = (LPB * N + NT - 1) / NT;
+  T ov0[MODE == XM_DAMP_INV ? IT : 1], ov1[MODE == XM_DAMP_INV ? IT : 1];
 #pragma unroll
-  for (int e = threadIdx.x; e < LPB * N; e += NT) {
+  for (int it = 0; it < IT; ++it) {
+    const int e = threadIdx.x + it * NT;
+    if (e >= LPB * N) break;
     const int l = e / N, x = e - l * N;
     C2<T> v = {0, 0};
     if (l < nl) {
@@ -274,6 +285,13 @@ __global__ __launch_bounds__(NT) void k_xlines_n(int nrows, int nlines, XArgs<T>
         v = {re, im};
       } else {
         v = spec[L * N + x];
+        if constexpr (MODE == XM_DAMP_INV) {
+          const long q = L / ny, j = L - q * ny;
+          const long k0 = 2 * q, k1 = k0 + 1;
+          const long o0 = x + (long)N * j + nxy * k0;
+          ov0[it] = (X.edges || (k0 != 0 && k0 != nz - 1)) ? out[o0] : (T)0;
+          ov1[it] = (k1 < nz - 1 || (X.edges && k1 < nz)) ? out[o0 + nxy] : (T)0;
+        }
       }
     }
     b0[l * LS + lpad(x)] = v;
@@ -316,15 +334,17 @@ __global__ __launch_bounds__(NT) void k_xlines_n(int nrows, int nlines, XArgs<T>
     }
   } else {
 #pragma unroll
-    for (int e = threadIdx.x; e < nl * N; e += NT) {
+    for (int it = 0; it < IT; ++it) {
+      const int e = threadIdx.x + it * NT;
+      if (e >= nl * N) break;
       const int l = e / N, x = e - l * N;
       const long L = L0 + l;
       const long q = L / ny, j = L - q * ny;
       const long k0 = 2 * q, k1 = k0 + 1;
       const long o0 = x + (long)N * j + nxy * k0;
       const C2<T> v = r[l * LS + lpad(x)];
-      if (X.edges || (k0 != 0 && k0 != nz - 1)) out[o0] = out[o0] - v.x;
-      if (k1 < nz - 1 || (X.edges && k1 < nz)) out[o0 + nxy] = out[o0 + nxy] - v.y;
+      if (X.edges || (k0 != 0 && k0 != nz - 1)) out[o0] = ov0[it] - v.x;
+      if (k1 < nz - 1 || (X.edges && k1 < nz)) out[o0 + nxy] = ov1[it] - v.y;
     }
   }
 }
@@ -334,15 +354,24 @@ __global__ __launch_bounds__(NT) void k_xlines_n(int nrows, int nlines, XArgs<T>
 // a plane-pair spectrum (DAMP).  blockIdx.z selects the field.
 __host__ __device__ constexpr int pow2_floor(int v) { return v >= 2 ? 2 * pow2_floor(v / 2) : 1; }
 
+#ifndef SPARC_Y_TILE_BYTES
+#define SPARC_Y_TILE_BYTES 32768
+#endif
+#ifndef SPARC_Y_SEG_BYTES
+#define SPARC_Y_SEG_BYTES 128
+#endif
+#ifndef SPARC_YDAMP_NT
+#define SPARC_YDAMP_NT 512  // measured 112 vs 124 us with 256 at 256x256x300 FP32
+#endif
 template <typename T, int NY, int MODE>
 struct YCfg {
   // 32 KiB of complex values per block; row segments of 128 bytes at most;
   // derivative/filter tiles use 512 threads (measured 4.7 vs 3.7 TB/s with 256)
-  static constexpr int LINES = pow2_floor((32768 / (int)sizeof(C2<T>)) / NY);
-  static constexpr int CAP = 128 / (int)sizeof(C2<T>);  // complex lines of 128 B
+  static constexpr int LINES = pow2_floor((SPARC_Y_TILE_BYTES / (int)sizeof(C2<T>)) / NY);
+  static constexpr int CAP = SPARC_Y_SEG_BYTES / (int)sizeof(C2<T>);  // complex lines per row segment
   static constexpr int NL = LINES < CAP ? LINES : CAP;
   static constexpr int TX = (MODE == YM_DAMP) ? NL : 2 * NL;
-  static constexpr int NT = (MODE == YM_DAMP || NL * NY < 4096) ? 256 : 512;
+  static constexpr int NT = MODE == YM_DAMP ? SPARC_YDAMP_NT : (NL * NY < 4096 ? 256 : 512);
 };
 
 template <typename T, int NY, int MODE, int NL = YCfg<T, NY, MODE>::NL,
@@ -352,7 +381,7 @@ __global__ __launch_bounds__(NT) void k_ylines_n(int nx, int nz, XArgs<T> X, T b
                                                  const C2<T>* __restrict__ twg,
                                                  const T* __restrict__ rates) {
   constexpr int TX = (MODE == YM_DAMP) ? NL : 2 * NL;
-  constexpr int LS = LineStride<NY>::LS;
+  constexpr int LS = YLineStride<NY>::LS;
   __shared__ C2<T> b0[NL * LS];
   __shared__ C2<T> tw[NY];
   const int fld = blockIdx.z;
@@ -375,7 +404,7 @@ __global__ __launch_bounds__(NT) void k_ylines_n(int nx, int nz, XArgs<T> X, T b
       b0r[2 * ((xx >> 1) * LS + lpad(y)) + (xx & 1)] = v;
     }
     __syncthreads();
-    rfft<T, NY, -1, NL, NT>(b0, tw);
+    rfft<T, NY, -1, NL, NT, LS>(b0, tw);
     C2<T>* r = b0;
 #pragma unroll
     for (int e = threadIdx.x; e < NL * NY; e += NT) {
@@ -389,7 +418,7 @@ __global__ __launch_bounds__(NT) void k_ylines_n(int nx, int nz, XArgs<T> X, T b
       r[l * LS + lpad(y)] = v;
     }
     __syncthreads();
-    rfft<T, NY, +1, NL, NT>(b0, tw);
+    rfft<T, NY, +1, NL, NT, LS>(b0, tw);
     const T* rr = reinterpret_cast<const T*>(b0);
 #pragma unroll
     for (int e = threadIdx.x; e < TX * NY; e += NT) {
@@ -405,20 +434,19 @@ __global__ __launch_bounds__(NT) void k_ylines_n(int nx, int nz, XArgs<T> X, T b
       b0[mm * LS + lpad(y)] = spec[(q * NY + y) * nx + i0 + mm];
     }
     __syncthreads();
-    rfft<T, NY, -1, NL, NT>(b0, tw);
+    rfft<T, NY, -1, NL, NT, LS>(b0, tw);
     C2<T>* r = b0;
-    const int nh = nx / 2 + 1;
 #pragma unroll 4
     for (int e = threadIdx.x; e < NL * NY; e += NT) {
       const int mm = e / NY, y = e - mm * NY;
       const int m = i0 + mm;
       const int mi = (m <= nx / 2) ? m : nx - m;
-      const T c = rates[mi + (long)nh * y];
+      const T c = rates[y + (long)NY * mi];
       const int o = mm * LS + lpad(y);
       r[o] = {c * r[o].x, c * r[o].y};
     }
     __syncthreads();
-    rfft<T, NY, +1, NL, NT>(b0, tw);
+    rfft<T, NY, +1, NL, NT, LS>(b0, tw);
 #pragma unroll 4
     for (int e = threadIdx.x; e < NL * NY; e += NT) {
       const int y = e / NL, mm = e - y * NL;

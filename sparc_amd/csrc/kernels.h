@@ -232,12 +232,11 @@ __global__ __launch_bounds__(256) void k_ylines(int nx, int ny, int nz, int TX,
     }
     __syncthreads();
     C2<T>* r = lds_fft<T, -1>(b0, b1, nl, P, tw);
-    const int nh = nx / 2 + 1;
     for (int e = threadIdx.x; e < nl * ny; e += blockDim.x) {
       const int mm = e / ny, y = e - mm * ny;
       const int m = i0 + mm;
       const int mi = (m <= nx / 2) ? m : nx - m;
-      const T c = rates[mi + (long)nh * y];
+      const T c = rates[y + (long)ny * mi];
       r[e] = {c * r[e].x, c * r[e].y};
     }
     __syncthreads();

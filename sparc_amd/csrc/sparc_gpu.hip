@@ -25,6 +25,7 @@ struct sparc_ctx {
   hipStream_t stream = nullptr;
   hipEvent_t xev = nullptr;  // orders this stream against another context's (cross-context calls)
   int device = 0;
+  int lds_max = 65536;  // dynamic LDS bytes one block may take
   long N = 0, nxy = 0, NP = 0;
   int nzpml = 0;
   void* a[SPARC_MAXFIELDS] = {};
@@ -361,6 +362,8 @@ extern "C" int sparc_create(const sparc_config* cfg, sparc_ctx** out) {
   c->nzpml = cfg->npmlbot + cfg->npmltop;
   c->NP = c->nxy * std::max(c->nzpml, 1);
   HIPCHK(c, hipGetDevice(&c->device));
+  HIPCHK(c, hipDeviceGetAttribute(&c->lds_max, hipDeviceAttributeMaxSharedMemoryPerBlock,
+                                  c->device));
   HIPCHK(c, hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
   HIPCHK(c, hipEventCreateWithFlags(&c->xev, hipEventDisableTiming));
   const size_t ts = c->ts, N = c->N;
@@ -572,7 +575,12 @@ extern "C" int sparc_set_damping(sparc_ctx* c, const double* r, size_t n) {
     int rc = upload(c, c->dmul2, t.data(), nx);
     if (rc) return rc;
   }
-  return upload(c, c->rates, r, n);
+  // kept transposed, rates_t[y + ny*m], so the y kernels read it along y
+  const int nh = c->cfg.nx / 2 + 1, ny = c->cfg.ny;
+  std::vector<double> t((size_t)nh * ny);
+  for (int y = 0; y < ny; ++y)
+    for (int m = 0; m < nh; ++m) t[y + (size_t)ny * m] = r[m + (size_t)nh * y];
+  return upload(c, c->rates, t.data(), n);
 }
 
 extern "C" int sparc_set_filter(sparc_ctx* c, const double* dx, const double* dy) {
@@ -718,11 +726,17 @@ static bool fast_size(int n) {
          n == 512;
 }
 
+#ifndef SPARC_XDAMP_LPB_DIV
+#define SPARC_XDAMP_LPB_DIV 1
+#endif
 template <typename T, int MODE, int N>
 static void launch_xn(sparc_ctx* c, int nf, const XArgs<T>& X, T beta, const T* mult, int nrows,
                       int nlines) {
-  constexpr int LPB = XCfg<T, N>::LPB;
-  hipLaunchKernelGGL((k_xlines_n<T, N, MODE>), dim3(cdiv(nlines, LPB), nf), dim3(XCfg<T, N>::NT), 0,
+  constexpr int LPB0 = XCfg<T, N>::LPB;
+  constexpr int LPB = (MODE == XM_DAMP_INV || MODE == XM_DAMP_FWD)
+                          ? (LPB0 / SPARC_XDAMP_LPB_DIV > 0 ? LPB0 / SPARC_XDAMP_LPB_DIV : 1)
+                          : LPB0;
+  hipLaunchKernelGGL((k_xlines_n<T, N, MODE, LPB>), dim3(cdiv(nlines, LPB), nf), dim3(XCfg<T, N>::NT), 0,
                      c->stream, nrows, nlines, X, beta, mult, (const C2<T>*)c->twx, c->cfg.ny,
                      c->cfg.nz, c->nxy);
 }
@@ -845,6 +859,18 @@ static int launch_y(sparc_ctx* c, const T* a, const T* s, T beta, T* out, const 
   return launch_ym<T, MODE>(c, 1, X, beta, mult);
 }
 
+// z-solve variant: ud in LDS (1) or in a global scratch array (0), and the
+// segments per column of the LDS variant.  Measured at 256x256x300 FP32 on
+// MI355X (us per launch, momentum / pressure epilogue): global ud with 4
+// segments 280 / 174, LDS with 4 segments 280 / 215 (two blocks of four
+// waves per CU), LDS with 8 segments 260 / 168.
+#ifndef SPARC_ZS_ULDS
+#define SPARC_ZS_ULDS 1
+#endif
+#ifndef SPARC_ZS_ULDS_NSEG
+#define SPARC_ZS_ULDS_NSEG 8
+#endif
+
 template <typename T, int EPI>
 static int launch_z(sparc_ctx* c, const Fld<T>& F, int ibc, const T* inA, const T* inS, T beta,
                     const T* prov, T provscale, T* out) {
@@ -877,19 +903,24 @@ static int launch_z(sparc_ctx* c, const Fld<T>& F, int ibc, const T* inA, const 
                nb * (double)c->N * sizeof(T));
   // segments per column: L = nz/NSEG planes each, at least ~48 (>= 2x warm-up)
   const int nseg = nz >= 192 ? 4 : (nz >= 96 ? 2 : 1);
-  const size_t lds = 5 * (size_t)nz * sizeof(T);
   const TriCoef<T> tcv = make_tc<T>(c, ibc);
   const dim3 grid(cdiv(c->nxy, 64));
-#define ZL(NS)                                                                                 \
-  hipLaunchKernelGGL((k_zsolve<T, EPI, NS>), grid, dim3(64 * NS), lds, c->stream, F, tcv, tri, \
-                     tri + nz, tri + 2 * nz, ibc, inA, inS, beta, (T*)c->W[7], prov, provscale,  \
+  // ud in LDS when two blocks of 64 columns fit in one CU's LDS
+  const size_t lds_ud = 64 * (size_t)nz * sizeof(T);
+  const bool ulds = SPARC_ZS_ULDS && nseg == 4 && lds_ud <= 81920 && lds_ud <= (size_t)c->lds_max;
+  const size_t lds = ulds ? lds_ud : 5 * (size_t)nz * sizeof(T);
+#define ZL(NS, U)                                                                                 \
+  hipLaunchKernelGGL((k_zsolve<T, EPI, NS, U>), grid, dim3(64 * NS), lds, c->stream, F, tcv, tri, \
+                     tri + nz, tri + 2 * nz, ibc, inA, inS, beta, (T*)c->W[7], prov, provscale,     \
                      out)
-  if (nseg == 4)
-    ZL(4);
+  if (ulds)
+    ZL(SPARC_ZS_ULDS_NSEG, true);
+  else if (nseg == 4)
+    ZL(4, false);
   else if (nseg == 2)
-    ZL(2);
+    ZL(2, false);
   else
-    ZL(1);
+    ZL(1, false);
 #undef ZL
   LAUNCHCHK(c);
   return SPARC_OK;
@@ -944,20 +975,21 @@ static int run_stage(sparc_ctx* c, Fld<T>& F, int s) {
     // ddxyz(p, bc=0) (displacement.f90:547)
     if ((rc = launch_x<T, XM_DERIV>(c, F.W[3], nullptr, 0, F.W[0], dmx, nullptr))) return rc;
     if ((rc = launch_y<T, YM_DERIV>(c, F.W[3], nullptr, 0, F.W[1], dmy, nullptr))) return rc;
-    const bool damp = c->cfg.damp_waves && c->damping_set;
-    XArgs<T> DX{};
-    for (int f = 0; f < 3; ++f) {
-      DX.a[f] = F.a[3 + f];
-      DX.s[f] = S(3 + f);
-      DX.out[f] = F.scr[3 + f];
-      DX.spec[f] = (C2<T>*)c->spec[f];
-    }
-    if (damp) {  // DAMP_VELOCITY forward half, on the stage's temp v (damping.f90:135-160)
-      if ((rc = launch_xm<T, XM_DAMP_FWD>(c, 3, DX, beta, nullptr))) return rc;
-      if ((rc = launch_ym<T, YM_DAMP>(c, 3, DX, 0, nullptr))) return rc;
-    }
     if ((rc = launch_z<T, EPI_RHS3D>(c, F, 0, F.W[3], nullptr, 0, nullptr, 0, nullptr))) return rc;
-    if (damp && (rc = launch_xm<T, XM_DAMP_INV>(c, 3, DX, 0, nullptr))) return rc;
+    if (c->cfg.damp_waves && c->damping_set) {
+      // DAMP_VELOCITY (damping.f90:125-185) on the stage's temp v, which the
+      // epilogue above has just stored as RHS_xi = v (zero on planes 1 and nz,
+      // whose damping output is discarded)
+      XArgs<T> DX{};
+      for (int f = 0; f < 3; ++f) {
+        DX.a[f] = F.scr[f];
+        DX.out[f] = F.scr[3 + f];
+        DX.spec[f] = (C2<T>*)c->spec[f];
+      }
+      if ((rc = launch_xm<T, XM_DAMP_FWD>(c, 3, DX, 0, nullptr))) return rc;
+      if ((rc = launch_ym<T, YM_DAMP>(c, 3, DX, 0, nullptr))) return rc;
+      if ((rc = launch_xm<T, XM_DAMP_INV>(c, 3, DX, 0, nullptr))) return rc;
+    }
   } else if (c->cfg.option == SPARC_OPT_MHD_PML_3D_DISPL) {
     // MP_MHD_PML_3D_DISPL (displacement.f90:663-849): 18 derivative passes
     // (ddx, ddy, compact ddz) joined by pointwise kernels

@@ -27,12 +27,12 @@ struct EIn {
 };
 
 template <typename T, int EPI>
-__device__ __forceinline__ void load_epi(const Fld<T>& F, const T* __restrict__ ud, int c, int k,
-                                         bool own, EIn<T>& e) {
+__device__ __forceinline__ void load_epi(const Fld<T>& F, const T* __restrict__ ud, long uoff,
+                                         long ustr, int c, int k, bool own, EIn<T>& e) {
   const int n = F.nz;
   const long nxy = F.nxy;
   const long q = c + nxy * k;
-  e.ud = ud[q];
+  e.ud = ud[uoff + ustr * k];
   if (!own) return;
   const bool st = F.beta != (T)0;
   if (EPI == EPI_P3D || EPI == EPI_P2D) {
@@ -242,8 +242,12 @@ struct ZWarm {
   static constexpr int W = sizeof(T) == 4 ? 20 : 42;
 };
 
-template <typename T, int EPI, int NSEG>
-__global__ __launch_bounds__(64 * NSEG) void k_zsolve(Fld<T> F, TriCoef<T> tc,
+// ULDS: the forward-sweep result ud lives in LDS (64 x nz values per block)
+// instead of a global scratch array, and the per-plane tables are read with
+// wave-uniform (scalar) loads to leave the LDS to ud.
+template <typename T, int EPI, int NSEG, bool ULDS>
+__global__ __launch_bounds__(64 * NSEG) __attribute__((amdgpu_waves_per_eu(4))) void k_zsolve(
+    Fld<T> F, TriCoef<T> tc,
                                                       const T* __restrict__ lowr,
                                                       const T* __restrict__ invb,
                                                       const T* __restrict__ gam, int ibc,
@@ -257,24 +261,46 @@ __global__ __launch_bounds__(64 * NSEG) void k_zsolve(Fld<T> F, TriCoef<T> tc,
   constexpr int ZW = ZWarm<T>::W;
   extern __shared__ unsigned char zsm[];
   const int n = F.nz;
+  constexpr bool HAS_G = (EPI == EPI_RHS3D || EPI == EPI_RHS2D || EPI == EPI_V2);
   T* s_st = reinterpret_cast<T*>(zsm);  // stretch
   T* s_gm = s_st + n;                   // gam
   T* s_lw = s_gm + n;                   // lowr
   T* s_ib = s_lw + n;                   // 1/betad
   T* s_g = s_ib + n;                    // g
-  for (int k = threadIdx.x; k < n; k += blockDim.x) {
-    s_st[k] = F.stretch[k];
-    s_gm[k] = gam[k];
-    s_lw[k] = lowr[k];
-    s_ib[k] = invb[k];
-    s_g[k] = (EPI == EPI_RHS3D || EPI == EPI_RHS2D || EPI == EPI_V2) ? F.g[k] : (T)0;
+  T* s_ud = reinterpret_cast<T*>(zsm);  // ULDS: ud[k * 64 + lane]
+  const T* __restrict__ g_st = F.stretch;
+  const T* __restrict__ g_g = F.g;
+  if constexpr (!ULDS) {
+    for (int k = threadIdx.x; k < n; k += blockDim.x) {
+      s_st[k] = F.stretch[k];
+      s_gm[k] = gam[k];
+      s_lw[k] = lowr[k];
+      s_ib[k] = invb[k];
+      s_g[k] = HAS_G ? F.g[k] : (T)0;
+    }
+    __syncthreads();
   }
-  __syncthreads();
+  auto ST = [&](int k) -> T { if constexpr (ULDS) return g_st[k]; else return s_st[k]; };
+  auto GM = [&](int k) -> T { if constexpr (ULDS) return gam[k]; else return s_gm[k]; };
+  auto LW = [&](int k) -> T { if constexpr (ULDS) return lowr[k]; else return s_lw[k]; };
+  auto IB = [&](int k) -> T { if constexpr (ULDS) return invb[k]; else return s_ib[k]; };
+  auto GG = [&](int k) -> T {
+    if constexpr (ULDS) return HAS_G ? g_g[k] : (T)0; else return s_g[k];
+  };
   const int lane = threadIdx.x & 63;
-  const int seg = threadIdx.x >> 6;
+  const int seg = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
   const int c = blockIdx.x * 64 + lane;
   const bool active = c < F.nxy;
   const long nxy = F.nxy;
+  auto UDW = [&](int k, T v) {
+    if constexpr (ULDS)
+      s_ud[k * 64 + lane] = v;
+    else
+      ud[c + nxy * k] = v;
+  };
+  const T* udr = ULDS ? (const T*)s_ud : (const T*)ud;
+  const long uoff = ULDS ? (long)lane : (long)c;
+  const long ustr = ULDS ? 64L : nxy;
   const int L = (n + NSEG - 1) / NSEG;
   const int ks = seg * L;
   const int ke = min(n, ks + L);
@@ -305,10 +331,10 @@ __global__ __launch_bounds__(64 * NSEG) void k_zsolve(Fld<T> F, TriCoef<T> tc,
         r0 = prov ? prov[c] * provscale : (T)0;
       const T r1 = tc.c1 * w0 + tc.c2 * w1 + tc.c3 * w2 + tc.c4 * w3 + tc.c5 * w4;
       if (ibc & 1) r0 = (tc.alpha2 * r0 - tc.beta1 * r1) * tc.cst;
-      u = r0 * s_ib[0];
-      if (ks == 0) ud[c] = u;
-      u = (r1 - s_lw[1] * u) * s_ib[1];
-      if (ks <= 1 && ke > 1) ud[c + nxy] = u;
+      u = r0 * IB(0);
+      if (ks == 0) UDW(0, u);
+      u = (r1 - LW(1) * u) * IB(1);
+      if (ks <= 1 && ke > 1) UDW(1, u);
       k0 = 2;
       wm1 = 0;
     } else {
@@ -348,8 +374,8 @@ __global__ __launch_bounds__(64 * NSEG) void k_zsolve(Fld<T> F, TriCoef<T> tc,
         const int k = kb + j;
         if (k < kend) {
           const T r = tc.ci1 * (w3 - w1) + tc.ci2 * (w4 - w0);
-          u = (r - s_lw[k] * u) * s_ib[k];
-          if (k >= ks) ud[c + nxy * k] = u;
+          u = (r - LW(k) * u) * IB(k);
+          if (k >= ks) UDW(k, u);
           wm1 = w0;
           w0 = w1;
           w1 = w2;
@@ -362,8 +388,8 @@ __global__ __launch_bounds__(64 * NSEG) void k_zsolve(Fld<T> F, TriCoef<T> tc,
     if (ke == n) {
       // window holds B(n-5..n-1) in wm1..w3
       const T rn2 = -(tc.c1 * w3 + tc.c2 * w2 + tc.c3 * w1 + tc.c4 * w0 + tc.c5 * wm1);
-      u = (rn2 - s_lw[n - 2] * u) * s_ib[n - 2];
-      ud[c + nxy * (n - 2)] = u;
+      u = (rn2 - LW(n - 2) * u) * IB(n - 2);
+      UDW(n - 2, u);
       T rn1;
       if (ibc == 0)
         rn1 = 0;
@@ -373,8 +399,8 @@ __global__ __launch_bounds__(64 * NSEG) void k_zsolve(Fld<T> F, TriCoef<T> tc,
       } else {
         rn1 = prov ? prov[c] * provscale : (T)0;
       }
-      u = (rn1 - s_lw[n - 1] * u) * s_ib[n - 1];
-      ud[c + nxy * (n - 1)] = u;  // x_{n-1}: read back by every wave that starts exactly
+      u = (rn1 - LW(n - 1) * u) * IB(n - 1);
+      UDW(n - 1, u);  // x_{n-1}: read back by every wave that starts exactly
     }
   }
   __syncthreads();
@@ -386,7 +412,7 @@ __global__ __launch_bounds__(64 * NSEG) void k_zsolve(Fld<T> F, TriCoef<T> tc,
 #pragma unroll
     for (int j = 0; j < UB; ++j) {
       const int k = kstart - j;
-      if (k >= ks) load_epi<T, EPI>(F, ud, c, k, k < ke, nx[j]);
+      if (k >= ks) load_epi<T, EPI>(F, udr, uoff, ustr, c, k, k < ke, nx[j]);
     }
     T x = 0;
     bool first = true;
@@ -398,7 +424,7 @@ __global__ __launch_bounds__(64 * NSEG) void k_zsolve(Fld<T> F, TriCoef<T> tc,
 #pragma unroll
         for (int j = 0; j < UB; ++j) {
           const int k = kh - UB - j;
-          if (k >= ks) load_epi<T, EPI>(F, ud, c, k, k < ke, nx[j]);
+          if (k >= ks) load_epi<T, EPI>(F, udr, uoff, ustr, c, k, k < ke, nx[j]);
         }
       }
 #pragma unroll
@@ -409,9 +435,9 @@ __global__ __launch_bounds__(64 * NSEG) void k_zsolve(Fld<T> F, TriCoef<T> tc,
             x = cu[j].ud;  // exact x_{n-1} = u_{n-1}, or the cold start
             first = false;
           } else {
-            x = cu[j].ud - s_gm[k + 1] * x;
+            x = cu[j].ud - GM(k + 1) * x;
           }
-          if (k < ke) epilogue<T, EPI>(F, c, k, x * s_st[k], s_g[k], cu[j], out);
+          if (k < ke) epilogue<T, EPI>(F, c, k, x * ST(k), GG(k), cu[j], out);
         }
       }
     }
