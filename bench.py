@@ -32,11 +32,11 @@ KERNEL_SYMBOLS = {
     "xlines": "sparc::k_xlines_n<float, 256, 0 (XM_DERIV), 8, 256> -- x derivative",
     "ylines": "sparc::k_ylines_n<float, 256, 0 (YM_DERIV), 16, 512> -- y derivative",
     "damp": "sparc::k_xlines_n<float, 256, 2 (XM_DAMP_FWD), 8, 256> -- damping, forward x",
-    "damp_y": "sparc::k_ylines_n<float, 256, 2 (YM_DAMP), 16, 256> -- damping, y pass",
+    "damp_y": "sparc::k_ylines_n<float, 256, 2 (YM_DAMP), 16, 512> -- damping, y pass",
     "damp_xi": "sparc::k_xlines_n<float, 256, 3 (XM_DAMP_INV), 8, 256> -- damping, inverse x",
-    "zsolve": "sparc::k_zsolve<float, 2 (EPI_RHS3D), 4> -- z solve + momentum/continuity RHS",
-    "zsolve_p": "sparc::k_zsolve<float, 1 (EPI_P3D), 4> -- z solve + pressure/density",
-    "update": "sparc::k_update_filter_c -- a += dt*scr with the z filter (hi+lo state)",
+    "zsolve": "sparc::k_zsolve<float, 2 (EPI_RHS3D), 8, true> -- z solve (ud in LDS) + momentum/continuity RHS",
+    "zsolve_p": "sparc::k_zsolve<float, 1 (EPI_P3D), 8, true> -- z solve (ud in LDS) + pressure/density",
+    "update": "sparc::k_update_filter_c -- a += dt*scr with the z filter (hi+lo state), all fields per launch",
 }
 
 

@@ -258,12 +258,23 @@ __global__ __launch_bounds__(256) void k_ylines(int nx, int ny, int nz, int TX,
 // column with an 11-value register window; output plane k is written after
 // input plane k+5 is read, so the update is in place.
 // ---------------------------------------------------------------------------
+// Fields of one update launch, selected by blockIdx.y (scr == nullptr: the
+// filter alone).
 template <typename T>
-__global__ __launch_bounds__(64) void k_update_filter(T* a, const T* __restrict__ scr, T dt,
-                                                      int nxy, int nz, int kst) {
-  constexpr int U = 16;  // planes loaded one chunk ahead of use
+struct UpdArgs {
+  T* a[SPARC_MAXFIELDS];
+  float* lo[SPARC_MAXFIELDS];
+  const T* scr[SPARC_MAXFIELDS];
+};
+
+template <typename T>
+__global__ __launch_bounds__(64) void k_update_filter(UpdArgs<T> A, T dt, int nxy, int nz,
+                                                      int kst) {
+  constexpr int U = 8;  // planes loaded one chunk ahead of use
   const int c = blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= nxy) return;
+  T* a = A.a[blockIdx.y];
+  const T* __restrict__ scr = A.scr[blockIdx.y];
   const T c0 = (T)0.75390625, h1 = (T)(0.5 * 0.41015625), h2 = (T)(0.5 * -0.234375),
           h3 = (T)(0.5 * 0.087890625), h4 = (T)(0.5 * -0.01953125),
           h5 = (T)(0.5 * 0.001953125);
@@ -327,12 +338,14 @@ __global__ __launch_bounds__(64) void k_update_filter(T* a, const T* __restrict_
 // filter run in FP64 on hi + lo + dt*scr and the result is split back into
 // hi = fl(v), lo = fl(v - hi), so the state carries ~48 significant bits and
 // the per-step rounding of the sum does not accumulate over a solve.
-__global__ __launch_bounds__(64) void k_update_filter_c(float* a, float* lo,
-                                                        const float* __restrict__ scr, double dt,
-                                                        int nxy, int nz, int kst) {
-  constexpr int U = 16;
+__global__ __launch_bounds__(64) void k_update_filter_c(UpdArgs<float> A, double dt, int nxy,
+                                                        int nz, int kst) {
+  constexpr int U = 8;
   const int c = blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= nxy) return;
+  float* a = A.a[blockIdx.y];
+  float* lo = A.lo[blockIdx.y];
+  const float* __restrict__ scr = A.scr[blockIdx.y];
   const double c0 = 0.75390625, h1 = 0.5 * 0.41015625, h2 = 0.5 * -0.234375,
                h3 = 0.5 * 0.087890625, h4 = 0.5 * -0.01953125, h5 = 0.5 * 0.001953125;
   auto V = [&](float x, float l, float s) -> double {

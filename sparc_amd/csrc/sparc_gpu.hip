@@ -7,7 +7,9 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <algorithm>
 #include <string>
+#include <utility>
 #include <vector>
 
 #include "../../include/sparc_gpu.h"
@@ -926,6 +928,15 @@ static int launch_z(sparc_ctx* c, const Fld<T>& F, int ibc, const T* inA, const 
   return SPARC_OK;
 }
 
+// FILTER_VAR_Z alone (kernels.f90:845-881) on one field, in place
+template <typename T>
+static void zfilter_only(sparc_ctx* c, T* f) {
+  UpdArgs<T> A{};
+  A.a[0] = f;
+  hipLaunchKernelGGL((k_update_filter<T>), dim3(cdiv(c->nxy, 64), 1), dim3(64), 0, c->stream, A,
+                     (T)0, (int)c->nxy, c->cfg.nz, 2);
+}
+
 // Lagrange weights of LAGRANGE_INTERP (physics.f90:680-734) for one time.
 static LagArgs lagrange_args(const sparc_ctx* c, int time) {
   LagArgs la{};
@@ -1270,15 +1281,28 @@ static int step_impl(sparc_ctx* c, int t0, int nsteps) {
     // a += deltat*scr with the z filter (step.f90:114-118)
     const T dt = (T)c->cfg.deltat;
     const int nfld = c->nf;  // dimfive
-    for (int f = 0; f < nfld; ++f) {
-      ProfScope ps(c, SPARC_KC_UPDATE, (c->comp ? 5.0 : 3.0) * (double)c->N * sizeof(T));
+    {
+      // all fields in one launch (blockIdx.y = field)
+      ProfScope ps(c, SPARC_KC_UPDATE,
+                   nfld * (c->comp ? 5.0 : 3.0) * (double)c->N * sizeof(T));
+      UpdArgs<T> A{};
+      for (int f = 0; f < nfld; ++f) {
+        A.a[f] = F.a[f];
+        A.lo[f] = (float*)c->alo[f];
+        A.scr[f] = (const T*)F.scr[f];
+      }
       if (c->comp) {
-        hipLaunchKernelGGL(k_update_filter_c, dim3(cdiv(nxy, 64)), dim3(64), 0, c->stream,
-                           (float*)F.a[f], (float*)c->alo[f], (const float*)F.scr[f],
-                           c->cfg.deltat, nxy, c->cfg.nz, is2d ? 6 : 2);
+        UpdArgs<float> Af{};
+        for (int f = 0; f < nfld; ++f) {
+          Af.a[f] = (float*)F.a[f];
+          Af.lo[f] = (float*)c->alo[f];
+          Af.scr[f] = (const float*)F.scr[f];
+        }
+        hipLaunchKernelGGL(k_update_filter_c, dim3(cdiv(nxy, 64), nfld), dim3(64), 0, c->stream,
+                           Af, c->cfg.deltat, nxy, c->cfg.nz, is2d ? 6 : 2);
       } else {
-        hipLaunchKernelGGL((k_update_filter<T>), dim3(cdiv(nxy, 64)), dim3(64), 0, c->stream,
-                           F.a[f], (const T*)F.scr[f], dt, nxy, c->cfg.nz, is2d ? 6 : 2);
+        hipLaunchKernelGGL((k_update_filter<T>), dim3(cdiv(nxy, 64), nfld), dim3(64), 0,
+                           c->stream, A, dt, nxy, c->cfg.nz, is2d ? 6 : 2);
       }
       LAUNCHCHK(c);
     }
@@ -1597,8 +1621,7 @@ static int magnetic_kernels(sparc_ctx* c, Fld<T>& F, const T* fx, const T* fy, c
   auto dzk = [&](const T* in, T* out) -> int {  // ddzkern with ibc 0
     int r = launch_z<T, EPI_STORE>(c, F, 0, in, nullptr, 0, nullptr, 0, out);
     if (r) return r;
-    hipLaunchKernelGGL((k_update_filter<T>), dim3(cdiv(nxy, 64)), dim3(64), 0, c->stream, out,
-                       (const T*)nullptr, (T)0, (int)nxy, nz, 2);
+    zfilter_only<T>(c, out);
     return SPARC_OK;
   };
   auto curl = [&](const T* x, const T* y, const T* z, T* ox, T* oy, T* oz) -> int {
@@ -1640,8 +1663,7 @@ static int magnetic_kernels(sparc_ctx* c, Fld<T>& F, const T* fx, const T* fy, c
   const bool div_y = c->cfg.ny > 4;
   if (div_y && (rc = launch_y<T, YM_DERIV>(c, ay, nullptr, 0, W[1], dmy, nullptr))) return rc;
   if ((rc = launch_z<T, EPI_STORE>(c, F, 1, az, nullptr, 0, nullptr, 0, W[2]))) return rc;
-  hipLaunchKernelGGL((k_update_filter<T>), dim3(cdiv(nxy, 64)), dim3(64), 0, c->stream, W[2],
-                     (const T*)nullptr, (T)0, (int)nxy, nz, 2);
+  zfilter_only<T>(c, W[2]);
   hipLaunchKernelGGL((k_sum3<T>), gN, b256, 0, c->stream, W[6], (const T*)W[0],
                      div_y ? (const T*)W[1] : nullptr, (const T*)W[2], N);
   cross(xi, J0, M[0], M[1], M[2]);
@@ -1673,8 +1695,7 @@ static int kernel_acc_impl(sparc_ctx* c, const sparc_ctx* src, int slot) {
     if ((r = launch_x<T, XM_DERIV>(c, fx, nullptr, 0, F.W[0], dmx, nullptr))) return r;
     if (has_y && (r = launch_y<T, YM_DERIV>(c, fy, nullptr, 0, F.W[1], dmy, nullptr))) return r;
     if ((r = launch_z<T, EPI_STORE>(c, F, 1, fz, nullptr, 0, nullptr, 0, F.W[2]))) return r;
-    hipLaunchKernelGGL((k_update_filter<T>), dim3(cdiv(nxy, 64)), dim3(64), 0, c->stream, F.W[2],
-                       (const T*)nullptr, (T)0, (int)nxy, nz, 2);
+    zfilter_only<T>(c, F.W[2]);
     LAUNCHCHK(c);
     hipLaunchKernelGGL((k_sum3<T>), dim3(cdiv(N, 256)), dim3(256), 0, c->stream, outdiv,
                        (const T*)F.W[0], has_y ? (const T*)F.W[1] : nullptr, (const T*)F.W[2], N);
@@ -1692,8 +1713,7 @@ static int kernel_acc_impl(sparc_ctx* c, const sparc_ctx* src, int slot) {
   if (has_y2 && (rc = launch_y<T, YM_DERIV>(c, F.W[3], nullptr, 0, F.W[1], dmy, nullptr)))
     return rc;
   if ((rc = launch_z<T, EPI_STORE>(c, F, 1, F.W[3], nullptr, 0, nullptr, 0, F.W[2]))) return rc;
-  hipLaunchKernelGGL((k_update_filter<T>), dim3(cdiv(nxy, 64)), dim3(64), 0, c->stream, F.W[2],
-                     (const T*)nullptr, (T)0, (int)nxy, nz, 2);
+  zfilter_only<T>(c, F.W[2]);
   LAUNCHCHK(c);
   const double sff = 1.0 / src->snap_scale, sfa = 1.0 / c->cfg.field_scale;
   hipLaunchKernelGGL((k_kern_accum<T>), dim3(cdiv(N, 256)), dim3(256), 0, c->stream,
@@ -1715,8 +1735,7 @@ static int kernel_acc_impl(sparc_ctx* c, const sparc_ctx* src, int slot) {
       if (has_y && (rc = launch_y<T, YM_DERIV>(c, fv, nullptr, 0, F.W[1], dmy, nullptr)))
         return rc;
       if ((rc = launch_z<T, EPI_STORE>(c, F, 1, fv, nullptr, 0, nullptr, 0, F.W[2]))) return rc;
-      hipLaunchKernelGGL((k_update_filter<T>), dim3(cdiv(nxy, 64)), dim3(64), 0, c->stream,
-                         F.W[2], (const T*)nullptr, (T)0, (int)nxy, nz, 2);
+      zfilter_only<T>(c, F.W[2]);
       LAUNCHCHK(c);
       hipLaunchKernelGGL((k_kern_flow<T>), dim3(cdiv(N, 256)), dim3(256), 0, c->stream,
                          c->kern[SPARC_KERN_VX], c->kern[SPARC_KERN_VY], c->kern[SPARC_KERN_VZ],
@@ -1798,8 +1817,7 @@ static int psi_kernel(sparc_ctx* c) {
   hipLaunchKernelGGL((k_from_double<T>), gN, b256, 0, c->stream, W[1],
                      (const double*)c->kern[SPARC_KERN_BZ], N);
   if ((rc = launch_z<T, EPI_STORE>(c, F, 0, W[0], nullptr, 0, nullptr, 0, W[2]))) return rc;
-  hipLaunchKernelGGL((k_update_filter<T>), dim3(cdiv(nxy, 64)), dim3(64), 0, c->stream, W[2],
-                     (const T*)nullptr, (T)0, (int)nxy, nz, 2);
+  zfilter_only<T>(c, W[2]);
   if ((rc = launch_x<T, XM_DERIV>(c, W[1], nullptr, 0, W[3], (const T*)c->dmulx, nullptr)))
     return rc;
   hipLaunchKernelGGL((k_curl_y<T>), gN, b256, 0, c->stream, c->kern[SPARC_KERN_PSI],
@@ -1854,8 +1872,7 @@ static int apply_op_impl(sparc_ctx* c, int op, int ibc, const double* in, double
     }
     case SPARC_OP_ZFILTER:
       HIPCHK(c, hipMemcpyAsync(dst, src, N * sizeof(T), hipMemcpyDeviceToDevice, c->stream));
-      hipLaunchKernelGGL((k_update_filter<T>), dim3(cdiv(nxy, 64)), dim3(64), 0, c->stream, dst,
-                         (const T*)nullptr, (T)0, (int)nxy, nz, 2);
+      zfilter_only<T>(c, dst);
       LAUNCHCHK(c);
       rc = SPARC_OK;
       break;
