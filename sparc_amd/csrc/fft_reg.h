@@ -232,7 +232,20 @@ struct XArgs {
   T* out[3];
   C2<T>* spec[3];
   int edges;  // damping inverse also on planes 1 and nz (option 2 zeroes before damping)
+  // derivative written as a momentum right-hand side, -rhoinv * d, zero on
+  // planes 1 and nz (displacement.f90:589-597,639-642): rhoinv as a z profile
+  // (rz) or a 3-D array (r3)
+  int rhs;
+  const T* rz;
+  const T* r3;
 };
+
+template <typename T>
+__device__ __forceinline__ T rhs_of(const XArgs<T>& X, long k, long o, int nz, T d) {
+  if (k == 0 || k == nz - 1) return (T)0;
+  const T ri = X.r3 ? X.r3[o] : X.rz[k];
+  return -ri * d;
+}
 
 template <typename T, int N, int MODE, int LPB = XCfg<T, N>::LPB, int NT = XCfg<T, N>::NT>
 __global__ __launch_bounds__(NT) void k_xlines_n(int nrows, int nlines, XArgs<T> X, T beta,
@@ -329,8 +342,13 @@ __global__ __launch_bounds__(NT) void k_xlines_n(int nrows, int nlines, XArgs<T>
       const long L = L0 + l;
       const long r0 = 2 * L, r1 = r0 + 1;
       const C2<T> v = r[l * LS + lpad(x)];
-      out[r0 * N + x] = v.x;
-      if (r1 < nrows) out[r1 * N + x] = v.y;
+      if (MODE == XM_DERIV && X.rhs) {  // row r = j + ny*k
+        out[r0 * N + x] = rhs_of(X, r0 / ny, r0 * N + x, nz, v.x);
+        if (r1 < nrows) out[r1 * N + x] = rhs_of(X, r1 / ny, r1 * N + x, nz, v.y);
+      } else {
+        out[r0 * N + x] = v.x;
+        if (r1 < nrows) out[r1 * N + x] = v.y;
+      }
     }
   } else {
 #pragma unroll
@@ -424,7 +442,8 @@ __global__ __launch_bounds__(NT) void k_ylines_n(int nx, int nz, XArgs<T> X, T b
     for (int e = threadIdx.x; e < TX * NY; e += NT) {
       const int y = e / TX, xx = e - y * TX;
       const long o = (i0 + xx) + (long)nx * y + nxy * k;
-      out[o] = rr[2 * ((xx >> 1) * LS + lpad(y)) + (xx & 1)];
+      const T d = rr[2 * ((xx >> 1) * LS + lpad(y)) + (xx & 1)];
+      out[o] = (MODE == YM_DERIV && X.rhs) ? rhs_of(X, k, o, nz, d) : d;
     }
   } else {
     const long q = k;

@@ -828,6 +828,18 @@ __global__ void k_kern_flow(double* __restrict__ kx, double* __restrict__ ky,
   kz[q] = kz[q] + (double)dz[q] * sff * a;
 }
 
+// -rhoinv * d with planes 1 and nz zeroed: a derivative turned into a
+// momentum right-hand side (the generic-size path of XArgs::rhs)
+template <typename T>
+__global__ void k_rhs_grad(T* __restrict__ out, const T* __restrict__ d,
+                           const T* __restrict__ rz, const T* __restrict__ r3, long nxy,
+                           int nz, long n) {
+  long q = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (q >= n) return;
+  const long k = q / nxy;
+  out[q] = (k == 0 || k == nz - 1) ? (T)0 : -(r3 ? r3[q] : rz[k]) * d[q];
+}
+
 template <typename T>
 __global__ void k_from_double(T* __restrict__ out, const double* __restrict__ in, long n) {
   long q = blockIdx.x * (long)blockDim.x + threadIdx.x;

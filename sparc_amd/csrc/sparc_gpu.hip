@@ -786,6 +786,11 @@ static int launch_xm(sparc_ctx* c, int nf, const XArgs<T>& X, T beta, const T* m
                        n, (int)nrows, (int)nlines, LPB, X.a[f], X.s[f], beta, X.out[f], mult,
                        c->px, (const C2<T>*)c->twx, X.spec[f], c->cfg.ny, c->cfg.nz, c->nxy);
     LAUNCHCHK(c);
+    if (MODE == XM_DERIV && X.rhs) {
+      hipLaunchKernelGGL((k_rhs_grad<T>), dim3(cdiv(c->N, 256)), dim3(256), 0, c->stream,
+                         X.out[f], (const T*)X.out[f], X.rz, X.r3, c->nxy, c->cfg.nz, c->N);
+      LAUNCHCHK(c);
+    }
   }
   return SPARC_OK;
 }
@@ -846,6 +851,11 @@ static int launch_ym(sparc_ctx* c, int nf, const XArgs<T>& X, T beta, const T* m
                        ny, c->cfg.nz, TX, X.a[f], X.s[f], beta, X.out[f], mult, c->py,
                        (const C2<T>*)c->twy, X.spec[f], (const T*)c->rates);
     LAUNCHCHK(c);
+    if (MODE == YM_DERIV && X.rhs) {
+      hipLaunchKernelGGL((k_rhs_grad<T>), dim3(cdiv(c->N, 256)), dim3(256), 0, c->stream,
+                         X.out[f], (const T*)X.out[f], X.rz, X.r3, c->nxy, c->cfg.nz, c->N);
+      LAUNCHCHK(c);
+    }
   }
   return SPARC_OK;
 }
@@ -892,8 +902,8 @@ static int launch_z(sparc_ctx* c, const Fld<T>& F, int ibc, const T* inA, const 
     nb = in + 2 + nbg({SPARC_BG_C2RHO0, SPARC_BG_GRADP0_Z, SPARC_BG_RHO0, SPARC_BG_GRADRHO0_Z}) + 2;
   else if (EPI == EPI_P2D)
     nb = in + 1 + nbg({SPARC_BG_C2RHO0, SPARC_BG_GRADP0_Z, SPARC_BG_RHO0, SPARC_BG_GRADRHO0_Z}) + 2;
-  else if (EPI == EPI_RHS3D)
-    nb = in + 3 + (F.beta != (T)0 ? 6 : 3) + nbg({SPARC_BG_RHOINV}) + 6;
+  else if (EPI == EPI_RHS3D)  // rho temp, a (+ scr) of v; RHS of xi, v_z, rho
+    nb = in + 1 + (F.beta != (T)0 ? 6 : 3) + nbg({SPARC_BG_RHOINV}) + 4;
   else if (EPI == EPI_C2)  // v_z again (own plane), dvxdx, dvydy -> W4, W5
     nb = in + 2 + nbg({SPARC_BG_C2RHO0, SPARC_BG_GRADP0_Z, SPARC_BG_RHO0, SPARC_BG_GRADRHO0_Z}) + 2;
   else if (EPI == EPI_V2)  // gradp_x, gradp_y, W4, W5, rho temp -> scr of 5 fields
@@ -983,10 +993,21 @@ static int run_stage(sparc_ctx* c, Fld<T>& F, int s) {
     if ((rc = launch_x<T, XM_DERIV>(c, F.a[0], S(0), beta, F.W[0], dmx, nullptr))) return rc;
     if ((rc = launch_y<T, YM_DERIV>(c, F.a[1], S(1), beta, F.W[1], dmy, nullptr))) return rc;
     if ((rc = launch_z<T, EPI_P3D>(c, F, 1, F.a[2], S(2), beta, nullptr, 0, nullptr))) return rc;
-    // ddxyz(p, bc=0) (displacement.f90:547)
-    if ((rc = launch_x<T, XM_DERIV>(c, F.W[3], nullptr, 0, F.W[0], dmx, nullptr))) return rc;
-    if ((rc = launch_y<T, YM_DERIV>(c, F.W[3], nullptr, 0, F.W[1], dmy, nullptr))) return rc;
+    // ddxyz(p, bc=0) (displacement.f90:547): the z part with the RHS epilogue
+    // (RHS_xi = temp v, RHS_vz, RHS_rho terms; reads the previous RHS_v), then
+    // the x and y parts written straight into RHS_vx, RHS_vy as -rhoinv*dp
     if ((rc = launch_z<T, EPI_RHS3D>(c, F, 0, F.W[3], nullptr, 0, nullptr, 0, nullptr))) return rc;
+    {
+      XArgs<T> G{};
+      G.a[0] = F.W[3];
+      G.rhs = 1;
+      G.rz = F.bg3d[SPARC_BG_RHOINV] ? nullptr : F.bg[SPARC_BG_RHOINV];
+      G.r3 = F.bg3d[SPARC_BG_RHOINV] ? F.bg[SPARC_BG_RHOINV] : nullptr;
+      G.out[0] = F.scr[SPARC_V_X];
+      if ((rc = launch_xm<T, XM_DERIV>(c, 1, G, 0, dmx))) return rc;
+      G.out[0] = F.scr[SPARC_V_Y];
+      if ((rc = launch_ym<T, YM_DERIV>(c, 1, G, 0, dmy))) return rc;
+    }
     if (c->cfg.damp_waves && c->damping_set) {
       // DAMP_VELOCITY (damping.f90:125-185) on the stage's temp v, which the
       // epilogue above has just stored as RHS_xi = v (zero on planes 1 and nz,

@@ -45,8 +45,7 @@ __device__ __forceinline__ void load_epi(const Fld<T>& F, const T* __restrict__ 
     if (F.bg3d[SPARC_BG_RHO0]) e.g2 = F.bg[SPARC_BG_RHO0][q];
     if (F.bg3d[SPARC_BG_GRADRHO0_Z]) e.g3 = F.bg[SPARC_BG_GRADRHO0_Z][q];
   } else if (EPI == EPI_RHS3D || EPI == EPI_RHS2D) {
-    e.w0 = F.W[0][q];
-    if (EPI == EPI_RHS3D) e.w1 = F.W[1][q];
+    if (EPI == EPI_RHS2D) e.w0 = F.W[0][q];
     e.w4 = F.W[4][q];
     e.a0 = F.a[SPARC_V_X][q];
     e.a2 = F.a[SPARC_V_Z][q];
@@ -184,14 +183,16 @@ __device__ __forceinline__ void epilogue(const Fld<T>& F, int c, int k, T val, T
     const T ri = bg_or(F, SPARC_BG_RHOINV, e.g0, k);
     const T rho = e.w4;
     if (EPI == EPI_RHS3D) {
-      T rvx = -ri * e.w0;
-      T rvy = -ri * e.w1;
+      // RHS_vx, RHS_vy = -rhoinv * (dp/dx, dp/dy) come from the x and y
+      // derivative passes that follow this solve
       T rvz = -ri * gz;
       rvz = rvz - rho * gk * ri;
       if (k == F.src_plane) rvz = rvz + F.forcing[c] / bgv(F, SPARC_BG_RHO0, c, k) * F.delta_width;
       if (k == 0 || k == n - 1) {
-#pragma unroll
-        for (int f = 0; f < SPARC_NFIELDS; ++f) F.scr[f][q] = 0;
+        F.scr[SPARC_XI_X][q] = 0;
+        F.scr[SPARC_XI_Y][q] = 0;
+        F.scr[SPARC_XI_Z][q] = 0;
+        F.scr[SPARC_V_Z][q] = 0;
       } else {
         T tvx = e.a0, tvy = e.a1, tvz = e.a2;
         if (st) {
@@ -202,8 +203,6 @@ __device__ __forceinline__ void epilogue(const Fld<T>& F, int c, int k, T val, T
         F.scr[SPARC_XI_X][q] = tvx;
         F.scr[SPARC_XI_Y][q] = tvy;
         F.scr[SPARC_XI_Z][q] = tvz;
-        F.scr[SPARC_V_X][q] = rvx;
-        F.scr[SPARC_V_Y][q] = rvy;
         F.scr[SPARC_V_Z][q] = rvz;
       }
     } else {
