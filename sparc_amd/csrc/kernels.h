@@ -80,6 +80,7 @@ __device__ __forceinline__ int pml_index(int k, int nz, int nbot, int ntop) {
 }
 
 #include "fft_reg.h"
+#include "fft_r16.h"
 
 // ---------------------------------------------------------------------------
 // x-lines: batched FFT along the contiguous index.  Real rows are packed two

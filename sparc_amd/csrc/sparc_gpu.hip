@@ -55,7 +55,8 @@ struct sparc_ctx {
   void* dmuly = nullptr;
   void* fmulx = nullptr;
   void* fmuly = nullptr;
-  void* rates = nullptr;
+  void* rates = nullptr;    // damping rates, transposed: [y + ny*m]
+  void* rates_o = nullptr;  // damping rates as given: [m + (nx/2+1)*y]
   bool damping_set = false, filter_set = false;
   void* tri[4] = {};  // per ibc 0..3: lowr | invb | gam, nz each
   double tc[14] = {};
@@ -305,7 +306,7 @@ static size_t footprint_bytes(const sparc_config* cfg) {
   if (!is_2d(cfg->option) && cfg->damp_waves)
     b += 3 * spec_elems(cfg) * 2 * ts;      // damping spectra
   b += (size_t)(cfg->nx + cfg->ny) * 6 * ts + 4 * 3 * cfg->nz * ts + SPARC_PROF_COUNT * cfg->nz * ts;
-  b += (size_t)(cfg->nx / 2 + 1) * cfg->ny * ts + 2 * nxy * ts;
+  b += 2 * (size_t)(cfg->nx / 2 + 1) * cfg->ny * ts + 2 * nxy * ts;  // rates (two layouts)
   return b;
 }
 
@@ -427,6 +428,7 @@ extern "C" int sparc_create(const sparc_config* cfg, sparc_ctx** out) {
   if ((rc = dalloc(c, &c->fmulx, cfg->nx * ts))) return rc;
   if ((rc = dalloc(c, &c->fmuly, cfg->ny * ts))) return rc;
   if ((rc = dalloc(c, &c->rates, (size_t)(cfg->nx / 2 + 1) * cfg->ny * ts))) return rc;
+  if ((rc = dalloc(c, &c->rates_o, (size_t)(cfg->nx / 2 + 1) * cfg->ny * ts))) return rc;
   // tridiagonal factors for ibc 0..3 and RHS coefficients (dbyd2.f90:88-146)
   for (int ibc = 0; ibc < 4; ++ibc) {
     std::vector<double> lowr, invb, gam;
@@ -582,6 +584,8 @@ extern "C" int sparc_set_damping(sparc_ctx* c, const double* r, size_t n) {
   std::vector<double> t((size_t)nh * ny);
   for (int y = 0; y < ny; ++y)
     for (int m = 0; m < nh; ++m) t[y + (size_t)ny * m] = r[m + (size_t)nh * y];
+  int rc = upload(c, c->rates_o, r, n);
+  if (rc) return rc;
   return upload(c, c->rates, t.data(), n);
 }
 
@@ -731,9 +735,26 @@ static bool fast_size(int n) {
 #ifndef SPARC_XDAMP_LPB_DIV
 #define SPARC_XDAMP_LPB_DIV 1
 #endif
+// Length-256 lines use the register four-step kernels of fft_r16.h for the
+// y passes and the damping inverse x pass.  Measured at 256x256x300 FP32 (us
+// per launch, radix-16 Stockham of fft_reg.h -> four-step): y derivative
+// 59.3 -> 50.9, damping y 108.1 -> 97.9, damping inverse x 131.9 -> 125.0;
+// but x derivative 65.1 -> 78.6 and damping forward x 99.5 -> 124.4 (lanes
+// owning every 16th element read 64-byte pieces of a row instead of whole
+// rows), so those keep the Stockham kernels.
+#ifndef SPARC_R16
+#define SPARC_R16 1
+#endif
+
 template <typename T, int MODE, int N>
 static void launch_xn(sparc_ctx* c, int nf, const XArgs<T>& X, T beta, const T* mult, int nrows,
                       int nlines) {
+  if constexpr (N == 256 && SPARC_R16 && MODE == XM_DAMP_INV) {
+    hipLaunchKernelGGL((k_x256<T, MODE>), dim3(cdiv(nlines, 16), nf), dim3(256), 0, c->stream,
+                       nrows, nlines, X, beta, mult, (const C2<T>*)c->twx, c->cfg.ny, c->cfg.nz,
+                       c->nxy);
+    return;
+  }
   constexpr int LPB0 = XCfg<T, N>::LPB;
   constexpr int LPB = (MODE == XM_DAMP_INV || MODE == XM_DAMP_FWD)
                           ? (LPB0 / SPARC_XDAMP_LPB_DIV > 0 ? LPB0 / SPARC_XDAMP_LPB_DIV : 1)
@@ -810,6 +831,15 @@ template <typename T, int MODE, int NY>
 static void launch_yn(sparc_ctx* c, int nf, const XArgs<T>& X, T beta, const T* mult) {
   constexpr int TX = YCfg<T, NY, MODE>::TX;
   const int planes = (MODE == YM_DAMP) ? (c->cfg.nz + 1) / 2 : c->cfg.nz;
+  if constexpr (NY == 256 && SPARC_R16) {
+    const int bx = MODE == YM_DAMP ? 16 : 32;
+    if (c->cfg.nx % bx == 0) {
+      hipLaunchKernelGGL((k_y256<T, MODE>), dim3(c->cfg.nx / bx, planes, nf), dim3(256), 0,
+                         c->stream, c->cfg.nx, c->cfg.nz, X, beta, mult, (const C2<T>*)c->twy,
+                         (const T*)c->rates_o);
+      return;
+    }
+  }
   hipLaunchKernelGGL((k_ylines_n<T, NY, MODE>), dim3(c->cfg.nx / TX, planes, nf),
                      dim3(YCfg<T, NY, MODE>::NT), 0,
                      c->stream, c->cfg.nx, c->cfg.nz, X, beta, mult, (const C2<T>*)c->twy,

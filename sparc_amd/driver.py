@@ -93,6 +93,7 @@ def workload(name: str, precision: int | None = None) -> Workload:
     box2l  384 x 384 x 300, 300 Mm (the large grid of the serving mix)
     tiny   32 x 32 x 40, 25 Mm (parity tests)
     tall   32 x 32 x 200, 25 Mm (parity tests of the segmented z solve)
+    wide   256 x 256 x 40, 200 Mm (parity tests of the length-256 line kernels)
     column 32 x 32 x 100, 25 Mm, the vertical grid of box1 (adjoint/gradient tests:
            receivers and source lie below the top PML and filter layers)
     two_d  256 x 1 x 300, 400 Mm, option 17 (the reference's wired 2-D case)
@@ -115,6 +116,9 @@ def workload(name: str, precision: int | None = None) -> Workload:
     elif name == "tall":
         nx = ny = 32
         nz, L, nsteps, prec = 200, 25.0, 10, abi.FP64
+    elif name == "wide":
+        nx = ny = 256
+        nz, L, nsteps, prec = 40, 200.0, 10, abi.FP64
     elif name == "column":
         nx = ny = 32
         nz, L, nsteps, prec = 100, 25.0, 400, abi.FP64

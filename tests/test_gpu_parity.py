@@ -25,6 +25,13 @@ def tiny():
 
 
 @pytest.fixture(scope="module")
+def wide():
+    W = D.workload("wide")
+    S = D.setup_for(W)
+    return W, S
+
+
+@pytest.fixture(scope="module")
 def tall():
     W = D.workload("tall")
     S = D.setup_for(W)
@@ -90,9 +97,12 @@ def _field(shape, seed):
     return np.asfortranarray(f)
 
 
+@pytest.mark.parametrize("grid", ["tiny", "wide"])
 @pytest.mark.parametrize("prec,tol", [(abi.FP64, TOL64), (abi.FP32, TOL32_OP)])
-def test_operators_match_oracle(tiny, prec, tol):
-    W, S = tiny
+def test_operators_match_oracle(request, grid, prec, tol):
+    """x, y, z derivatives, filters and the damping transform; the 256-wide
+    grid runs the length-256 register four-step line kernels."""
+    W, S = request.getfixturevalue(grid)
     eng, cfg = D.make_engine(W, S, precision=prec)
     try:
         f = _field(eng.shape, 1)
@@ -145,6 +155,24 @@ def test_step_3d_matches_oracle(tiny, prec, comp):
         for f in range(2):
             a, b = eng.get(abi.ARR_PMLVARS, f), o.get(abi.ARR_PMLVARS, f)
             assert (rel_err(a, b) < TOL64) if prec == abi.FP64 else (rel_l2(a, b) < 1e-4)
+    finally:
+        eng.close()
+        o.close()
+
+
+@pytest.mark.parametrize("prec", [abi.FP64, abi.FP32])
+def test_step_3d_wide_matches_oracle(wide, prec):
+    """Option 4 on 256 x 256 x 40: the length-256 x/y kernels in the stage
+    (derivatives, RHS epilogue of the pressure gradient, damping) and the
+    spectral filter at time 0."""
+    W, S = wide
+    eng, o = _run_pair(W, S, 3, prec)
+    try:
+        for arr, fields in ((abi.ARR_A, range(6)), (abi.ARR_SCR, range(6))):
+            for f in fields:
+                a, b = eng.get(arr, f), o.get(arr, f)
+                assert np.linalg.norm(b) > 0, (arr, f)
+                assert (rel_err(a, b) < TOL64) if prec == abi.FP64 else (rel_l2(a, b) < TOL32_L2)
     finally:
         eng.close()
         o.close()
