@@ -175,17 +175,50 @@ def observe_all(spec_for_source, setup, sources: list, eng_factory=None) -> list
 
 
 def gradient_pass(spec_for_source, setup, sources: list, observations: list,
-                  eng_factory=None, **kw):
+                  eng_factory=None, kernel_dir: str | None = None, **kw):
     """full.py:48-97: forward, travel times, adjoint and kernels for every
     source.  Returns the per-source SolveResults (traces, misfit, kernel
-    sums) and RingMeasurements (windows reused by the line search)."""
+    sums) and RingMeasurements (windows reused by the line search).  With
+    kernel_dir, each source's kernel files and misfit line are also written
+    there as PRODUCE_KERNELS and MISFIT_ALL name them (kernel_c_SS.fits,
+    hessian_SS.fits, ..., misfit_SS_00)."""
     from . import pipeline as PL
     results, meas = [], []
-    for s, obs in zip(sources, observations):
-        r, m = PL.full_solve(spec_for_source(s), setup, obs, eng_factory=eng_factory, **kw)
+    for i, (s, obs) in enumerate(zip(sources, observations)):
+        spec = spec_for_source(s)
+        r, m = PL.full_solve(spec, setup, obs, eng_factory=eng_factory, **kw)
         results.append(r)
         meas.append(m)
+        if kernel_dir is not None:
+            write_solve_files(kernel_dir, i + 1, r, setup, spec)
     return results, meas
+
+
+def write_solve_files(directory: str, contrib: int, res, setup, spec) -> list:
+    """One source's kernel files and misfit line in the reference's layout."""
+    from . import abi
+    from . import formats as FM
+    files = FM.kernel_files(res.kernel_raw, setup.rho0, setup.c2, stepskern=spec.steps,
+                            timestep=spec.W.params.timestep,
+                            magnetic=abi.KERN_BX in res.kernel_raw)
+    return FM.write_kernel_files(directory, contrib, files, misfit=res.misfit,
+                                 nmeas=len(spec.receivers))
+
+
+def read_gradient_inputs(directory: str, nsources: int, psi: bool = True):
+    """The per-source files grad.py reads (grad.py:158-164): kernel_c_SS,
+    kernel_vectorpsi_SS (when psi) and hessian_SS.  Returns (kernels_c,
+    kernels_psi or None, hessians)."""
+    import os
+    from . import formats as FM
+
+    def rd(name, s):
+        return FM.fits_read(os.path.join(directory, "%s_%02d.fits" % (name, s)))
+    srcs = range(1, nsources + 1)
+    kc = [rd("kernel_c", s) for s in srcs]
+    kp = [rd("kernel_vectorpsi", s) for s in srcs] if psi else None
+    hs = [rd("hessian", s) for s in srcs]
+    return kc, kp, hs
 
 
 def linesearch_pass(spec_for_source, setups: list, sources: list, observations: list,

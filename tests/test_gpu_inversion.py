@@ -16,7 +16,7 @@ from sparc_amd import pipeline as PL
 pytestmark = pytest.mark.gpu
 
 
-def test_one_iteration_lowers_the_misfit():
+def test_one_iteration_lowers_the_misfit(tmp_path):
     W = D.workload("tiny")
     S0 = D.setup_for(W)
     rec = np.array([[16, 16], [18, 16], [16, 13]])
@@ -28,9 +28,15 @@ def test_one_iteration_lowers_the_misfit():
 
     true = PL.bump_c2(S0, W, amp=0.05, radius_mm=3.0, depth_mm=2.0)
     obs = INV.observe_all(spec_for, D.setup_for(W, c2_3d=true), sources)
-    res, meas = INV.gradient_pass(spec_for, S0, sources, obs)
+    res, meas = INV.gradient_pass(spec_for, S0, sources, obs, kernel_dir=str(tmp_path))
     chi0 = INV.total_misfit([r.misfit for r in res])
     assert chi0 > 0
+    # the kernel files are the reference's kernel_c (a fixed multiple of K_lnc)
+    kc, _, hs = INV.read_gradient_inputs(str(tmp_path), len(sources), psi=False)
+    for r, k, h in zip(res, kc, hs):
+        ratio = k[r.K_lnc != 0] / r.K_lnc[r.K_lnc != 0]
+        assert np.allclose(ratio, ratio[0], rtol=1e-12)
+        assert np.array_equal(h, r.kernel_raw[abi.KERN_HESSIAN])
     grad, _ = INV.gradients([r.K_lnc for r in res], None,
                             [r.kernel_raw[abi.KERN_HESSIAN] for r in res], W.tab[:, 2])
     assert np.all(np.isfinite(grad))

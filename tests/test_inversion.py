@@ -78,3 +78,28 @@ def test_linesearch_models_and_jobs():
     assert jobs[14] == (5, 3) and jobs[15] is None
     assert INV.total_misfit([1.0, 2.5]) == 3.5
     assert INV.linesearch_totals(np.arange(15.0), 3) == [3.0, 12.0, 21.0, 30.0, 39.0]
+
+
+def test_kernel_files_round_trip_through_grad_inputs(tmp_path):
+    """Kernel files written as PRODUCE_KERNELS names them are what the
+    gradient step reads back (grad.py:158-164)."""
+    from types import SimpleNamespace
+    from sparc_amd import abi
+    rng = np.random.default_rng(2)
+    shape = (6, 5, 20)
+    setup = SimpleNamespace(rho0=np.ones(shape) * 2.0, c2=np.ones(shape) * 3.0)
+    spec = SimpleNamespace(steps=15, receivers=np.zeros((4, 2)),
+                           W=SimpleNamespace(params=SimpleNamespace(timestep=2.0)))
+    res = []
+    for s in range(2):
+        raw = {k: rng.standard_normal(shape) for k in (abi.KERN_C2, abi.KERN_RHO, abi.KERN_HESSIAN)}
+        r = SimpleNamespace(kernel_raw=raw, misfit=0.5 * (s + 1))
+        INV.write_solve_files(str(tmp_path), s + 1, r, setup, spec)
+        res.append(r)
+    kc, kp, hs = INV.read_gradient_inputs(str(tmp_path), 2, psi=False)
+    assert kp is None
+    for s in range(2):
+        assert np.array_equal(hs[s], res[s].kernel_raw[abi.KERN_HESSIAN])
+        assert np.allclose(kc[s] / res[s].kernel_raw[abi.KERN_C2], kc[0] / res[0].kernel_raw[abi.KERN_C2])
+    lines = [np.loadtxt(str(tmp_path / ("misfit_%02d_00" % (s + 1))), usecols=[2]) for s in range(2)]
+    assert INV.total_misfit(lines) == 1.5
