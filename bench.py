@@ -57,6 +57,10 @@ def main() -> int:
                     help="serving benchmark instead: the first N requests of the arrival "
                          "stream through the FP32 serving loop, then through the plain FP64 path")
     ap.add_argument("--serving-rate", type=float, default=1.0, help="arrivals per second")
+    ap.add_argument("--solve", type=int, default=0, metavar="STEPS",
+                    help="one forward + adjoint + kernel solve of STEPS steps on the workload "
+                         "(observed data from a +2%% sound-speed bump, 8 receivers on a 10 Mm "
+                         "ring) instead of the step benchmark")
     args = ap.parse_args()
     if args.copy_probe:
         return copy_probe(args.copy_probe)
@@ -65,6 +69,11 @@ def main() -> int:
             print("bench.py measures one GPU only", file=sys.stderr)
             return 2
         return serving_bench(args.serving, args.serving_rate)
+    if args.solve:
+        if args.gpus != 1:
+            print("bench.py measures one GPU only", file=sys.stderr)
+            return 2
+        return solve_bench(args.workload, args.precision, args.solve)
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (this build has no multi-GPU path); "
               f"--gpus {args.gpus} is not supported", file=sys.stderr)
@@ -127,8 +136,13 @@ def main() -> int:
             "traffic": traffic, "algorithmic_bytes_per_launch": by / nl,
             "us_per_launch": 1e3 * ms / nl, "kernels": kernels}
     # bytes per point and step: 18 field arrays moved per field and step, 5
-    # 3-D background arrays, 2 more per field for the compensated state
-    step_algo = (18 * 6 + (12 if cfg.compensated else 0) + 5 * 5) * (4 if prec == abi.FP32 else 8)
+    # 3-D background arrays per stage (15 with a magnetic field), 2 more per
+    # field for the compensated state
+    mhd = P.option in (abi.OPT_MHD, abi.OPT_MHD8)
+    nbg = 15 if mhd else 5
+    step_algo = (18 * 6 + (12 if cfg.compensated else 0) + 5 * nbg) * (4 if prec == abi.FP32 else 8)
+    opt = "option 10" if P.option == abi.OPT_MHD else ("option 8" if P.option == abi.OPT_MHD8
+                                                        else "option 4")
     eng.close()
 
     cpu = None
@@ -136,7 +150,7 @@ def main() -> int:
         cpu = cpu_baseline(W)
 
     line = {
-        "metric": "Mpt-updates/s per RK step (forward, option 4, 256x256x300)",
+        "metric": f"Mpt-updates/s per RK step (forward, {opt}, {P.nx}x{P.ny}x{P.nz})",
         "value": round(mpts, 2),
         "unit": "Mpt/s",
         "n_gpus": 1,
@@ -147,10 +161,11 @@ def main() -> int:
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f32" if prec == abi.FP32 else "f64",
-        "data": "synthetic (polytrope background, Gaussian x Ricker source)",
+        "data": "synthetic (polytrope background" + (" with a 3 kG flux tube" if mhd else "")
+                + ", Gaussian x Ricker source)",
         "config": {"workload": f"{W.name}: {P.nx}x{P.ny}x{P.nz} polytrope box, "
-                               f"{P.xlength / 1e8:.0f} Mm, option 4 with PML 10+10, damping, "
-                               f"filters, dt={P.timestep:.4f}s",
+                               f"{P.xlength / 1e8:.0f} Mm, {opt} with PML {P.npmlbot}+{P.npmltop}, "
+                               f"damping, filters, dt={P.timestep:.4f}s",
                    "nx": P.nx, "ny": P.ny, "nz": P.nz, "parallelism": "single GPU",
                    "finite": finite,
                    "compensated_fp32_state": bool(cfg.compensated),
@@ -225,6 +240,48 @@ def copy_probe(mib: int) -> int:
     torch.cuda.synchronize()
     print(json.dumps({"copy_probe_bytes_read": 4 * n, "copy_probe_bytes_written": 4 * n,
                       "copies": 5}))
+    return 0
+
+
+def solve_bench(workload: str, precision: str, nsteps: int) -> int:
+    """Wall time of one forward + adjoint + kernel solve (the kernel-mode
+    sequence of driver.f90) on the workload, with the kernels it returns."""
+    import numpy as np
+
+    from sparc_amd import abi
+    from sparc_amd import driver as D
+    from sparc_amd import pipeline as PL
+    from sparc_amd.measure import ring_pixels
+
+    prec = abi.FP32 if precision == "fp32" else abi.FP64
+    W = D.workload(workload, precision=prec)
+    P = W.params
+    S0 = D.setup_for(W)
+    rec = ring_pixels(P.nx, P.ny, P.xlength, P.ylength, W.src_xy_mm[0], W.src_xy_mm[1], 10.0, 8)
+    maxtime = max(P.steps, nsteps // P.steps * P.steps)
+    spec = PL.SolveSpec(W=W, receivers=rec, maxtime=maxtime, precision=prec)
+    print(f"[solve] workload={W.name} grid={P.nx}x{P.ny}x{P.nz} option={P.option} "
+          f"steps={maxtime + 1} precision={precision}", file=sys.stderr, flush=True)
+    t0 = time.perf_counter()
+    obs = PL.observe(spec, D.setup_for(W, c2_3d=PL.bump_c2(S0, W)))
+    t1 = time.perf_counter()
+    print(f"[solve] observed data in {t1 - t0:.1f}s", file=sys.stderr, flush=True)
+    res, _ = PL.full_solve(spec, S0, obs)
+    t2 = time.perf_counter()
+    names = {abi.KERN_C2: "c2", abi.KERN_RHO: "rho", abi.KERN_HESSIAN: "hessian",
+             abi.KERN_VX: "vx", abi.KERN_VY: "vy", abi.KERN_VZ: "vz", abi.KERN_BX: "bx",
+             abi.KERN_BY: "by", abi.KERN_BZ: "bz", abi.KERN_PSI: "psi"}
+    kern = {names[k]: {"l2": float(np.linalg.norm(v)), "finite": bool(np.all(np.isfinite(v)))}
+            for k, v in res.kernel_raw.items()}
+    line = {"metric": "forward + adjoint + kernel solve wall time", "value": round(t2 - t1, 3),
+            "unit": "s", "higher_is_better": False, "n_gpus": 1,
+            "dtype": "f32" if prec == abi.FP32 else "f64", "data": "synthetic",
+            "config": {"workload": W.name, "grid": [P.nx, P.ny, P.nz], "option": P.option,
+                       "steps_per_solve": maxtime + 1, "receivers": len(rec)},
+            "fwd_plus_adj_mpt_per_s": round(P.nx * P.ny * P.nz * (maxtime + 1) * 2 / (t2 - t1) / 1e6, 1),
+            "misfit": res.misfit, "tau_s": [round(float(t), 4) for t in res.tau],
+            "kernels": kern}
+    print(json.dumps(line))
     return 0
 
 

@@ -163,10 +163,12 @@ def nearest_index(z: np.ndarray, target: float) -> int:
     return best
 
 
-def init_rhs(z, c2, rho0, p0, g, npmlbot: int, npmltop: int, is2d: bool) -> dict:
+def init_rhs(z, c2, rho0, p0, g, npmlbot: int, npmltop: int, is2d: bool,
+             tempin: np.ndarray | None = None) -> dict:
     """Derived background of mp_Initialize_RHS (physics.f90:204-303) or
     mp_Initialize_RHS_2d (physics2d.f90:235-342), PML profiles included.
-    c2, rho0, p0 are (nx, ny, nz); z, g are (nz,)."""
+    c2, rho0, p0 are (nx, ny, nz); z, g are (nz,); tempin is the PML speed
+    (sqrt(c2) by default, the fast speed with a magnetic field)."""
     nx, ny, nz = c2.shape
     unstretch = dbyd2(np.asarray(z)[None, :], 1)[0]
     stretch = 1.0 / unstretch
@@ -175,7 +177,8 @@ def init_rhs(z, c2, rho0, p0, g, npmlbot: int, npmltop: int, is2d: bool) -> dict
     gradrho0_z = ddz(rho0, stretch, 1)
     gradp0_z = ddz(p0, stretch, 1)
     g = np.array(g, dtype=np.float64)
-    tempin = c2 ** 0.5
+    if tempin is None:
+        tempin = c2 ** 0.5
     dampz = np.zeros_like(c2)
     pd = PI_R * sp(0.005) * DIML / DIMC
     alphaz = np.full(nz, pd)
@@ -337,6 +340,8 @@ class Setup:
     rates: np.ndarray | None = None
     decay_x: np.ndarray | None = None
     decay_y: np.ndarray | None = None
+    mhd: dict | None = None           # magnetic backgrounds by abi id (3-D)
+    rho_q: np.ndarray | None = None   # quiet density profile (Wilson depression search)
 
     @property
     def deltat(self) -> float:
@@ -389,3 +394,159 @@ def background_arrays(S: Setup) -> dict:
                      (abi.BG_BZPML, d["bzpml"]), (abi.BG_C2, S.c2)):
         out[bid] = np.ascontiguousarray(arr[0, 0, :]) if is_uniform(arr) else arr
     return out
+
+
+# --------------------------------------------------------------------------
+# Magnetic backgrounds: a flux tube from a vector potential, and the
+# reference's magnetic set-up of mp_Initialize_RHS
+# --------------------------------------------------------------------------
+def ddh(f: np.ndarray, axis: int, stretch: float) -> np.ndarray:
+    """ddx / ddy (derivatives.f90:28-170) on the host in FP64: periodic
+    spectral derivative along axis 0 or 1 with the Nyquist bin dropped,
+    w.r.t. x/Rsun (stretch = Rsun/length)."""
+    n = f.shape[axis]
+    k = np.fft.rfftfreq(n) * n
+    mult = 2.0 * np.pi * k * stretch
+    if n % 2 == 0:
+        mult[-1] = 0.0
+    shape = [1] * f.ndim
+    shape[axis] = k.size
+    return np.fft.irfft(np.fft.rfft(f, axis=axis) * (1j * mult).reshape(shape), n=n, axis=axis)
+
+
+def _solve_h(div: np.ndarray, sx: float, sy: float) -> np.ndarray:
+    """chi with lap_h(chi) = div per plane (spectral, periodic, chi mean 0),
+    using the squared wavenumbers of ddh so that ddh(ddh(chi)) summed over x
+    and y reproduces div exactly for every mode ddh keeps."""
+    nx, ny = div.shape[:2]
+    kx = np.fft.fftfreq(nx) * nx
+    ky = np.fft.fftfreq(ny) * ny
+    mx = 2.0 * np.pi * kx * sx
+    my = 2.0 * np.pi * ky * sy
+    if nx % 2 == 0:
+        mx[nx // 2] = 0.0
+    if ny % 2 == 0:
+        my[ny // 2] = 0.0
+    lap = -(mx[:, None] ** 2 + my[None, :] ** 2)
+    F = np.fft.fft2(div, axes=(0, 1))
+    with np.errstate(divide="ignore", invalid="ignore"):
+        G = np.where(lap[:, :, None] != 0.0, F / lap[:, :, None], 0.0)
+    return np.fft.ifft2(G, axes=(0, 1)).real
+
+
+def flux_tube(tab: np.ndarray, nx: int, ny: int, xlength: float, ylength: float,
+              b_ph: float = 3000.0, r_ph_mm: float = 4.0, decay_mm: float = 10.0,
+              beta_min: float = 1.5):
+    """Axisymmetric Gaussian flux tube in the polytrope of ``tab``, on the
+    centred (nx, ny) grid, in cgs (B in gauss).
+
+    Field: B_z = B0(z) exp(-r^2/R(z)^2) with R = r_ph sqrt(b_ph/B0(z)), so
+    the flux pi R^2 B0 is the same at every height.  On the axis B0 is the
+    smooth minimum (4-norm) of b_ph exp(z/decay) -- decaying with depth like
+    code_python/tubex.py's exp(z/decay_z) -- and sqrt(8 pi p0/beta_min), the
+    field of plasma beta beta_min, normalised to b_ph at the photosphere
+    z = 0.  The horizontal field is the horizontal gradient that cancels
+    d(B_z)/dz in the divergence with the package's own operators (spectral
+    x, y; compact z), so div B = 0 on the grid up to each plane's mean (the
+    flux the box truncates, ~1e-6 of the tube's dB/dz); it is the discrete
+    form of the vector potential's B_r = -r (dB0/dz)/(2 B0) B_z.
+
+    Gas: lateral (thin-tube) pressure balance p = p0 - B_z^2/(8 pi), which
+    beta_min > 1 keeps positive, at the temperature outside (rho = rho0 p/p0,
+    so c is unchanged and the tube is less dense: a Wilson depression).  The
+    full magnetohydrostatic balance of tubex.py (lines 73-96, pressure from
+    the horizontal Lorentz force and density from the vertical one) has no
+    positive solution for a 3 kG tube in this polytrope's top 0.5 Mm: the
+    field has to weaken upward there, and the horizontal field that
+    weakening needs pulls the tube's pressure below zero.
+
+    Returns dict(rho, p, c, bx, by, bz, z) of (nx, ny, nz) arrays (z in cm).
+    """
+    nz = tab.shape[0]
+    zc = (tab[:, 0] - 1.0) * RSUN_R               # cm, photosphere at 0
+    zmm = zc / M_CM
+    x = (np.arange(nx) / nx - 0.5) * xlength / M_CM
+    y = (np.arange(ny) / ny - 0.5) * ylength / M_CM
+    r2 = x[:, None, None] ** 2 + y[None, :, None] ** 2
+    p0, rho0 = tab[:, 3], tab[:, 2]
+    cap = np.sqrt(8.0 * np.pi * p0 / beta_min)
+
+    def smin(a, b):
+        return (a ** -4.0 + b ** -4.0) ** -0.25
+
+    cap0 = float(np.interp(0.0, zc, cap))
+    # amplitude of the exponential branch that puts B0(0) at b_ph
+    a0 = (b_ph ** -4.0 - cap0 ** -4.0) ** -0.25
+    b0 = smin(a0 * np.exp(zmm / decay_mm), cap)
+    R = r_ph_mm * np.sqrt(b_ph / b0)
+    bz = b0[None, None, :] * np.exp(-r2 / R[None, None, :] ** 2)
+    sx, sy = RSUN_R / xlength, RSUN_R / ylength
+    unstretch = dbyd2(np.asarray(tab[:, 0])[None, :], 1)[0]
+    stretch = 1.0 / unstretch
+    # lap_h chi = -dBz/dz in x/Rsun units; B_h = grad_h chi (per cm)
+    chi = _solve_h(-ddz(bz, stretch, 1) * RSUN_R, sx, sy)
+    bx = ddh(chi, 0, sx) / RSUN_R
+    by = ddh(chi, 1, sy) / RSUN_R
+    p = p0[None, None, :] - bz ** 2 / (8.0 * np.pi)
+    if p.min() <= 0:
+        raise ValueError("flux tube: negative pressure (beta_min must exceed 1)")
+    rho = rho0[None, None, :] * p / p0[None, None, :]
+    c = np.broadcast_to(tab[:, 1][None, None, :], p.shape)
+    return dict(rho=np.asfortranarray(rho), p=np.asfortranarray(p), c=np.asfortranarray(c),
+                bx=np.asfortranarray(bx), by=np.asfortranarray(by), bz=np.asfortranarray(bz),
+                z=zc)
+
+
+def magnetic_setup(tab: np.ndarray, fields: dict, nx: int, ny: int, option: int,
+                   xlength: float, ylength: float, timestep: float, npmlbot: int, npmltop: int,
+                   cap2: float = 8.0) -> "Setup":
+    """mp_Initialize_RHS with MAGNETIC (physics.f90:113-160, 204-316) from 3-D
+    rho, c, p (cgs) and B (gauss) as READ_IN_FIELD reads them
+    (all_modules.f90:486-513):
+
+    * nondimensionalisation (B/dimb, p/(dimrho dimc^2), rho/dimrho, c/dimc);
+    * the Alfven-speed reduction cap2/(cap2 + B^2/(rho0 cq^2)) with the quiet
+      sound speed cq(z), times ((k-nz)/10)^4 on the top 11 planes, applied to
+      B as its square root (cap2 = 8 in the reference, i.e. v_A <= sqrt(8) c);
+    * curl B0 as the reference forms it: curlboz = dx boy - dy box,
+      curlbox = dy boz and curlboy = -dx boz (its dz terms are computed and
+      then overwritten with zero, physics.f90:133-157);
+    * horizontal gradients of rho0 and p0 (physics.f90:310-316);
+    * the PML profiles from the fast speed sqrt(c^2 + B^2/rho0)
+      (physics.f90:258), the z gradients of rho0 and p0, rhoinv, c2rho0.
+    Returns a Setup whose ``mhd`` holds the extra backgrounds by abi id."""
+    from . import abi
+    sd = solar_data(tab)
+    nz = tab.shape[0]
+    rho0 = fields["rho"] / DIMRHO
+    p0 = fields["p"] / (DIMRHO * DIMC ** 2.0)
+    c2 = (fields["c"] / DIMC) ** 2.0
+    box, boy, boz = (fields[k] / DIMB for k in ("bx", "by", "bz"))
+    cq = sd["c"]
+    red = (box ** 2 + boy ** 2 + boz ** 2) / (rho0 * cq[None, None, :] ** 2)
+    red = cap2 / (cap2 + red)
+    for k1 in range(nz - 10, nz + 1):
+        red[:, :, k1 - 1] *= ((k1 - nz) / 10.0) ** 4.0
+    sq = red ** 0.5
+    box, boy, boz = box * sq, boy * sq, boz * sq
+    sx, sy = RSUN_R / xlength, RSUN_R / ylength
+    curlbz = ddh(boy, 0, sx) - ddh(box, 1, sy)
+    curlbx = ddh(boz, 1, sy)
+    curlby = -ddh(boz, 0, sx)
+    tempin = (c2 + (box ** 2 + boy ** 2 + boz ** 2) / rho0) ** 0.5
+    der = init_rhs(sd["z"], c2, rho0, p0, sd["g"], npmlbot, npmltop, False, tempin=tempin)
+    S = Setup(nx=nx, ny=ny, nz=nz, option=option, xlength=xlength, ylength=ylength,
+              timestep=timestep, npmlbot=npmlbot, npmltop=npmltop, z=sd["z"],
+              c2=np.asfortranarray(c2), rho0=np.asfortranarray(rho0),
+              p0=np.asfortranarray(p0), g_raw=sd["g"], derived=der)
+    S.rates = damping_rates(nx, ny, xlength, ylength, False)
+    S.decay_x = decay_coeff(nx)
+    S.decay_y = decay_coeff(ny)
+    S.mhd = {
+        abi.BG_BX: box, abi.BG_BY: boy, abi.BG_BZ: boz,
+        abi.BG_CURLBX: curlbx, abi.BG_CURLBY: curlby, abi.BG_CURLBZ: curlbz,
+        abi.BG_GRADP0_X: ddh(p0, 0, sx), abi.BG_GRADP0_Y: ddh(p0, 1, sy),
+        abi.BG_GRADRHO0_X: ddh(rho0, 0, sx), abi.BG_GRADRHO0_Y: ddh(rho0, 1, sy),
+    }
+    S.rho_q = sd["rho0"]
+    return S

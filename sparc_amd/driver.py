@@ -58,12 +58,19 @@ class Params:
         return int(math.floor(60.0 / self.timestep + 1e-9))  # driver.f90:270
 
 
+# Alfven-speed cap of the flux-tube cases: reduction cap2/(cap2 + v_A^2/c^2)
+# bounds v_A by sqrt(cap2) c; 16 gives the 4 c of the 256^2 x 300 MHD case
+# (the reference's own constant is 8, physics.f90:121)
+TUBE_CAP2 = 16.0
+
+
 def cfl_timestep(tab: np.ndarray, dx_cm: float, dy_cm: float, cfl: float,
-                 cadence: float = 30.0) -> float:
+                 cadence: float = 30.0, speed_factor: float = 1.0) -> float:
     """Largest timestep <= cfl * min(dz/c, dx/c_max, dy/c_max) that divides
-    the output cadence an integer number of times."""
+    the output cadence an integer number of times; speed_factor scales c
+    (the fast speed of a capped magnetic field)."""
     z_cm = (tab[:, 0] - 1.0) * RSUN_R
-    c = tab[:, 1]
+    c = tab[:, 1] * speed_factor
     dz = np.diff(z_cm)
     cmid = np.maximum(c[1:], c[:-1])
     lim = min(float(np.min(dz / cmid)), dx_cm / float(c.max()), dy_cm / float(c.max()))
@@ -94,6 +101,10 @@ def workload(name: str, precision: int | None = None) -> Workload:
     tiny   32 x 32 x 40, 25 Mm (parity tests)
     tall   32 x 32 x 200, 25 Mm (parity tests of the segmented z solve)
     wide   256 x 256 x 40, 200 Mm (parity tests of the length-256 line kernels)
+    box3   box2 with option 10 and a flux tube (setup_for builds it): radius
+           4 Mm, 3 kG at the photosphere, v_A capped at 4 c
+    tube   32 x 32 x 100, 25 Mm, the flux tube of box3 with a 2 Mm radius
+           (parity tests of the magnetic set-up)
     column 32 x 32 x 100, 25 Mm, the vertical grid of box1 (adjoint/gradient tests:
            receivers and source lie below the top PML and filter layers)
     two_d  256 x 1 x 300, 400 Mm, option 17 (the reference's wired 2-D case)
@@ -119,6 +130,20 @@ def workload(name: str, precision: int | None = None) -> Workload:
     elif name == "wide":
         nx = ny = 256
         nz, L, nsteps, prec = 40, 200.0, 10, abi.FP64
+    elif name in ("box3", "tube"):
+        big = name == "box3"
+        nx = ny = 256 if big else 32
+        nz, L, nsteps = (300, 200.0, 2000) if big else (100, 25.0, 400)
+        tab = bgm.polytrope_table(nz, 0.5, -12.0)
+        dx = L * M_CM / nx
+        # the time step is set by the fast speed of the capped field
+        dt = cfl_timestep(tab, dx, dx, 0.5, speed_factor=math.sqrt(1.0 + TUBE_CAP2))
+        P = Params(nx=nx, ny=ny, nz=nz, xlength=L * M_CM, ylength=L * M_CM, timestep=dt,
+                   option=abi.OPT_MHD, npmltop=0,
+                   precision=(abi.FP32 if big else abi.FP64) if precision is None else precision)
+        return Workload(name, P, tab, nsteps,
+                        extra={"tube": dict(b_ph=3000.0, r_ph_mm=4.0 if big else 2.0,
+                                            decay_mm=10.0), "cap2": TUBE_CAP2})
     elif name == "column":
         nx = ny = 32
         nz, L, nsteps, prec = 100, 25.0, 400, abi.FP64
@@ -187,12 +212,28 @@ def load_setup(eng, S: bgm.Setup) -> None:
     eng.set_profile(abi.PROF_Z, S.z)
     if eng.cfg.option in (abi.OPT_RHOVP, abi.OPT_MHD8):
         eng.set_profile(abi.PROF_SOURCE_DEP, bgm.source_dep(S.z, eng.cfg.e_rad))
+    if S.mhd is not None:
+        # MHD backgrounds, the top sponge and the Wilson-depression source
+        # planes (physics.f90:66-107) from a magnetic set-up
+        for bid, arr in S.mhd.items():
+            eng.set_background(bid, arr)
+        if eng.cfg.option == abi.OPT_MHD:
+            eng.set_profile(abi.PROF_SPONGE, bgm.sponge_z(S.z))
+        if S.rho_q is not None and eng.cfg.option in (abi.OPT_MHD, abi.OPT_MHD8):
+            k_ref = eng.cfg.e_rad if eng.cfg.mode == abi.FORWARD else eng.cfg.o_rad
+            eng.set_source_planes(bgm.wilson_planes(S.rho0, S.rho_q, k_ref))
     eng.set_damping(S.rates)
     eng.set_filter(S.decay_x, S.decay_y)
 
 
 def setup_for(W: Workload, c2_3d=None, rho0_3d=None) -> bgm.Setup:
     P = W.params
+    if "tube" in W.extra:
+        fields = bgm.flux_tube(W.tab, P.nx, P.ny, P.xlength, P.ylength, **W.extra["tube"])
+        if c2_3d is not None:
+            fields["c"] = np.sqrt(c2_3d) * bgm.DIMC
+        return bgm.magnetic_setup(W.tab, fields, P.nx, P.ny, P.option, P.xlength, P.ylength,
+                                  P.timestep, P.npmlbot, P.npmltop, cap2=W.extra["cap2"])
     return bgm.build_setup(W.tab, P.nx, P.ny, P.option, P.xlength, P.ylength, P.timestep,
                            P.npmlbot, P.npmltop, c2_3d=c2_3d, rho0_3d=rho0_3d)
 

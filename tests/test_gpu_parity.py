@@ -605,3 +605,26 @@ def test_option10_wilson_depression_planes_match_oracle():
     finally:
         eng.close()
         o.close()
+
+
+@pytest.mark.parametrize("prec", [abi.FP64, abi.FP32])
+def test_flux_tube_option10_matches_oracle(prec):
+    """Option 10 on the flux-tube background of the magnetic set-up (Alfven
+    cap, curl terms, horizontal gradients, fast-speed PML, top sponge,
+    Wilson-depression source planes) through load_setup, 20 steps."""
+    W = D.workload("tube")
+    S = D.setup_for(W)
+    eng, cfg = D.make_engine(W, S, precision=prec, nsteps=20)
+    o, _ = make_oracle(W, S, nsteps=20)
+    try:
+        mask, series = D.forward_source(W, 28)
+        for x in (eng, o):
+            x.set_source(abi.SRC_SERIES, mask, series, 10 ** 9)
+            x.step(0, 20)
+        for f in range(6):
+            a, b = eng.get(abi.ARR_A, f), o.get(abi.ARR_A, f)
+            assert np.linalg.norm(b) > 0, f
+            assert (rel_err(a, b) < TOL64) if prec == abi.FP64 else (rel_l2(a, b) < TOL32_L2), f
+    finally:
+        eng.close()
+        o.close()
