@@ -187,7 +187,10 @@ typedef struct sparc_config {
                              reference state is FP64); 0 = plain FP32 state     */
   int32_t flow_kernels;   /* snapshots also keep v (9 fields) and the kernel sums
                              add the flow kernels SPARC_KERN_VX..VZ            */
-  int32_t reserved_i[6];
+  int32_t use_graphs;     /* replay each step's five stages and update as a
+                             captured HIP graph (forcing and spectral filter
+                             stay eager); 0 = launch every kernel eagerly    */
+  int32_t reserved_i[5];
   double xlength, ylength;/* box sides, cm (params.i:27)                        */
   double timestep;        /* s (params.i:27)                                    */
   double deltat;          /* timestep*dimc/diml (initialize.f90:472)            */

@@ -72,7 +72,8 @@ class SparcConfig(C.Structure):
         ("num_steps", C.c_int32),
         ("compensated", C.c_int32),
         ("flow_kernels", C.c_int32),
-        ("reserved_i", C.c_int32 * 6),
+        ("use_graphs", C.c_int32),
+        ("reserved_i", C.c_int32 * 5),
         ("xlength", C.c_double), ("ylength", C.c_double),
         ("timestep", C.c_double),
         ("deltat", C.c_double),

@@ -8,6 +8,7 @@
 #include <cstdio>
 #include <cstring>
 #include <algorithm>
+#include <mutex>
 #include <string>
 #include <utility>
 #include <vector>
@@ -26,6 +27,8 @@ struct sparc_ctx {
   size_t ts = 8;
   hipStream_t stream = nullptr;
   hipEvent_t xev = nullptr;  // orders this stream against another context's (cross-context calls)
+  // captured step bodies (five stages + update), without / with the source plane
+  hipGraphExec_t gexec[2] = {nullptr, nullptr};
   int device = 0;
   int lds_max = 65536;  // dynamic LDS bytes one block may take
   long N = 0, nxy = 0, NP = 0;
@@ -150,10 +153,20 @@ static int fail(sparc_ctx* c, int code, const char* fmt, ...) {
                   __LINE__, hipGetErrorString(e_));                                   \
   } while (0)
 
+// Graph capture (the first replayed step of a context) must not overlap a
+// device allocation or free made by another thread for another context: HIP
+// refuses such calls while a stream captures.  Captures are short (one step's
+// launches), so one process-wide lock covers capture, hipMalloc and hipFree.
+static std::mutex g_capture_mu;
+
 static int dalloc(sparc_ctx* c, void** p, size_t bytes) {
   *p = nullptr;
   if (bytes == 0) bytes = 16;
-  hipError_t e = hipMalloc(p, bytes);
+  hipError_t e;
+  {
+    std::lock_guard<std::mutex> lk(g_capture_mu);
+    e = hipMalloc(p, bytes);
+  }
   if (e != hipSuccess) {
     *p = nullptr;
     return fail(c, SPARC_ENOMEM, "hipMalloc(%zu) failed: %s", bytes, hipGetErrorString(e));
@@ -187,9 +200,11 @@ static int upload(sparc_ctx* c, void* dst, const double* h, size_t n) {
   if (c->fp32) {
     std::vector<float> t(n);
     for (size_t i = 0; i < n; ++i) t[i] = (float)h[i];
-    HIPCHK(c, hipMemcpy(dst, t.data(), n * 4, hipMemcpyHostToDevice));
+    HIPCHK(c, hipMemcpyAsync(dst, t.data(), n * 4, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));  // t is released on return
   } else {
-    HIPCHK(c, hipMemcpy(dst, h, n * 8, hipMemcpyHostToDevice));
+    HIPCHK(c, hipMemcpyAsync(dst, h, n * 8, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
   }
   return SPARC_OK;
 }
@@ -198,10 +213,12 @@ static int download(sparc_ctx* c, double* h, const void* src, size_t n, double s
   HIPCHK(c, hipStreamSynchronize(c->stream));
   if (c->fp32) {
     std::vector<float> t(n);
-    HIPCHK(c, hipMemcpy(t.data(), src, n * 4, hipMemcpyDeviceToHost));
+    HIPCHK(c, hipMemcpyAsync(t.data(), src, n * 4, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
     for (size_t i = 0; i < n; ++i) h[i] = (double)t[i] * scale;
   } else {
-    HIPCHK(c, hipMemcpy(h, src, n * 8, hipMemcpyDeviceToHost));
+    HIPCHK(c, hipMemcpyAsync(h, src, n * 8, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
     if (scale != 1.0)
       for (size_t i = 0; i < n; ++i) h[i] *= scale;
   }
@@ -493,7 +510,19 @@ extern "C" int sparc_create(const sparc_config* cfg, sparc_ctx** out) {
   return SPARC_OK;
 }
 
+// Captured steps hold the kernel arguments of their capture; anything that
+// can change them (backgrounds, profiles, damping, sources) drops them.
+static void drop_graphs(sparc_ctx* c) {
+  if ((c->gexec[0] || c->gexec[1]) && c->stream) (void)hipStreamSynchronize(c->stream);
+  for (auto& g : c->gexec)
+    if (g) {
+      (void)hipGraphExecDestroy(g);
+      g = nullptr;
+    }
+}
+
 extern "C" void sparc_destroy(sparc_ctx* c) {
+  if (c) drop_graphs(c);
   if (!c) return;
   if (c->stream) (void)hipStreamSynchronize(c->stream);
   for (auto& r : c->recs) {
@@ -501,8 +530,11 @@ extern "C" void sparc_destroy(sparc_ctx* c) {
     (void)hipEventDestroy(r.e1);
   }
   for (auto e : c->pool) (void)hipEventDestroy(e);
-  for (void* p : c->allocs)
-    if (p) (void)hipFree(p);
+  {
+    std::lock_guard<std::mutex> lk(g_capture_mu);
+    for (void* p : c->allocs)
+      if (p) (void)hipFree(p);
+  }
   if (c->xev) (void)hipEventDestroy(c->xev);
   if (c->stream) (void)hipStreamDestroy(c->stream);
   delete c;
@@ -537,6 +569,7 @@ extern "C" int sparc_reset(sparc_ctx* c) {
 }
 
 extern "C" int sparc_set_background(sparc_ctx* c, int id, const double* h, size_t n) {
+  if (c) drop_graphs(c);
   if (!c || !h || id < 0 || id >= SPARC_BG_COUNT)
     return fail(c, SPARC_EINVAL, "bad background id %d", id);
   int is3d;
@@ -560,6 +593,7 @@ extern "C" int sparc_set_background(sparc_ctx* c, int id, const double* h, size_
 }
 
 extern "C" int sparc_set_profile(sparc_ctx* c, int id, const double* h, int n) {
+  if (c) drop_graphs(c);
   if (!c || !h || id < 0 || id >= SPARC_PROF_COUNT || n != c->cfg.nz)
     return fail(c, SPARC_EINVAL, "bad profile id %d or length %d", id, n);
   c->profset[id] = true;
@@ -569,6 +603,7 @@ extern "C" int sparc_set_profile(sparc_ctx* c, int id, const double* h, int n) {
 }
 
 extern "C" int sparc_set_damping(sparc_ctx* c, const double* r, size_t n) {
+  if (c) drop_graphs(c);
   if (!c || !r || n != (size_t)(c->cfg.nx / 2 + 1) * c->cfg.ny)
     return fail(c, SPARC_EINVAL, "damping rates need (nx/2+1)*ny values");
   c->damping_set = true;
@@ -590,6 +625,7 @@ extern "C" int sparc_set_damping(sparc_ctx* c, const double* r, size_t n) {
 }
 
 extern "C" int sparc_set_filter(sparc_ctx* c, const double* dx, const double* dy) {
+  if (c) drop_graphs(c);
   if (!c || !dx || !dy) return fail(c, SPARC_EINVAL, "null filter");
   // full-spectrum multipliers decay(min(m, n-m)); decay folds 1/n (initialize.f90:215-221)
   auto mk = [&](int n, const double* d, void* dst) -> int {
@@ -606,6 +642,7 @@ extern "C" int sparc_set_filter(sparc_ctx* c, const double* dx, const double* dy
 }
 
 extern "C" int sparc_set_source_planes(sparc_ctx* c, const int32_t* k_xy, size_t n) {
+  if (c) drop_graphs(c);
   if (!c) return SPARC_EINVAL;
   if (n == 0) {
     c->src_planes = nullptr;  // the device buffers stay allocated for reuse
@@ -629,7 +666,8 @@ extern "C" int sparc_set_source_planes(sparc_ctx* c, const int32_t* k_xy, size_t
     c->planes_buf = p;
     if ((rc = dalloc(c, &c->src_dw, n * c->ts))) return rc;
   }
-  HIPCHK(c, hipMemcpy(p, k0.data(), n * sizeof(int32_t), hipMemcpyHostToDevice));
+  HIPCHK(c, hipMemcpyAsync(p, k0.data(), n * sizeof(int32_t), hipMemcpyHostToDevice, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
   if ((rc = upload(c, c->src_dw, dw.data(), n))) return rc;
   c->src_planes = (int32_t*)p;
   return SPARC_OK;
@@ -637,6 +675,7 @@ extern "C" int sparc_set_source_planes(sparc_ctx* c, const int32_t* k_xy, size_t
 
 extern "C" int sparc_set_source(sparc_ctx* c, int kind, const double* mask, size_t nmask,
                                 const double* series, int nt, int active_until) {
+  if (c) drop_graphs(c);
   if (!c) return SPARC_EINVAL;
   c->src_kind = SPARC_SRC_NONE;
   if (kind == SPARC_SRC_NONE) return SPARC_OK;
@@ -1303,6 +1342,54 @@ static int run_stage(sparc_ctx* c, Fld<T>& F, int s) {
   return SPARC_OK;
 }
 
+// The five stages of one step and the update (step.f90:49-118 without the
+// spectral filter): the part of a step whose launches do not change from step
+// to step, replayed as a captured graph when cfg.use_graphs is set.
+template <typename T>
+static int step_body(sparc_ctx* c, Fld<T>& F) {
+  const int nxy = (int)c->nxy;
+  const bool is2d = is_2d(c->cfg.option);
+  int rc;
+  for (int s = 0; s < 5; ++s)
+    if ((rc = run_stage<T>(c, F, s))) return rc;
+  // a += deltat*scr with the z filter (step.f90:114-118)
+  const T dt = (T)c->cfg.deltat;
+  const int nfld = c->nf;  // dimfive
+  {
+    // all fields in one launch (blockIdx.y = field)
+    ProfScope ps(c, SPARC_KC_UPDATE,
+                 nfld * (c->comp ? 5.0 : 3.0) * (double)c->N * sizeof(T));
+    UpdArgs<T> A{};
+    for (int f = 0; f < nfld; ++f) {
+      A.a[f] = F.a[f];
+      A.lo[f] = (float*)c->alo[f];
+      A.scr[f] = (const T*)F.scr[f];
+    }
+    if (c->comp) {
+      UpdArgs<float> Af{};
+      for (int f = 0; f < nfld; ++f) {
+        Af.a[f] = (float*)F.a[f];
+        Af.lo[f] = (float*)c->alo[f];
+        Af.scr[f] = (const float*)F.scr[f];
+      }
+      hipLaunchKernelGGL(k_update_filter_c, dim3(cdiv(nxy, 64), nfld), dim3(64), 0, c->stream,
+                         Af, c->cfg.deltat, nxy, c->cfg.nz, is2d ? 6 : 2);
+    } else {
+      hipLaunchKernelGGL((k_update_filter<T>), dim3(cdiv(nxy, 64), nfld), dim3(64), 0,
+                         c->stream, A, dt, nxy, c->cfg.nz, is2d ? 6 : 2);
+    }
+    LAUNCHCHK(c);
+  }
+  if (c->nzpml > 0)
+    for (int f = 0; f < c->npv; ++f) {
+      ProfScope ps(c, SPARC_KC_OTHER, 3.0 * (double)c->NP * sizeof(T));
+      hipLaunchKernelGGL((k_axpy<T>), dim3(cdiv(c->NP, 256)), dim3(256), 0, c->stream,
+                         F.pmlvars[f], (const T*)F.scrpml[f], dt, c->NP);
+      LAUNCHCHK(c);
+    }
+  return SPARC_OK;
+}
+
 template <typename T>
 static int step_impl(sparc_ctx* c, int t0, int nsteps) {
   Fld<T> F = make_fld<T>(c);
@@ -1327,43 +1414,32 @@ static int step_impl(sparc_ctx* c, int t0, int nsteps) {
       }
       LAUNCHCHK(c);
     }
-    for (int s = 0; s < 5; ++s)
-      if ((rc = run_stage<T>(c, F, s))) return rc;
-    // a += deltat*scr with the z filter (step.f90:114-118)
-    const T dt = (T)c->cfg.deltat;
-    const int nfld = c->nf;  // dimfive
-    {
-      // all fields in one launch (blockIdx.y = field)
-      ProfScope ps(c, SPARC_KC_UPDATE,
-                   nfld * (c->comp ? 5.0 : 3.0) * (double)c->N * sizeof(T));
-      UpdArgs<T> A{};
-      for (int f = 0; f < nfld; ++f) {
-        A.a[f] = F.a[f];
-        A.lo[f] = (float*)c->alo[f];
-        A.scr[f] = (const T*)F.scr[f];
-      }
-      if (c->comp) {
-        UpdArgs<float> Af{};
-        for (int f = 0; f < nfld; ++f) {
-          Af.a[f] = (float*)F.a[f];
-          Af.lo[f] = (float*)c->alo[f];
-          Af.scr[f] = (const float*)F.scr[f];
+    if (c->cfg.use_graphs && !c->profiling) {
+      const int gi = F.src_plane >= 0 ? 1 : 0;
+      if (!c->gexec[gi]) {
+        // relaxed: other threads driving other contexts (the serving loop)
+        // keep making stream-ordered calls meanwhile; allocations and frees
+        // wait for the lock
+        std::lock_guard<std::mutex> lk(g_capture_mu);
+        HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeRelaxed));
+        rc = step_body<T>(c, F);
+        hipGraph_t g = nullptr;
+        const hipError_t e = hipStreamEndCapture(c->stream, &g);
+        if (rc || e != hipSuccess) {
+          if (g) (void)hipGraphDestroy(g);
+          return rc ? rc : fail(c, SPARC_EHIP, "graph capture: %s", hipGetErrorString(e));
         }
-        hipLaunchKernelGGL(k_update_filter_c, dim3(cdiv(nxy, 64), nfld), dim3(64), 0, c->stream,
-                           Af, c->cfg.deltat, nxy, c->cfg.nz, is2d ? 6 : 2);
-      } else {
-        hipLaunchKernelGGL((k_update_filter<T>), dim3(cdiv(nxy, 64), nfld), dim3(64), 0,
-                           c->stream, A, dt, nxy, c->cfg.nz, is2d ? 6 : 2);
+        const hipError_t ei = hipGraphInstantiate(&c->gexec[gi], g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        if (ei != hipSuccess) {
+          c->gexec[gi] = nullptr;
+          return fail(c, SPARC_EHIP, "graph instantiate: %s", hipGetErrorString(ei));
+        }
       }
-      LAUNCHCHK(c);
+      HIPCHK(c, hipGraphLaunch(c->gexec[gi], c->stream));
+    } else if ((rc = step_body<T>(c, F))) {
+      return rc;
     }
-    if (c->nzpml > 0)
-      for (int f = 0; f < c->npv; ++f) {
-        ProfScope ps(c, SPARC_KC_OTHER, 3.0 * (double)c->NP * sizeof(T));
-        hipLaunchKernelGGL((k_axpy<T>), dim3(cdiv(c->NP, 256)), dim3(256), 0, c->stream,
-                           F.pmlvars[f], (const T*)F.scrpml[f], dt, c->NP);
-        LAUNCHCHK(c);
-      }
     // spectral low-pass (physics.f90:792-830; physics2d.f90:563-585)
     const int ff = c->cfg.freq_filtering;
     if (ff > 0 && c->filter_set) {
@@ -1884,8 +1960,8 @@ extern "C" int sparc_kernel_get(sparc_ctx* c, int which, double* h) {
     const int rc = c->fp32 ? psi_kernel<float>(c) : psi_kernel<double>(c);
     if (rc) return rc;
   }
+  HIPCHK(c, hipMemcpyAsync(h, c->kern[which], c->N * 8, hipMemcpyDeviceToHost, c->stream));
   HIPCHK(c, hipStreamSynchronize(c->stream));
-  HIPCHK(c, hipMemcpy(h, c->kern[which], c->N * 8, hipMemcpyDeviceToHost));
   return SPARC_OK;
 }
 

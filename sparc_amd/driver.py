@@ -38,6 +38,7 @@ class Params:
     field_scale: float = 1.0
     compensated: bool = True       # FP32: hi+lo state updated in FP64 arithmetic
     flow_kernels: bool = False     # snapshots keep v; kernel sums add K_v (kernels.f90:317-332)
+    use_graphs: bool = True        # replay each step's stages + update as a captured HIP graph
 
     @property
     def is2d(self) -> bool:
@@ -198,6 +199,7 @@ def make_config(P: Params, S: bgm.Setup, mode: int, nsteps_total: int) -> abi.Sp
     c.field_scale = P.field_scale if P.precision == abi.FP32 else 1.0
     c.compensated = int(P.compensated and P.precision == abi.FP32)
     c.flow_kernels = int(P.flow_kernels)
+    c.use_graphs = int(P.use_graphs)
     return c
 
 
@@ -296,10 +298,13 @@ def write_bundle(path: str, cfg: abi.SparcConfig, S: bgm.Setup, kind: int, mask:
 
 
 def make_engine(W: Workload, S: bgm.Setup, mode: int = abi.FORWARD, nsteps: int | None = None,
-                precision: int | None = None, compensated: bool | None = None):
+                precision: int | None = None, compensated: bool | None = None,
+                use_graphs: bool | None = None):
     P = W.params if precision is None else replace(W.params, precision=precision)
     if compensated is not None:
         P = replace(P, compensated=compensated)
+    if use_graphs is not None:
+        P = replace(P, use_graphs=use_graphs)
     cfg = make_config(P, S, mode, nsteps if nsteps is not None else W.nsteps)
     eng = abi.Engine(cfg)
     load_setup(eng, S)
