@@ -255,8 +255,14 @@ __global__ __launch_bounds__(64 * NSEG) __attribute__((amdgpu_waves_per_eu(4))) 
                                                       T* __restrict__ ud,
                                                       const T* __restrict__ prov, T provscale,
                                                       T* __restrict__ out) {
-  constexpr int UF = 8;  // forward chunk (planes loaded one chunk ahead)
-  constexpr int UB = 2;  // backward chunk
+#ifndef SPARC_ZS_UF
+#define SPARC_ZS_UF 8
+#endif
+#ifndef SPARC_ZS_UB
+#define SPARC_ZS_UB 3
+#endif
+  constexpr int UF = SPARC_ZS_UF;  // forward chunk (planes loaded one chunk ahead)
+  constexpr int UB = SPARC_ZS_UB;  // backward chunk
   constexpr int ZW = ZWarm<T>::W;
   extern __shared__ unsigned char zsm[];
   const int n = F.nz;
