@@ -261,3 +261,166 @@ __global__ __launch_bounds__(256) void k_y256(int nx, int nz, XArgs<T> X, T beta
     }
   }
 }
+
+// ---------------------------------------------------------------------------
+// x-derivative lines of length 256, one wavefront per complex line (a row
+// pair), no LDS tile.  Lane l holds v[j] = x[l + 64 j], j = 0..3, so every
+// load and store is a full 256-byte piece of a row.  With n = n0 + 4 n1 +
+// 16 n2 + 64 n3 (lane l = n0 + 4 n1 + 16 n2, register j = n3), the forward
+// transform is four radix-4 stages (decimation in frequency):
+//   S1  DFT4 over the registers (n3 -> k0), times W256^(l k0)
+//   T1  swap the register digit with lane digit n2 (lane bits 4-5)
+//   S2  DFT4 (n2 -> k1), times W64^((l & 15) k1)
+//   T2  swap with lane digit n1 (bits 2-3)
+//   S3  DFT4 (n1 -> k2), times W16^((l & 3) k2)
+//   T3  swap with lane digit n0 (bits 0-1)
+//   S4  DFT4 (n0 -> k3)
+// leaving X[k0 + 4 k1 + 16 k2 + 64 k3] in lane k2 + 4 k1 + 16 k0, register
+// k3.  The spectral multiplier is applied there and the inverse runs the
+// same steps backwards with conjugate twiddles, ending in natural order.  A
+// digit swap is two rounds of pairwise exchanges (lane ^ 2b, then lane ^ b)
+// in which each lane sends the two values whose register bit differs from
+// its lane bit.
+// selects between values (a conditional on array lvalues would select
+// between their addresses and keep the array in memory)
+template <typename T>
+__device__ __forceinline__ C2<T> xw_sel(bool c, C2<T> a, C2<T> b) {
+  return C2<T>{c ? a.x : b.x, c ? a.y : b.y};
+}
+
+template <typename T, int B>
+__device__ __forceinline__ void xw_swap(C2<T> (&v)[4], int lane) {
+#pragma unroll
+  for (int h = 1; h >= 0; --h) {  // register bit h <-> lane bit log2(B) + h
+    const int m = B << h;
+    const bool hi = (lane & m) != 0;
+#pragma unroll
+    for (int o = 0; o < 2; ++o) {
+      // registers i0 (bit h clear) and i1 (bit h set), other bit = o
+      const int i0 = h ? o : 2 * o, i1 = i0 + (1 << h);
+      const C2<T> snd = xw_sel(hi, v[i0], v[i1]);
+      const C2<T> rcv = {__shfl_xor(snd.x, m), __shfl_xor(snd.y, m)};
+      v[i0] = xw_sel(hi, rcv, v[i0]);
+      v[i1] = xw_sel(hi, v[i1], rcv);
+    }
+  }
+}
+
+template <typename T>
+__device__ __forceinline__ void xw_twiddle(C2<T> (&v)[4], const C2<T> (&w)[3], bool conj) {
+#pragma unroll
+  for (int j = 1; j < 4; ++j) {
+    C2<T> t = w[j - 1];
+    if (conj) t.y = -t.y;
+    v[j] = cmul(v[j], t);
+  }
+}
+
+// Same contract as k_xlines_n<T, 256, XM_DERIV>: line L = rows 2L, 2L+1
+// (x fastest), out = i k mult(k) applied, or the momentum RHS when X.rhs.
+// LPW line pairs per wave, the next pair's loads issued before the current
+// pair's transform.
+template <typename T, int LPW>
+__global__ __launch_bounds__(256) void k_xw256(int nrows, int nlines, XArgs<T> X, T beta,
+                                               const T* __restrict__ mult,
+                                               const C2<T>* __restrict__ twg, int ny, int nz) {
+  constexpr int N = 256;
+  const int fld = blockIdx.y;
+  const T* __restrict__ a = X.a[fld];
+  const T* __restrict__ s = X.s[fld];
+  T* __restrict__ out = X.out[fld];
+  const int lane = threadIdx.x & 63;
+  const int wv = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+  const long L0 = ((long)blockIdx.x * 4 + wv) * LPW;
+  // per-lane twiddles of the three twiddled stages and the multiplier of the
+  // four spectral values this lane holds
+  C2<T> w1[3], w2[3], w3[3];
+#pragma unroll
+  for (int k = 1; k < 4; ++k) {
+    w1[k - 1] = twg[(lane * k) & 255];
+    w2[k - 1] = twg[(4 * (lane & 15) * k) & 255];
+    w3[k - 1] = twg[(16 * (lane & 3) * k) & 255];
+  }
+  const int kl = (lane >> 4) + 4 * ((lane >> 2) & 3) + 16 * (lane & 3);
+  T ml[4];
+#pragma unroll
+  for (int j = 0; j < 4; ++j) ml[j] = mult[kl + 64 * j];
+  C2<T> nv[4];
+#define XW_LOAD(LL)                                                  \
+  {                                                                  \
+    const long r0_ = 2 * (LL);                                       \
+    const bool h1_ = r0_ + 1 < nrows;                                \
+    const T* __restrict__ a0_ = a + r0_ * N;                         \
+    const T* __restrict__ s0_ = s ? s + r0_ * N : nullptr;           \
+    _Pragma("unroll") for (int j = 0; j < 4; ++j) {                  \
+      const int x = lane + 64 * j;                                   \
+      T re = a0_[x];                                                 \
+      if (s0_) re = re + beta * s0_[x];                              \
+      T im = 0;                                                      \
+      if (h1_) {                                                     \
+        im = a0_[N + x];                                             \
+        if (s0_) im = im + beta * s0_[N + x];                        \
+      }                                                              \
+      nv[j] = C2<T>{re, im};                                         \
+    }                                                                \
+  }
+  if (L0 < nlines) XW_LOAD(L0);
+  for (int p = 0; p < LPW; ++p) {
+    const long L = L0 + p;
+    if (L >= nlines) break;
+    C2<T> v[4];
+#pragma unroll
+    for (int j = 0; j < 4; ++j) v[j] = nv[j];
+    if (p + 1 < LPW && L + 1 < nlines) XW_LOAD(L + 1);
+    dft_reg<T, 4, -1>(v);
+    xw_twiddle(v, w1, false);
+    xw_swap<T, 16>(v, lane);
+    dft_reg<T, 4, -1>(v);
+    xw_twiddle(v, w2, false);
+    xw_swap<T, 4>(v, lane);
+    dft_reg<T, 4, -1>(v);
+    xw_twiddle(v, w3, false);
+    xw_swap<T, 1>(v, lane);
+    dft_reg<T, 4, -1>(v);
+#pragma unroll
+    for (int j = 0; j < 4; ++j) v[j] = C2<T>{-ml[j] * v[j].y, ml[j] * v[j].x};
+    dft_reg<T, 4, +1>(v);
+    xw_swap<T, 1>(v, lane);
+    xw_twiddle(v, w3, true);
+    dft_reg<T, 4, +1>(v);
+    xw_swap<T, 4>(v, lane);
+    xw_twiddle(v, w2, true);
+    dft_reg<T, 4, +1>(v);
+    xw_swap<T, 16>(v, lane);
+    xw_twiddle(v, w1, true);
+    dft_reg<T, 4, +1>(v);
+    const long r0 = 2 * L;
+    const bool has1 = r0 + 1 < nrows;
+    T* __restrict__ o0 = out + r0 * N;
+    if (X.rhs) {
+      // -rhoinv * d, zero on planes 1 and nz (rhs_of)
+      const int k0 = (int)(r0 / ny), k1 = (int)((r0 + 1) / ny);
+      const bool e0 = k0 == 0 || k0 == nz - 1, e1 = k1 == 0 || k1 == nz - 1;
+      const T* __restrict__ q3 = X.r3 ? X.r3 + r0 * N : nullptr;
+      const T z0 = X.r3 ? (T)0 : X.rz[k0], z1 = X.r3 ? (T)0 : X.rz[k1];
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        const int x = lane + 64 * j;
+        const T ri0 = q3 ? q3[x] : z0;
+        o0[x] = e0 ? (T)0 : -ri0 * v[j].x;
+        if (has1) {
+          const T ri1 = q3 ? q3[N + x] : z1;
+          o0[N + x] = e1 ? (T)0 : -ri1 * v[j].y;
+        }
+      }
+    } else {
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        const int x = lane + 64 * j;
+        o0[x] = v[j].x;
+        if (has1) o0[N + x] = v[j].y;
+      }
+    }
+  }
+#undef XW_LOAD
+}

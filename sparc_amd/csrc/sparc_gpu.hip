@@ -16,6 +16,20 @@
 #include "../../include/sparc_gpu.h"
 #include "kernels.h"
 
+// Exact segmented z solve (k_zsolve_b: input and sweep results in LDS) for the
+// epilogues whose bit is set in SPARC_ZB_EPIS (bit EPI_*), when 2 x 64 x nz
+// values fit one block's LDS; otherwise the warm-up variants of k_zsolve.
+#ifndef SPARC_ZB_EPIS
+#define SPARC_ZB_EPIS 0x2a
+#endif
+#ifndef SPARC_ZB_NSEG
+#define SPARC_ZB_NSEG 16
+#endif
+// FP64 takes 8 segments (512 threads): at 1024 threads its epilogues spill
+#ifndef SPARC_ZB_NSEG64
+#define SPARC_ZB_NSEG64 8
+#endif
+
 using namespace sparc;
 
 // single-precision literal of initialize.f90:38 (no -r8 in makefile:27)
@@ -61,7 +75,7 @@ struct sparc_ctx {
   void* rates = nullptr;    // damping rates, transposed: [y + ny*m]
   void* rates_o = nullptr;  // damping rates as given: [m + (nx/2+1)*y]
   bool damping_set = false, filter_set = false;
-  void* tri[4] = {};  // per ibc 0..3: lowr | invb | gam, nz each
+  void* tri[4] = {};  // per ibc 0..3: lowr | invb | gam | fcum | gcum, nz each
   double tc[14] = {};
   // per ibc and precision (0 fp64, 1 fp32): [kc0, kc1) rows with constant factors
   int kc[4][2] = {};
@@ -322,7 +336,7 @@ static size_t footprint_bytes(const sparc_config* cfg) {
   b += 8 * N * ts;                          // work
   if (!is_2d(cfg->option) && cfg->damp_waves)
     b += 3 * spec_elems(cfg) * 2 * ts;      // damping spectra
-  b += (size_t)(cfg->nx + cfg->ny) * 6 * ts + 4 * 3 * cfg->nz * ts + SPARC_PROF_COUNT * cfg->nz * ts;
+  b += (size_t)(cfg->nx + cfg->ny) * 6 * ts + 4 * 5 * cfg->nz * ts + SPARC_PROF_COUNT * cfg->nz * ts;
   b += 2 * (size_t)(cfg->nx / 2 + 1) * cfg->ny * ts + 2 * nxy * ts;  // rates (two layouts)
   return b;
 }
@@ -450,14 +464,29 @@ extern "C" int sparc_create(const sparc_config* cfg, sparc_ctx** out) {
   for (int ibc = 0; ibc < 4; ++ibc) {
     std::vector<double> lowr, invb, gam;
     tri_factors(cfg->nz, ibc, lowr, invb, gam);
-    std::vector<double> t(3 * cfg->nz);
+    std::vector<double> t(5 * cfg->nz);
     for (int k = 0; k < cfg->nz; ++k) {
       t[k] = lowr[k];
       t[cfg->nz + k] = invb[k];
       t[2 * cfg->nz + k] = gam[k];
     }
-    if ((rc = dalloc(c, &c->tri[ibc], 3 * cfg->nz * ts))) return rc;
-    if ((rc = upload(c, c->tri[ibc], t.data(), 3 * cfg->nz))) return rc;
+    // segment coupling products of k_zsolve_b (SPARC_ZB_NSEG segments of L rows):
+    // fcum[k] = prod_{i=first..k} (-lowr_i invb_i), gcum[k] = prod_{i=k+1..end} (-gam_i)
+    {
+      const int nseg = c->fp32 ? SPARC_ZB_NSEG : SPARC_ZB_NSEG64;
+      const int n = cfg->nz, L = (n + nseg - 1) / nseg;
+      double* fc = t.data() + 3 * n;
+      double* gc = t.data() + 4 * n;
+      for (int a = 0; a < n; a += L) {
+        const int b = std::min(n, a + L);
+        double p = 1.0;
+        for (int k = a; k < b; ++k) fc[k] = (p *= -lowr[k] * invb[k]);
+        p = 1.0;
+        for (int k = b - 1; k >= a; --k) gc[k] = (b == n) ? 0.0 : (p *= -gam[k + 1]);
+      }
+    }
+    if ((rc = dalloc(c, &c->tri[ibc], 5 * cfg->nz * ts))) return rc;
+    if ((rc = upload(c, c->tri[ibc], t.data(), 5 * cfg->nz))) return rc;
     // the widest range around the middle row where lowr, invb and gam (as
     // stored in the working precision) equal the middle row's values bitwise
     const int n = cfg->nz, mid = n / 2;
@@ -775,7 +804,7 @@ static bool fast_size(int n) {
 #define SPARC_XDAMP_LPB_DIV 1
 #endif
 // Length-256 lines use the register four-step kernels of fft_r16.h for the
-// y passes and the damping inverse x pass.  Measured at 256x256x300 FP32 (us
+// y passes and the damping inverse x pass (x derivatives: k_xw256 below).  Measured at 256x256x300 FP32 (us
 // per launch, radix-16 Stockham of fft_reg.h -> four-step): y derivative
 // 59.3 -> 50.9, damping y 108.1 -> 97.9, damping inverse x 131.9 -> 125.0;
 // but x derivative 65.1 -> 78.6 and damping forward x 99.5 -> 124.4 (lanes
@@ -785,9 +814,28 @@ static bool fast_size(int n) {
 #define SPARC_R16 1
 #endif
 
+// x derivatives of length 256: one wavefront per row pair with cross-lane
+// shuffles (k_xw256), SPARC_XW_LPW row pairs per wavefront.  Measured at
+// 256x256x300 FP32, us per launch averaged over the xi_x and p derivatives:
+// Stockham 67.6, k_xw256 56.4 (1 or 2 pairs per wavefront; 4 pairs 62.5,
+// 8 pairs 63.1).  DPP and v_permlane16/32_swap exchanges in place of the
+// shuffles measured the same (57.8): the kernel is not bound by them.
+#ifndef SPARC_XW
+#define SPARC_XW 1
+#endif
+#ifndef SPARC_XW_LPW
+#define SPARC_XW_LPW 2
+#endif
+
 template <typename T, int MODE, int N>
 static void launch_xn(sparc_ctx* c, int nf, const XArgs<T>& X, T beta, const T* mult, int nrows,
                       int nlines) {
+  if constexpr (N == 256 && SPARC_XW && MODE == XM_DERIV) {
+    hipLaunchKernelGGL((k_xw256<T, SPARC_XW_LPW>), dim3(cdiv(nlines, 4 * SPARC_XW_LPW), nf),
+                       dim3(256), 0, c->stream, nrows, nlines, X, beta, mult,
+                       (const C2<T>*)c->twx, c->cfg.ny, c->cfg.nz);
+    return;
+  }
   if constexpr (N == 256 && SPARC_R16 && MODE == XM_DAMP_INV) {
     hipLaunchKernelGGL((k_x256<T, MODE>), dim3(cdiv(nlines, 16), nf), dim3(256), 0, c->stream,
                        nrows, nlines, X, beta, mult, (const C2<T>*)c->twx, c->cfg.ny, c->cfg.nz,
@@ -986,6 +1034,23 @@ static int launch_z(sparc_ctx* c, const Fld<T>& F, int ibc, const T* inA, const 
   const int nseg = nz >= 192 ? 4 : (nz >= 96 ? 2 : 1);
   const TriCoef<T> tcv = make_tc<T>(c, ibc);
   const dim3 grid(cdiv(c->nxy, 64));
+  const size_t lds_b = 2 * 64 * (size_t)nz * sizeof(T);
+  if (((SPARC_ZB_EPIS >> EPI) & 1) && nz >= 5 && lds_b <= (size_t)c->lds_max) {
+    constexpr int NS = sizeof(T) == 4 ? SPARC_ZB_NSEG : SPARC_ZB_NSEG64;
+    const bool bg3 = c->bg3d[SPARC_BG_C2RHO0] || c->bg3d[SPARC_BG_GRADP0_Z] ||
+                     c->bg3d[SPARC_BG_RHO0] || c->bg3d[SPARC_BG_GRADRHO0_Z];
+#define ZB(G)                                                                                    \
+  hipLaunchKernelGGL((k_zsolve_b<T, EPI, NS, G>), grid, dim3(64 * NS), lds_b, c->stream, F, tcv, \
+                     tri, tri + nz, tri + 2 * nz, tri + 3 * nz, tri + 4 * nz, ibc, inA, inS, beta, \
+                     prov, provscale, out)
+    if (bg3)
+      ZB(false);
+    else
+      ZB(true);
+#undef ZB
+    LAUNCHCHK(c);
+    return SPARC_OK;
+  }
   // ud in LDS when two blocks of 64 columns fit in one CU's LDS
   const size_t lds_ud = 64 * (size_t)nz * sizeof(T);
   const bool ulds = SPARC_ZS_ULDS && nseg == 4 && lds_ud <= 81920 && lds_ud <= (size_t)c->lds_max;

@@ -26,24 +26,32 @@ struct EIn {
   T pv, ps, paz, pbz;  // PML psi and coefficients on PML planes
 };
 
-template <typename T, int EPI>
+// NOUD: ud is not read here.  BKEEP: the epilogue's own copy of the solved
+// field's stage temp (a2 + beta*s2) is supplied by the caller, not loaded.
+// NOBG3: the caller guarantees that the epilogue's backgrounds are z
+// profiles, so no 3-D background is read.
+template <typename T, int EPI, bool NOUD = false, bool BKEEP = false, bool NOBG3 = false>
 __device__ __forceinline__ void load_epi(const Fld<T>& F, const T* __restrict__ ud, long uoff,
                                          long ustr, int c, int k, bool own, EIn<T>& e) {
   const int n = F.nz;
   const long nxy = F.nxy;
   const long q = c + nxy * k;
-  e.ud = ud[uoff + ustr * k];
+  if (!NOUD) e.ud = ud[uoff + ustr * k];
   if (!own) return;
   const bool st = F.beta != (T)0;
   if (EPI == EPI_P3D || EPI == EPI_P2D) {
     e.w0 = F.W[0][q];
     if (EPI == EPI_P3D) e.w1 = F.W[1][q];
-    e.a2 = F.a[SPARC_XI_Z][q];
-    if (st) e.s2 = F.scr[SPARC_XI_Z][q];
-    if (F.bg3d[SPARC_BG_C2RHO0]) e.g0 = F.bg[SPARC_BG_C2RHO0][q];
-    if (F.bg3d[SPARC_BG_GRADP0_Z]) e.g1 = F.bg[SPARC_BG_GRADP0_Z][q];
-    if (F.bg3d[SPARC_BG_RHO0]) e.g2 = F.bg[SPARC_BG_RHO0][q];
-    if (F.bg3d[SPARC_BG_GRADRHO0_Z]) e.g3 = F.bg[SPARC_BG_GRADRHO0_Z][q];
+    if (!BKEEP) {
+      e.a2 = F.a[SPARC_XI_Z][q];
+      if (st) e.s2 = F.scr[SPARC_XI_Z][q];
+    }
+    if (!NOBG3) {
+      if (F.bg3d[SPARC_BG_C2RHO0]) e.g0 = F.bg[SPARC_BG_C2RHO0][q];
+      if (F.bg3d[SPARC_BG_GRADP0_Z]) e.g1 = F.bg[SPARC_BG_GRADP0_Z][q];
+      if (F.bg3d[SPARC_BG_RHO0]) e.g2 = F.bg[SPARC_BG_RHO0][q];
+      if (F.bg3d[SPARC_BG_GRADRHO0_Z]) e.g3 = F.bg[SPARC_BG_GRADRHO0_Z][q];
+    }
   } else if (EPI == EPI_RHS3D || EPI == EPI_RHS2D) {
     if (EPI == EPI_RHS2D) e.w0 = F.W[0][q];
     e.w4 = F.W[4][q];
@@ -59,12 +67,16 @@ __device__ __forceinline__ void load_epi(const Fld<T>& F, const T* __restrict__ 
   } else if (EPI == EPI_C2) {  // div v, RHScont, RHSp of option 2
     e.w0 = F.W[0][q];
     e.w1 = F.W[1][q];
-    e.a2 = F.a[SPARC_V_Z - 2][q];  // v_z in slot 3
-    if (st) e.s2 = F.scr[SPARC_V_Z - 2][q];
-    if (F.bg3d[SPARC_BG_C2RHO0]) e.g0 = F.bg[SPARC_BG_C2RHO0][q];
-    if (F.bg3d[SPARC_BG_GRADP0_Z]) e.g1 = F.bg[SPARC_BG_GRADP0_Z][q];
-    if (F.bg3d[SPARC_BG_RHO0]) e.g2 = F.bg[SPARC_BG_RHO0][q];
-    if (F.bg3d[SPARC_BG_GRADRHO0_Z]) e.g3 = F.bg[SPARC_BG_GRADRHO0_Z][q];
+    if (!BKEEP) {
+      e.a2 = F.a[SPARC_V_Z - 2][q];  // v_z in slot 3
+      if (st) e.s2 = F.scr[SPARC_V_Z - 2][q];
+    }
+    if (!NOBG3) {
+      if (F.bg3d[SPARC_BG_C2RHO0]) e.g0 = F.bg[SPARC_BG_C2RHO0][q];
+      if (F.bg3d[SPARC_BG_GRADP0_Z]) e.g1 = F.bg[SPARC_BG_GRADP0_Z][q];
+      if (F.bg3d[SPARC_BG_RHO0]) e.g2 = F.bg[SPARC_BG_RHO0][q];
+      if (F.bg3d[SPARC_BG_GRADRHO0_Z]) e.g3 = F.bg[SPARC_BG_GRADRHO0_Z][q];
+    }
   } else if (EPI == EPI_V2) {  // RHSv of option 2; moves RHScont, RHSp into scr
     e.w0 = F.W[0][q];
     e.w1 = F.W[1][q];
@@ -261,6 +273,9 @@ __global__ __launch_bounds__(64 * NSEG) __attribute__((amdgpu_waves_per_eu(4))) 
 #ifndef SPARC_ZS_UB
 #define SPARC_ZS_UB 3
 #endif
+#ifndef SPARC_ZS_RING
+#define SPARC_ZS_RING 0
+#endif
   constexpr int UF = SPARC_ZS_UF;  // forward chunk (planes loaded one chunk ahead)
   constexpr int UB = SPARC_ZS_UB;  // backward chunk
   constexpr int ZW = ZWarm<T>::W;
@@ -421,6 +436,27 @@ __global__ __launch_bounds__(64 * NSEG) __attribute__((amdgpu_waves_per_eu(4))) 
     }
     T x = 0;
     bool first = true;
+#if SPARC_ZS_RING
+    // ring of UB planes in flight: slot j is consumed, then refilled with
+    // the plane UB further down
+    for (int kh = kstart; kh >= ks; kh -= UB) {
+#pragma unroll
+      for (int j = 0; j < UB; ++j) {
+        const int k = kh - j;
+        if (k >= ks) {
+          EIn<T> cu = nx[j];
+          if (k - UB >= ks) load_epi<T, EPI>(F, udr, uoff, ustr, c, k - UB, k - UB < ke, nx[j]);
+          if (first) {
+            x = cu.ud;  // exact x_{n-1} = u_{n-1}, or the cold start
+            first = false;
+          } else {
+            x = cu.ud - GM(k + 1) * x;
+          }
+          if (k < ke) epilogue<T, EPI>(F, c, k, x * ST(k), GG(k), cu, out);
+        }
+      }
+    }
+#else
     for (int kh = kstart; kh >= ks; kh -= UB) {
       EIn<T> cu[UB];
 #pragma unroll
@@ -443,6 +479,256 @@ __global__ __launch_bounds__(64 * NSEG) __attribute__((amdgpu_waves_per_eu(4))) 
             x = cu[j].ud - GM(k + 1) * x;
           }
           if (k < ke) epilogue<T, EPI>(F, c, k, x * ST(k), GG(k), cu[j], out);
+        }
+      }
+    }
+#endif
+  }
+}
+
+// Exact segmented solve with the column's input and sweep results in LDS
+// (64 x n values each).  Block = 64 columns x NSEG waves; wave s owns planes
+// [s*L, min(n, (s+1)*L)).  Because the matrix is the same for every column,
+// the coupling between segments is carried by per-plane products of the LU
+// factors instead of a warm-up:
+//   P1  B = a + beta*s of the owned planes -> LDS (each input read once).
+//   P2  forward sweep from u = 0 before the segment's first row (u~); the
+//       five-point right-hand side takes its halo planes from LDS.
+//   P3  U = the true u on the previous segment's last row, chained over the
+//       earlier segments: U <- u~_last(j) + fc[last(j)] U.  The true u is
+//       u~ + fc U, fc[k] = prod_{i = first .. k} (-lowr_i / betad_i).
+//   P4  backward sweep from x = u on the segment's last row (x~).
+//   P5  X = the true x on the next segment's first row, chained from the
+//       last segment down: X <- x~_first(j) + gc[first(j)] X.  The true x is
+//       x~ + gc X, gc[k] = prod_{i = k+1 .. segment end} (-gam_i) (0 in the
+//       last segment); epilogue.
+// In exact arithmetic this equals the sequential LU solve.  BKEEP epilogues
+// take the solved field's stage temp from the LDS copy of B.
+#ifndef SPARC_ZB_UB
+#define SPARC_ZB_UB 3
+#endif
+#ifndef SPARC_ZB_UBP
+#define SPARC_ZB_UBP 14
+#endif
+#ifndef SPARC_ZB_UBP3
+#define SPARC_ZB_UBP3 10
+#endif
+#ifndef SPARC_ZB_U1
+#define SPARC_ZB_U1 20
+#endif
+#ifndef SPARC_ZB_WIN
+#define SPARC_ZB_WIN 0
+#endif
+#ifndef SPARC_ZB_CHU
+#define SPARC_ZB_CHU 0
+#endif
+#ifndef SPARC_ZB_P4U
+#define SPARC_ZB_P4U 4
+#endif
+template <typename T, int EPI, int NSEG, bool NOBG3>
+__global__ __launch_bounds__(64 * NSEG) void k_zsolve_b(
+    Fld<T> F, TriCoef<T> tc, const T* __restrict__ lowr, const T* __restrict__ invb,
+    const T* __restrict__ gam, const T* __restrict__ fcum, const T* __restrict__ gcum, int ibc,
+    const T* __restrict__ inA, const T* __restrict__ inS, T beta_in, const T* __restrict__ prov,
+    T provscale, T* __restrict__ out) {
+  constexpr bool HAS_G = (EPI == EPI_RHS3D || EPI == EPI_RHS2D || EPI == EPI_V2);
+  constexpr bool BKEEP = (EPI == EPI_P3D || EPI == EPI_P2D || EPI == EPI_C2);
+  constexpr int U1 = SPARC_ZB_U1;                       // P1 planes in flight
+  // P5 planes in flight (bounded by 128 VGPRs at 16 waves per CU)
+  constexpr int UB = BKEEP ? (NOBG3 ? SPARC_ZB_UBP : SPARC_ZB_UBP3) : SPARC_ZB_UB;
+  extern __shared__ unsigned char zsm[];
+  const int n = F.nz;
+  T* sB = reinterpret_cast<T*>(zsm);  // B[k * 64 + lane]
+  T* sX = sB + 64 * n;                // u~, then x~
+  const int lane = threadIdx.x & 63;
+  const int seg = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+  const int c = blockIdx.x * 64 + lane;
+  const bool active = c < F.nxy;
+  const long nxy = F.nxy;
+  const int L = (n + NSEG - 1) / NSEG;
+  const int ks = seg * L;
+  const int ke = min(n, ks + L);
+  // ---------------- P1: B of the owned planes
+  {
+    T na[U1], ns[U1];
+    auto ld = [&](int kb) {
+#pragma unroll
+      for (int j = 0; j < U1; ++j) {
+        const int k = kb + j;
+        if (active && k < ke) {
+          const long q = c + nxy * k;
+          na[j] = inA[q];
+          if (inS) ns[j] = inS[q];
+        }
+      }
+    };
+    ld(ks);
+    for (int kb = ks; kb < ke; kb += U1) {
+      T ca[U1], cs[U1];
+#pragma unroll
+      for (int j = 0; j < U1; ++j) {
+        ca[j] = na[j];
+        cs[j] = ns[j];
+      }
+      if (kb + U1 < ke) ld(kb + U1);
+#pragma unroll
+      for (int j = 0; j < U1; ++j) {
+        const int k = kb + j;
+        if (k < ke) sB[k * 64 + lane] = active ? (inS ? ca[j] + beta_in * cs[j] : ca[j]) : (T)0;
+      }
+    }
+  }
+  // the epilogue's first planes are requested now and arrive during P2-P4
+  EIn<T> nx[UB];
+  if (active && ks < ke) {
+#pragma unroll
+    for (int j = 0; j < UB; ++j) {
+      const int k = ks + j;
+      if (k < ke) load_epi<T, EPI, true, BKEEP, NOBG3>(F, nullptr, 0, 0, c, k, true, nx[j]);
+    }
+  }
+  __syncthreads();
+  auto Bv = [&](int k) -> T { return sB[k * 64 + lane]; };
+  // ---------------- P2: local forward sweep
+  if (ks < ke) {
+    T u = 0;
+    int k = ks;
+    if (k < 2) {
+      // rows 0 and 1 (dbyd2.f90:120-146)
+      const T r1 = tc.c1 * Bv(0) + tc.c2 * Bv(1) + tc.c3 * Bv(2) + tc.c4 * Bv(3) + tc.c5 * Bv(4);
+      if (k == 0) {
+        T r0;
+        if (ibc == 0)
+          r0 = 0;
+        else if (ibc == 1 || ibc == 3)
+          r0 = tc.b1 * Bv(0) + tc.b2 * Bv(1) + tc.b3 * Bv(2) + tc.b4 * Bv(3);
+        else
+          r0 = (prov && active) ? prov[c] * provscale : (T)0;
+        if (ibc & 1) r0 = (tc.alpha2 * r0 - tc.beta1 * r1) * tc.cst;
+        u = r0 * invb[0];
+        sX[lane] = u;
+        k = 1;
+      }
+      if (k < ke) {
+        u = (r1 - lowr[1] * u) * invb[1];
+        sX[64 + lane] = u;
+        k = 2;
+      }
+    }
+    const int kend = min(ke, n - 2);
+#if SPARC_ZB_WIN
+    if (k < kend) {
+      // window B(k-2..k+1) in registers, one LDS read per row
+      T w0 = Bv(k - 2), w1 = Bv(k - 1), w2 = Bv(k), w3 = Bv(k + 1);
+#pragma unroll 4
+      for (; k < kend; ++k) {
+        const T w4 = Bv(k + 2);
+        const T r = tc.ci1 * (w3 - w1) + tc.ci2 * (w4 - w0);
+        u = (r - lowr[k] * u) * invb[k];
+        sX[k * 64 + lane] = u;
+        w0 = w1;
+        w1 = w2;
+        w2 = w3;
+        w3 = w4;
+      }
+    }
+#else
+    for (; k < kend; ++k) {
+      const T r = tc.ci1 * (Bv(k + 1) - Bv(k - 1)) + tc.ci2 * (Bv(k + 2) - Bv(k - 2));
+      u = (r - lowr[k] * u) * invb[k];
+      sX[k * 64 + lane] = u;
+    }
+#endif
+    if (k < ke) {
+      // rows n-2 and n-1
+      const T rn2 = -(tc.c1 * Bv(n - 1) + tc.c2 * Bv(n - 2) + tc.c3 * Bv(n - 3) +
+                      tc.c4 * Bv(n - 4) + tc.c5 * Bv(n - 5));
+      if (k == n - 2) {
+        u = (rn2 - lowr[n - 2] * u) * invb[n - 2];
+        sX[(n - 2) * 64 + lane] = u;
+        k = n - 1;
+      }
+      if (k < ke) {
+        T rn1;
+        if (ibc == 0)
+          rn1 = 0;
+        else if (ibc == 1 || ibc == 2) {
+          rn1 = -(tc.b1 * Bv(n - 1) + tc.b2 * Bv(n - 2) + tc.b3 * Bv(n - 3) + tc.b4 * Bv(n - 4));
+          rn1 = (tc.alpha2 * rn1 - tc.beta1 * rn2) * tc.cst;
+        } else {
+          rn1 = (prov && active) ? prov[c] * provscale : (T)0;
+        }
+        u = (rn1 - lowr[n - 1] * u) * invb[n - 1];
+        sX[(n - 1) * 64 + lane] = u;
+      }
+    }
+  }
+  __syncthreads();
+  // ---------------- P3: true u on the previous segment's last row
+  T U = 0;
+  if (ks < ke) {
+#if SPARC_ZB_CHU
+#pragma unroll
+    for (int j = 0; j < NSEG - 1; ++j) {
+      if (j < seg) {
+        const int bj = (j + 1) * L - 1;
+        U = sX[bj * 64 + lane] + fcum[bj] * U;
+      }
+    }
+#else
+    for (int j = 0; j < seg; ++j) {
+      const int bj = (j + 1) * L - 1;
+      U = sX[bj * 64 + lane] + fcum[bj] * U;
+    }
+#endif
+  }
+  __syncthreads();
+  // ---------------- P4: corrected u, local backward sweep
+  if (ks < ke) {
+    T x = sX[(ke - 1) * 64 + lane] + fcum[ke - 1] * U;
+    sX[(ke - 1) * 64 + lane] = x;
+#pragma unroll SPARC_ZB_P4U
+    for (int k = ke - 2; k >= ks; --k) {
+      const T u = sX[k * 64 + lane] + fcum[k] * U;
+      x = u - gam[k + 1] * x;
+      sX[k * 64 + lane] = x;
+    }
+  }
+  __syncthreads();
+  // ---------------- P5: true x on the next segment's first row, epilogue
+  if (active && ks < ke) {
+    T X = 0;
+    const int nsu = (n + L - 1) / L;
+#if SPARC_ZB_CHU
+#pragma unroll
+    for (int j = NSEG - 1; j > 0; --j) {
+      if (j < nsu && j > seg) {
+        const int aj = j * L;
+        X = sX[aj * 64 + lane] + gcum[aj] * X;
+      }
+    }
+#else
+    for (int j = nsu - 1; j > seg; --j) {
+      const int aj = j * L;
+      X = sX[aj * 64 + lane] + gcum[aj] * X;
+    }
+#endif
+    const T* __restrict__ st = F.stretch;
+    // ring of UB planes in flight: slot j is consumed, then refilled with
+    // the plane UB further on
+    for (int kb = ks; kb < ke; kb += UB) {
+#pragma unroll
+      for (int j = 0; j < UB; ++j) {
+        const int k = kb + j;
+        if (k < ke) {
+          EIn<T> cu = nx[j];
+          if (k + UB < ke) load_epi<T, EPI, true, BKEEP, NOBG3>(F, nullptr, 0, 0, c, k + UB, true, nx[j]);
+          const T x = sX[k * 64 + lane] + gcum[k] * X;
+          if (BKEEP) {
+            cu.a2 = sB[k * 64 + lane];
+            cu.s2 = 0;
+          }
+          epilogue<T, EPI>(F, c, k, x * st[k], HAS_G ? F.g[k] : (T)0, cu, out);
         }
       }
     }

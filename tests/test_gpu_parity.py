@@ -101,7 +101,8 @@ def _field(shape, seed):
 @pytest.mark.parametrize("prec,tol", [(abi.FP64, TOL64), (abi.FP32, TOL32_OP)])
 def test_operators_match_oracle(request, grid, prec, tol):
     """x, y, z derivatives, filters and the damping transform; the 256-wide
-    grid runs the length-256 register four-step line kernels."""
+    grid runs the length-256 register four-step line kernels and the
+    wavefront-per-line x derivative."""
     W, S = request.getfixturevalue(grid)
     eng, cfg = D.make_engine(W, S, precision=prec)
     try:
