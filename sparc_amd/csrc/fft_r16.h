@@ -316,19 +316,36 @@ __device__ __forceinline__ void xw_twiddle(C2<T> (&v)[4], const C2<T> (&w)[3], b
   }
 }
 
-// Same contract as k_xlines_n<T, 256, XM_DERIV>: line L = rows 2L, 2L+1
-// (x fastest), out = i k mult(k) applied, or the momentum RHS when X.rhs.
-// LPW line pairs per wave, the next pair's loads issued before the current
-// pair's transform.
-template <typename T, int LPW>
+// Same contracts as k_xlines_n<T, 256, MODE> for MODE = XM_DERIV (line L =
+// rows 2L, 2L+1, x fastest; out = i k mult(k) applied, or the momentum RHS
+// when X.rhs) and XM_DAMP_FWD (line L = row j of planes 2q, 2q+1, L = j +
+// ny q; spec[L N + k] = the forward transform).  LPW lines per wave, the
+// next line's loads issued before the current line's transform.
+template <typename T, int MODE, int LPW>
 __global__ __launch_bounds__(256) void k_xw256(int nrows, int nlines, XArgs<T> X, T beta,
                                                const T* __restrict__ mult,
-                                               const C2<T>* __restrict__ twg, int ny, int nz) {
+                                               const C2<T>* __restrict__ twg, int ny, int nz,
+                                               long nxy) {
+  static_assert(MODE == XM_DERIV || MODE == XM_DAMP_FWD, "k_xw256 modes");
   constexpr int N = 256;
   const int fld = blockIdx.y;
   const T* __restrict__ a = X.a[fld];
   const T* __restrict__ s = X.s[fld];
   T* __restrict__ out = X.out[fld];
+  C2<T>* __restrict__ spec = X.spec[fld];
+  // offset of the line's first row, stride to its second, whether it exists
+  auto rows = [&](long L, long& o, long& st, bool& h1) {
+    if (MODE == XM_DERIV) {
+      o = 2 * L * N;
+      st = N;
+      h1 = 2 * L + 1 < nrows;
+    } else {
+      const long q = L / ny, j = L - q * ny;
+      o = N * j + nxy * (2 * q);
+      st = nxy;
+      h1 = 2 * q + 1 < nz;
+    }
+  };
   const int lane = threadIdx.x & 63;
   const int wv = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
   const long L0 = ((long)blockIdx.x * 4 + wv) * LPW;
@@ -343,23 +360,26 @@ __global__ __launch_bounds__(256) void k_xw256(int nrows, int nlines, XArgs<T> X
   }
   const int kl = (lane >> 4) + 4 * ((lane >> 2) & 3) + 16 * (lane & 3);
   T ml[4];
+  if (MODE == XM_DERIV) {
 #pragma unroll
-  for (int j = 0; j < 4; ++j) ml[j] = mult[kl + 64 * j];
+    for (int j = 0; j < 4; ++j) ml[j] = mult[kl + 64 * j];
+  }
   C2<T> nv[4];
 #define XW_LOAD(LL)                                                  \
   {                                                                  \
-    const long r0_ = 2 * (LL);                                       \
-    const bool h1_ = r0_ + 1 < nrows;                                \
-    const T* __restrict__ a0_ = a + r0_ * N;                         \
-    const T* __restrict__ s0_ = s ? s + r0_ * N : nullptr;           \
+    long o_, st_;                                                    \
+    bool h1_;                                                        \
+    rows((LL), o_, st_, h1_);                                        \
+    const T* __restrict__ a0_ = a + o_;                              \
+    const T* __restrict__ s0_ = s ? s + o_ : nullptr;                \
     _Pragma("unroll") for (int j = 0; j < 4; ++j) {                  \
       const int x = lane + 64 * j;                                   \
       T re = a0_[x];                                                 \
       if (s0_) re = re + beta * s0_[x];                              \
       T im = 0;                                                      \
       if (h1_) {                                                     \
-        im = a0_[N + x];                                             \
-        if (s0_) im = im + beta * s0_[N + x];                        \
+        im = a0_[st_ + x];                                           \
+        if (s0_) im = im + beta * s0_[st_ + x];                      \
       }                                                              \
       nv[j] = C2<T>{re, im};                                         \
     }                                                                \
@@ -382,6 +402,12 @@ __global__ __launch_bounds__(256) void k_xw256(int nrows, int nlines, XArgs<T> X
     xw_twiddle(v, w3, false);
     xw_swap<T, 1>(v, lane);
     dft_reg<T, 4, -1>(v);
+    if (MODE == XM_DAMP_FWD) {
+      // natural order: each store fills one 512-byte block, lanes permuted
+#pragma unroll
+      for (int j = 0; j < 4; ++j) spec[L * N + kl + 64 * j] = v[j];
+      continue;
+    }
 #pragma unroll
     for (int j = 0; j < 4; ++j) v[j] = C2<T>{-ml[j] * v[j].y, ml[j] * v[j].x};
     dft_reg<T, 4, +1>(v);
