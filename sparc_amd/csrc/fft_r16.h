@@ -318,15 +318,17 @@ __device__ __forceinline__ void xw_twiddle(C2<T> (&v)[4], const C2<T> (&w)[3], b
 
 // Same contracts as k_xlines_n<T, 256, MODE> for MODE = XM_DERIV (line L =
 // rows 2L, 2L+1, x fastest; out = i k mult(k) applied, or the momentum RHS
-// when X.rhs) and XM_DAMP_FWD (line L = row j of planes 2q, 2q+1, L = j +
-// ny q; spec[L N + k] = the forward transform).  LPW lines per wave, the
-// next line's loads issued before the current line's transform.
+// when X.rhs), XM_DAMP_FWD (line L = row j of planes 2q, 2q+1, L = j +
+// ny q; spec[L N + k] = the forward transform) and XM_DAMP_INV (the same
+// rows of out minus the inverse transform of spec, planes 1 and nz skipped
+// unless X.edges).  LPW lines per wave, the next line's loads issued before
+// the current line's transform.
 template <typename T, int MODE, int LPW>
 __global__ __launch_bounds__(256) void k_xw256(int nrows, int nlines, XArgs<T> X, T beta,
                                                const T* __restrict__ mult,
                                                const C2<T>* __restrict__ twg, int ny, int nz,
                                                long nxy) {
-  static_assert(MODE == XM_DERIV || MODE == XM_DAMP_FWD, "k_xw256 modes");
+  static_assert(MODE == XM_DERIV || MODE == XM_DAMP_FWD || MODE == XM_DAMP_INV, "k_xw256 modes");
   constexpr int N = 256;
   const int fld = blockIdx.y;
   const T* __restrict__ a = X.a[fld];
@@ -365,8 +367,29 @@ __global__ __launch_bounds__(256) void k_xw256(int nrows, int nlines, XArgs<T> X
     for (int j = 0; j < 4; ++j) ml[j] = mult[kl + 64 * j];
   }
   C2<T> nv[4];
-#define XW_LOAD(LL)                                                  \
+  T no0[MODE == XM_DAMP_INV ? 4 : 1], no1[MODE == XM_DAMP_INV ? 4 : 1];
+  // damping inverse: lane l takes X[kl + 64 j], the layout the inverse
+  // stages start from, and the RHS values it updates
+  auto edge_w = [&](long L, bool& w0, bool& w1) {
+    const long k0 = 2 * (L / ny), k1 = k0 + 1;
+    w0 = X.edges || (k0 != 0 && k0 != nz - 1);
+    w1 = k1 < nz - 1 || (X.edges && k1 < nz);
+  };
+#define XW_LOAD_INV(LL)                                              \
   {                                                                  \
+    long o_, st_;                                                    \
+    bool h1_, w0_, w1_;                                              \
+    rows((LL), o_, st_, h1_);                                        \
+    edge_w((LL), w0_, w1_);                                          \
+    _Pragma("unroll") for (int j = 0; j < 4; ++j) {                  \
+      const int x = lane + 64 * j;                                   \
+      nv[j] = spec[(LL) * N + kl + 64 * j];                          \
+      no0[j] = w0_ ? out[o_ + x] : (T)0;                             \
+      no1[j] = w1_ ? out[o_ + st_ + x] : (T)0;                       \
+    }                                                                \
+  }
+#define XW_LOAD(LL)                                                  \
+  if (MODE == XM_DAMP_INV) XW_LOAD_INV(LL) else {                    \
     long o_, st_;                                                    \
     bool h1_;                                                        \
     rows((LL), o_, st_, h1_);                                        \
@@ -389,27 +412,36 @@ __global__ __launch_bounds__(256) void k_xw256(int nrows, int nlines, XArgs<T> X
     const long L = L0 + p;
     if (L >= nlines) break;
     C2<T> v[4];
+    T ov0[MODE == XM_DAMP_INV ? 4 : 1], ov1[MODE == XM_DAMP_INV ? 4 : 1];
 #pragma unroll
-    for (int j = 0; j < 4; ++j) v[j] = nv[j];
-    if (p + 1 < LPW && L + 1 < nlines) XW_LOAD(L + 1);
-    dft_reg<T, 4, -1>(v);
-    xw_twiddle(v, w1, false);
-    xw_swap<T, 16>(v, lane);
-    dft_reg<T, 4, -1>(v);
-    xw_twiddle(v, w2, false);
-    xw_swap<T, 4>(v, lane);
-    dft_reg<T, 4, -1>(v);
-    xw_twiddle(v, w3, false);
-    xw_swap<T, 1>(v, lane);
-    dft_reg<T, 4, -1>(v);
-    if (MODE == XM_DAMP_FWD) {
-      // natural order: each store fills one 512-byte block, lanes permuted
-#pragma unroll
-      for (int j = 0; j < 4; ++j) spec[L * N + kl + 64 * j] = v[j];
-      continue;
+    for (int j = 0; j < 4; ++j) {
+      v[j] = nv[j];
+      if (MODE == XM_DAMP_INV) {
+        ov0[j] = no0[j];
+        ov1[j] = no1[j];
+      }
     }
+    if (p + 1 < LPW && L + 1 < nlines) XW_LOAD(L + 1);
+    if (MODE != XM_DAMP_INV) {  // forward transform
+      dft_reg<T, 4, -1>(v);
+      xw_twiddle(v, w1, false);
+      xw_swap<T, 16>(v, lane);
+      dft_reg<T, 4, -1>(v);
+      xw_twiddle(v, w2, false);
+      xw_swap<T, 4>(v, lane);
+      dft_reg<T, 4, -1>(v);
+      xw_twiddle(v, w3, false);
+      xw_swap<T, 1>(v, lane);
+      dft_reg<T, 4, -1>(v);
+      if (MODE == XM_DAMP_FWD) {
+        // natural order: each store fills one 512-byte block, lanes permuted
 #pragma unroll
-    for (int j = 0; j < 4; ++j) v[j] = C2<T>{-ml[j] * v[j].y, ml[j] * v[j].x};
+        for (int j = 0; j < 4; ++j) spec[L * N + kl + 64 * j] = v[j];
+        continue;
+      }
+#pragma unroll
+      for (int j = 0; j < 4; ++j) v[j] = C2<T>{-ml[j] * v[j].y, ml[j] * v[j].x};
+    }
     dft_reg<T, 4, +1>(v);
     xw_swap<T, 1>(v, lane);
     xw_twiddle(v, w3, true);
@@ -420,6 +452,19 @@ __global__ __launch_bounds__(256) void k_xw256(int nrows, int nlines, XArgs<T> X
     xw_swap<T, 16>(v, lane);
     xw_twiddle(v, w1, true);
     dft_reg<T, 4, +1>(v);
+    if (MODE == XM_DAMP_INV) {
+      long o, st;
+      bool h1, w0, w1;
+      rows(L, o, st, h1);
+      edge_w(L, w0, w1);
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        const int x = lane + 64 * j;
+        if (w0) out[o + x] = ov0[j] - v[j].x;
+        if (w1) out[o + st + x] = ov1[j] - v[j].y;
+      }
+      continue;
+    }
     const long r0 = 2 * L;
     const bool has1 = r0 + 1 < nrows;
     T* __restrict__ o0 = out + r0 * N;
@@ -449,4 +494,5 @@ __global__ __launch_bounds__(256) void k_xw256(int nrows, int nlines, XArgs<T> X
     }
   }
 #undef XW_LOAD
+#undef XW_LOAD_INV
 }

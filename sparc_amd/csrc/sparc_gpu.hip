@@ -820,7 +820,8 @@ static bool fast_size(int n) {
 // Stockham 67.6, k_xw256 56.4 (1 or 2 pairs per wavefront; 4 pairs 62.5,
 // 8 pairs 63.1).  DPP and v_permlane16/32_swap exchanges in place of the
 // shuffles measured the same (57.8): the kernel is not bound by them.  The
-// damping forward x pass also runs on it (SPARC_XW_DF): 103.3 -> 94.9 us.
+// damping forward x pass also runs on it (SPARC_XW_DF): 103.3 -> 94.9 us,
+// and the damping inverse (SPARC_XW_DI, in place of k_x256): 125.4 -> 121.7.
 #ifndef SPARC_XW
 #define SPARC_XW 1
 #endif
@@ -830,11 +831,16 @@ static bool fast_size(int n) {
 #ifndef SPARC_XW_DF
 #define SPARC_XW_DF 1
 #endif
+#ifndef SPARC_XW_DI
+#define SPARC_XW_DI 1
+#endif
 
 template <typename T, int MODE, int N>
 static void launch_xn(sparc_ctx* c, int nf, const XArgs<T>& X, T beta, const T* mult, int nrows,
                       int nlines) {
-  if constexpr (N == 256 && SPARC_XW && (MODE == XM_DERIV || (SPARC_XW_DF && MODE == XM_DAMP_FWD))) {
+  if constexpr (N == 256 && SPARC_XW &&
+                (MODE == XM_DERIV || (SPARC_XW_DF && MODE == XM_DAMP_FWD) ||
+                 (SPARC_XW_DI && MODE == XM_DAMP_INV))) {
     hipLaunchKernelGGL((k_xw256<T, MODE, SPARC_XW_LPW>), dim3(cdiv(nlines, 4 * SPARC_XW_LPW), nf),
                        dim3(256), 0, c->stream, nrows, nlines, X, beta, mult,
                        (const C2<T>*)c->twx, c->cfg.ny, c->cfg.nz, c->nxy);
