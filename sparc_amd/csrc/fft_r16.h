@@ -388,6 +388,9 @@ __global__ __launch_bounds__(256) void k_xw256(int nrows, int nlines, XArgs<T> X
       no1[j] = w1_ ? out[o_ + st_ + x] : (T)0;                       \
     }                                                                \
   }
+  // raw a and s values of the next line: the stage temp a + beta s is formed
+  // when the line is used, so the loads stay in flight meanwhile
+  C2<T> ns[4];
 #define XW_LOAD(LL)                                                  \
   if (MODE == XM_DAMP_INV) XW_LOAD_INV(LL) else {                    \
     long o_, st_;                                                    \
@@ -397,14 +400,10 @@ __global__ __launch_bounds__(256) void k_xw256(int nrows, int nlines, XArgs<T> X
     const T* __restrict__ s0_ = s ? s + o_ : nullptr;                \
     _Pragma("unroll") for (int j = 0; j < 4; ++j) {                  \
       const int x = lane + 64 * j;                                   \
-      T re = a0_[x];                                                 \
-      if (s0_) re = re + beta * s0_[x];                              \
-      T im = 0;                                                      \
-      if (h1_) {                                                     \
-        im = a0_[st_ + x];                                           \
-        if (s0_) im = im + beta * s0_[st_ + x];                      \
-      }                                                              \
-      nv[j] = C2<T>{re, im};                                         \
+      nv[j].x = a0_[x];                                              \
+      if (s0_) ns[j].x = s0_[x];                                     \
+      nv[j].y = h1_ ? a0_[st_ + x] : (T)0;                           \
+      if (s0_) ns[j].y = h1_ ? s0_[st_ + x] : (T)0;                  \
     }                                                                \
   }
   if (L0 < nlines) XW_LOAD(L0);
@@ -416,6 +415,7 @@ __global__ __launch_bounds__(256) void k_xw256(int nrows, int nlines, XArgs<T> X
 #pragma unroll
     for (int j = 0; j < 4; ++j) {
       v[j] = nv[j];
+      if (MODE != XM_DAMP_INV && s) v[j] = C2<T>{v[j].x + beta * ns[j].x, v[j].y + beta * ns[j].y};
       if (MODE == XM_DAMP_INV) {
         ov0[j] = no0[j];
         ov1[j] = no1[j];

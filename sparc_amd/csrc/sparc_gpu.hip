@@ -173,20 +173,31 @@ static int fail(sparc_ctx* c, int code, const char* fmt, ...) {
 // launches), so one process-wide lock covers capture, hipMalloc and hipFree.
 static std::mutex g_capture_mu;
 
+// Large allocations start on 2 MiB boundaries, so the same index of two
+// field arrays shares its low address bits.  The n-th allocation is moved
+// (n mod 16) * SPARC_STAGGER bytes past its start (a multiple of 256 keeps
+// every alignment the kernels use); allocs keeps the pointer hipMalloc gave.
+#ifndef SPARC_STAGGER
+#define SPARC_STAGGER 2304
+#endif
 static int dalloc(sparc_ctx* c, void** p, size_t bytes) {
+  static_assert(SPARC_STAGGER % 256 == 0, "stagger keeps 256-byte alignment");
   *p = nullptr;
   if (bytes == 0) bytes = 16;
+  const size_t off = (size_t)(c->allocs.size() % 16) * SPARC_STAGGER;
+  void* raw = nullptr;
   hipError_t e;
   {
     std::lock_guard<std::mutex> lk(g_capture_mu);
-    e = hipMalloc(p, bytes);
+    e = hipMalloc(&raw, bytes + off);
   }
   if (e != hipSuccess) {
     *p = nullptr;
     return fail(c, SPARC_ENOMEM, "hipMalloc(%zu) failed: %s", bytes, hipGetErrorString(e));
   }
-  c->allocs.push_back(*p);
-  c->bytes += bytes;
+  c->allocs.push_back(raw);
+  c->bytes += bytes + off;
+  *p = (char*)raw + off;
   return SPARC_OK;
 }
 
