@@ -276,8 +276,16 @@ __global__ __launch_bounds__(64 * NSEG) __attribute__((amdgpu_waves_per_eu(4))) 
 #ifndef SPARC_ZS_RING
 #define SPARC_ZS_RING 0
 #endif
-  constexpr int UF = SPARC_ZS_UF;  // forward chunk (planes loaded one chunk ahead)
-  constexpr int UB = SPARC_ZS_UB;  // backward chunk
+#ifndef SPARC_ZS_UF64
+#define SPARC_ZS_UF64 8
+#endif
+#ifndef SPARC_ZS_UB64
+#define SPARC_ZS_UB64 1
+#endif
+  // forward chunk (planes loaded one chunk ahead) and backward chunk; FP64
+  // values take two registers each, so its chunks are shorter
+  constexpr int UF = sizeof(T) == 4 ? SPARC_ZS_UF : SPARC_ZS_UF64;
+  constexpr int UB = sizeof(T) == 4 ? SPARC_ZS_UB : SPARC_ZS_UB64;
   constexpr int ZW = ZWarm<T>::W;
   extern __shared__ unsigned char zsm[];
   const int n = F.nz;
