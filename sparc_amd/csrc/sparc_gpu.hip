@@ -25,6 +25,14 @@
 #ifndef SPARC_ZB_NSEG
 #define SPARC_ZB_NSEG 16
 #endif
+// z-solve variant: ud in LDS (1) or in a global scratch array (0), and the
+// segments per column of the LDS variant.
+#ifndef SPARC_ZS_ULDS
+#define SPARC_ZS_ULDS 1
+#endif
+#ifndef SPARC_ZS_ULDS_NSEG
+#define SPARC_ZS_ULDS_NSEG 8
+#endif
 // FP64 takes 8 segments (512 threads): at 1024 threads its epilogues spill
 #ifndef SPARC_ZB_NSEG64
 #define SPARC_ZB_NSEG64 8
@@ -75,7 +83,7 @@ struct sparc_ctx {
   void* rates = nullptr;    // damping rates, transposed: [y + ny*m]
   void* rates_o = nullptr;  // damping rates as given: [m + (nx/2+1)*y]
   bool damping_set = false, filter_set = false;
-  void* tri[4] = {};  // per ibc 0..3: lowr | invb | gam | fcum | gcum, nz each
+  void* tri[4] = {};  // per ibc 0..3: lowr | invb | gam | fcum, gcum (2 partitions), nz each
   double tc[14] = {};
   // per ibc and precision (0 fp64, 1 fp32): [kc0, kc1) rows with constant factors
   int kc[4][2] = {};
@@ -347,7 +355,7 @@ static size_t footprint_bytes(const sparc_config* cfg) {
   b += 8 * N * ts;                          // work
   if (!is_2d(cfg->option) && cfg->damp_waves)
     b += 3 * spec_elems(cfg) * 2 * ts;      // damping spectra
-  b += (size_t)(cfg->nx + cfg->ny) * 6 * ts + 4 * 5 * cfg->nz * ts + SPARC_PROF_COUNT * cfg->nz * ts;
+  b += (size_t)(cfg->nx + cfg->ny) * 6 * ts + 4 * 7 * cfg->nz * ts + SPARC_PROF_COUNT * cfg->nz * ts;
   b += 2 * (size_t)(cfg->nx / 2 + 1) * cfg->ny * ts + 2 * nxy * ts;  // rates (two layouts)
   return b;
 }
@@ -475,19 +483,19 @@ extern "C" int sparc_create(const sparc_config* cfg, sparc_ctx** out) {
   for (int ibc = 0; ibc < 4; ++ibc) {
     std::vector<double> lowr, invb, gam;
     tri_factors(cfg->nz, ibc, lowr, invb, gam);
-    std::vector<double> t(5 * cfg->nz);
+    std::vector<double> t(7 * cfg->nz);
     for (int k = 0; k < cfg->nz; ++k) {
       t[k] = lowr[k];
       t[cfg->nz + k] = invb[k];
       t[2 * cfg->nz + k] = gam[k];
     }
-    // segment coupling products of k_zsolve_b (SPARC_ZB_NSEG segments of L rows):
-    // fcum[k] = prod_{i=first..k} (-lowr_i invb_i), gcum[k] = prod_{i=k+1..end} (-gam_i)
-    {
-      const int nseg = c->fp32 ? SPARC_ZB_NSEG : SPARC_ZB_NSEG64;
+    // segment coupling products of the exact segmented solves for a partition
+    // into nseg segments of L rows: fcum[k] = prod_{i=first..k} (-lowr_i
+    // invb_i), gcum[k] = prod_{i=k+1..end} (-gam_i), 0 in the last segment;
+    // rows 3-4 for k_zsolve_b, rows 5-6 for the exact mode of k_zsolve
+    // (SPARC_ZS_ULDS_NSEG segments)
+    auto coupling = [&](int nseg, double* fc, double* gc) {
       const int n = cfg->nz, L = (n + nseg - 1) / nseg;
-      double* fc = t.data() + 3 * n;
-      double* gc = t.data() + 4 * n;
       for (int a = 0; a < n; a += L) {
         const int b = std::min(n, a + L);
         double p = 1.0;
@@ -495,9 +503,12 @@ extern "C" int sparc_create(const sparc_config* cfg, sparc_ctx** out) {
         p = 1.0;
         for (int k = b - 1; k >= a; --k) gc[k] = (b == n) ? 0.0 : (p *= -gam[k + 1]);
       }
-    }
-    if ((rc = dalloc(c, &c->tri[ibc], 5 * cfg->nz * ts))) return rc;
-    if ((rc = upload(c, c->tri[ibc], t.data(), 5 * cfg->nz))) return rc;
+    };
+    coupling(c->fp32 ? SPARC_ZB_NSEG : SPARC_ZB_NSEG64, t.data() + 3 * cfg->nz,
+             t.data() + 4 * cfg->nz);
+    coupling(SPARC_ZS_ULDS_NSEG, t.data() + 5 * cfg->nz, t.data() + 6 * cfg->nz);
+    if ((rc = dalloc(c, &c->tri[ibc], 7 * cfg->nz * ts))) return rc;
+    if ((rc = upload(c, c->tri[ibc], t.data(), 7 * cfg->nz))) return rc;
     // the widest range around the middle row where lowr, invb and gam (as
     // stored in the working precision) equal the middle row's values bitwise
     const int n = cfg->nz, mid = n / 2;
@@ -1009,17 +1020,10 @@ static int launch_y(sparc_ctx* c, const T* a, const T* s, T beta, T* out, const 
   return launch_ym<T, MODE>(c, 1, X, beta, mult);
 }
 
-// z-solve variant: ud in LDS (1) or in a global scratch array (0), and the
-// segments per column of the LDS variant.  Measured at 256x256x300 FP32 on
-// MI355X (us per launch, momentum / pressure epilogue): global ud with 4
-// segments 280 / 174, LDS with 4 segments 280 / 215 (two blocks of four
-// waves per CU), LDS with 8 segments 260 / 168.
-#ifndef SPARC_ZS_ULDS
-#define SPARC_ZS_ULDS 1
-#endif
-#ifndef SPARC_ZS_ULDS_NSEG
-#define SPARC_ZS_ULDS_NSEG 8
-#endif
+// z-solve variants (SPARC_ZS_ULDS, SPARC_ZS_ULDS_NSEG above).  Measured at
+// 256x256x300 FP32 on MI355X (us per launch, momentum / pressure epilogue):
+// global ud with 4 segments 280 / 174, LDS with 4 segments 280 / 215 (two
+// blocks of four waves per CU), LDS with 8 segments 260 / 168.
 
 template <typename T, int EPI>
 static int launch_z(sparc_ctx* c, const Fld<T>& F, int ibc, const T* inA, const T* inS, T beta,
@@ -1079,7 +1083,7 @@ static int launch_z(sparc_ctx* c, const Fld<T>& F, int ibc, const T* inA, const 
 #define ZL(NS, U)                                                                                 \
   hipLaunchKernelGGL((k_zsolve<T, EPI, NS, U>), grid, dim3(64 * NS), lds, c->stream, F, tcv, tri, \
                      tri + nz, tri + 2 * nz, ibc, inA, inS, beta, (T*)c->W[7], prov, provscale,     \
-                     out)
+                     out, tri + 5 * nz, tri + 6 * nz)
   if (ulds)
     ZL(SPARC_ZS_ULDS_NSEG, true);
   else if (nseg == 4)

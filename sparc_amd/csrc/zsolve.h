@@ -255,7 +255,12 @@ struct ZWarm {
 
 // ULDS: the forward-sweep result ud lives in LDS (64 x nz values per block)
 // instead of a global scratch array, and the per-plane tables are read with
-// wave-uniform (scalar) loads to leave the LDS to ud.
+// wave-uniform (scalar) loads to leave the LDS to ud.  With ud in LDS the
+// segments are coupled exactly instead of by warm-up (SPARC_ZS_EXACT): the
+// forward sweep starts from u = 0 at its own first row, the true u on the
+// previous segment's last row is chained from the segment ends (fcum), the
+// local backward sweep runs in place in LDS, and the epilogue pass adds the
+// next segment's share of x (gcum), as in k_zsolve_b.
 template <typename T, int EPI, int NSEG, bool ULDS>
 __global__ __launch_bounds__(64 * NSEG) __attribute__((amdgpu_waves_per_eu(4))) void k_zsolve(
     Fld<T> F, TriCoef<T> tc,
@@ -266,7 +271,9 @@ __global__ __launch_bounds__(64 * NSEG) __attribute__((amdgpu_waves_per_eu(4))) 
                                                       const T* __restrict__ inS, T beta_in,
                                                       T* __restrict__ ud,
                                                       const T* __restrict__ prov, T provscale,
-                                                      T* __restrict__ out) {
+                                                      T* __restrict__ out,
+                                                      const T* __restrict__ fcum,
+                                                      const T* __restrict__ gcum) {
 #ifndef SPARC_ZS_UF
 #define SPARC_ZS_UF 8
 #endif
@@ -279,6 +286,9 @@ __global__ __launch_bounds__(64 * NSEG) __attribute__((amdgpu_waves_per_eu(4))) 
 #ifndef SPARC_ZS_UF64
 #define SPARC_ZS_UF64 8
 #endif
+#ifndef SPARC_ZS_EXACT
+#define SPARC_ZS_EXACT 1
+#endif
 #ifndef SPARC_ZS_UB64
 #define SPARC_ZS_UB64 1
 #endif
@@ -287,6 +297,9 @@ __global__ __launch_bounds__(64 * NSEG) __attribute__((amdgpu_waves_per_eu(4))) 
   constexpr int UF = sizeof(T) == 4 ? SPARC_ZS_UF : SPARC_ZS_UF64;
   constexpr int UB = sizeof(T) == 4 ? SPARC_ZS_UB : SPARC_ZS_UB64;
   constexpr int ZW = ZWarm<T>::W;
+  // EX: segments coupled exactly (the products fcum, gcum of this partition)
+  // instead of by warm-up; needs ud in LDS and at least 3 planes per segment
+  constexpr bool EX = ULDS && SPARC_ZS_EXACT;
   extern __shared__ unsigned char zsm[];
   const int n = F.nz;
   constexpr bool HAS_G = (EPI == EPI_RHS3D || EPI == EPI_RHS2D || EPI == EPI_V2);
@@ -341,7 +354,7 @@ __global__ __launch_bounds__(64 * NSEG) __attribute__((amdgpu_waves_per_eu(4))) 
   // ---------------- forward sweep
   if (active && ks < ke) {
     // rows 2..min(ke, n-2)-1 use the interior formula; rows 0, 1, n-2, n-1 are special
-    const bool exact = ks - ZW <= 2;
+    const bool exact = EX ? ks == 0 : ks - ZW <= 2;
     int k0;  // first interior row of the loop
     T u, wm1, w0, w1, w2, w3, w4;
     if (exact) {
@@ -366,7 +379,7 @@ __global__ __launch_bounds__(64 * NSEG) __attribute__((amdgpu_waves_per_eu(4))) 
       k0 = 2;
       wm1 = 0;
     } else {
-      k0 = ks - ZW;
+      k0 = EX ? ks : ks - ZW;
       u = 0;
       wm1 = B(k0 - 3);
       w0 = B(k0 - 2);
@@ -432,10 +445,36 @@ __global__ __launch_bounds__(64 * NSEG) __attribute__((amdgpu_waves_per_eu(4))) 
     }
   }
   __syncthreads();
-  // ---------------- backward sweep + epilogue
+  T XN = 0;  // EX: the true x on the next segment's first row
+  if constexpr (EX) {
+    // true u on the previous segment's last row, chained over the earlier
+    // segments; then the local backward sweep in place; then the true x on
+    // the next segment's first row, chained down from the last segment
+    T U = 0;
+    if (active && ks < ke)
+      for (int j = 0; j < seg; ++j) {
+        const int bj = (j + 1) * L - 1;
+        U = s_ud[bj * 64 + lane] + fcum[bj] * U;
+      }
+    __syncthreads();
+    if (active && ks < ke) {
+      T x = s_ud[(ke - 1) * 64 + lane] + fcum[ke - 1] * U;
+      s_ud[(ke - 1) * 64 + lane] = x;
+      for (int k = ke - 2; k >= ks; --k) {
+        x = s_ud[k * 64 + lane] + fcum[k] * U - GM(k + 1) * x;
+        s_ud[k * 64 + lane] = x;
+      }
+    }
+    __syncthreads();
+    if (active && ks < ke) {
+      const int nsu = (n + L - 1) / L;
+      for (int j = nsu - 1; j > seg; --j) XN = s_ud[j * L * 64 + lane] + gcum[j * L] * XN;
+    }
+  }
+  // ---------------- backward sweep + epilogue (EX: epilogue pass)
   if (active && ks < ke) {
-    const bool exact = ke - 1 + ZW >= n - 1;
-    const int kstart = exact ? n - 1 : ke - 1 + ZW;
+    const bool exact = EX || ke - 1 + ZW >= n - 1;
+    const int kstart = EX ? ke - 1 : (exact ? n - 1 : ke - 1 + ZW);
     EIn<T> nx[UB];
 #pragma unroll
     for (int j = 0; j < UB; ++j) {
@@ -480,7 +519,9 @@ __global__ __launch_bounds__(64 * NSEG) __attribute__((amdgpu_waves_per_eu(4))) 
       for (int j = 0; j < UB; ++j) {
         const int k = kh - j;
         if (k >= ks) {
-          if (first) {
+          if (EX) {
+            x = cu[j].ud + gcum[k] * XN;  // x~ + the next segment's share
+          } else if (first) {
             x = cu[j].ud;  // exact x_{n-1} = u_{n-1}, or the cold start
             first = false;
           } else {
