@@ -280,9 +280,6 @@ __global__ __launch_bounds__(64 * NSEG) __attribute__((amdgpu_waves_per_eu(4))) 
 #ifndef SPARC_ZS_UB
 #define SPARC_ZS_UB 3
 #endif
-#ifndef SPARC_ZS_RING
-#define SPARC_ZS_RING 0
-#endif
 #ifndef SPARC_ZS_UF64
 #define SPARC_ZS_UF64 8
 #endif
@@ -390,11 +387,12 @@ __global__ __launch_bounds__(64 * NSEG) __attribute__((amdgpu_waves_per_eu(4))) 
     }
     const int kend = min(ke, n - 2);  // interior rows k0..kend-1
     T na[UF], ns[UF];
+    // the window takes B(k + 3) at row k, so no plane past kend + 2 is used
     auto load_chunk = [&](int kb) {
 #pragma unroll
       for (int j = 0; j < UF; ++j) {
         const int kk = kb + j;
-        if (kk < n) {
+        if (kk < n && kk < kend + 3) {
           const long q = c + nxy * kk;
           na[j] = inA[q];
           if (inS) ns[j] = inS[q];
@@ -483,27 +481,6 @@ __global__ __launch_bounds__(64 * NSEG) __attribute__((amdgpu_waves_per_eu(4))) 
     }
     T x = 0;
     bool first = true;
-#if SPARC_ZS_RING
-    // ring of UB planes in flight: slot j is consumed, then refilled with
-    // the plane UB further down
-    for (int kh = kstart; kh >= ks; kh -= UB) {
-#pragma unroll
-      for (int j = 0; j < UB; ++j) {
-        const int k = kh - j;
-        if (k >= ks) {
-          EIn<T> cu = nx[j];
-          if (k - UB >= ks) load_epi<T, EPI>(F, udr, uoff, ustr, c, k - UB, k - UB < ke, nx[j]);
-          if (first) {
-            x = cu.ud;  // exact x_{n-1} = u_{n-1}, or the cold start
-            first = false;
-          } else {
-            x = cu.ud - GM(k + 1) * x;
-          }
-          if (k < ke) epilogue<T, EPI>(F, c, k, x * ST(k), GG(k), cu, out);
-        }
-      }
-    }
-#else
     for (int kh = kstart; kh >= ks; kh -= UB) {
       EIn<T> cu[UB];
 #pragma unroll
@@ -531,7 +508,6 @@ __global__ __launch_bounds__(64 * NSEG) __attribute__((amdgpu_waves_per_eu(4))) 
         }
       }
     }
-#endif
   }
 }
 
@@ -564,12 +540,6 @@ __global__ __launch_bounds__(64 * NSEG) __attribute__((amdgpu_waves_per_eu(4))) 
 #endif
 #ifndef SPARC_ZB_U1
 #define SPARC_ZB_U1 20
-#endif
-#ifndef SPARC_ZB_WIN
-#define SPARC_ZB_WIN 0
-#endif
-#ifndef SPARC_ZB_CHU
-#define SPARC_ZB_CHU 0
 #endif
 #ifndef SPARC_ZB_P4U
 #define SPARC_ZB_P4U 4
@@ -665,29 +635,11 @@ __global__ __launch_bounds__(64 * NSEG) void k_zsolve_b(
       }
     }
     const int kend = min(ke, n - 2);
-#if SPARC_ZB_WIN
-    if (k < kend) {
-      // window B(k-2..k+1) in registers, one LDS read per row
-      T w0 = Bv(k - 2), w1 = Bv(k - 1), w2 = Bv(k), w3 = Bv(k + 1);
-#pragma unroll 4
-      for (; k < kend; ++k) {
-        const T w4 = Bv(k + 2);
-        const T r = tc.ci1 * (w3 - w1) + tc.ci2 * (w4 - w0);
-        u = (r - lowr[k] * u) * invb[k];
-        sX[k * 64 + lane] = u;
-        w0 = w1;
-        w1 = w2;
-        w2 = w3;
-        w3 = w4;
-      }
-    }
-#else
     for (; k < kend; ++k) {
       const T r = tc.ci1 * (Bv(k + 1) - Bv(k - 1)) + tc.ci2 * (Bv(k + 2) - Bv(k - 2));
       u = (r - lowr[k] * u) * invb[k];
       sX[k * 64 + lane] = u;
     }
-#endif
     if (k < ke) {
       // rows n-2 and n-1
       const T rn2 = -(tc.c1 * Bv(n - 1) + tc.c2 * Bv(n - 2) + tc.c3 * Bv(n - 3) +
@@ -716,20 +668,10 @@ __global__ __launch_bounds__(64 * NSEG) void k_zsolve_b(
   // ---------------- P3: true u on the previous segment's last row
   T U = 0;
   if (ks < ke) {
-#if SPARC_ZB_CHU
-#pragma unroll
-    for (int j = 0; j < NSEG - 1; ++j) {
-      if (j < seg) {
-        const int bj = (j + 1) * L - 1;
-        U = sX[bj * 64 + lane] + fcum[bj] * U;
-      }
-    }
-#else
     for (int j = 0; j < seg; ++j) {
       const int bj = (j + 1) * L - 1;
       U = sX[bj * 64 + lane] + fcum[bj] * U;
     }
-#endif
   }
   __syncthreads();
   // ---------------- P4: corrected u, local backward sweep
@@ -748,20 +690,10 @@ __global__ __launch_bounds__(64 * NSEG) void k_zsolve_b(
   if (active && ks < ke) {
     T X = 0;
     const int nsu = (n + L - 1) / L;
-#if SPARC_ZB_CHU
-#pragma unroll
-    for (int j = NSEG - 1; j > 0; --j) {
-      if (j < nsu && j > seg) {
-        const int aj = j * L;
-        X = sX[aj * 64 + lane] + gcum[aj] * X;
-      }
-    }
-#else
     for (int j = nsu - 1; j > seg; --j) {
       const int aj = j * L;
       X = sX[aj * 64 + lane] + gcum[aj] * X;
     }
-#endif
     const T* __restrict__ st = F.stretch;
     // ring of UB planes in flight: slot j is consumed, then refilled with
     // the plane UB further on
