@@ -1,0 +1,214 @@
+"""CPU checks of the computation schedules two HIP kernels are built on,
+against independent methods (the GPU parity tests run the kernels
+themselves):
+
+* the exact segmented tridiagonal solve of the z derivatives (k_zsolve_b
+  and the LDS variant of k_zsolve): local sweeps per segment coupled through
+  products of the LU factors must equal the sequential LU solve;
+* the length-256 x-line transform of k_xw256: radix-4 stages over four
+  registers per lane with register/lane digit swaps by pairwise lane
+  exchanges must equal numpy's FFT, leave the spectrum in the documented
+  digit-reversed lane layout and invert back to natural order.
+"""
+import numpy as np
+import pytest
+
+# ---------------------------------------------------------------------------
+# segmented solve
+# ---------------------------------------------------------------------------
+
+
+def _sequential(r, lw, ib, gm):
+    n = len(r)
+    u = np.zeros(n)
+    p = 0.0
+    for k in range(n):
+        p = (r[k] - lw[k] * p) * ib[k]
+        u[k] = p
+    x = np.zeros(n)
+    x[n - 1] = u[n - 1]
+    for k in range(n - 2, -1, -1):
+        x[k] = u[k] - gm[k + 1] * x[k + 1]
+    return x
+
+
+def _coupling(lw, ib, gm, nseg):
+    """fcum, gcum as the host builds them for nseg segments of L rows."""
+    n = len(lw)
+    L = (n + nseg - 1) // nseg
+    fc, gc = np.zeros(n), np.zeros(n)
+    for a in range(0, n, L):
+        b = min(n, a + L)
+        p = 1.0
+        for k in range(a, b):
+            p *= -lw[k] * ib[k]
+            fc[k] = p
+        p = 1.0
+        for k in range(b - 1, a - 1, -1):
+            if b == n:
+                gc[k] = 0.0
+            else:
+                p *= -gm[k + 1]
+                gc[k] = p
+    return fc, gc, L
+
+
+def _segmented(r, lw, ib, gm, nseg, chain=True):
+    """The kernels' phases: local forward sweeps, the chain of segment ends,
+    corrected local backward sweeps, the chain of segment starts (chain=False
+    leaves the segments uncoupled)."""
+    n = len(r)
+    fc, gc, L = _coupling(lw, ib, gm, nseg)
+    segs = [(a, min(n, a + L)) for a in range(0, n, L)]
+    X = np.zeros(n)
+    for a, b in segs:  # forward from u = 0 before the segment
+        u = 0.0
+        for k in range(a, b):
+            u = (r[k] - lw[k] * u) * ib[k]
+            X[k] = u
+    U = []
+    for s in range(len(segs)):  # true u on the previous segment's last row
+        v = 0.0
+        for j in range(s):
+            bj = (j + 1) * L - 1
+            v = X[bj] + fc[bj] * v
+        U.append(v if chain else 0.0)
+    for s, (a, b) in enumerate(segs):  # local backward sweep, in place
+        x = X[b - 1] + fc[b - 1] * U[s]
+        X[b - 1] = x
+        for k in range(b - 2, a - 1, -1):
+            x = X[k] + fc[k] * U[s] - gm[k + 1] * x
+            X[k] = x
+    out = np.zeros(n)
+    for s, (a, b) in enumerate(segs):  # true x on the next segment's first row
+        xn = 0.0
+        for j in range(len(segs) - 1, s, -1):
+            xn = X[j * L] + gc[j * L] * xn
+        out[a:b] = X[a:b] + gc[a:b] * (xn if chain else 0.0)
+    return out
+
+
+@pytest.mark.parametrize("n", [12, 13, 40, 200, 300])
+@pytest.mark.parametrize("nseg", [1, 4, 8, 16, 32])
+def test_segment_coupling_equals_sequential_lu(n, nseg):
+    rng = np.random.default_rng(n * 100 + nseg)
+    # LU factors with the contraction of the compact scheme's (|lw ib|, |gm| < 1)
+    lw = rng.uniform(0.2, 0.4, n)
+    ib = rng.uniform(0.6, 1.2, n)
+    gm = rng.uniform(0.2, 0.4, n)
+    r = rng.standard_normal(n)
+    want = _sequential(r, lw, ib, gm)
+    got = _segmented(r, lw, ib, gm, nseg)
+    assert np.abs(got - want).max() <= 1e-13 * np.abs(want).max()
+
+
+def test_segment_coupling_catches_a_dropped_chain():
+    """Uncoupled segments are not the solve: the check above can fail."""
+    n, nseg = 300, 16
+    rng = np.random.default_rng(7)
+    lw, ib, gm = rng.uniform(0.2, 0.4, n), rng.uniform(0.6, 1.2, n), rng.uniform(0.2, 0.4, n)
+    r = rng.standard_normal(n)
+    want = _sequential(r, lw, ib, gm)
+    bad = _segmented(r, lw, ib, gm, nseg, chain=False)
+    assert np.abs(bad - want).max() > 1e-3 * np.abs(want).max()
+
+
+# ---------------------------------------------------------------------------
+# length-256 x lines, one wavefront per line
+# ---------------------------------------------------------------------------
+N = 256
+LANE = np.arange(64)
+W256 = np.exp(-2j * np.pi * np.arange(N) / N)
+
+
+def _dft4(v, sign):
+    w = np.exp(sign * 2j * np.pi * np.outer(np.arange(4), np.arange(4)) / 4)
+    return w @ v
+
+
+def _swap(v, b):
+    """Register digit <-> lane digit at bits log2(b), log2(b)+1: two rounds
+    of exchanges with lane ^ 2b and lane ^ b, each lane sending the values
+    whose register bit differs from its lane bit."""
+    v = v.copy()
+    for h in (1, 0):
+        m = b << h
+        hi = (LANE & m) != 0
+        for o in (0, 1):
+            i0 = o if h else 2 * o
+            i1 = i0 + (1 << h)
+            snd = np.where(hi, v[i0], v[i1])
+            rcv = snd[LANE ^ m]
+            v[i0], v[i1] = np.where(hi, rcv, v[i0]), np.where(hi, v[i1], rcv)
+    return v
+
+
+def _twiddle(v, w, conj):
+    v = v.copy()
+    for j in range(1, 4):
+        v[j] = v[j] * (np.conj(w[j - 1]) if conj else w[j - 1])
+    return v
+
+
+_W1 = [W256[(LANE * k) & 255] for k in (1, 2, 3)]
+_W2 = [W256[(4 * (LANE & 15) * k) & 255] for k in (1, 2, 3)]
+_W3 = [W256[(16 * (LANE & 3) * k) & 255] for k in (1, 2, 3)]
+# lane l holds X[kl + 64 j] after the forward stages
+KL = (LANE >> 4) + 4 * ((LANE >> 2) & 3) + 16 * (LANE & 3)
+
+
+def _forward(x):
+    v = np.stack([x[LANE + 64 * j] for j in range(4)])
+    v = _swap(_twiddle(_dft4(v, -1), _W1, False), 16)
+    v = _swap(_twiddle(_dft4(v, -1), _W2, False), 4)
+    v = _swap(_twiddle(_dft4(v, -1), _W3, False), 1)
+    return _dft4(v, -1)
+
+
+def _inverse(v):
+    v = _twiddle(_swap(_dft4(v, 1), 1), _W3, True)
+    v = _twiddle(_swap(_dft4(v, 1), 4), _W2, True)
+    v = _twiddle(_swap(_dft4(v, 1), 16), _W1, True)
+    v = _dft4(v, 1)
+    y = np.zeros(N, complex)
+    for j in range(4):
+        y[LANE + 64 * j] = v[j]
+    return y
+
+
+def test_lane_shuffle_forward_is_the_fft_in_digit_reversed_lanes():
+    rng = np.random.default_rng(3)
+    x = rng.standard_normal(N) + 1j * rng.standard_normal(N)
+    v = _forward(x)
+    X = np.fft.fft(x)
+    for j in range(4):
+        np.testing.assert_allclose(v[j], X[KL + 64 * j], rtol=0, atol=1e-12 * np.abs(X).max())
+    # the digit reversal is a permutation of the 64 lanes
+    assert sorted(KL.tolist()) == list(range(64))
+
+
+def test_lane_shuffle_derivative_round_trip():
+    """Spectral multiplier applied in the digit-reversed layout, inverse back
+    to natural order: the unnormalised inverse of (i m) X."""
+    rng = np.random.default_rng(4)
+    x = rng.standard_normal(N) + 1j * rng.standard_normal(N)
+    m = rng.standard_normal(N)
+    v = _forward(x)
+    v = np.stack([1j * m[KL + 64 * j] * v[j] for j in range(4)])
+    got = _inverse(v)
+    want = np.fft.ifft(1j * m * np.fft.fft(x)) * N
+    np.testing.assert_allclose(got, want, rtol=0, atol=1e-11 * np.abs(want).max())
+
+
+def test_lane_shuffle_damping_layout_round_trip():
+    """The damping forward pass stores the spectrum in natural order and the
+    inverse pass loads it back into the digit-reversed lanes."""
+    rng = np.random.default_rng(5)
+    x = rng.standard_normal(N) + 1j * rng.standard_normal(N)
+    v = _forward(x)
+    spec = np.zeros(N, complex)
+    for j in range(4):
+        spec[KL + 64 * j] = v[j]
+    np.testing.assert_allclose(spec, np.fft.fft(x), rtol=0, atol=1e-12 * np.abs(spec).max())
+    back = np.stack([spec[KL + 64 * j] for j in range(4)])
+    np.testing.assert_allclose(_inverse(back) / N, x, rtol=0, atol=1e-13 * np.abs(x).max())
