@@ -207,16 +207,26 @@ __global__ __launch_bounds__(256) void k_y256(int nx, int nz, XArgs<T> X, T beta
 #pragma unroll
     for (int n = 0; n < 16; ++n) v[n] = spec[(k * NY + 16 * n + w) * nx + i0 + c];
   } else {
+    // all 16 (or 32) loads are issued before the stage temp a + beta s is
+    // formed; combining them inside the loop made every row wait for its
+    // own loads
     const long x0 = (long)blockIdx.x * 32 + 2 * c;
+    if (s) {
+      C2<T> t[16];
 #pragma unroll
-    for (int n = 0; n < 16; ++n) {
-      const long o = x0 + (long)nx * (16 * n + w) + nxy * k;
-      C2<T> z = *reinterpret_cast<const C2<T>*>(a + o);
-      if (s) {
-        const C2<T> t = *reinterpret_cast<const C2<T>*>(s + o);
-        z = {z.x + beta * t.x, z.y + beta * t.y};
+      for (int n = 0; n < 16; ++n) {
+        const long o = x0 + (long)nx * (16 * n + w) + nxy * k;
+        v[n] = *reinterpret_cast<const C2<T>*>(a + o);
+        t[n] = *reinterpret_cast<const C2<T>*>(s + o);
       }
-      v[n] = z;
+#pragma unroll
+      for (int n = 0; n < 16; ++n) v[n] = {v[n].x + beta * t[n].x, v[n].y + beta * t[n].y};
+    } else {
+#pragma unroll
+      for (int n = 0; n < 16; ++n) {
+        const long o = x0 + (long)nx * (16 * n + w) + nxy * k;
+        v[n] = *reinterpret_cast<const C2<T>*>(a + o);
+      }
     }
   }
   __syncthreads();  // tables
@@ -342,8 +352,9 @@ __global__ __launch_bounds__(256) void k_xw256(int nrows, int nlines, XArgs<T> X
       st = N;
       h1 = 2 * L + 1 < nrows;
     } else {
-      const long q = L / ny, j = L - q * ny;
-      o = N * j + nxy * (2 * q);
+      // lines number (nz + 1) / 2 * ny < 2^31: 32-bit division
+      const int q = (int)((unsigned)L / (unsigned)ny), j = (int)L - q * ny;
+      o = (long)N * j + nxy * (2 * q);
       st = nxy;
       h1 = 2 * q + 1 < nz;
     }
@@ -371,7 +382,7 @@ __global__ __launch_bounds__(256) void k_xw256(int nrows, int nlines, XArgs<T> X
   // damping inverse: lane l takes X[kl + 64 j], the layout the inverse
   // stages start from, and the RHS values it updates
   auto edge_w = [&](long L, bool& w0, bool& w1) {
-    const long k0 = 2 * (L / ny), k1 = k0 + 1;
+    const int k0 = 2 * (int)((unsigned)L / (unsigned)ny), k1 = k0 + 1;
     w0 = X.edges || (k0 != 0 && k0 != nz - 1);
     w1 = k1 < nz - 1 || (X.edges && k1 < nz);
   };
@@ -470,7 +481,8 @@ __global__ __launch_bounds__(256) void k_xw256(int nrows, int nlines, XArgs<T> X
     T* __restrict__ o0 = out + r0 * N;
     if (X.rhs) {
       // -rhoinv * d, zero on planes 1 and nz (rhs_of)
-      const int k0 = (int)(r0 / ny), k1 = (int)((r0 + 1) / ny);
+      // rows number ny * nz < 2^31: 32-bit division
+      const int k0 = (int)((unsigned)r0 / (unsigned)ny), k1 = (int)((unsigned)(r0 + 1) / (unsigned)ny);
       const bool e0 = k0 == 0 || k0 == nz - 1, e1 = k1 == 0 || k1 == nz - 1;
       const T* __restrict__ q3 = X.r3 ? X.r3 + r0 * N : nullptr;
       const T z0 = X.r3 ? (T)0 : X.rz[k0], z1 = X.r3 ? (T)0 : X.rz[k1];

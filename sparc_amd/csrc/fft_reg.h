@@ -263,41 +263,38 @@ __global__ __launch_bounds__(NT) void k_xlines_n(int nrows, int nlines, XArgs<T>
   for (int m = threadIdx.x; m < N; m += NT) tw[m] = twg[m];
   const int L0 = blockIdx.x * LPB;
   const int nl = min(LPB, nlines - L0);
-  // damping inverse: the RHS values it updates, loaded with the spectrum
+  // damping inverse: the RHS values it updates, loaded with the spectrum.
+  // Every load is issued before the first use (a + beta s, the LDS store).
   constexpr int IT = (LPB * N + NT - 1) / NT;
+  constexpr bool RL = MODE == XM_DERIV || MODE == XM_FILTER || MODE == XM_DAMP_FWD;
   T ov0[MODE == XM_DAMP_INV ? IT : 1], ov1[MODE == XM_DAMP_INV ? IT : 1];
+  T ra[RL ? IT : 1], rs[RL ? IT : 1], ia[RL ? IT : 1], is[RL ? IT : 1];
+  C2<T> sv[RL ? 1 : IT];
 #pragma unroll
   for (int it = 0; it < IT; ++it) {
     const int e = threadIdx.x + it * NT;
-    if (e >= LPB * N) break;
     const int l = e / N, x = e - l * N;
-    C2<T> v = {0, 0};
-    if (l < nl) {
+    if (e < LPB * N && l < nl) {
       const long L = L0 + l;
-      if (MODE == XM_DERIV || MODE == XM_FILTER) {
-        const long r0 = 2 * L, r1 = r0 + 1;
-        T re = a[r0 * N + x];
-        if (s) re = re + beta * s[r0 * N + x];
-        T im = 0;
-        if (r1 < nrows) {
-          im = a[r1 * N + x];
-          if (s) im = im + beta * s[r1 * N + x];
+      if (RL) {
+        long o0, st;
+        bool h1;
+        if (MODE == XM_DAMP_FWD) {
+          const long q = L / ny, j = L - q * ny;
+          o0 = x + (long)N * j + nxy * (2 * q);
+          st = nxy;
+          h1 = 2 * q + 1 < nz;
+        } else {
+          o0 = 2 * L * N + x;
+          st = N;
+          h1 = 2 * L + 1 < nrows;
         }
-        v = {re, im};
-      } else if (MODE == XM_DAMP_FWD) {
-        const long q = L / ny, j = L - q * ny;
-        const long k0 = 2 * q, k1 = k0 + 1;
-        const long o0 = x + (long)N * j + nxy * k0;
-        T re = a[o0];
-        if (s) re = re + beta * s[o0];
-        T im = 0;
-        if (k1 < nz) {
-          im = a[o0 + nxy];
-          if (s) im = im + beta * s[o0 + nxy];
-        }
-        v = {re, im};
+        ra[it] = a[o0];
+        if (s) rs[it] = s[o0];
+        ia[it] = h1 ? a[o0 + st] : (T)0;
+        if (s) is[it] = h1 ? s[o0 + st] : (T)0;
       } else {
-        v = spec[L * N + x];
+        sv[it] = spec[L * N + x];
         if constexpr (MODE == XM_DAMP_INV) {
           const long q = L / ny, j = L - q * ny;
           const long k0 = 2 * q, k1 = k0 + 1;
@@ -307,7 +304,21 @@ __global__ __launch_bounds__(NT) void k_xlines_n(int nrows, int nlines, XArgs<T>
         }
       }
     }
-    b0[l * LS + lpad(x)] = v;
+  }
+#pragma unroll
+  for (int it = 0; it < IT; ++it) {
+    const int e = threadIdx.x + it * NT;
+    const int l = e / N, x = e - l * N;
+    if (e < LPB * N) {
+      C2<T> v = {0, 0};
+      if (l < nl) {
+        if (RL)
+          v = s ? C2<T>{ra[it] + beta * rs[it], ia[it] + beta * is[it]} : C2<T>{ra[it], ia[it]};
+        else
+          v = sv[it];
+      }
+      b0[l * LS + lpad(x)] = v;
+    }
   }
   __syncthreads();
   if (MODE == XM_DAMP_INV)
@@ -413,13 +424,27 @@ __global__ __launch_bounds__(NT) void k_ylines_n(int nx, int nz, XArgs<T> X, T b
   const long k = blockIdx.y;
   if (MODE == YM_DERIV || MODE == YM_FILTER) {
     T* b0r = reinterpret_cast<T*>(b0);
+    // every load issued before the first use (a + beta s, the LDS store)
+    constexpr int IT = (TX * NY + NT - 1) / NT;
+    T va[IT], vs[IT];
 #pragma unroll
-    for (int e = threadIdx.x; e < TX * NY; e += NT) {
-      const int y = e / TX, xx = e - y * TX;
-      const long o = (i0 + xx) + (long)nx * y + nxy * k;
-      T v = a[o];
-      if (s) v = v + beta * s[o];
-      b0r[2 * ((xx >> 1) * LS + lpad(y)) + (xx & 1)] = v;
+    for (int it = 0; it < IT; ++it) {
+      const int e = threadIdx.x + it * NT;
+      if (e < TX * NY) {
+        const int y = e / TX, xx = e - y * TX;
+        const long o = (i0 + xx) + (long)nx * y + nxy * k;
+        va[it] = a[o];
+        if (s) vs[it] = s[o];
+      }
+    }
+#pragma unroll
+    for (int it = 0; it < IT; ++it) {
+      const int e = threadIdx.x + it * NT;
+      if (e < TX * NY) {
+        const int y = e / TX, xx = e - y * TX;
+        const T v = s ? va[it] + beta * vs[it] : va[it];
+        b0r[2 * ((xx >> 1) * LS + lpad(y)) + (xx & 1)] = v;
+      }
     }
     __syncthreads();
     rfft<T, NY, -1, NL, NT, LS>(b0, tw);
@@ -447,10 +472,19 @@ __global__ __launch_bounds__(NT) void k_ylines_n(int nx, int nz, XArgs<T> X, T b
     }
   } else {
     const long q = k;
-#pragma unroll 4
-    for (int e = threadIdx.x; e < NL * NY; e += NT) {
+    constexpr int IT = (NL * NY + NT - 1) / NT;
+    C2<T> sv[IT];
+#pragma unroll
+    for (int it = 0; it < IT; ++it) {
+      const int e = threadIdx.x + it * NT;
       const int y = e / NL, mm = e - y * NL;
-      b0[mm * LS + lpad(y)] = spec[(q * NY + y) * nx + i0 + mm];
+      if (e < NL * NY) sv[it] = spec[(q * NY + y) * nx + i0 + mm];
+    }
+#pragma unroll
+    for (int it = 0; it < IT; ++it) {
+      const int e = threadIdx.x + it * NT;
+      const int y = e / NL, mm = e - y * NL;
+      if (e < NL * NY) b0[mm * LS + lpad(y)] = sv[it];
     }
     __syncthreads();
     rfft<T, NY, -1, NL, NT, LS>(b0, tw);

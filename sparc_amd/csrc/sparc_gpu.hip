@@ -1076,9 +1076,13 @@ static int launch_z(sparc_ctx* c, const Fld<T>& F, int ibc, const T* inA, const 
     LAUNCHCHK(c);
     return SPARC_OK;
   }
-  // ud in LDS when two blocks of 64 columns fit in one CU's LDS
+  // ud in LDS when two blocks of 64 columns fit in one CU's LDS; with the
+  // exact coupling any segment of at least 3 planes works, the warm-up
+  // needs long columns (nz >= 192)
   const size_t lds_ud = 64 * (size_t)nz * sizeof(T);
-  const bool ulds = SPARC_ZS_ULDS && nseg == 4 && lds_ud <= 81920 && lds_ud <= (size_t)c->lds_max;
+  const bool seg_ok = SPARC_ZS_EXACT ? (nz + SPARC_ZS_ULDS_NSEG - 1) / SPARC_ZS_ULDS_NSEG >= 3
+                                     : nseg == 4;
+  const bool ulds = SPARC_ZS_ULDS && seg_ok && lds_ud <= 81920 && lds_ud <= (size_t)c->lds_max;
   const size_t lds = ulds ? lds_ud : 5 * (size_t)nz * sizeof(T);
 #define ZL(NS, U)                                                                                 \
   hipLaunchKernelGGL((k_zsolve<T, EPI, NS, U>), grid, dim3(64 * NS), lds, c->stream, F, tcv, tri, \

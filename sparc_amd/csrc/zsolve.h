@@ -290,9 +290,10 @@ __global__ __launch_bounds__(64 * NSEG) __attribute__((amdgpu_waves_per_eu(4))) 
 #define SPARC_ZS_UB64 1
 #endif
   // forward chunk (planes loaded one chunk ahead) and backward chunk; FP64
-  // values take two registers each, so its chunks are shorter
+  // values take two registers each, so its chunks are shorter, and the
+  // global-ud variants keep 2 planes (3 spill there)
   constexpr int UF = sizeof(T) == 4 ? SPARC_ZS_UF : SPARC_ZS_UF64;
-  constexpr int UB = sizeof(T) == 4 ? SPARC_ZS_UB : SPARC_ZS_UB64;
+  constexpr int UB = sizeof(T) == 4 ? (ULDS ? SPARC_ZS_UB : 2) : SPARC_ZS_UB64;
   constexpr int ZW = ZWarm<T>::W;
   // EX: segments coupled exactly (the products fcum, gcum of this partition)
   // instead of by warm-up; needs ud in LDS and at least 3 planes per segment
