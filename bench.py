@@ -31,7 +31,7 @@ HBM_PEAK_GBS = 8000.0
 KERNEL_SYMBOLS = {
     "xlines": "sparc::k_xw256<float, 0 (XM_DERIV), 2> -- x derivative, one wavefront per row pair",
     "ylines": "sparc::k_y256<float, 0 (YM_DERIV)> -- y derivative, register four-step",
-    "damp": "sparc::k_xw256<float, 2 (XM_DAMP_FWD), 2> -- damping, forward x",
+    "damp": "sparc::k_xw256<float, 2 (XM_DAMP_FWD), 4> -- damping, forward x",
     "damp_y": "sparc::k_y256<float, 2 (YM_DAMP)> -- damping, y pass",
     "damp_xi": "sparc::k_xw256<float, 3 (XM_DAMP_INV), 2> -- damping, inverse x",
     "zsolve": "sparc::k_zsolve<float, 2 (EPI_RHS3D), 8, true> -- z solve (ud in LDS) + momentum/continuity RHS",

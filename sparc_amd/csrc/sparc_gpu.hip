@@ -844,6 +844,9 @@ static bool fast_size(int n) {
 // shuffles measured the same (57.8): the kernel is not bound by them.  The
 // damping forward x pass also runs on it (SPARC_XW_DF): 103.3 -> 94.9 us,
 // and the damping inverse (SPARC_XW_DI, in place of k_x256): 125.4 -> 121.7.
+// With the loads ordered, 1 / 2 / 4 lines per wavefront measured 51.5 /
+// 50.0 / 55.7 us (derivative), 110.9 / 98.6 / 94.9 (damping forward) and
+// 137.6 / 122.8 / 126.8 (damping inverse): the forward damping pass takes 4.
 #ifndef SPARC_XW
 #define SPARC_XW 1
 #endif
@@ -863,9 +866,10 @@ static void launch_xn(sparc_ctx* c, int nf, const XArgs<T>& X, T beta, const T* 
   if constexpr (N == 256 && SPARC_XW &&
                 (MODE == XM_DERIV || (SPARC_XW_DF && MODE == XM_DAMP_FWD) ||
                  (SPARC_XW_DI && MODE == XM_DAMP_INV))) {
-    hipLaunchKernelGGL((k_xw256<T, MODE, SPARC_XW_LPW>), dim3(cdiv(nlines, 4 * SPARC_XW_LPW), nf),
-                       dim3(256), 0, c->stream, nrows, nlines, X, beta, mult,
-                       (const C2<T>*)c->twx, c->cfg.ny, c->cfg.nz, c->nxy);
+    constexpr int LPW = MODE == XM_DAMP_FWD ? 2 * SPARC_XW_LPW : SPARC_XW_LPW;
+    hipLaunchKernelGGL((k_xw256<T, MODE, LPW>), dim3(cdiv(nlines, 4 * LPW), nf), dim3(256), 0,
+                       c->stream, nrows, nlines, X, beta, mult, (const C2<T>*)c->twx,
+                       c->cfg.ny, c->cfg.nz, c->nxy);
     return;
   }
   if constexpr (N == 256 && SPARC_R16 && MODE == XM_DAMP_INV) {
