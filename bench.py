@@ -29,11 +29,11 @@ sys.path.insert(0, ROOT)
 HBM_PEAK_GBS = 8000.0
 # the kernel behind each timed class in the forward step of the 3-D box
 KERNEL_SYMBOLS = {
-    "xlines": "sparc::k_xw256<float, 0 (XM_DERIV), 2> -- x derivative, one wavefront per row pair",
+    "xlines": "sparc::k_xw256<float, 0 (XM_DERIV) | 4 (XM_DERIV_DAMPINV), 2> -- x derivative, one wavefront per line",
     "ylines": "sparc::k_y256<float, 0 (YM_DERIV)> -- y derivative, register four-step",
     "damp": "sparc::k_xw256<float, 2 (XM_DAMP_FWD), 4> -- damping, forward x",
     "damp_y": "sparc::k_y256<float, 2 (YM_DAMP)> -- damping, y pass",
-    "damp_xi": "sparc::k_xw256<float, 3 (XM_DAMP_INV), 2> -- damping, inverse x",
+    "damp_xi": "sparc::k_xw256<float, 3 (XM_DAMP_INV), 2> -- damping, inverse x (v_y, v_z; v_x in XM_DERIV_DAMPINV)",
     "zsolve": "sparc::k_zsolve<float, 2 (EPI_RHS3D), 8, true> -- z solve (ud in LDS) + momentum/continuity RHS",
     "zsolve_p": "sparc::k_zsolve_b<float, 1 (EPI_P3D), 16, true> -- exact segmented z solve (column in LDS) + pressure/density",
     "update": "sparc::k_update_filter_c -- a += dt*scr with the z filter (hi+lo state), all fields per launch",

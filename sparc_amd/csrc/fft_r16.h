@@ -326,19 +326,58 @@ __device__ __forceinline__ void xw_twiddle(C2<T> (&v)[4], const C2<T> (&w)[3], b
   }
 }
 
+// the four forward stages (natural order in, digit-reversed lanes out) and
+// the inverse stages (back to natural order), unnormalised
+template <typename T>
+__device__ __forceinline__ void xw_forward(C2<T> (&v)[4], const C2<T> (&w1)[3],
+                                           const C2<T> (&w2)[3], const C2<T> (&w3)[3], int lane) {
+  dft_reg<T, 4, -1>(v);
+  xw_twiddle(v, w1, false);
+  xw_swap<T, 16>(v, lane);
+  dft_reg<T, 4, -1>(v);
+  xw_twiddle(v, w2, false);
+  xw_swap<T, 4>(v, lane);
+  dft_reg<T, 4, -1>(v);
+  xw_twiddle(v, w3, false);
+  xw_swap<T, 1>(v, lane);
+  dft_reg<T, 4, -1>(v);
+}
+
+template <typename T>
+__device__ __forceinline__ void xw_inverse(C2<T> (&v)[4], const C2<T> (&w1)[3],
+                                           const C2<T> (&w2)[3], const C2<T> (&w3)[3], int lane) {
+  dft_reg<T, 4, +1>(v);
+  xw_swap<T, 1>(v, lane);
+  xw_twiddle(v, w3, true);
+  dft_reg<T, 4, +1>(v);
+  xw_swap<T, 4>(v, lane);
+  xw_twiddle(v, w2, true);
+  dft_reg<T, 4, +1>(v);
+  xw_swap<T, 16>(v, lane);
+  xw_twiddle(v, w1, true);
+  dft_reg<T, 4, +1>(v);
+}
+
 // Same contracts as k_xlines_n<T, 256, MODE> for MODE = XM_DERIV (line L =
 // rows 2L, 2L+1, x fastest; out = i k mult(k) applied, or the momentum RHS
 // when X.rhs), XM_DAMP_FWD (line L = row j of planes 2q, 2q+1, L = j +
 // ny q; spec[L N + k] = the forward transform) and XM_DAMP_INV (the same
 // rows of out minus the inverse transform of spec, planes 1 and nz skipped
-// unless X.edges).  LPW lines per wave, the next line's loads issued before
-// the current line's transform.
+// unless X.edges).  XM_DERIV_DAMPINV takes the damping lines (row j of
+// planes 2q, 2q+1) of the input X.a (no stage temp) and writes
+// -rhoinv d/dx minus the inverse transform of spec, which is what the
+// momentum-RHS derivative followed by the damping inverse leave there.
+// LPW lines per wave, the next line's loads issued before the current
+// line's transform.
 template <typename T, int MODE, int LPW>
 __global__ __launch_bounds__(256) void k_xw256(int nrows, int nlines, XArgs<T> X, T beta,
                                                const T* __restrict__ mult,
                                                const C2<T>* __restrict__ twg, int ny, int nz,
                                                long nxy) {
-  static_assert(MODE == XM_DERIV || MODE == XM_DAMP_FWD || MODE == XM_DAMP_INV, "k_xw256 modes");
+  static_assert(MODE == XM_DERIV || MODE == XM_DAMP_FWD || MODE == XM_DAMP_INV ||
+                    MODE == XM_DERIV_DAMPINV,
+                "k_xw256 modes");
+  constexpr bool DD = MODE == XM_DERIV_DAMPINV;
   constexpr int N = 256;
   const int fld = blockIdx.y;
   const T* __restrict__ a = X.a[fld];
@@ -373,7 +412,7 @@ __global__ __launch_bounds__(256) void k_xw256(int nrows, int nlines, XArgs<T> X
   }
   const int kl = (lane >> 4) + 4 * ((lane >> 2) & 3) + 16 * (lane & 3);
   T ml[4];
-  if (MODE == XM_DERIV) {
+  if (MODE == XM_DERIV || DD) {
 #pragma unroll
     for (int j = 0; j < 4; ++j) ml[j] = mult[kl + 64 * j];
   }
@@ -402,6 +441,7 @@ __global__ __launch_bounds__(256) void k_xw256(int nrows, int nlines, XArgs<T> X
   // raw a and s values of the next line: the stage temp a + beta s is formed
   // when the line is used, so the loads stay in flight meanwhile
   C2<T> ns[4];
+  C2<T> nsp[DD ? 4 : 1];  // DD: the damping spectrum, digit-reversed lanes
 #define XW_LOAD(LL)                                                  \
   if (MODE == XM_DAMP_INV) XW_LOAD_INV(LL) else {                    \
     long o_, st_;                                                    \
@@ -415,17 +455,19 @@ __global__ __launch_bounds__(256) void k_xw256(int nrows, int nlines, XArgs<T> X
       if (s0_) ns[j].x = s0_[x];                                     \
       nv[j].y = h1_ ? a0_[st_ + x] : (T)0;                           \
       if (s0_) ns[j].y = h1_ ? s0_[st_ + x] : (T)0;                  \
+      if constexpr (DD) nsp[j] = spec[(LL) * N + kl + 64 * j];        \
     }                                                                \
   }
   if (L0 < nlines) XW_LOAD(L0);
   for (int p = 0; p < LPW; ++p) {
     const long L = L0 + p;
     if (L >= nlines) break;
-    C2<T> v[4];
+    C2<T> v[4], sp[DD ? 4 : 1];
     T ov0[MODE == XM_DAMP_INV ? 4 : 1], ov1[MODE == XM_DAMP_INV ? 4 : 1];
 #pragma unroll
     for (int j = 0; j < 4; ++j) {
       v[j] = nv[j];
+      if constexpr (DD) sp[j] = nsp[j];
       if (MODE != XM_DAMP_INV && s) v[j] = C2<T>{v[j].x + beta * ns[j].x, v[j].y + beta * ns[j].y};
       if (MODE == XM_DAMP_INV) {
         ov0[j] = no0[j];
@@ -434,16 +476,7 @@ __global__ __launch_bounds__(256) void k_xw256(int nrows, int nlines, XArgs<T> X
     }
     if (p + 1 < LPW && L + 1 < nlines) XW_LOAD(L + 1);
     if (MODE != XM_DAMP_INV) {  // forward transform
-      dft_reg<T, 4, -1>(v);
-      xw_twiddle(v, w1, false);
-      xw_swap<T, 16>(v, lane);
-      dft_reg<T, 4, -1>(v);
-      xw_twiddle(v, w2, false);
-      xw_swap<T, 4>(v, lane);
-      dft_reg<T, 4, -1>(v);
-      xw_twiddle(v, w3, false);
-      xw_swap<T, 1>(v, lane);
-      dft_reg<T, 4, -1>(v);
+      xw_forward(v, w1, w2, w3, lane);
       if (MODE == XM_DAMP_FWD) {
         // natural order: each store fills one 512-byte block, lanes permuted
 #pragma unroll
@@ -453,16 +486,32 @@ __global__ __launch_bounds__(256) void k_xw256(int nrows, int nlines, XArgs<T> X
 #pragma unroll
       for (int j = 0; j < 4; ++j) v[j] = C2<T>{-ml[j] * v[j].y, ml[j] * v[j].x};
     }
-    dft_reg<T, 4, +1>(v);
-    xw_swap<T, 1>(v, lane);
-    xw_twiddle(v, w3, true);
-    dft_reg<T, 4, +1>(v);
-    xw_swap<T, 4>(v, lane);
-    xw_twiddle(v, w2, true);
-    dft_reg<T, 4, +1>(v);
-    xw_swap<T, 16>(v, lane);
-    xw_twiddle(v, w1, true);
-    dft_reg<T, 4, +1>(v);
+    xw_inverse(v, w1, w2, w3, lane);
+    if constexpr (DD) {
+      xw_inverse(sp, w1, w2, w3, lane);
+      long o, st;
+      bool h1, w0, w1e;
+      rows(L, o, st, h1);
+      edge_w(L, w0, w1e);
+      const int k0 = 2 * (int)((unsigned)L / (unsigned)ny), k1 = k0 + 1;
+      const bool e0 = k0 == 0 || k0 == nz - 1, e1 = k1 == nz - 1;
+      const T z0 = X.r3 ? (T)0 : X.rz[k0], z1 = (X.r3 || !h1) ? (T)0 : X.rz[k1];
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        const int x = lane + 64 * j;
+        const T ri0 = X.r3 ? X.r3[o + x] : z0;
+        T r0 = e0 ? (T)0 : -ri0 * v[j].x;
+        if (w0) r0 = r0 - sp[j].x;
+        out[o + x] = r0;
+        if (h1) {
+          const T ri1 = X.r3 ? X.r3[o + st + x] : z1;
+          T r1 = e1 ? (T)0 : -ri1 * v[j].y;
+          if (w1e) r1 = r1 - sp[j].y;
+          out[o + st + x] = r1;
+        }
+      }
+      continue;
+    }
     if (MODE == XM_DAMP_INV) {
       long o, st;
       bool h1, w0, w1;

@@ -13,7 +13,9 @@
 namespace sparc {
 
 enum { EPI_STORE = 0, EPI_P3D = 1, EPI_RHS3D = 2, EPI_P2D = 3, EPI_RHS2D = 4, EPI_C2 = 5, EPI_V2 = 6 };
-enum { XM_DERIV = 0, XM_FILTER = 1, XM_DAMP_FWD = 2, XM_DAMP_INV = 3 };
+// XM_DERIV_DAMPINV: the momentum RHS from the x derivative with the damping
+// inverse of the same component subtracted, in one pass (k_xw256 only)
+enum { XM_DERIV = 0, XM_FILTER = 1, XM_DAMP_FWD = 2, XM_DAMP_INV = 3, XM_DERIV_DAMPINV = 4 };
 enum { YM_DERIV = 0, YM_FILTER = 1, YM_DAMP = 2 };
 
 template <typename T>

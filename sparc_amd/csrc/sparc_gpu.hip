@@ -939,6 +939,25 @@ static int launch_xm(sparc_ctx* c, int nf, const XArgs<T>& X, T beta, const T* m
   return SPARC_OK;
 }
 
+// x derivative of p as the momentum RHS with the damping inverse of the
+// same component subtracted (k_xw256 XM_DERIV_DAMPINV): lines of 256 only
+#ifndef SPARC_XW_FUSE
+#define SPARC_XW_FUSE 1
+#endif
+template <typename T>
+static int launch_x_rhs_damp(sparc_ctx* c, const XArgs<T>& X, const T* mult) {
+  const long nlines = (long)((c->cfg.nz + 1) / 2) * c->cfg.ny;
+  const long nrows = (long)c->cfg.ny * c->cfg.nz;
+  // p, the damping spectrum, the RHS written
+  ProfScope ps(c, SPARC_KC_XLINES, 3.0 * (double)c->N * sizeof(T));
+  hipLaunchKernelGGL((k_xw256<T, XM_DERIV_DAMPINV, SPARC_XW_LPW>),
+                     dim3(cdiv(nlines, 4 * SPARC_XW_LPW), 1), dim3(256), 0, c->stream, (int)nrows,
+                     (int)nlines, X, (T)0, mult, (const C2<T>*)c->twx, c->cfg.ny, c->cfg.nz,
+                     c->nxy);
+  LAUNCHCHK(c);
+  return SPARC_OK;
+}
+
 template <typename T, int MODE>
 static int launch_x(sparc_ctx* c, const T* a, const T* s, T beta, T* out, const T* mult,
                     C2<T>* spec) {
@@ -1164,18 +1183,23 @@ static int run_stage(sparc_ctx* c, Fld<T>& F, int s) {
     // (RHS_xi = temp v, RHS_vz, RHS_rho terms; reads the previous RHS_v), then
     // the x and y parts written straight into RHS_vx, RHS_vy as -rhoinv*dp
     if ((rc = launch_z<T, EPI_RHS3D>(c, F, 0, F.W[3], nullptr, 0, nullptr, 0, nullptr))) return rc;
-    {
-      XArgs<T> G{};
-      G.a[0] = F.W[3];
-      G.rhs = 1;
-      G.rz = F.bg3d[SPARC_BG_RHOINV] ? nullptr : F.bg[SPARC_BG_RHOINV];
-      G.r3 = F.bg3d[SPARC_BG_RHOINV] ? F.bg[SPARC_BG_RHOINV] : nullptr;
+    const bool damp = c->cfg.damp_waves && c->damping_set;
+    // with damping on 256-wide x lines, RHS_vx is written once, by the pass
+    // that also subtracts its damping (after the damping forward and y passes,
+    // which read RHS_xi only)
+    const bool fuse = damp && SPARC_XW && SPARC_XW_FUSE && c->cfg.nx == 256;
+    XArgs<T> G{};
+    G.a[0] = F.W[3];
+    G.rhs = 1;
+    G.rz = F.bg3d[SPARC_BG_RHOINV] ? nullptr : F.bg[SPARC_BG_RHOINV];
+    G.r3 = F.bg3d[SPARC_BG_RHOINV] ? F.bg[SPARC_BG_RHOINV] : nullptr;
+    if (!fuse) {
       G.out[0] = F.scr[SPARC_V_X];
       if ((rc = launch_xm<T, XM_DERIV>(c, 1, G, 0, dmx))) return rc;
-      G.out[0] = F.scr[SPARC_V_Y];
-      if ((rc = launch_ym<T, YM_DERIV>(c, 1, G, 0, dmy))) return rc;
     }
-    if (c->cfg.damp_waves && c->damping_set) {
+    G.out[0] = F.scr[SPARC_V_Y];
+    if ((rc = launch_ym<T, YM_DERIV>(c, 1, G, 0, dmy))) return rc;
+    if (damp) {
       // DAMP_VELOCITY (damping.f90:125-185) on the stage's temp v, which the
       // epilogue above has just stored as RHS_xi = v (zero on planes 1 and nz,
       // whose damping output is discarded)
@@ -1187,7 +1211,23 @@ static int run_stage(sparc_ctx* c, Fld<T>& F, int s) {
       }
       if ((rc = launch_xm<T, XM_DAMP_FWD>(c, 3, DX, 0, nullptr))) return rc;
       if ((rc = launch_ym<T, YM_DAMP>(c, 3, DX, 0, nullptr))) return rc;
-      if ((rc = launch_xm<T, XM_DAMP_INV>(c, 3, DX, 0, nullptr))) return rc;
+      if (fuse) {
+        XArgs<T> H = G;
+        H.out[0] = F.scr[SPARC_V_X];
+        H.spec[0] = DX.spec[0];
+        H.edges = DX.edges;
+        if ((rc = launch_x_rhs_damp<T>(c, H, dmx))) return rc;
+        XArgs<T> D2{};
+        for (int f = 0; f < 2; ++f) {
+          D2.a[f] = DX.a[f + 1];
+          D2.out[f] = DX.out[f + 1];
+          D2.spec[f] = DX.spec[f + 1];
+        }
+        D2.edges = DX.edges;
+        if ((rc = launch_xm<T, XM_DAMP_INV>(c, 2, D2, 0, nullptr))) return rc;
+      } else if ((rc = launch_xm<T, XM_DAMP_INV>(c, 3, DX, 0, nullptr))) {
+        return rc;
+      }
     }
   } else if (c->cfg.option == SPARC_OPT_MHD_PML_3D_DISPL) {
     // MP_MHD_PML_3D_DISPL (displacement.f90:663-849): 18 derivative passes
