@@ -180,6 +180,32 @@ def test_step_3d_wide_matches_oracle(wide, prec):
 
 
 @pytest.mark.parametrize("prec", [abi.FP64, abi.FP32])
+def test_step_3d_wide_3d_density_matches_oracle(wide, prec):
+    """Option 4 on 256 x 256 x 40 with a 3-D density model (a 2 % bump): the
+    density backgrounds, rhoinv among them, are 3-D arrays, so the 256-wide
+    x kernels read rhoinv per point in the momentum-RHS derivative and in
+    the pass that writes RHS_vx with its damping."""
+    W, S0 = wide
+    nx, ny, nz = S0.rho0.shape
+    x = np.linspace(-1, 1, nx)[:, None, None]
+    y = np.linspace(-1, 1, ny)[None, :, None]
+    z = np.linspace(-1, 1, nz)[None, None, :]
+    rho3 = S0.rho0 * (1.0 + 0.02 * np.exp(-(x ** 2 + y ** 2 + z ** 2) / 0.1))
+    S = D.setup_for(W, rho0_3d=rho3)
+    assert S.rho0.ndim == 3 and np.ptp(S.rho0[:, :, nz // 2]) > 0
+    eng, o = _run_pair(W, S, 3, prec)
+    try:
+        for arr, fields in ((abi.ARR_A, range(6)), (abi.ARR_SCR, range(6))):
+            for f in fields:
+                a, b = eng.get(arr, f), o.get(arr, f)
+                assert np.linalg.norm(b) > 0, (arr, f)
+                assert (rel_err(a, b) < TOL64) if prec == abi.FP64 else (rel_l2(a, b) < TOL32_L2)
+    finally:
+        eng.close()
+        o.close()
+
+
+@pytest.mark.parametrize("prec", [abi.FP64, abi.FP32])
 def test_step_2d_option17_matches_oracle(two_d, prec):
     W, S = two_d
     eng, o = _run_pair(W, S, 90, prec)  # (time+1) % 86 == 0 fires the 2-D spectral filter
