@@ -266,7 +266,12 @@ __global__ __launch_bounds__(256) void k_y256(int nx, int nz, XArgs<T> X, T beta
     for (int n = 0; n < 16; ++n) {
       const long o = x0 + (long)nx * (w + 16 * n) + nxy * k;
       C2<T> z = v[n];
-      if (MODE == YM_DERIV && X.rhs) z = {rhs_of(X, k, o, nz, z.x), rhs_of(X, k, o + 1, nz, z.y)};
+      if (MODE == YM_DERIV && X.rhs) {
+        z = {rhs_of(X, k, o, nz, z.x), rhs_of(X, k, o + 1, nz, z.y)};
+      } else if (X.acc) {
+        const C2<T> w = *reinterpret_cast<const C2<T>*>(out + o);
+        z = {X.ald * z.x + X.gout * w.x, X.ald * z.y + X.gout * w.y};
+      }
       *reinterpret_cast<C2<T>*>(out + o) = z;
     }
   }
@@ -549,8 +554,8 @@ __global__ __launch_bounds__(256) void k_xw256(int nrows, int nlines, XArgs<T> X
 #pragma unroll
       for (int j = 0; j < 4; ++j) {
         const int x = lane + 64 * j;
-        o0[x] = v[j].x;
-        if (has1) o0[N + x] = v[j].y;
+        store_d(X, o0, x, v[j].x);
+        if (has1) store_d(X, o0, N + x, v[j].y);
       }
     }
   }

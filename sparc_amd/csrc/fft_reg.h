@@ -238,7 +238,16 @@ struct XArgs {
   int rhs;
   const T* rz;
   const T* r3;
+  // derivative combined into the output (curl assembly): out = ald d + gout out
+  int acc;
+  T ald, gout;
 };
+
+// plain derivative store: d, or combined with what out holds (X.acc)
+template <typename T>
+__device__ __forceinline__ void store_d(const XArgs<T>& X, T* out, long o, T d) {
+  out[o] = X.acc ? X.ald * d + X.gout * out[o] : d;
+}
 
 template <typename T>
 __device__ __forceinline__ T rhs_of(const XArgs<T>& X, long k, long o, int nz, T d) {
@@ -357,8 +366,8 @@ __global__ __launch_bounds__(NT) void k_xlines_n(int nrows, int nlines, XArgs<T>
         out[r0 * N + x] = rhs_of(X, r0 / ny, r0 * N + x, nz, v.x);
         if (r1 < nrows) out[r1 * N + x] = rhs_of(X, r1 / ny, r1 * N + x, nz, v.y);
       } else {
-        out[r0 * N + x] = v.x;
-        if (r1 < nrows) out[r1 * N + x] = v.y;
+        store_d(X, out, r0 * N + x, v.x);
+        if (r1 < nrows) store_d(X, out, r1 * N + x, v.y);
       }
     }
   } else {
@@ -468,7 +477,10 @@ __global__ __launch_bounds__(NT) void k_ylines_n(int nx, int nz, XArgs<T> X, T b
       const int y = e / TX, xx = e - y * TX;
       const long o = (i0 + xx) + (long)nx * y + nxy * k;
       const T d = rr[2 * ((xx >> 1) * LS + lpad(y)) + (xx & 1)];
-      out[o] = (MODE == YM_DERIV && X.rhs) ? rhs_of(X, k, o, nz, d) : d;
+      if (MODE == YM_DERIV && X.rhs)
+        out[o] = rhs_of(X, k, o, nz, d);
+      else
+        store_d(X, out, o, d);
     }
   } else {
     const long q = k;

@@ -944,6 +944,10 @@ static int launch_xm(sparc_ctx* c, int nf, const XArgs<T>& X, T beta, const T* m
 #ifndef SPARC_XW_FUSE
 #define SPARC_XW_FUSE 1
 #endif
+// option 10: curl pieces combined as the line kernels produce them
+#ifndef SPARC_MHD_BACC
+#define SPARC_MHD_BACC 1
+#endif
 template <typename T>
 static int launch_x_rhs_damp(sparc_ctx* c, const XArgs<T>& X, const T* mult) {
   const long nlines = (long)((c->cfg.nz + 1) / 2) * c->cfg.ny;
@@ -1266,29 +1270,78 @@ static int run_stage(sparc_ctx* c, Fld<T>& F, int s) {
     if ((rc = launch_y<T, YM_DERIV>(c, F.W[3], nullptr, 0, M[4], dmy, nullptr))) return rc;
     if ((rc = dz(F.W[3], nullptr, 0, 0, M[5]))) return rc;
     if ((rc = pmlb(M[5], 1))) return rc;
-    // b = curl(flux): dx f3, dy f3, dz f2 (psiinductionbx), dx f2, dy f1, dz f1 (psiinductionby)
-    if ((rc = launch_x<T, XM_DERIV>(c, M[2], nullptr, 0, M[6], dmx, nullptr))) return rc;
-    if ((rc = launch_y<T, YM_DERIV>(c, M[2], nullptr, 0, M[7], dmy, nullptr))) return rc;
-    if ((rc = dz(M[1], nullptr, 0, 1, M[8]))) return rc;
-    if ((rc = pmlb(M[8], 4))) return rc;  // psiinductionbx
-    if ((rc = launch_x<T, XM_DERIV>(c, M[1], nullptr, 0, M[9], dmx, nullptr))) return rc;
-    if ((rc = launch_y<T, YM_DERIV>(c, M[0], nullptr, 0, M[10], dmy, nullptr))) return rc;
-    if ((rc = dz(M[0], nullptr, 0, 1, M[11]))) return rc;
-    if ((rc = pmlb(M[11], 5))) return rc;  // psiinductionby
-    {
-      ProfScope ps(c, SPARC_KC_OTHER, 9.0 * (double)N * sizeof(T));
-      hipLaunchKernelGGL((k_mhd_b<T>), gN, b256, 0, c->stream, F, N);
-      LAUNCHCHK(c);
+    // b = curl(flux): dx f3, dy f3, dz f2 (psiinductionbx), dx f2, dy f1, dz f1
+    // (psiinductionby).  With the register line kernels the pieces are
+    // combined as they are produced (out = ald d + gout out) into M9..M11,
+    // after the z pieces and their PML: bx = dy f3 - dz f2, by = dz f1 -
+    // dx f3, bz = dx f2 - dy f1.  The curl of b then goes to the freed slots
+    // and k_mhd_rhs reads them through a remapped table.  Otherwise the six
+    // pieces go to M6..M11 and k_mhd_b combines them into M0..M2.
+    Fld<T> FR = F;  // the arrays k_mhd_rhs reads as M0..M11
+    const bool bacc = SPARC_MHD_BACC && fast_size(c->cfg.nx) && fast_size(c->cfg.ny);
+    auto xacc = [&](const T* in, T* out, T ald, T gout) -> int {
+      XArgs<T> X{};
+      X.a[0] = in;
+      X.out[0] = out;
+      X.acc = 1;
+      X.ald = ald;
+      X.gout = gout;
+      return launch_xm<T, XM_DERIV>(c, 1, X, 0, dmx);
+    };
+    auto yacc = [&](const T* in, T* out, T ald, T gout) -> int {
+      XArgs<T> X{};
+      X.a[0] = in;
+      X.out[0] = out;
+      X.acc = 1;
+      X.ald = ald;
+      X.gout = gout;
+      return launch_ym<T, YM_DERIV>(c, 1, X, 0, dmy);
+    };
+    if (bacc) {
+      if ((rc = dz(M[1], nullptr, 0, 1, M[9]))) return rc;
+      if ((rc = pmlb(M[9], 4))) return rc;  // psiinductionbx
+      if ((rc = yacc(M[2], M[9], 1, -1))) return rc;   // bx = dy f3 - dz f2
+      if ((rc = dz(M[0], nullptr, 0, 1, M[10]))) return rc;
+      if ((rc = pmlb(M[10], 5))) return rc;  // psiinductionby
+      if ((rc = xacc(M[2], M[10], -1, 1))) return rc;  // by = dz f1 - dx f3
+      if ((rc = launch_x<T, XM_DERIV>(c, M[1], nullptr, 0, M[11], dmx, nullptr))) return rc;
+      if ((rc = yacc(M[0], M[11], -1, 1))) return rc;  // bz = dx f2 - dy f1
+      // J = curl(b) pieces dx by, dy bz, dz bx, dx bz, dy bx, dz by into M0,
+      // M1, M2, M6, M7, M8 (flux no longer needed)
+      if ((rc = launch_x<T, XM_DERIV>(c, M[10], nullptr, 0, M[0], dmx, nullptr))) return rc;
+      if ((rc = launch_y<T, YM_DERIV>(c, M[11], nullptr, 0, M[1], dmy, nullptr))) return rc;
+      if ((rc = dz(M[9], nullptr, 0, 1, M[2]))) return rc;
+      if ((rc = pmlb(M[2], 2))) return rc;  // psidzbx
+      if ((rc = launch_x<T, XM_DERIV>(c, M[11], nullptr, 0, M[6], dmx, nullptr))) return rc;
+      if ((rc = launch_y<T, YM_DERIV>(c, M[9], nullptr, 0, M[7], dmy, nullptr))) return rc;
+      if ((rc = dz(M[10], nullptr, 0, 1, M[8]))) return rc;
+      if ((rc = pmlb(M[8], 3))) return rc;  // psidzby
+      T* const perm[12] = {M[9], M[10], M[11], M[3], M[4], M[5], M[0], M[1], M[2], M[6], M[7], M[8]};
+      for (int w = 0; w < 12; ++w) FR.M[w] = perm[w];
+    } else {
+      if ((rc = launch_x<T, XM_DERIV>(c, M[2], nullptr, 0, M[6], dmx, nullptr))) return rc;
+      if ((rc = launch_y<T, YM_DERIV>(c, M[2], nullptr, 0, M[7], dmy, nullptr))) return rc;
+      if ((rc = dz(M[1], nullptr, 0, 1, M[8]))) return rc;
+      if ((rc = pmlb(M[8], 4))) return rc;  // psiinductionbx
+      if ((rc = launch_x<T, XM_DERIV>(c, M[1], nullptr, 0, M[9], dmx, nullptr))) return rc;
+      if ((rc = launch_y<T, YM_DERIV>(c, M[0], nullptr, 0, M[10], dmy, nullptr))) return rc;
+      if ((rc = dz(M[0], nullptr, 0, 1, M[11]))) return rc;
+      if ((rc = pmlb(M[11], 5))) return rc;  // psiinductionby
+      {
+        ProfScope ps(c, SPARC_KC_OTHER, 9.0 * (double)N * sizeof(T));
+        hipLaunchKernelGGL((k_mhd_b<T>), gN, b256, 0, c->stream, F, N);
+        LAUNCHCHK(c);
+      }
+      // J = curl(b): dx by, dy bz, dz bx (psidzbx), dx bz, dy bx, dz by (psidzby)
+      if ((rc = launch_x<T, XM_DERIV>(c, M[1], nullptr, 0, M[6], dmx, nullptr))) return rc;
+      if ((rc = launch_y<T, YM_DERIV>(c, M[2], nullptr, 0, M[7], dmy, nullptr))) return rc;
+      if ((rc = dz(M[0], nullptr, 0, 1, M[8]))) return rc;
+      if ((rc = pmlb(M[8], 2))) return rc;  // psidzbx
+      if ((rc = launch_x<T, XM_DERIV>(c, M[2], nullptr, 0, M[9], dmx, nullptr))) return rc;
+      if ((rc = launch_y<T, YM_DERIV>(c, M[0], nullptr, 0, M[10], dmy, nullptr))) return rc;
+      if ((rc = dz(M[1], nullptr, 0, 1, M[11]))) return rc;
+      if ((rc = pmlb(M[11], 3))) return rc;  // psidzby
     }
-    // J = curl(b): dx by, dy bz, dz bx (psidzbx), dx bz, dy bx, dz by (psidzby)
-    if ((rc = launch_x<T, XM_DERIV>(c, M[1], nullptr, 0, M[6], dmx, nullptr))) return rc;
-    if ((rc = launch_y<T, YM_DERIV>(c, M[2], nullptr, 0, M[7], dmy, nullptr))) return rc;
-    if ((rc = dz(M[0], nullptr, 0, 1, M[8]))) return rc;
-    if ((rc = pmlb(M[8], 2))) return rc;  // psidzbx
-    if ((rc = launch_x<T, XM_DERIV>(c, M[2], nullptr, 0, M[9], dmx, nullptr))) return rc;
-    if ((rc = launch_y<T, YM_DERIV>(c, M[0], nullptr, 0, M[10], dmy, nullptr))) return rc;
-    if ((rc = dz(M[1], nullptr, 0, 1, M[11]))) return rc;
-    if ((rc = pmlb(M[11], 3))) return rc;  // psidzby
     const bool damp = c->cfg.damp_waves && c->damping_set;
     XArgs<T> DX{};
     for (int f = 0; f < 3; ++f) {
@@ -1303,7 +1356,7 @@ static int run_stage(sparc_ctx* c, Fld<T>& F, int s) {
     }
     {
       ProfScope ps(c, SPARC_KC_OTHER, 25.0 * (double)N * sizeof(T));
-      hipLaunchKernelGGL((k_mhd_rhs<T>), gN, b256, 0, c->stream, F,
+      hipLaunchKernelGGL((k_mhd_rhs<T>), gN, b256, 0, c->stream, FR,
                          (const T*)c->prof[SPARC_PROF_SPONGE], N);
       LAUNCHCHK(c);
     }
