@@ -102,6 +102,8 @@ def workload(name: str, precision: int | None = None) -> Workload:
     tiny   32 x 32 x 40, 25 Mm (parity tests)
     tall   32 x 32 x 200, 25 Mm (parity tests of the segmented z solve)
     wide   256 x 256 x 40, 200 Mm (parity tests of the length-256 line kernels)
+    wide_odd  256 x 256 x 41: an odd plane count (the damping's plane pairs end
+           with a single plane; z segments of unequal length)
     box3   box2 with option 10 and a flux tube (setup_for builds it): radius
            4 Mm, 3 kG at the photosphere, v_A capped at 4 c
     tube   32 x 32 x 100, 25 Mm, the flux tube of box3 with a 2 Mm radius
@@ -128,9 +130,9 @@ def workload(name: str, precision: int | None = None) -> Workload:
     elif name == "tall":
         nx = ny = 32
         nz, L, nsteps, prec = 200, 25.0, 10, abi.FP64
-    elif name == "wide":
+    elif name in ("wide", "wide_odd"):
         nx = ny = 256
-        nz, L, nsteps, prec = 40, 200.0, 10, abi.FP64
+        nz, L, nsteps, prec = (40 if name == "wide" else 41), 200.0, 10, abi.FP64
     elif name in ("box3", "tube"):
         big = name == "box3"
         nx = ny = 256 if big else 32

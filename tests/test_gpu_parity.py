@@ -40,8 +40,10 @@ def tall():
 
 @pytest.mark.parametrize("prec,tol", [(abi.FP64, TOL64), (abi.FP32, TOL32_OP)])
 def test_segmented_zsolve_matches_oracle(tall, prec, tol):
-    """nz = 200 runs the z solve in column segments with cold-start warm-up;
-    every ibc must still match DBYD2 to rounding."""
+    """nz = 200 runs the z solve in column segments: coupled exactly with ud
+    in LDS in FP32, by cold-start warm-up with ud in global memory in FP64
+    (the column is too long for LDS); every ibc must still match DBYD2 to
+    rounding."""
     W, S = tall
     eng, cfg = D.make_engine(W, S, precision=prec)
     try:
@@ -167,6 +169,27 @@ def test_step_3d_wide_matches_oracle(wide, prec):
     (derivatives, RHS epilogue of the pressure gradient, damping) and the
     spectral filter at time 0."""
     W, S = wide
+    eng, o = _run_pair(W, S, 3, prec)
+    try:
+        for arr, fields in ((abi.ARR_A, range(6)), (abi.ARR_SCR, range(6))):
+            for f in fields:
+                a, b = eng.get(arr, f), o.get(arr, f)
+                assert np.linalg.norm(b) > 0, (arr, f)
+                assert (rel_err(a, b) < TOL64) if prec == abi.FP64 else (rel_l2(a, b) < TOL32_L2)
+    finally:
+        eng.close()
+        o.close()
+
+
+@pytest.mark.parametrize("prec", [abi.FP64, abi.FP32])
+def test_step_3d_wide_odd_planes_matches_oracle(prec):
+    """Option 4 on 256 x 256 x 41: the damping passes pair planes 2q, 2q+1,
+    so the last pair holds one plane (in the damping transforms and in the
+    pass that writes RHS_vx with its damping), and the z solves' segments
+    are of unequal length."""
+    W = D.workload("wide_odd")
+    S = D.setup_for(W)
+    assert S.rho0.shape[2] == 41
     eng, o = _run_pair(W, S, 3, prec)
     try:
         for arr, fields in ((abi.ARR_A, range(6)), (abi.ARR_SCR, range(6))):
