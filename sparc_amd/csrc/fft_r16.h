@@ -480,6 +480,29 @@ __global__ __launch_bounds__(256) void k_xw256(int nrows, int nlines, XArgs<T> X
       }
     }
     if (p + 1 < LPW && L + 1 < nlines) XW_LOAD(L + 1);
+    // DD: rows k0, k1 of the pair, which of them take the RHS (not planes 1
+    // and nz) and the damping, and rhoinv as a z profile.  When both rows
+    // take the damping and rhoinv is a profile (constant along the line),
+    // the rows are scaled by -rhoinv before the transform and the damping
+    // spectrum is subtracted from the derivative's: one inverse transform.
+    long dd_o = 0, dd_st = 0;
+    bool dd_h1 = false, dd_w0 = false, dd_w1 = false, dd_e0 = false, dd_e1 = false, one = false;
+    T dd_z0 = 0, dd_z1 = 0;
+    if constexpr (DD) {
+      rows(L, dd_o, dd_st, dd_h1);
+      edge_w(L, dd_w0, dd_w1);
+      const int k0 = 2 * (int)((unsigned)L / (unsigned)ny), k1 = k0 + 1;
+      dd_e0 = k0 == 0 || k0 == nz - 1;
+      dd_e1 = k1 == nz - 1;
+      dd_z0 = X.r3 ? (T)0 : X.rz[k0];
+      dd_z1 = (X.r3 || !dd_h1) ? (T)0 : X.rz[k1];
+      one = !X.r3 && dd_w0 && dd_w1;
+      if (one) {
+        const T s0 = dd_e0 ? (T)0 : -dd_z0, s1 = dd_e1 ? (T)0 : -dd_z1;
+#pragma unroll
+        for (int j = 0; j < 4; ++j) v[j] = C2<T>{s0 * v[j].x, s1 * v[j].y};
+      }
+    }
     if (MODE != XM_DAMP_INV) {  // forward transform
       xw_forward(v, w1, w2, w3, lane);
       if (MODE == XM_DAMP_FWD) {
@@ -490,29 +513,37 @@ __global__ __launch_bounds__(256) void k_xw256(int nrows, int nlines, XArgs<T> X
       }
 #pragma unroll
       for (int j = 0; j < 4; ++j) v[j] = C2<T>{-ml[j] * v[j].y, ml[j] * v[j].x};
+      if constexpr (DD) {
+        if (one) {
+#pragma unroll
+          for (int j = 0; j < 4; ++j) v[j] = C2<T>{v[j].x - sp[j].x, v[j].y - sp[j].y};
+        }
+      }
     }
     xw_inverse(v, w1, w2, w3, lane);
     if constexpr (DD) {
+      if (one) {
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+          const int x = lane + 64 * j;
+          out[dd_o + x] = v[j].x;
+          if (dd_h1) out[dd_o + dd_st + x] = v[j].y;
+        }
+        continue;
+      }
       xw_inverse(sp, w1, w2, w3, lane);
-      long o, st;
-      bool h1, w0, w1e;
-      rows(L, o, st, h1);
-      edge_w(L, w0, w1e);
-      const int k0 = 2 * (int)((unsigned)L / (unsigned)ny), k1 = k0 + 1;
-      const bool e0 = k0 == 0 || k0 == nz - 1, e1 = k1 == nz - 1;
-      const T z0 = X.r3 ? (T)0 : X.rz[k0], z1 = (X.r3 || !h1) ? (T)0 : X.rz[k1];
 #pragma unroll
       for (int j = 0; j < 4; ++j) {
         const int x = lane + 64 * j;
-        const T ri0 = X.r3 ? X.r3[o + x] : z0;
-        T r0 = e0 ? (T)0 : -ri0 * v[j].x;
-        if (w0) r0 = r0 - sp[j].x;
-        out[o + x] = r0;
-        if (h1) {
-          const T ri1 = X.r3 ? X.r3[o + st + x] : z1;
-          T r1 = e1 ? (T)0 : -ri1 * v[j].y;
-          if (w1e) r1 = r1 - sp[j].y;
-          out[o + st + x] = r1;
+        const T ri0 = X.r3 ? X.r3[dd_o + x] : dd_z0;
+        T r0 = dd_e0 ? (T)0 : -ri0 * v[j].x;
+        if (dd_w0) r0 = r0 - sp[j].x;
+        out[dd_o + x] = r0;
+        if (dd_h1) {
+          const T ri1 = X.r3 ? X.r3[dd_o + dd_st + x] : dd_z1;
+          T r1 = dd_e1 ? (T)0 : -ri1 * v[j].y;
+          if (dd_w1) r1 = r1 - sp[j].y;
+          out[dd_o + dd_st + x] = r1;
         }
       }
       continue;
