@@ -448,21 +448,25 @@ __global__ __launch_bounds__(256) void k_xw256(int nrows, int nlines, XArgs<T> X
   C2<T> ns[4];
   C2<T> nsp[DD ? 4 : 1];  // DD: the damping spectrum, digit-reversed lanes
 #define XW_LOAD(LL)                                                  \
-  if (MODE == XM_DAMP_INV) XW_LOAD_INV(LL) else {                    \
-    long o_, st_;                                                    \
-    bool h1_;                                                        \
-    rows((LL), o_, st_, h1_);                                        \
-    const T* __restrict__ a0_ = a + o_;                              \
-    const T* __restrict__ s0_ = s ? s + o_ : nullptr;                \
-    _Pragma("unroll") for (int j = 0; j < 4; ++j) {                  \
-      const int x = lane + 64 * j;                                   \
-      nv[j].x = a0_[x];                                              \
-      if (s0_) ns[j].x = s0_[x];                                     \
-      nv[j].y = h1_ ? a0_[st_ + x] : (T)0;                           \
-      if (s0_) ns[j].y = h1_ ? s0_[st_ + x] : (T)0;                  \
-      if constexpr (DD) nsp[j] = spec[(LL) * N + kl + 64 * j];        \
-    }                                                                \
-  }
+  do {                                                               \
+    if constexpr (MODE == XM_DAMP_INV) {                             \
+      XW_LOAD_INV(LL)                                                \
+    } else {                                                         \
+      long o_, st_;                                                \
+      bool h1_;                                                    \
+      rows((LL), o_, st_, h1_);                                    \
+      const T* __restrict__ a0_ = a + o_;                          \
+      const T* __restrict__ s0_ = s ? s + o_ : nullptr;            \
+      _Pragma("unroll") for (int j = 0; j < 4; ++j) {              \
+        const int x = lane + 64 * j;                               \
+        nv[j].x = a0_[x];                                          \
+        if (s0_) ns[j].x = s0_[x];                                 \
+        nv[j].y = h1_ ? a0_[st_ + x] : (T)0;                       \
+        if (s0_) ns[j].y = h1_ ? s0_[st_ + x] : (T)0;              \
+        if constexpr (DD) nsp[j] = spec[(LL) * N + kl + 64 * j];   \
+      }                                                            \
+    }                                                              \
+  } while (0)
   if (L0 < nlines) XW_LOAD(L0);
   for (int p = 0; p < LPW; ++p) {
     const long L = L0 + p;

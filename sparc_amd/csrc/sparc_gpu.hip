@@ -1901,8 +1901,7 @@ extern "C" int sparc_kernel_begin(sparc_ctx* c, const double* grav, int nz) {
 template <typename T>
 static int magnetic_kernels(sparc_ctx* c, Fld<T>& F, const T* fx, const T* fy, const T* fz,
                             const T* ax, const T* ay, const T* az, double scale) {
-  const long N = c->N, nxy = c->nxy;
-  const int nz = c->cfg.nz;
+  const long N = c->N;
   const dim3 gN(cdiv(N, 256)), b256(256);
   const bool has_y = c->cfg.ny > 1;
   const T* dmx = (const T*)c->dmulx;
@@ -1997,7 +1996,6 @@ static int magnetic_kernels(sparc_ctx* c, Fld<T>& F, const T* fx, const T* fy, c
 template <typename T>
 static int kernel_acc_impl(sparc_ctx* c, const sparc_ctx* src, int slot) {
   const long N = c->N, nxy = c->nxy;
-  const int nz = c->cfg.nz;
   Fld<T> F = make_fld<T>(c);
   T* s = (T*)src->snap + (size_t)slot * snap_fields(src) * N;
   const T *fxx = s, *fxy = s + N, *fxz = s + 2 * N, *fax = s + 3 * N, *fay = s + 4 * N,
@@ -2125,8 +2123,7 @@ extern "C" int sparc_checkpoint_load(sparc_ctx* c, int slot) { return ckpt_copy(
 template <typename T>
 static int psi_kernel(sparc_ctx* c) {
   Fld<T> F = make_fld<T>(c);
-  const long N = c->N, nxy = c->nxy;
-  const int nz = c->cfg.nz;
+  const long N = c->N;
   const dim3 gN(cdiv(N, 256)), b256(256);
   T** W = F.W;
   int rc;
