@@ -212,3 +212,40 @@ def test_lane_shuffle_damping_layout_round_trip():
     np.testing.assert_allclose(spec, np.fft.fft(x), rtol=0, atol=1e-12 * np.abs(spec).max())
     back = np.stack([spec[KL + 64 * j] for j in range(4)])
     np.testing.assert_allclose(_inverse(back) / N, x, rtol=0, atol=1e-13 * np.abs(x).max())
+
+
+def _odd_multiplier(rng):
+    """An odd multiplier with the Nyquist bin zero, like the derivative's."""
+    k = np.fft.fftfreq(N) * N
+    m = rng.standard_normal(N // 2 + 1)
+    mult = np.sign(k) * m[np.abs(k).astype(int)]
+    mult[N // 2] = 0.0
+    return mult
+
+
+def test_single_inverse_for_scaled_rows_and_damping():
+    """The RHS_vx pass on interior plane pairs: rows scaled by -rhoinv (one
+    constant per row) before the forward transform, (i m) X minus the
+    damping spectrum, one inverse transform; each row must be its own
+    scaled derivative minus its own damping, as with separate inverses."""
+    rng = np.random.default_rng(8)
+    p0, p1 = rng.standard_normal(N), rng.standard_normal(N)
+    s0, s1 = -1.7, -0.4
+    mult = _odd_multiplier(rng)
+    d0, d1 = rng.standard_normal(N), rng.standard_normal(N)  # damping rows
+    rates = np.abs(rng.standard_normal(N))
+    rates = 0.5 * (rates + rates[(-np.arange(N)) % N])  # even in k
+    B = rates * np.fft.fft(d0 + 1j * d1)  # the damping spectrum of the pair
+    # one inverse, through the lane schedule
+    v = _forward(s0 * p0 + 1j * s1 * p1)
+    v = np.stack([1j * mult[KL + 64 * j] * v[j] - B[KL + 64 * j] for j in range(4)])
+    got = _inverse(v) / N
+    # separate real rows
+    deriv = lambda f: np.fft.ifft(1j * mult * np.fft.fft(f)).real
+    damp = np.fft.ifft(B)
+    want0 = s0 * deriv(p0) - damp.real
+    want1 = s1 * deriv(p1) - damp.imag
+    np.testing.assert_allclose(got.real, want0, rtol=0, atol=1e-12 * np.abs(want0).max())
+    np.testing.assert_allclose(got.imag, want1, rtol=0, atol=1e-12 * np.abs(want1).max())
+    assert np.abs(np.fft.ifft(1j * mult * np.fft.fft(p0)).imag).max() < 1e-12
+
