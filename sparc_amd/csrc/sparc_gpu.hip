@@ -357,6 +357,7 @@ static size_t footprint_bytes(const sparc_config* cfg) {
     b += 3 * spec_elems(cfg) * 2 * ts;      // damping spectra
   b += (size_t)(cfg->nx + cfg->ny) * 6 * ts + 4 * 7 * cfg->nz * ts + SPARC_PROF_COUNT * cfg->nz * ts;
   b += 2 * (size_t)(cfg->nx / 2 + 1) * cfg->ny * ts + 2 * nxy * ts;  // rates (two layouts)
+  b += 128 * 15 * (size_t)SPARC_STAGGER;  // allocation stagger, up to 128 allocations
   return b;
 }
 
