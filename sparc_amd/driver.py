@@ -108,6 +108,8 @@ def workload(name: str, precision: int | None = None) -> Workload:
            4 Mm, 3 kG at the photosphere, v_A capped at 4 c
     tube   32 x 32 x 100, 25 Mm, the flux tube of box3 with a 2 Mm radius
            (parity tests of the magnetic set-up)
+    tube_wide  256 x 256 x 40, 200 Mm, the flux tube of box3 (parity tests of
+           option 10 on the length-256 line kernels)
     column 32 x 32 x 100, 25 Mm, the vertical grid of box1 (adjoint/gradient tests:
            receivers and source lie below the top PML and filter layers)
     two_d  256 x 1 x 300, 400 Mm, option 17 (the reference's wired 2-D case)
@@ -133,10 +135,11 @@ def workload(name: str, precision: int | None = None) -> Workload:
     elif name in ("wide", "wide_odd"):
         nx = ny = 256
         nz, L, nsteps, prec = (40 if name == "wide" else 41), 200.0, 10, abi.FP64
-    elif name in ("box3", "tube"):
+    elif name in ("box3", "tube", "tube_wide"):
         big = name == "box3"
-        nx = ny = 256 if big else 32
-        nz, L, nsteps = (300, 200.0, 2000) if big else (100, 25.0, 400)
+        nx = ny = 32 if name == "tube" else 256
+        nz, L, nsteps = {"box3": (300, 200.0, 2000), "tube": (100, 25.0, 400),
+                         "tube_wide": (40, 200.0, 20)}[name]
         tab = bgm.polytrope_table(nz, 0.5, -12.0)
         dx = L * M_CM / nx
         # the time step is set by the fast speed of the capped field
@@ -145,7 +148,7 @@ def workload(name: str, precision: int | None = None) -> Workload:
                    option=abi.OPT_MHD, npmltop=0,
                    precision=(abi.FP32 if big else abi.FP64) if precision is None else precision)
         return Workload(name, P, tab, nsteps,
-                        extra={"tube": dict(b_ph=3000.0, r_ph_mm=4.0 if big else 2.0,
+                        extra={"tube": dict(b_ph=3000.0, r_ph_mm=2.0 if name == "tube" else 4.0,
                                             decay_mm=10.0), "cap2": TUBE_CAP2})
     elif name == "column":
         nx = ny = 32

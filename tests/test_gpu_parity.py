@@ -657,6 +657,39 @@ def test_option10_wilson_depression_planes_match_oracle():
         o.close()
 
 
+@pytest.fixture(scope="module")
+def tube_wide_oracle():
+    """Option 10 on the 256-wide flux-tube grid, 2 steps of the oracle (run
+    once for both precisions)."""
+    W = D.workload("tube_wide")
+    S = D.setup_for(W)
+    o, _ = make_oracle(W, S, nsteps=2)
+    mask, series = D.forward_source(W, 10)
+    o.set_source(abi.SRC_SERIES, mask, series, 10 ** 9)
+    o.step(0, 2)
+    want = [o.get(abi.ARR_A, f) for f in range(6)] + [o.get(abi.ARR_SCR, f) for f in range(6)]
+    o.close()
+    return W, S, (mask, series), want
+
+
+@pytest.mark.parametrize("prec", [abi.FP64, abi.FP32])
+def test_flux_tube_wide_option10_matches_oracle(tube_wide_oracle, prec):
+    """Option 10 on 256 x 256 x 40 with the flux tube: the length-256 x and
+    y kernels store the curl pieces combined onto the z pieces (b = curl
+    of the flux assembled in place)."""
+    W, S, (mask, series), want = tube_wide_oracle
+    eng, cfg = D.make_engine(W, S, precision=prec, nsteps=2)
+    try:
+        eng.set_source(abi.SRC_SERIES, mask, series, 10 ** 9)
+        eng.step(0, 2)
+        got = [eng.get(abi.ARR_A, f) for f in range(6)] + [eng.get(abi.ARR_SCR, f) for f in range(6)]
+        for i, (a, b) in enumerate(zip(got, want)):
+            assert np.linalg.norm(b) > 0, i
+            assert (rel_err(a, b) < TOL64) if prec == abi.FP64 else (rel_l2(a, b) < TOL32_L2), i
+    finally:
+        eng.close()
+
+
 @pytest.mark.parametrize("prec", [abi.FP64, abi.FP32])
 def test_flux_tube_option10_matches_oracle(prec):
     """Option 10 on the flux-tube background of the magnetic set-up (Alfven
