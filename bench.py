@@ -31,7 +31,8 @@ HBM_PEAK_GBS = 8000.0
 KERNEL_SYMBOLS = {
     "xlines": "sparc::k_xw256<float, 0 (XM_DERIV) | 4 (XM_DERIV_DAMPINV), 2> -- x derivative, one wavefront per line",
     "ylines": "sparc::k_y256<float, 0 (YM_DERIV)> -- y derivative, register four-step",
-    "damp": "sparc::k_xw256<float, 2 (XM_DAMP_FWD), 4> -- damping, forward x",
+    "damp": "sparc::k_xw256<float, 2 (XM_DAMP_FWD), 4> -- damping, forward x "
+            "(with --damp2d: sparc::k_damp2d<float>, the whole damping, one 256x256 plane per block)",
     "damp_y": "sparc::k_y256<float, 2 (YM_DAMP)> -- damping, y pass",
     "damp_xi": "sparc::k_xw256<float, 3 (XM_DAMP_INV), 2> -- damping, inverse x (v_y, v_z; v_x in XM_DERIV_DAMPINV)",
     "zsolve": "sparc::k_zsolve<float, 2 (EPI_RHS3D), 8, true> -- z solve (ud in LDS) + momentum/continuity RHS",
@@ -61,7 +62,12 @@ def main() -> int:
                     help="one forward + adjoint + kernel solve of STEPS steps on the workload "
                          "(observed data from a +2%% sound-speed bump, 8 receivers on a 10 Mm "
                          "ring) instead of the step benchmark")
+    ap.add_argument("--damp2d", action="store_true",
+                    help="the damping of 256x256 FP32 planes by k_damp2d, one pass per plane "
+                         "(sets SPARC_DAMP2D=1 for the engine)")
     args = ap.parse_args()
+    if args.damp2d:
+        os.environ["SPARC_DAMP2D"] = "1"
     if args.copy_probe:
         return copy_probe(args.copy_probe)
     if args.serving:

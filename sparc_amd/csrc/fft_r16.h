@@ -321,6 +321,75 @@ __device__ __forceinline__ void xw_swap(C2<T> (&v)[4], int lane) {
   }
 }
 
+#ifndef SPARC_XW_PL
+#define SPARC_XW_PL 3
+#endif
+// The same digit swaps for FP32 without the LDS permute: lane bits 5 and 4
+// by v_permlane32_swap / v_permlane16_swap (which swap the upper half of one
+// register's lane group with the lower half of the other's: exactly the
+// exchange above), lane bits 0-3 by DPP row shifts folded into the select
+// (a lane with the bit set reads lane - m, one with it clear lane + m).
+template <int CTRL>
+__device__ __forceinline__ float xw_dpp(float x) {
+  return __builtin_bit_cast(
+      float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, x), CTRL, 0xF, 0xF, false));
+}
+
+template <int M>
+__device__ __forceinline__ void xw_pair_pl(float& a, float& b, bool hi) {
+  if constexpr (M == 32) {
+    const auto r = __builtin_amdgcn_permlane32_swap(__builtin_bit_cast(unsigned, a),
+                                                    __builtin_bit_cast(unsigned, b), false, false);
+    a = __builtin_bit_cast(float, (unsigned)r[0]);
+    b = __builtin_bit_cast(float, (unsigned)r[1]);
+  } else if constexpr (M == 16) {
+    const auto r = __builtin_amdgcn_permlane16_swap(__builtin_bit_cast(unsigned, a),
+                                                    __builtin_bit_cast(unsigned, b), false, false);
+    a = __builtin_bit_cast(float, (unsigned)r[0]);
+    b = __builtin_bit_cast(float, (unsigned)r[1]);
+  } else {
+    static_assert(M == 1 || M == 2 || M == 4 || M == 8, "row shifts within 16 lanes");
+    // both shifted reads by every lane, then the selects: a conditional
+    // read would run with the partner lanes switched off
+    const float sb = xw_dpp<0x110 + M>(b);  // row_shr:M, lane - M
+    const float sa = xw_dpp<0x100 + M>(a);  // row_shl:M, lane + M
+    const float na = hi ? sb : a;
+    const float nb = hi ? b : sa;
+    a = na;
+    b = nb;
+  }
+}
+
+template <int B>
+__device__ __forceinline__ void xw_swap_pl(C2<float> (&v)[4], int lane) {
+#pragma unroll
+  for (int h = 1; h >= 0; --h) {
+    const bool hi = (lane & (B << h)) != 0;
+#pragma unroll
+    for (int o = 0; o < 2; ++o) {
+      const int i0 = h ? o : 2 * o, i1 = i0 + (1 << h);
+      if (h) {
+        xw_pair_pl<B * 2>(v[i0].x, v[i1].x, hi);
+        xw_pair_pl<B * 2>(v[i0].y, v[i1].y, hi);
+      } else {
+        xw_pair_pl<B>(v[i0].x, v[i1].x, hi);
+        xw_pair_pl<B>(v[i0].y, v[i1].y, hi);
+      }
+    }
+  }
+}
+
+template <typename T, int B>
+__device__ __forceinline__ void xw_swap_any(C2<T> (&v)[4], int lane) {
+  // SPARC_XW_PL: 1 = lane bits 4, 5 by permlane swaps; 2 = lane bits 0-3
+  // by DPP; 3 = both
+  constexpr bool PL = sizeof(T) == 4 && (B == 16 ? (SPARC_XW_PL & 1) : (SPARC_XW_PL & 2));
+  if constexpr (PL)
+    xw_swap_pl<B>(reinterpret_cast<C2<float>(&)[4]>(v), lane);
+  else
+    xw_swap<T, B>(v, lane);
+}
+
 template <typename T>
 __device__ __forceinline__ void xw_twiddle(C2<T> (&v)[4], const C2<T> (&w)[3], bool conj) {
 #pragma unroll
@@ -338,13 +407,13 @@ __device__ __forceinline__ void xw_forward(C2<T> (&v)[4], const C2<T> (&w1)[3],
                                            const C2<T> (&w2)[3], const C2<T> (&w3)[3], int lane) {
   dft_reg<T, 4, -1>(v);
   xw_twiddle(v, w1, false);
-  xw_swap<T, 16>(v, lane);
+  xw_swap_any<T, 16>(v, lane);
   dft_reg<T, 4, -1>(v);
   xw_twiddle(v, w2, false);
-  xw_swap<T, 4>(v, lane);
+  xw_swap_any<T, 4>(v, lane);
   dft_reg<T, 4, -1>(v);
   xw_twiddle(v, w3, false);
-  xw_swap<T, 1>(v, lane);
+  xw_swap_any<T, 1>(v, lane);
   dft_reg<T, 4, -1>(v);
 }
 
@@ -352,13 +421,13 @@ template <typename T>
 __device__ __forceinline__ void xw_inverse(C2<T> (&v)[4], const C2<T> (&w1)[3],
                                            const C2<T> (&w2)[3], const C2<T> (&w3)[3], int lane) {
   dft_reg<T, 4, +1>(v);
-  xw_swap<T, 1>(v, lane);
+  xw_swap_any<T, 1>(v, lane);
   xw_twiddle(v, w3, true);
   dft_reg<T, 4, +1>(v);
-  xw_swap<T, 4>(v, lane);
+  xw_swap_any<T, 4>(v, lane);
   xw_twiddle(v, w2, true);
   dft_reg<T, 4, +1>(v);
-  xw_swap<T, 16>(v, lane);
+  xw_swap_any<T, 16>(v, lane);
   xw_twiddle(v, w1, true);
   dft_reg<T, 4, +1>(v);
 }
@@ -596,4 +665,151 @@ __global__ __launch_bounds__(256) void k_xw256(int nrows, int nlines, XArgs<T> X
   }
 #undef XW_LOAD
 #undef XW_LOAD_INV
+}
+
+// DAMP_VELOCITY (damping.f90:125-185) on one 256 x 256 plane per block:
+// out -= IDFT2(rates DFT2(in)), unnormalised transforms, rates[m + 129 ky]
+// with 1/(nx ny) folded and m = min(kx, 256 - kx) (even in kx and ky).
+// 16 wavefronts; wave w keeps row pairs q = 8w .. 8w+7 in registers as
+// complex rows z_q = in[2q] + i in[2q+1] (lane l, register j: x = l + 64j)
+// and transforms them along x (spectrum in the digit-reversed lanes of
+// xw_forward).  The y lines of the half spectrum kx = 0..128 run in two
+// chunks of x-bins closed under kx -> 256 - kx ({0..64, 192..255}, then
+// {65..191}), staged through an LDS tile [bin][q]: each line unpacks V_y[kx]
+// = (z_q[kx] + conj z_q[-kx]) / 2 (y = 2q) or -i (z_q[kx] - conj z_q[-kx]) / 2
+// (y = 2q+1), is transformed, scaled, transformed back and repacked into
+// both bins (the outputs are real, so bin -kx holds the conjugates).  The
+// inverse x transforms then leave D in[2q] + i D in[2q+1].  Planes 1 and nz
+// are skipped unless X.edges.
+// kx = KL(lane) + 64 j: chunk 0 is registers 0 and 3 and bin 64 (lane 0,
+// register 1), chunk 1 registers 1 and 2 without it; with c and j known at
+// compile time the other chunk's registers are dead while the tile holds them
+__device__ __forceinline__ bool d2_member(int c, int j, int kx) {
+  return c == 0 ? (j == 0 || j == 3 || kx == 64) : ((j == 1 || j == 2) && kx != 64);
+}
+__device__ __forceinline__ int d2_col(int c, int kx) {
+  return c == 0 ? (kx <= 64 ? kx : kx - 127) : kx - 65;
+}
+
+template <typename T>
+__global__ __launch_bounds__(1024) void k_damp2d(XArgs<T> X, const C2<T>* __restrict__ twg,
+                                                 const T* __restrict__ rates, int nz, long nxy) {
+  constexpr int N = 256, LQ = 129;
+  __shared__ C2<T> tile[129 * LQ];
+  const int k = blockIdx.x, fld = blockIdx.y;
+  if (!X.edges && (k == 0 || k == nz - 1)) return;  // whole block
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const T* __restrict__ in = X.a[fld] + nxy * k;
+  T* __restrict__ out = X.out[fld] + nxy * k;
+  C2<T> w1[3], w2[3], w3[3];
+#pragma unroll
+  for (int kk = 1; kk < 4; ++kk) {
+    w1[kk - 1] = twg[(lane * kk) & 255];
+    w2[kk - 1] = twg[(4 * (lane & 15) * kk) & 255];
+    w3[kk - 1] = twg[(16 * (lane & 3) * kk) & 255];
+  }
+  C2<T> z[8][4];
+#pragma unroll
+  for (int r = 0; r < 8; ++r) {
+    const long o = (long)(2 * (8 * wv + r)) * N + lane;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) z[r][j] = C2<T>{in[o + 64 * j], in[o + N + 64 * j]};
+  }
+#pragma unroll
+  for (int r = 0; r < 8; ++r) xw_forward(z[r], w1, w2, w3, lane);
+  const int kl = (lane >> 4) + 4 * ((lane >> 2) & 3) + 16 * (lane & 3);
+  const bool odd = lane & 1;
+#pragma unroll
+  for (int c = 0; c < 2; ++c) {
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const int kx = kl + 64 * j;
+      if (d2_member(c, j, kx)) {
+        C2<T>* t = tile + d2_col(c, kx) * LQ + 8 * wv;
+#pragma unroll
+        for (int r = 0; r < 8; ++r) t[r] = z[r][j];
+      }
+    }
+    __syncthreads();
+    const int ncol = c ? 64 : 65, kx0 = c ? 65 : 0;
+    for (int i = wv; i < ncol; i += 16) {  // wave-uniform
+      const int kx = kx0 + i, mk = (N - kx) & (N - 1);
+      T rt[4];  // loaded ahead of the transform
+#pragma unroll
+      for (int j = 0; j < 4; ++j) rt[j] = rates[kx + 129 * (kl + 64 * j)];
+      C2<T>* ta = tile + d2_col(c, kx) * LQ;
+      C2<T>* tb = tile + d2_col(c, mk) * LQ;
+      C2<T> v[4];
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        const int q = (lane + 64 * j) >> 1;
+        const C2<T> A = ta[q], B = tb[q];
+        v[j] = odd ? C2<T>{A.y + B.y, B.x - A.x} : C2<T>{A.x + B.x, A.y - B.y};
+      }
+      xw_forward(v, w1, w2, w3, lane);
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        const T r = (T)0.5 * rt[j];
+        v[j] = C2<T>{r * v[j].x, r * v[j].y};
+      }
+      xw_inverse(v, w1, w2, w3, lane);
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        const C2<T> P = {__shfl_xor(v[j].x, 1), __shfl_xor(v[j].y, 1)};
+        const C2<T> E = odd ? P : v[j], O = odd ? v[j] : P;
+        const int q = (lane + 64 * j) >> 1;
+        if (mk == kx) {
+          if (!odd) ta[q] = C2<T>{E.x, O.x};
+        } else if (!odd) {
+          ta[q] = C2<T>{E.x - O.y, E.y + O.x};
+        } else {
+          tb[q] = C2<T>{E.x + O.y, O.x - E.y};
+        }
+      }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const int kx = kl + 64 * j;
+      if (d2_member(c, j, kx)) {
+        const C2<T>* t = tile + d2_col(c, kx) * LQ + 8 * wv;
+#pragma unroll
+        for (int r = 0; r < 8; ++r) z[r][j] = t[r];
+      }
+    }
+    __syncthreads();  // tile reused by the next chunk
+  }
+  // the RHS rows of the next row pair are loaded before this one's inverse
+  const long ob = (long)(2 * 8 * wv) * N + lane;
+  T o0[4], o1[4];
+#pragma unroll
+  for (int j = 0; j < 4; ++j) {
+    o0[j] = out[ob + 64 * j];
+    o1[j] = out[ob + N + 64 * j];
+  }
+#pragma unroll
+  for (int r = 0; r < 8; ++r) {
+    const long o = ob + (long)(2 * r) * N;
+    T n0[4], n1[4];
+    if (r < 7) {
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        n0[j] = out[o + 2 * N + 64 * j];
+        n1[j] = out[o + 3 * N + 64 * j];
+      }
+    }
+    xw_inverse(z[r], w1, w2, w3, lane);
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      out[o + 64 * j] = o0[j] - z[r][j].x;
+      out[o + N + 64 * j] = o1[j] - z[r][j].y;
+    }
+    if (r < 7) {
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        o0[j] = n0[j];
+        o1[j] = n1[j];
+      }
+    }
+  }
 }

@@ -6,6 +6,7 @@
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <algorithm>
 #include <mutex>
@@ -83,6 +84,10 @@ struct sparc_ctx {
   void* rates = nullptr;    // damping rates, transposed: [y + ny*m]
   void* rates_o = nullptr;  // damping rates as given: [m + (nx/2+1)*y]
   bool damping_set = false, filter_set = false;
+  // option 4 on 256 x 256 FP32 planes: the damping by k_damp2d, one pass per
+  // plane, instead of the forward x, y and inverse x passes (SPARC_DAMP2D=1
+  // in the environment at sparc_create)
+  bool damp2d = false;
   void* tri[4] = {};  // per ibc 0..3: lowr | invb | gam | fcum, gcum (2 partitions), nz each
   double tc[14] = {};
   // per ibc and precision (0 fp64, 1 fp32): [kc0, kc1) rows with constant factors
@@ -401,6 +406,7 @@ extern "C" int sparc_create(const sparc_config* cfg, sparc_ctx** out) {
   *out = nullptr;
   sparc_ctx* c = new sparc_ctx();
   c->cfg = *cfg;
+  if (const char* e = std::getenv("SPARC_DAMP2D")) c->damp2d = std::atoi(e) != 0;
   if (c->cfg.field_scale == 0.0) c->cfg.field_scale = 1.0;
   std::string why;
   if (check_config(cfg, &why) != SPARC_OK) {
@@ -949,6 +955,24 @@ static int launch_xm(sparc_ctx* c, int nf, const XArgs<T>& X, T beta, const T* m
 #ifndef SPARC_MHD_BACC
 #define SPARC_MHD_BACC 1
 #endif
+// the damping of three fields, one 256 x 256 plane per block (X.a the
+// velocities, X.out the RHS they are subtracted from)
+template <typename T>
+static int launch_damp2d(sparc_ctx* c, const XArgs<T>& X) {
+  if constexpr (sizeof(T) != 4) {  // an FP64 plane does not fit the LDS tile
+    return fail(c, SPARC_EINVAL, "k_damp2d is FP32 only");
+  } else {
+    if (c->cfg.nx != 256 || c->cfg.ny != 256 || c->nxy != 65536)
+      return fail(c, SPARC_EINVAL, "k_damp2d needs 256 x 256 planes");
+    // the velocities read, the RHS read and written
+    ProfScope ps(c, SPARC_KC_DAMP, 9.0 * (double)c->N * sizeof(T));
+    hipLaunchKernelGGL((k_damp2d<T>), dim3(c->cfg.nz, 3), dim3(1024), 0, c->stream, X,
+                       (const C2<T>*)c->twx, (const T*)c->rates_o, c->cfg.nz, c->nxy);
+    LAUNCHCHK(c);
+    return SPARC_OK;
+  }
+}
+
 template <typename T>
 static int launch_x_rhs_damp(sparc_ctx* c, const XArgs<T>& X, const T* mult) {
   const long nlines = (long)((c->cfg.nz + 1) / 2) * c->cfg.ny;
@@ -1192,7 +1216,10 @@ static int run_stage(sparc_ctx* c, Fld<T>& F, int s) {
     // with damping on 256-wide x lines, RHS_vx is written once, by the pass
     // that also subtracts its damping (after the damping forward and y passes,
     // which read RHS_xi only)
-    const bool fuse = damp && SPARC_XW && SPARC_XW_FUSE && c->cfg.nx == 256;
+    // FP32 on 256 x 256 planes: the whole damping in one pass per plane
+    const bool d2d = damp && c->damp2d && sizeof(T) == 4 && c->cfg.nx == 256 &&
+                     c->cfg.ny == 256;
+    const bool fuse = damp && !d2d && SPARC_XW && SPARC_XW_FUSE && c->cfg.nx == 256;
     XArgs<T> G{};
     G.a[0] = F.W[3];
     G.rhs = 1;
@@ -1214,6 +1241,7 @@ static int run_stage(sparc_ctx* c, Fld<T>& F, int s) {
         DX.out[f] = F.scr[3 + f];
         DX.spec[f] = (C2<T>*)c->spec[f];
       }
+      if (d2d) return launch_damp2d<T>(c, DX);
       if ((rc = launch_xm<T, XM_DAMP_FWD>(c, 3, DX, 0, nullptr))) return rc;
       if ((rc = launch_ym<T, YM_DAMP>(c, 3, DX, 0, nullptr))) return rc;
       if (fuse) {

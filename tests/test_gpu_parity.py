@@ -181,6 +181,24 @@ def test_step_3d_wide_matches_oracle(wide, prec):
         o.close()
 
 
+def test_step_3d_wide_plane_damping_matches_oracle(wide, monkeypatch):
+    """Option 4 on 256 x 256 x 40 in FP32 with SPARC_DAMP2D=1: the damping of
+    v_x, v_y, v_z by k_damp2d (one pass per plane) instead of the forward x,
+    y and inverse x passes, and the x derivative of p written plainly."""
+    W, S = wide
+    monkeypatch.setenv("SPARC_DAMP2D", "1")
+    eng, o = _run_pair(W, S, 3, abi.FP32)
+    try:
+        for arr, fields in ((abi.ARR_A, range(6)), (abi.ARR_SCR, range(6))):
+            for f in fields:
+                a, b = eng.get(arr, f), o.get(arr, f)
+                assert np.linalg.norm(b) > 0, (arr, f)
+                assert rel_l2(a, b) < TOL32_L2, (arr, f)
+    finally:
+        eng.close()
+        o.close()
+
+
 @pytest.mark.parametrize("prec", [abi.FP64, abi.FP32])
 def test_step_3d_wide_odd_planes_matches_oracle(prec):
     """Option 4 on 256 x 256 x 41: the damping passes pair planes 2q, 2q+1,

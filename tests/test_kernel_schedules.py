@@ -157,19 +157,28 @@ _W3 = [W256[(16 * (LANE & 3) * k) & 255] for k in (1, 2, 3)]
 KL = (LANE >> 4) + 4 * ((LANE >> 2) & 3) + 16 * (LANE & 3)
 
 
-def _forward(x):
-    v = np.stack([x[LANE + 64 * j] for j in range(4)])
+def _fwd_lanes(v):
+    """lane l, register j holds x[l + 64 j] -> X[KL[l] + 64 j]"""
     v = _swap(_twiddle(_dft4(v, -1), _W1, False), 16)
     v = _swap(_twiddle(_dft4(v, -1), _W2, False), 4)
     v = _swap(_twiddle(_dft4(v, -1), _W3, False), 1)
     return _dft4(v, -1)
 
 
-def _inverse(v):
+def _inv_lanes(v):
+    """X[KL[l] + 64 j] -> unnormalised inverse at l + 64 j"""
     v = _twiddle(_swap(_dft4(v, 1), 1), _W3, True)
     v = _twiddle(_swap(_dft4(v, 1), 4), _W2, True)
     v = _twiddle(_swap(_dft4(v, 1), 16), _W1, True)
-    v = _dft4(v, 1)
+    return _dft4(v, 1)
+
+
+def _forward(x):
+    return _fwd_lanes(np.stack([x[LANE + 64 * j] for j in range(4)]))
+
+
+def _inverse(v):
+    v = _inv_lanes(v)
     y = np.zeros(N, complex)
     for j in range(4):
         y[LANE + 64 * j] = v[j]
@@ -249,3 +258,95 @@ def test_single_inverse_for_scaled_rows_and_damping():
     np.testing.assert_allclose(got.imag, want1, rtol=0, atol=1e-12 * np.abs(want1).max())
     assert np.abs(np.fft.ifft(1j * mult * np.fft.fft(p0)).imag).max() < 1e-12
 
+
+
+# ---------------------------------------------------------------------------
+# one 256 x 256 plane of the damping per block (k_damp2d)
+# ---------------------------------------------------------------------------
+
+
+def _chunk_member(c, kx):
+    return ((kx <= 64) | (kx >= 192)) if c == 0 else ((kx >= 65) & (kx <= 191))
+
+
+def _chunk_col(c, kx):
+    """column of the LDS tile holding x-bin kx in chunk c"""
+    if c == 0:
+        return np.where(kx <= 64, kx, kx - 127)
+    return kx - 65
+
+
+def _damp2d(v, rates):
+    """The kernel's schedule on a real plane v[y, x]: row pairs q as complex
+    rows, forward x transforms in the lane layout, two chunks of x-bins
+    closed under kx -> 256 - kx staged in a tile [column, q], the y lines of
+    kx = 0..128 unpacked from z_q[kx] and z_q[-kx], transformed, scaled by
+    rates[ky, kx] / 2, transformed back and repacked into both bins, then
+    inverse x transforms.  Returns the unnormalised IDFT2(rates DFT2(v))."""
+    regs = np.stack([_forward(v[2 * q] + 1j * v[2 * q + 1]) for q in range(128)])
+    kxs = KL[None, :] + 64 * np.arange(4)[:, None]
+    y = LANE[None, :] + 64 * np.arange(4)[:, None]
+    qy, even = y >> 1, (y & 1) == 0
+    ky = kxs
+    for c in (0, 1):
+        mem = _chunk_member(c, kxs)
+        col_of = _chunk_col(c, kxs)
+        tile = np.full((129, 128), np.nan + 0j)
+        for q in range(128):
+            tile[col_of[mem], q] = regs[q][mem]
+        for kx in (range(0, 65) if c == 0 else range(65, 129)):
+            mk = (256 - kx) % 256
+            ca, cb = int(_chunk_col(c, np.array(kx))), int(_chunk_col(c, np.array(mk)))
+            A, B = tile[ca, qy], tile[cb, qy]
+            col = np.where(even, (A.real + B.real) + 1j * (A.imag - B.imag),
+                           (A.imag + B.imag) - 1j * (A.real - B.real))
+            col = _fwd_lanes(col)
+            col = col * 0.5 * rates[ky, kx]
+            col = _inv_lanes(col)
+            P = col[:, LANE ^ 1]
+            E, O = np.where(even, col, P), np.where(even, P, col)
+            if mk == kx:
+                tile[ca, qy[even]] = (E.real + 1j * O.real)[even]
+            else:
+                tile[ca, qy[even]] = (E + 1j * O)[even]
+                tile[cb, qy[~even]] = (np.conj(E) + 1j * np.conj(O))[~even]
+        assert not np.isnan(tile[: (129 if c == 0 else 127)]).any()
+        for q in range(128):
+            regs[q][mem] = tile[col_of[mem], q]
+    out = np.zeros((256, 256))
+    for q in range(128):
+        z = _inv_lanes(regs[q])
+        for j in range(4):
+            out[2 * q, LANE + 64 * j] = z[j].real
+            out[2 * q + 1, LANE + 64 * j] = z[j].imag
+    return out
+
+
+def test_plane_damping_schedule_is_the_2d_transform():
+    """The per-plane damping pass equals IDFT2(r DFT2(v)) with rates even in
+    ky and indexed by min(kx, 256 - kx), as the three-pass chain computes."""
+    rng = np.random.default_rng(11)
+    v = rng.standard_normal((256, 256))
+    r = rng.uniform(0.1, 1.0, (256, 129))
+    r = 0.5 * (r + r[(-np.arange(256)) % 256])  # even in ky
+    got = _damp2d(v, r)
+    kx = np.arange(256)
+    R = r[:, np.minimum(kx, 256 - kx)]
+    want = np.fft.ifft2(R * np.fft.fft2(v)) * 65536
+    assert np.abs(want.imag).max() < 1e-9 * np.abs(want.real).max()
+    np.testing.assert_allclose(got, want.real, rtol=0, atol=1e-11 * np.abs(want.real).max())
+
+
+def test_plane_damping_chunks_by_register():
+    """The kernel tests chunk membership by register index (registers 0 and 3
+    and bin 64 in chunk 0, registers 1 and 2 without it in chunk 1): the same
+    sets as the bin ranges, each closed under kx -> 256 - kx."""
+    kxs = KL[None, :] + 64 * np.arange(4)[:, None]
+    j = np.arange(4)[:, None] + 0 * KL[None, :]
+    by_reg = [(j == 0) | (j == 3) | (kxs == 64), ((j == 1) | (j == 2)) & (kxs != 64)]
+    for c in (0, 1):
+        np.testing.assert_array_equal(by_reg[c], _chunk_member(c, kxs))
+        ks = set(kxs[by_reg[c]].tolist())
+        assert ks == {(256 - k) % 256 for k in ks}
+        cols = _chunk_col(c, kxs[by_reg[c]])
+        assert sorted(cols.tolist()) == list(range(len(ks)))
