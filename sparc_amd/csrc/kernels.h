@@ -96,7 +96,8 @@ __global__ __launch_bounds__(256) void k_xlines(int n, int nrows, int nlines, in
                                                 const T* __restrict__ a, const T* __restrict__ s,
                                                 T beta, T* out, const T* __restrict__ mult,
                                                 FFTPlan P, const C2<T>* __restrict__ tw,
-                                                C2<T>* spec, int ny, int nz, long nxy) {
+                                                C2<T>* spec, int ny, int nz, long nxy,
+                                                int edges) {
   extern __shared__ unsigned char smem_raw[];
   C2<T>* b0 = reinterpret_cast<C2<T>*>(smem_raw);
   C2<T>* b1 = b0 + LPB * n;
@@ -164,15 +165,16 @@ __global__ __launch_bounds__(256) void k_xlines(int n, int nrows, int nlines, in
       out[r0 * n + x] = r[e].x;
       if (r1 < nrows) out[r1 * n + x] = r[e].y;
     }
-  } else {  // XM_DAMP_INV: RHSv -= term, boundary planes excluded (they are zeroed)
+  } else {  // XM_DAMP_INV: RHSv -= term; planes 1 and nz only with edges (options
+            // 2 and 8 zero them before the damping, pml.f90:56-74,193-206)
     for (int e = threadIdx.x; e < nl * n; e += blockDim.x) {
       const int l = e / n, x = e - l * n;
       const long L = L0 + l;
       const long q = L / ny, j = L - q * ny;
       const long k0 = 2 * q, k1 = k0 + 1;
       const long o0 = x + (long)n * j + nxy * k0;
-      if (k0 != 0 && k0 != nz - 1) out[o0] = out[o0] - r[e].x;
-      if (k1 < nz - 1) out[o0 + nxy] = out[o0 + nxy] - r[e].y;
+      if (edges || (k0 != 0 && k0 != nz - 1)) out[o0] = out[o0] - r[e].x;
+      if (k1 < nz - 1 || (edges && k1 < nz)) out[o0 + nxy] = out[o0 + nxy] - r[e].y;
     }
   }
 }

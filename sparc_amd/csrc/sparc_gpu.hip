@@ -935,7 +935,8 @@ static int launch_xm(sparc_ctx* c, int nf, const XArgs<T>& X, T beta, const T* m
   for (int f = 0; f < nf; ++f) {
     hipLaunchKernelGGL((k_xlines<T, MODE>), dim3(cdiv(nlines, LPB)), dim3(256), lds, c->stream,
                        n, (int)nrows, (int)nlines, LPB, X.a[f], X.s[f], beta, X.out[f], mult,
-                       c->px, (const C2<T>*)c->twx, X.spec[f], c->cfg.ny, c->cfg.nz, c->nxy);
+                       c->px, (const C2<T>*)c->twx, X.spec[f], c->cfg.ny, c->cfg.nz, c->nxy,
+                       X.edges);
     LAUNCHCHK(c);
     if (MODE == XM_DERIV && X.rhs) {
       hipLaunchKernelGGL((k_rhs_grad<T>), dim3(cdiv(c->N, 256)), dim3(256), 0, c->stream,
