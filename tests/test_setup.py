@@ -81,3 +81,31 @@ def test_workload_timesteps_divide_the_cadence():
         P = D.workload(name).params
         assert abs(P.steps * P.timestep - 30.0) < 1e-9
         assert P.cadforcing_step == P.steps
+
+
+def test_density_perturbation_set_up():
+    """background.perturb_density: zero perturbation leaves every array
+    unchanged; a bump moves c^2 rho0, 1/rho0 and rho0' with rho0 at fixed c
+    and changes p0' by -g d(rho0) (hydrostatic), outside the PML layers."""
+    W = D.workload("tiny")
+    S = D.setup_for(W)
+    P = W.params
+    same = B.perturb_density(S, np.zeros(S.c2.shape))
+    for k in ("rhoinv", "c2rho0", "gradp0_z", "gradrho0_z", "az", "bzpml", "g"):
+        assert rel_err(same.derived[k], np.broadcast_to(S.derived[k], same.derived[k].shape)) \
+            < 1e-15, k
+    nz = P.nz
+    h = np.zeros(S.c2.shape)
+    h[:, :, P.npmlbot + 5:nz - P.npmltop - 5] = 0.01
+    T = B.perturb_density(S, h)
+    rho = np.broadcast_to(S.rho0, T.rho0.shape)
+    assert np.allclose(T.rho0, rho * np.exp(h), rtol=1e-15)
+    assert np.allclose(T.derived["c2rho0"], S.c2 * T.rho0, rtol=1e-15)
+    assert np.allclose(T.derived["rhoinv"] * T.rho0, 1.0, rtol=1e-15)
+    dp = T.derived["gradp0_z"] - S.derived["gradp0_z"]
+    assert np.allclose(dp, -S.derived["g"][None, None, :] * (T.rho0 - rho), rtol=1e-12,
+                       atol=1e-14 * np.abs(S.derived["gradp0_z"]).max())
+    drho = B.ddz(np.asfortranarray(T.rho0 - rho), S.derived["stretch"], 1)
+    inner = slice(P.npmlbot, nz - P.npmltop)
+    assert np.allclose((T.derived["gradrho0_z"] - S.derived["gradrho0_z"])[:, :, inner],
+                       drho[:, :, inner], rtol=1e-10, atol=1e-12 * np.abs(drho).max())
