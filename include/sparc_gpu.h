@@ -271,6 +271,17 @@ int sparc_norm(sparc_ctx *ctx, int field, double *out);
 /* Samples a(i,j,k,field) at npts (0-based i,j; 1-based k) points -> host. */
 int sparc_sample(sparc_ctx *ctx, int field, int npts, const int32_t *ijk,
                  double *host);
+/* sparc_sample without any device allocation, for callers that cannot
+ * allocate in their loop (a serving loop sampling receivers every output
+ * cadence, code under stream capture).  No counterpart in the reference.
+ * The point list and the values are staged in a fixed device array of
+ * SPARC_SAMPLE_NOALLOC_MAX points (20 bytes each, 1.25 MiB per GPU, held
+ * until the process ends); larger npts returns SPARC_EINVAL.  Every context
+ * on a GPU shares that array: calls within one process are serialised by a
+ * lock, and the call returns only after the values are on the host. */
+#define SPARC_SAMPLE_NOALLOC_MAX 65536
+int sparc_sample_noalloc(sparc_ctx *ctx, int field, int npts, const int32_t *ijk,
+                         double *host);
 
 /* ---- sensitivity kernels in the time loop (kernels.f90:197-461) -------- */
 

@@ -198,6 +198,15 @@ module sparc_gpu
       integer(c_int32_t), intent(in) :: ijk(*)
       real(c_double), intent(out) :: host(*)
     end function
+    ! the same without device allocation (at most 65536 points per call)
+    integer(c_int) function sparc_sample_noalloc(ctx, field, npts, ijk, host) &
+        bind(C, name="sparc_sample_noalloc")
+      import :: c_int, c_ptr, c_double, c_int32_t
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: field, npts
+      integer(c_int32_t), intent(in) :: ijk(*)
+      real(c_double), intent(out) :: host(*)
+    end function
 
     ! snapshots, checkpoints and kernels (WRITE_OUT_PARTIAL_STATE, PRODUCE_KERNELS)
     integer(c_int) function sparc_snapshots_alloc(ctx, nslots) bind(C, name="sparc_snapshots_alloc")

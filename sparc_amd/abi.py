@@ -45,6 +45,7 @@ EXPORTS = (
     "sparc_footprint", "sparc_set_background", "sparc_set_profile", "sparc_set_damping",
     "sparc_set_filter", "sparc_set_source", "sparc_set_source_planes", "sparc_step", "sparc_sync", "sparc_reset",
     "sparc_get_state", "sparc_put_state", "sparc_get_plane", "sparc_norm", "sparc_sample",
+    "sparc_sample_noalloc",
     "sparc_snapshots_alloc", "sparc_snapshot_store", "sparc_snapshots_copy",
     "sparc_kernel_begin", "sparc_kernel_accumulate", "sparc_kernel_get", "sparc_apply_op",
     "sparc_profile_enable", "sparc_profile_read", "sparc_kernel_accumulate_from",
@@ -130,6 +131,7 @@ def lib() -> C.CDLL:
     L.sparc_get_plane.argtypes = [vp, C.c_int, C.c_int, DP]
     L.sparc_norm.argtypes = [vp, C.c_int, DP]
     L.sparc_sample.argtypes = [vp, C.c_int, C.c_int, C.POINTER(C.c_int32), DP]
+    L.sparc_sample_noalloc.argtypes = [vp, C.c_int, C.c_int, C.POINTER(C.c_int32), DP]
     L.sparc_snapshots_alloc.argtypes = [vp, C.c_int]
     L.sparc_snapshot_store.argtypes = [vp, C.c_int]
     L.sparc_snapshots_copy.argtypes = [vp, vp]
@@ -267,12 +269,14 @@ class Engine:
         self._chk(self.L.sparc_norm(self.h, field, C.byref(v)), "sparc_norm")
         return v.value
 
-    def sample(self, field: int, ijk: np.ndarray) -> np.ndarray:
+    def sample(self, field: int, ijk: np.ndarray, noalloc: bool = False) -> np.ndarray:
+        """a(i,j,k,field) at the points; noalloc stages them in the library's
+        fixed device array (sparc_sample_noalloc) instead of allocating."""
         ijk = np.ascontiguousarray(ijk, dtype=np.int32).reshape(-1, 3)
         out = np.empty(ijk.shape[0])
-        self._chk(self.L.sparc_sample(self.h, field, ijk.shape[0],
-                                      ijk.ctypes.data_as(C.POINTER(C.c_int32)), _dp(out)),
-                  "sparc_sample")
+        fn = self.L.sparc_sample_noalloc if noalloc else self.L.sparc_sample
+        self._chk(fn(self.h, field, ijk.shape[0], ijk.ctypes.data_as(C.POINTER(C.c_int32)),
+                     _dp(out)), "sparc_sample_noalloc" if noalloc else "sparc_sample")
         return out
 
     def snapshots_alloc(self, n: int):

@@ -73,6 +73,7 @@ struct sparc_ctx {
   bool profset[SPARC_PROF_COUNT] = {};
   std::vector<double> unstretch_h, z_h;
   void* planes_buf = nullptr;  // device storage behind src_planes
+  unsigned char* scratch = nullptr;  // device address of this device's sample scratch
   void* W[8] = {};
   void* spec[3] = {};
   void* twx = nullptr;
@@ -401,6 +402,33 @@ static void tri_factors(int n2, int ibc, std::vector<double>& lowr, std::vector<
   for (int j = 0; j < n2; ++j) invb[j] = 1.0 / betad[j];
 }
 
+// Staging of sparc_sample_noalloc: the point list and the gathered values of
+// one call, in a file-scope device array so that the call allocates nothing.
+// Each GPU holds its own copy until the process ends; host code reaches it
+// only through the address hipGetSymbolAddress returns for that device.
+#define SPARC_SAMPLE_SCRATCH_BYTES ((size_t)SPARC_SAMPLE_NOALLOC_MAX * (12 + 8))
+static __device__ __align__(256) unsigned char scratch[SPARC_SAMPLE_SCRATCH_BYTES];
+static_assert(SPARC_SAMPLE_NOALLOC_MAX % 32 == 0, "values start 256-byte aligned");
+static std::mutex g_scratch_mu;  // guards the table and serialises the calls
+static constexpr int kMaxDevices = 64;
+static unsigned char* g_scratch_ptr[kMaxDevices] = {};
+
+// Device address of `scratch` on the current device (looked up once per device).
+static int scratch_address(sparc_ctx* c) {
+  if (c->device < 0 || c->device >= kMaxDevices)
+    return fail(c, SPARC_EINVAL, "device %d beyond the scratch table", c->device);
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (!g_scratch_ptr[c->device]) {
+    void* ptr = nullptr;
+    hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !ptr)
+      return fail(c, SPARC_EHIP, "hipGetSymbolAddress(scratch) failed: %s", hipGetErrorString(e));
+    g_scratch_ptr[c->device] = (unsigned char*)ptr;
+  }
+  c->scratch = g_scratch_ptr[c->device];
+  return SPARC_OK;
+}
+
 extern "C" int sparc_create(const sparc_config* cfg, sparc_ctx** out) {
   if (!cfg || !out) return SPARC_EINVAL;
   *out = nullptr;
@@ -428,6 +456,7 @@ extern "C" int sparc_create(const sparc_config* cfg, sparc_ctx** out) {
   HIPCHK(c, hipEventCreateWithFlags(&c->xev, hipEventDisableTiming));
   const size_t ts = c->ts, N = c->N;
   int rc;
+  if ((rc = scratch_address(c))) return rc;
   c->comp = c->fp32 && cfg->compensated;
   c->nf = nfields_of(cfg->option);
   for (int f = 0; f < c->nf; ++f) {
@@ -1808,18 +1837,19 @@ __global__ void k_gather(double* out, const T* __restrict__ f, const int32_t* __
   out[p] = (double)f[q] * scale;
 }
 
-extern "C" int sparc_sample(sparc_ctx* c, int field, int npts, const int32_t* ijk, double* h) {
-  if (!c || !h || !ijk || npts < 0 || field < 0 || field >= c->nf)
-    return SPARC_EINVAL;
-  if (npts == 0) return SPARC_OK;
+static int sample_check(sparc_ctx* c, int field, int npts, const int32_t* ijk, const double* h) {
+  if (!c || !h || !ijk || npts < 0 || field < 0 || field >= c->nf) return SPARC_EINVAL;
   for (int p = 0; p < npts; ++p)
     if (ijk[3 * p] < 0 || ijk[3 * p] >= c->cfg.nx || ijk[3 * p + 1] < 0 ||
         ijk[3 * p + 1] >= c->cfg.ny || ijk[3 * p + 2] < 1 || ijk[3 * p + 2] > c->cfg.nz)
       return fail(c, SPARC_EINVAL, "sample point %d out of the grid", p);
-  int32_t* dijk;
-  double* dout;
-  HIPCHK(c, hipMallocAsync((void**)&dijk, 12 * (size_t)npts, c->stream));
-  HIPCHK(c, hipMallocAsync((void**)&dout, 8 * (size_t)npts, c->stream));
+  return SPARC_OK;
+}
+
+// Stages ijk into dijk, gathers into dout and copies the values to h, on the
+// context's stream; returns after the copy is done.
+static int sample_on(sparc_ctx* c, int field, int npts, const int32_t* ijk, double* h,
+                     int32_t* dijk, double* dout) {
   HIPCHK(c, hipMemcpyAsync(dijk, ijk, 12 * (size_t)npts, hipMemcpyHostToDevice, c->stream));
   const double sc = 1.0 / c->cfg.field_scale;
   if (c->fp32)
@@ -1830,10 +1860,37 @@ extern "C" int sparc_sample(sparc_ctx* c, int field, int npts, const int32_t* ij
                        (const double*)c->a[field], dijk, npts, c->cfg.nx, c->nxy, sc);
   LAUNCHCHK(c);
   HIPCHK(c, hipMemcpyAsync(h, dout, 8 * (size_t)npts, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return SPARC_OK;
+}
+
+extern "C" int sparc_sample(sparc_ctx* c, int field, int npts, const int32_t* ijk, double* h) {
+  int rc = sample_check(c, field, npts, ijk, h);
+  if (rc || npts == 0) return rc;
+  int32_t* dijk;
+  double* dout;
+  HIPCHK(c, hipMallocAsync((void**)&dijk, 12 * (size_t)npts, c->stream));
+  HIPCHK(c, hipMallocAsync((void**)&dout, 8 * (size_t)npts, c->stream));
+  rc = sample_on(c, field, npts, ijk, h, dijk, dout);
   HIPCHK(c, hipFreeAsync(dijk, c->stream));
   HIPCHK(c, hipFreeAsync(dout, c->stream));
   HIPCHK(c, hipStreamSynchronize(c->stream));
-  return SPARC_OK;
+  return rc;
+}
+
+extern "C" int sparc_sample_noalloc(sparc_ctx* c, int field, int npts, const int32_t* ijk,
+                                    double* h) {
+  int rc = sample_check(c, field, npts, ijk, h);
+  if (rc || npts == 0) return rc;
+  if (npts > SPARC_SAMPLE_NOALLOC_MAX)
+    return fail(c, SPARC_EINVAL, "sparc_sample_noalloc: %d points exceed the maximum %d", npts,
+                SPARC_SAMPLE_NOALLOC_MAX);
+  if (!c->scratch) return fail(c, SPARC_ESTATE, "sample scratch not set up");
+  // one call at a time per process: every context on a device shares its array
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  int32_t* dijk = (int32_t*)c->scratch;
+  double* dout = (double*)(c->scratch + 12 * (size_t)SPARC_SAMPLE_NOALLOC_MAX);
+  return sample_on(c, field, npts, ijk, h, dijk, dout);
 }
 
 // ---------------------------------------------------------------------------
