@@ -379,7 +379,7 @@ def build_setup(tab: np.ndarray, nx: int, ny: int, option: int, xlength: float, 
     return S
 
 
-def perturb_density(S: Setup, dlnrho: np.ndarray) -> Setup:
+def perturb_density(S: Setup, dlnrho: np.ndarray, hydrostatic: bool = True) -> Setup:
     """The set-up of a density perturbation rho0 -> rho0 exp(dlnrho) at fixed
     sound speed and gravity, with the background kept hydrostatic:
     d(p0')/dz changes by -g d(rho0) (p0' = -rho0 g).  This is the
@@ -387,14 +387,16 @@ def perturb_density(S: Setup, dlnrho: np.ndarray) -> Setup:
     derivative for: c^2 rho0 (the bulk modulus), rho0 (inertia), rho0' (the
     buoyancy of rho') and p0' (the Lagrangian pressure) all move with it.
     dlnrho must vanish in the PML layers (their profile factors multiply the
-    gradients there)."""
+    gradients there).  hydrostatic=False keeps p0' as it was (a background
+    out of balance; the density kernel is not its derivative)."""
     if S.mhd is not None:
         raise ValueError("perturb_density: magnetic set-ups are not covered")
     d = np.asarray(dlnrho, dtype=np.float64)
     rho_new = np.asfortranarray(np.broadcast_to(S.rho0, S.c2.shape) * np.exp(d))
     drho = rho_new - S.rho0
     der = init_rhs(S.z, S.c2, rho_new, S.p0, S.g_raw, S.npmlbot, S.npmltop, S.option == 17)
-    der["gradp0_z"] = S.derived["gradp0_z"] - S.derived["g"][None, None, :] * drho
+    der["gradp0_z"] = S.derived["gradp0_z"] - (S.derived["g"][None, None, :] * drho
+                                              if hydrostatic else 0.0)
     return replace(S, rho0=rho_new, derived=der)
 
 

@@ -158,6 +158,16 @@ def kernel_lnc(raw_c2: np.ndarray, S: bgm.Setup, spec: SolveSpec,
     return -2.0 * S.rho0 * S.c2 * raw_c2 * dtn
 
 
+def kernel_lnrho(raw_rho: np.ndarray, S: bgm.Setup, spec: SolveSpec,
+                 kstride: int | None = None) -> np.ndarray:
+    """Density kernel K_lnrho at fixed sound speed (per unit nondimensional
+    volume) from the accumulated sum (kernels.f90:530: -rho0 sum *
+    stepskern*timestep), so that d chi = sum_cells K_lnrho dlnrho dV for the
+    hydrostatic perturbation of background.perturb_density."""
+    dtn = (kstride or spec.steps) * S.deltat
+    return -S.rho0 * raw_rho * dtn
+
+
 def kernel_flow(raw_v: np.ndarray, S: bgm.Setup, spec: SolveSpec,
                 kstride: int | None = None) -> np.ndarray:
     """Flow kernel component (per unit nondimensional volume and velocity)
