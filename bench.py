@@ -12,8 +12,13 @@ Prints ONE JSON line.  Besides the metric it carries:
   roofline      the dominant kernel class, its algorithmic bytes per launch /
                 its mean launch time (HIP events on the engine's stream,
                 recorded over a second pass of the same steps), vs 8 TB/s;
+  fp64_step     the same workload stepped by the FP64 engine;
+  serving       solves/hour and p50/p99 latency of the FP32 serving loop on
+                one forward+adjoint+kernel request per grid of the mix
+                (128^2 x 150, 256^2 x 300, 384^2 x 300; 1000 steps each),
+                against the same requests one at a time in FP64;
   cpu_baseline  the FP64 CPU oracle (a C restatement of the reference, OpenMP)
-                on a bounded sample of the same workload (one RK step).
+                on a bounded sample of the same workload (25 s of RK steps).
 """
 from __future__ import annotations
 
@@ -49,6 +54,11 @@ def main() -> int:
     ap.add_argument("--workload", default="box2")
     ap.add_argument("--precision", default="fp32", choices=("fp32", "fp64"))
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-fp64", action="store_true",
+                    help="skip the FP64 step rate of the same workload")
+    ap.add_argument("--serving-mix", type=int, default=1000, metavar="STEPS",
+                    help="steps per request of the in-line serving figure (one request per "
+                         "grid of the mix; 0 = skip)")
     ap.add_argument("--traffic-json", default=os.path.join(ROOT, "profiles", "traffic.json"),
                     help="HBM bytes per launch measured by rocprofv3 --pmc (optional)")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
@@ -151,6 +161,13 @@ def main() -> int:
                                                         else "option 4")
     eng.close()
 
+    fp64 = None
+    if prec == abi.FP32 and not args.no_fp64:
+        fp64 = fp64_step_rate(W, S, args.warmup, min(args.steps, 20))
+    serving = None
+    if args.serving_mix:
+        serving = serving_mix(args.serving_mix)
+
     cpu = None
     if not args.no_cpu_baseline:
         cpu = cpu_baseline(W)
@@ -180,15 +197,74 @@ def main() -> int:
         "roofline": roof,
         "cpu_baseline": cpu,
     }
+    if fp64 is not None:
+        line["fp64_step"] = fp64
+    if serving is not None:
+        line["serving"] = serving
     print(json.dumps(line))
     return 0
+
+
+def fp64_step_rate(W, S, warmup: int, steps: int) -> dict:
+    """The same workload stepped by the FP64 engine (the plain path's
+    arithmetic): Mpt/s over `steps` steps after `warmup`."""
+    from sparc_amd import abi
+    from sparc_amd import driver as D
+    P = W.params
+    total = warmup + steps
+    eng, cfg = D.make_engine(W, S, precision=abi.FP64, nsteps=max(total, W.nsteps))
+    try:
+        mask, series = D.forward_source(W, max(total, W.nsteps) + 8)
+        eng.set_source(abi.SRC_SERIES, mask, series, 10 ** 9)
+        eng.step(0, warmup)
+        eng.sync()
+        t0 = time.perf_counter()
+        eng.step(warmup, steps)
+        eng.sync()
+        wall = time.perf_counter() - t0
+    finally:
+        eng.close()
+    N = P.nx * P.ny * P.nz
+    print(f"[bench] FP64: {steps} steps in {wall:.3f}s -> {N * steps / wall / 1e6:.1f} Mpt/s",
+          file=sys.stderr, flush=True)
+    return {"value": round(N * steps / wall / 1e6, 2), "unit": "Mpt/s", "steps": steps,
+            "ms_per_step": round(1e3 * wall / steps, 4), "dtype": "f64"}
+
+
+def serving_mix(nsteps: int) -> dict:
+    """Solves/hour of the FP32 serving loop and of the plain FP64 path (the
+    same requests one at a time) on serve.mix_sample: one request per grid of
+    the mix, 384x384x300 included."""
+    from sparc_amd import serve as SV
+
+    reqs = SV.mix_sample(nsteps)
+    cache = SV.ModelCache()
+    for r in reqs:
+        cache.setup(r.workload, r.alpha)
+    free, _ = SV.device_free_bytes()
+    srv = SV.Server(int(0.9 * free), max_inflight=8, cache=cache)
+    res = srv.run(reqs, log=lambda m: print(m, file=sys.stderr, flush=True))
+    fast = SV.summarize(res)
+    print(f"[bench] serving loop: {fast}", file=sys.stderr, flush=True)
+    plain = SV.plain_fp64(reqs, cache)
+    print(f"[bench] plain FP64: {plain}", file=sys.stderr, flush=True)
+    sph = fast.get("solves_per_hour", 0.0)
+    return {"metric": "forward+adjoint+kernel solves/hour", "unit": "solves/h",
+            "requests": [f"{r.workload} x {r.nsteps} steps" for r in reqs],
+            "errors": len(srv.errors),
+            "fp32_loop": {"solves_per_hour": round(sph, 1),
+                          "wall_s": round(fast.get("wall_s", 0.0), 2),
+                          "p50_latency_s": round(fast.get("p50_latency_s", 0.0), 2),
+                          "p99_latency_s": round(fast.get("p99_latency_s", 0.0), 2)},
+            "plain_fp64": {"solves_per_hour": round(plain["solves_per_hour"], 1),
+                           "wall_s": round(plain["wall_s"], 2),
+                           "per_solve_s": [round(x, 2) for x in plain["per_solve_s"]]},
+            "speedup_vs_plain_fp64": round(sph / plain["solves_per_hour"], 3)}
 
 
 def serving_bench(n: int, rate: float) -> int:
     """Solves/hour of the FP32 serving loop against the plain FP64 path on the
     same n requests (forward + adjoint + kernel each)."""
-    import torch
-
     from sparc_amd import serve as SV
 
     reqs = SV.c5_requests(n, rate_per_s=rate)
@@ -197,7 +273,7 @@ def serving_bench(n: int, rate: float) -> int:
     for r in reqs:  # the models of an inversion step exist before its solves
         cache.setup(r.workload, r.alpha)
     setup_s = time.perf_counter() - t0
-    free, total = torch.cuda.mem_get_info()
+    free, total = SV.device_free_bytes()
     budget = int(0.9 * free)
     print(f"[serving] {n} requests, set-up {setup_s:.1f}s, budget {budget / 2**30:.0f} GiB",
           file=sys.stderr, flush=True)
@@ -291,10 +367,22 @@ def solve_bench(workload: str, precision: str, nsteps: int) -> int:
     return 0
 
 
-def cpu_baseline(W, min_seconds: float = 10.0, max_steps: int = 8):
+def cpu_model() -> str:
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline(W, min_seconds: float = 25.0, max_steps: int = 10):
     """FP64 CPU oracle (C restatement of the reference step, OpenMP) timed on
     RK steps of the same workload: steps until min_seconds have passed (at
-    most max_steps), after one untimed step."""
+    most max_steps), after one untimed step.  The reference itself cannot be
+    built here (it needs FFTW3), so this port stands in for it."""
     try:
         from oracle import oracle as O
         from sparc_amd import abi
@@ -319,9 +407,11 @@ def cpu_baseline(W, min_seconds: float = 10.0, max_steps: int = 8):
         P = W.params
         threads = O.omp_threads()
         return {"value": round(P.nx * P.ny * P.nz * n / dt / 1e6, 4), "unit": "Mpt/s",
-                "cores": threads, "kind": "port",
+                "cores": threads, "kind": "port", "cpu_model": cpu_model(),
                 "sample": f"{n} RK steps (time 1..{n}) of {P.nx}x{P.ny}x{P.nz}, FP64, "
-                          f"{dt:.1f} s on {threads} OpenMP threads"}
+                          f"{dt:.1f} s on {threads} OpenMP threads of {cpu_model()}; "
+                          f"the oracle (C restatement of the reference's step), since the "
+                          f"reference needs FFTW3, which is absent"}
     except Exception as e:  # the baseline is reported, never required
         return {"value": None, "unit": "Mpt/s", "cores": None, "kind": "port",
                 "sample": f"unavailable: {e}"}

@@ -100,6 +100,29 @@ def c5_requests(n: int, rate_per_s: float, seed: int = 5, mix=GRID_MIX,
     return out
 
 
+def device_free_bytes() -> tuple[int, int]:
+    """(free, total) device memory of the current GPU from the HIP runtime
+    the engine itself links (hipMemGetInfo), so the query does not start a
+    second runtime in the process."""
+    import ctypes as C
+    abi.lib()  # loads libamdhip64 as a dependency of the engine
+    hip = C.CDLL("libamdhip64.so.7")
+    free, total = C.c_size_t(0), C.c_size_t(0)
+    rc = hip.hipMemGetInfo(C.byref(free), C.byref(total))
+    if rc != 0:
+        raise RuntimeError(f"hipMemGetInfo failed ({rc})")
+    return int(free.value), int(total.value)
+
+
+def mix_sample(nsteps: int = 1000, precision: int = abi.FP32) -> list[Request]:
+    """A fixed request set with one request per grid of the mix (128^2 x 150,
+    256^2 x 300, 384^2 x 300), all arriving at once, each nsteps long (the
+    short end of the stream's 1000-3000): the serving figure the step
+    benchmark reports beside its step rate."""
+    picks = (("box2s", (-6.0, 4.0), 1.0), ("box2", (5.0, -3.0), 0.5), ("box2l", (2.0, 7.0), 1.5))
+    return [Request(i, wl, xy, a, nsteps, 0.0, precision) for i, (wl, xy, a) in enumerate(picks)]
+
+
 class ModelCache:
     """FP64 set-ups per (grid, step size): base polytrope plus alpha times a
     fixed update direction (+2 % sound speed, Gaussian of radius 5 Mm at 3 Mm
