@@ -109,6 +109,17 @@ def call_ref_dbyd2(A: np.ndarray, B: np.ndarray, ibc: int) -> np.ndarray:
     return A
 
 
+def call_ref_dbyd1(A: np.ndarray, B: np.ndarray, ibc: int) -> np.ndarray:
+    """A(N1,N2) <- DBYD1(B) along the first index, Fortran order; A holds
+    caller boundary values (ibc 2, 3, 4)."""
+    R = ref_dbyd1()
+    n1, n2 = B.shape[0], B.shape[1]
+    A = np.asfortranarray(A.astype(np.float64))
+    B = np.asfortranarray(B.astype(np.float64))
+    R.dbyd1_(_dp(A), _dp(B), C.byref(C.c_int(n1)), C.byref(C.c_int(n2)), C.byref(C.c_int(ibc)))
+    return A
+
+
 def dbyd2(A: np.ndarray, B: np.ndarray, ibc: int) -> np.ndarray:
     L = lib()
     n1, n2 = B.shape[0], B.shape[1]

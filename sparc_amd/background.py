@@ -89,6 +89,88 @@ def dbyd2(B: np.ndarray, ibc: int, A: np.ndarray | None = None) -> np.ndarray:
     return U
 
 
+def _f32(x: float) -> float:
+    return float(np.float32(x))
+
+
+def dbyd1(B: np.ndarray, ibc: int, A: np.ndarray | None = None) -> np.ndarray:
+    """DBYD1 (dbyd1.f:8-242) + TRIDAG ispecial=0 (tridag.f): the sixth-order
+    compact derivative of B(n1, n2) along axis 0 (w.r.t. index/(n1-1)); A
+    carries caller boundary values for ibc 2, 3, 4.  dbyd1.f has no IMPLICIT
+    NONE and declares only A, B and the matrix REAL*8, so d1i, the stencil
+    coefficients, alpha and the preconditioning constants are single
+    precision, each computed in single arithmetic (dbyd1.f:27-28)."""
+    B = np.asarray(B, dtype=np.float64)
+    n1, n2 = B.shape
+    if not 0 <= ibc <= 4:
+        raise ValueError("dbyd1: ibc must be 0..4 (5, periodic, is not used here)")
+    A = np.zeros_like(B) if A is None else np.array(A, dtype=np.float64)
+    d1i = _f32(n1 - 1)
+    if n1 < 5:  # dbyd1.f:44-77
+        if ibc == 0:
+            A[:] = 0.0
+        elif ibc == 1:
+            A[:] = (B[n1 - 1] - B[0])[None, :]
+        elif ibc == 2:
+            A[:] = A[0][None, :]
+        elif ibc == 3:
+            A[:] = A[n1 - 1][None, :]
+        else:  # in place, row by row: later rows see the rewritten first row
+            for i in range(n1):
+                A[i] = (float(i) * A[0] + float(n1 - 1 - i) * A[n1 - 1]) / d1i
+        return A
+    f = np.float32
+    C1 = float(f(f(d1i) * f(7.0)) / f(9.0))
+    C2 = float(f(d1i) / f(36.0))
+    A[2:n1 - 2] = C1 * (B[3:n1 - 1] - B[1:n1 - 3]) + C2 * (B[4:n1] - B[0:n1 - 4])
+    c1 = float(f(-43.) / f(96.) * f(d1i))
+    c2 = float(f(-5.) / f(6.) * f(d1i))
+    c3 = float(f(9.) / f(8.) * f(d1i))
+    c4 = float(f(1.) / f(6.) * f(d1i))
+    c5 = float(f(-1.) / f(96.) * f(d1i))
+    A[1] = c1 * B[0] + c2 * B[1] + c3 * B[2] + c4 * B[3] + c5 * B[4]
+    A[n1 - 2] = -(c1 * B[n1 - 1] + c2 * B[n1 - 2] + c3 * B[n1 - 3] + c4 * B[n1 - 4]
+                  + c5 * B[n1 - 5])
+    b1 = float(f(-10) / f(3.) * f(d1i))
+    b2 = float(f(-3.0) * f(d1i))
+    b3 = float(f(6.) * f(d1i))
+    b4 = float(f(1.) / f(3.) * f(d1i))
+    if ibc == 0:
+        A[0] = 0.0
+        A[n1 - 1] = 0.0
+    if ibc in (1, 3):
+        A[0] = b1 * B[0] + b2 * B[1] + b3 * B[2] + b4 * B[3]
+    if ibc in (1, 2):
+        A[n1 - 1] = -(b1 * B[n1 - 1] + b2 * B[n1 - 2] + b3 * B[n1 - 3] + b4 * B[n1 - 4])
+    alpha = float(f(1.) / f(3.))
+    alpha2, gamma2, alpha1, beta1 = 0.75, 0.125, 6.0, 3.0
+    cst = float(f(1.) / f(f(alpha2) - f(f(beta1) * f(gamma2))))
+    up1 = float(f(f(f(alpha1) * f(alpha2)) - f(beta1)) * f(cst))
+    if ibc % 2:
+        A[0] = (alpha2 * A[0] - beta1 * A[1]) * cst
+    if ibc in (1, 2):
+        A[n1 - 1] = (alpha2 * A[n1 - 1] - beta1 * A[n1 - 2]) * cst
+    uppr = np.zeros(n1)
+    uppr[2:n1 - 2] = alpha
+    uppr[0] = 0.0 if ibc % 2 == 0 else up1
+    uppr[1] = alpha2
+    uppr[n1 - 2] = gamma2
+    lowr = uppr[::-1].copy()
+    lowr[n1 - 1] = up1 if ibc in (1, 2) else 0.0
+    # TRIDAG ispecial=0 (tridag.f), diag = 1
+    U = np.empty_like(A)
+    bet = 1.0
+    U[0] = A[0] / bet
+    gam = np.zeros(n1)
+    for j in range(1, n1):
+        gam[j] = uppr[j - 1] / bet
+        bet = 1.0 - lowr[j] * gam[j]
+        U[j] = (A[j] - lowr[j] * U[j - 1]) / bet
+    for j in range(n1 - 2, -1, -1):
+        U[j] = U[j] - gam[j + 1] * U[j + 1]
+    return U
+
+
 def ddz(f: np.ndarray, stretch: np.ndarray, ibc: int = 1) -> np.ndarray:
     """ddz (derivatives.f90:10-24) of a (nx, ny, nz) field."""
     nx, ny, nz = f.shape
@@ -455,6 +537,22 @@ def _solve_h(div: np.ndarray, sx: float, sy: float) -> np.ndarray:
     return np.fft.ifft2(G, axes=(0, 1)).real
 
 
+def _tube_profile(zc, zmm, p0, b_ph, r_ph_mm, decay_mm, beta_min):
+    """Axis field B0(z) and radius R(z) (Mm) of the flux tube: B0 is the
+    smooth minimum (4-norm) of b_ph exp(z/decay) and the field of plasma
+    beta beta_min, normalised to b_ph at z = 0; R = r_ph sqrt(b_ph/B0)."""
+    cap = np.sqrt(8.0 * np.pi * p0 / beta_min)
+
+    def smin(a, b):
+        return (a ** -4.0 + b ** -4.0) ** -0.25
+
+    cap0 = float(np.interp(0.0, zc, cap))
+    # amplitude of the exponential branch that puts B0(0) at b_ph
+    a0 = (b_ph ** -4.0 - cap0 ** -4.0) ** -0.25
+    b0 = smin(a0 * np.exp(zmm / decay_mm), cap)
+    return b0, r_ph_mm * np.sqrt(b_ph / b0)
+
+
 def flux_tube(tab: np.ndarray, nx: int, ny: int, xlength: float, ylength: float,
               b_ph: float = 3000.0, r_ph_mm: float = 4.0, decay_mm: float = 10.0,
               beta_min: float = 1.5):
@@ -490,16 +588,7 @@ def flux_tube(tab: np.ndarray, nx: int, ny: int, xlength: float, ylength: float,
     y = (np.arange(ny) / ny - 0.5) * ylength / M_CM
     r2 = x[:, None, None] ** 2 + y[None, :, None] ** 2
     p0, rho0 = tab[:, 3], tab[:, 2]
-    cap = np.sqrt(8.0 * np.pi * p0 / beta_min)
-
-    def smin(a, b):
-        return (a ** -4.0 + b ** -4.0) ** -0.25
-
-    cap0 = float(np.interp(0.0, zc, cap))
-    # amplitude of the exponential branch that puts B0(0) at b_ph
-    a0 = (b_ph ** -4.0 - cap0 ** -4.0) ** -0.25
-    b0 = smin(a0 * np.exp(zmm / decay_mm), cap)
-    R = r_ph_mm * np.sqrt(b_ph / b0)
+    b0, R = _tube_profile(zc, zmm, p0, b_ph, r_ph_mm, decay_mm, beta_min)
     bz = b0[None, None, :] * np.exp(-r2 / R[None, None, :] ** 2)
     sx, sy = RSUN_R / xlength, RSUN_R / ylength
     unstretch = dbyd2(np.asarray(tab[:, 0])[None, :], 1)[0]
@@ -516,6 +605,72 @@ def flux_tube(tab: np.ndarray, nx: int, ny: int, xlength: float, ylength: float,
     return dict(rho=np.asfortranarray(rho), p=np.asfortranarray(p), c=np.asfortranarray(c),
                 bx=np.asfortranarray(bx), by=np.asfortranarray(by), bz=np.asfortranarray(bz),
                 z=zc)
+
+
+def flux_tube_vector_potential(tab: np.ndarray, nx: int, ny: int, xlength: float,
+                               ylength: float, b_ph: float = 3000.0, r_ph_mm: float = 4.0,
+                               decay_mm: float = 10.0, beta_min: float = 1.5):
+    """The flux tube of ``flux_tube`` built from a vector potential, B =
+    B_mean z + curl A, with the package's own derivative operators (spectral
+    x and y, compact z), so that div B = 0 on the grid by construction: the x,
+    y and z operators act on different indices and commute.
+
+    The tube's flux function is psi(r, z) = r A_phi = (Phi/2 pi)(1 - exp(-r^2
+    / R(z)^2)), Phi = pi B0 R^2 the same at every height (B0(z), R(z) as in
+    flux_tube), whose B_z = B0 exp(-r^2/R^2).  A periodic box holds the tube's
+    net flux as the uniform field B_mean = Phi/(Lx Ly) (divergence-free and
+    constant in z); the rest is the periodic Coulomb-gauge potential
+    A = (-d phi/dy, d phi/dx, 0) with lap_h phi = B_z - B_mean per plane,
+    which is A_phi(r, z) phi_hat less the uniform field's (B_mean/2)(-y, x)
+    near the tube.  Then B_x = -dA_y/dz, B_y = dA_x/dz and B_z = dA_y/dx -
+    dA_x/dy + B_mean.  The gas is set as in flux_tube.
+
+    Returns flux_tube's dict plus ax, ay (gauss cm) and b_mean (gauss)."""
+    nz = tab.shape[0]
+    zc = (tab[:, 0] - 1.0) * RSUN_R
+    zmm = zc / M_CM
+    x = (np.arange(nx) / nx - 0.5) * xlength / M_CM
+    y = (np.arange(ny) / ny - 0.5) * ylength / M_CM
+    r2 = x[:, None, None] ** 2 + y[None, :, None] ** 2
+    p0, rho0 = tab[:, 3], tab[:, 2]
+    b0, R = _tube_profile(zc, zmm, p0, b_ph, r_ph_mm, decay_mm, beta_min)
+    bz_t = b0[None, None, :] * np.exp(-r2 / R[None, None, :] ** 2)
+    phi_flux = np.pi * b0 * (R * M_CM) ** 2          # Phi(z), gauss cm^2
+    b_mean = float(np.mean(phi_flux)) / (xlength * ylength)
+    sx, sy = RSUN_R / xlength, RSUN_R / ylength
+    # Rsun units: phi' in G Rsun^2, A' in G Rsun, derivatives w.r.t. x/Rsun
+    phi = _solve_h(bz_t - b_mean, sx, sy)
+    ax = -ddh(phi, 1, sy)
+    ay = ddh(phi, 0, sx)
+    unstretch = dbyd2(np.asarray(tab[:, 0])[None, :], 1)[0]
+    stretch = 1.0 / unstretch
+    bx = -ddz(ay, stretch, 1)
+    by = ddz(ax, stretch, 1)
+    bz = ddh(ay, 0, sx) - ddh(ax, 1, sy) + b_mean
+    p = p0[None, None, :] - bz ** 2 / (8.0 * np.pi)
+    if p.min() <= 0:
+        raise ValueError("flux tube: negative pressure (beta_min must exceed 1)")
+    rho = rho0[None, None, :] * p / p0[None, None, :]
+    c = np.broadcast_to(tab[:, 1][None, None, :], p.shape)
+    F = np.asfortranarray
+    return dict(rho=F(rho), p=F(p), c=F(c), bx=F(bx), by=F(by), bz=F(bz), z=zc,
+                ax=F(ax * RSUN_R), ay=F(ay * RSUN_R), b_mean=b_mean, flux=phi_flux)
+
+
+def magnetic_from_vectorpsi_2d(psi: np.ndarray, stretch: np.ndarray, stretchx: float):
+    """The 2-D field from the vector potential model_vectorpsi
+    (physics2d.f90:124-140): box = -ddz(psi, ibc 1), boz = DBYD1(psi,
+    ibc 0) * stretchx (the compact x derivative, not the FFT), and the curl
+    curlboy = ddz(box, ibc 1) - ddx(boz) (physics2d.f90:160-162).  psi is
+    (nx, 1, nz); returns box, boz, curlboy."""
+    psi = np.asarray(psi, dtype=np.float64)
+    nx, ny, nz = psi.shape
+    box = -ddz(psi, stretch, 1)
+    boz = np.empty_like(psi)
+    for j in range(ny):
+        boz[:, j, :] = dbyd1(psi[:, j, :], 0) * stretchx
+    curlboy = ddz(box, stretch, 1) - ddh(boz, 0, stretchx)
+    return box, boz, curlboy
 
 
 def magnetic_setup(tab: np.ndarray, fields: dict, nx: int, ny: int, option: int,

@@ -14,7 +14,7 @@ import numpy as np
 
 from . import abi
 from . import background as bgm
-from .background import M_CM, RSUN_R
+from .background import M_CM, RSUN_R, sp
 
 
 @dataclass
@@ -236,7 +236,8 @@ def load_setup(eng, S: bgm.Setup) -> None:
 def setup_for(W: Workload, c2_3d=None, rho0_3d=None) -> bgm.Setup:
     P = W.params
     if "tube" in W.extra:
-        fields = bgm.flux_tube(W.tab, P.nx, P.ny, P.xlength, P.ylength, **W.extra["tube"])
+        fields = bgm.flux_tube_vector_potential(W.tab, P.nx, P.ny, P.xlength, P.ylength,
+                                                **W.extra["tube"])
         if c2_3d is not None:
             fields["c"] = np.sqrt(c2_3d) * bgm.DIMC
         return bgm.magnetic_setup(W.tab, fields, P.nx, P.ny, P.option, P.xlength, P.ylength,
@@ -250,8 +251,8 @@ def forward_source(W: Workload, nt: int):
     (sigma 0.4 Mm) x Ricker (3 mHz); 2-D: the reference's sinc mask at the
     master pixel x = -5 Mm (data/master.pixels) x FORWARD_SOURCE."""
     P = W.params
-    if P.is2d:
-        mask = bgm.sinc_mask_2d(P.nx, P.xlength, -5.0)
+    if P.is2d:  # the source pixel of master.pixels (data/master.pixels holds -5 Mm)
+        mask = bgm.sinc_mask_2d(P.nx, P.xlength, W.extra.get("master_pixel_mm", -5.0))
         series = bgm.fwdsource_reference(nt, P.timestep)
     else:
         mask = bgm.gaussian_mask(P.nx, P.ny, P.xlength, P.ylength, W.src_xy_mm[0],
@@ -314,3 +315,35 @@ def make_engine(W: Workload, S: bgm.Setup, mode: int = abi.FORWARD, nsteps: int 
     eng = abi.Engine(cfg)
     load_setup(eng, S)
     return eng, cfg
+
+
+def reference_workload(params_text: str, data_dir: str, src: int, tab: np.ndarray,
+                       instruction: str | None = None, nsteps: int | None = None,
+                       precision: int | None = None):
+    """The reference's start-up from its own inputs (READ_PARAMETERS,
+    initialize.f90:964-1087; DETERMINE_STATUS, driver.f90:552-702):
+    params.i (as text) gives the grid, box, time step, PML, cadences and the
+    option (its switches and the Instruction file's 2D line); the Instruction
+    file gives the run mode; master.pixels in data_dir the source x of
+    source `src` (1-based, argv 1); params.N / filter.params.N in data_dir
+    the measurement windows and bands; `tab` is the background table named
+    by file_data.  Returns (workload, run mode, data directory).  nsteps
+    defaults to the reference's maxtime + 1 (solartime + 2/nupeak)."""
+    import os
+
+    from . import inputs as I
+    mode = I.read_instruction(instruction) if instruction else I.RunMode("forward", False)
+    v = I.parse_params_i(params_text)
+    P = I.params_from_reference(params_text, test_in_2d=mode.test_in_2d,
+                                precision=abi.FP64 if precision is None else precision)
+    if tab.shape[0] != P.nz:
+        raise ValueError(f"background table has {tab.shape[0]} rows, params.i nz = {P.nz}")
+    if nsteps is None:
+        sc = I.step_counts(float(v.get("solartime", 2.0)), float(v.get("nupeak", sp(0.0017))),
+                           P.timestep, P.cadforcing)
+        nsteps = sc.maxtime + 1
+    data = I.read_data_dir(data_dir)
+    x0 = I.master_pixel(os.path.join(data_dir, "master.pixels"), src)
+    W = Workload(f"reference_src{src:02d}", P, tab, nsteps,
+                 src_xy_mm=(x0, 0.0), extra={"master_pixel_mm": x0})
+    return W, mode, data

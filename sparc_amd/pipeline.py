@@ -96,6 +96,29 @@ def forward_solve(eng, spec: SolveSpec, store: bool = True, kstride: int | None 
     return traces
 
 
+def forward_planes(eng, spec: SolveSpec, store: bool = True) -> np.ndarray:
+    """The forward loop recording the whole observation plane: v_z(:,:,o_rad)
+    after every step with mod(time, steps) == 0, the vz_cc record of
+    TIMESTEPPING (driver.f90:455-476), as (nx, ny, nt); with store, the
+    partial state is kept at the same cadence."""
+    P = spec.W.params
+    steps = spec.steps
+    if store:
+        eng.snapshots_alloc(spec.maxtime // steps + 1)
+    out = np.zeros((P.nx, P.ny, spec.nt))
+    t = 0
+    for i in range(0, spec.maxtime + 1, steps):
+        t = _run_to(eng, t, i)
+        k = i // steps
+        if hasattr(eng, "plane"):
+            out[:, :, k] = eng.plane(abi.V_Z, eng.cfg.o_rad)
+        else:
+            out[:, :, k] = eng.get(abi.ARR_A, abi.V_Z)[:, :, eng.cfg.o_rad - 1]
+        if store:
+            eng.snapshot_store(k)
+    return out
+
+
 def adjoint_vr(spec: SolveSpec, adj: np.ndarray, S: bgm.Setup) -> np.ndarray:
     """vr(nx, ny, nt+8) of the adjoint run: the time-reversed adjoint source
     (adj(:,1,nt-j+1) = ..., driver.f90:876) at the receiver pixels, as a
@@ -266,6 +289,55 @@ def full_solve(spec: SolveSpec, S: bgm.Setup, obs: np.ndarray, meas: RingMeasure
         res = SolveResult(traces=syn, tau=tau, misfit=meas.misfit(tau), adj=adj, kernel_raw=raw,
                           K_lnc=K)
         return res, meas
+    finally:
+        fe.close()
+        ae.close()
+
+
+@dataclass
+class FilteredResult:
+    planes: np.ndarray      # (nx, ny, nt) synthetic v_z at the observation plane
+    meas: object            # measure_filt.FiltResult: tau per pixel, windows, adjoint source
+    kernel_raw: dict
+
+
+def filtered_solve(spec: SolveSpec, S: bgm.Setup, dat: np.ndarray, ridge: dict, bands: dict,
+                   src_mm: tuple, eng_factory=None, windows: dict | None = None,
+                   receivers: np.ndarray | None = None, t1_min: float | None = None
+                   ) -> FilteredResult:
+    """The reference's kernel-mode chain with the filtered measurement:
+    forward recording the observation plane, ADJOINT_SOURCE_FILT against the
+    data planes `dat` (ridge windows params.N, frequency bands
+    filter.params.N), and the adjoint run driven by that source as the
+    reference reads it back (vr(:,:,1:forcing_length) = source.fits,
+    driver.f90:653-656), with the kernel sums accumulated in the loop."""
+    from . import measure_filt as MF
+    P = spec.W.params
+    fe, ae = make_engines(spec, S, eng_factory)
+    try:
+        planes = forward_planes(fe, spec, store=True)
+        res = MF.adjoint_source_filt(planes, dat, spec.cad, P.xlength, P.ylength, src_mm,
+                                     ridge, bands, t1_min=t1_min, receivers=receivers,
+                                     windows=windows)
+        vr = np.zeros((P.nx, P.ny, spec.nt + 8))
+        vr[:, :, :spec.nt] = res.adj
+        if hasattr(ae, "kernel_accumulate_from"):
+            adjoint_solve(ae, spec, vr, S.g_raw, src=fe)
+        else:
+            copy_snapshots(ae, fe, spec.maxtime // spec.steps + 1)
+            adjoint_solve(ae, spec, vr, S.g_raw)
+        raw = {k: ae.kernel_get(k) for k in kernel_ids(ae.cfg)}
+        return FilteredResult(planes=planes, meas=res, kernel_raw=raw)
+    finally:
+        fe.close()
+        ae.close()
+
+
+def observe_planes(spec: SolveSpec, S: bgm.Setup, eng_factory=None) -> np.ndarray:
+    """Observation-plane v_z of a (true) model, every output cadence."""
+    fe, ae = make_engines(spec, S, eng_factory)
+    try:
+        return forward_planes(fe, spec, store=False)
     finally:
         fe.close()
         ae.close()

@@ -12,7 +12,7 @@ from sparc_amd import driver as D
 def tube():
     W = D.workload("tube")
     P = W.params
-    f = B.flux_tube(W.tab, P.nx, P.ny, P.xlength, P.ylength, **W.extra["tube"])
+    f = B.flux_tube_vector_potential(W.tab, P.nx, P.ny, P.xlength, P.ylength, **W.extra["tube"])
     S = D.setup_for(W)
     return W, f, S
 
