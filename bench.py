@@ -74,10 +74,8 @@ def main() -> int:
                          "ring) instead of the step benchmark")
     ap.add_argument("--damp2d", action="store_true",
                     help="the damping of 256x256 FP32 planes by k_damp2d, one pass per plane "
-                         "(sets SPARC_DAMP2D=1 for the engine)")
+                         "(cfg.plane_damping)")
     args = ap.parse_args()
-    if args.damp2d:
-        os.environ["SPARC_DAMP2D"] = "1"
     if args.copy_probe:
         return copy_probe(args.copy_probe)
     if args.serving:
@@ -102,6 +100,8 @@ def main() -> int:
 
     prec = abi.FP32 if args.precision == "fp32" else abi.FP64
     W = D.workload(args.workload, precision=prec)
+    if args.damp2d:
+        W.params.plane_damping = True
     P = W.params
     S = D.setup_for(W)
     total = args.warmup + args.steps

@@ -190,7 +190,11 @@ typedef struct sparc_config {
   int32_t use_graphs;     /* replay each step's five stages and update as a
                              captured HIP graph (forcing and spectral filter
                              stay eager); 0 = launch every kernel eagerly    */
-  int32_t reserved_i[5];
+  int32_t plane_damping;  /* option 4, FP32, 256 x 256 planes: DAMP_VELOCITY
+                             in one pass per plane (k_damp2d) instead of the
+                             forward x, y and inverse x passes; ignored
+                             elsewhere.  0 = the three passes (default)      */
+  int32_t reserved_i[4];
   double xlength, ylength;/* box sides, cm (params.i:27)                        */
   double timestep;        /* s (params.i:27)                                    */
   double deltat;          /* timestep*dimc/diml (initialize.f90:472)            */

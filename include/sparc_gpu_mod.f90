@@ -61,7 +61,8 @@ module sparc_gpu
     integer(c_int32_t) :: compensated
     integer(c_int32_t) :: flow_kernels
     integer(c_int32_t) :: use_graphs
-    integer(c_int32_t) :: reserved_i(5)
+    integer(c_int32_t) :: plane_damping
+    integer(c_int32_t) :: reserved_i(4)
     real(c_double) :: xlength, ylength
     real(c_double) :: timestep
     real(c_double) :: deltat

@@ -74,7 +74,8 @@ class SparcConfig(C.Structure):
         ("compensated", C.c_int32),
         ("flow_kernels", C.c_int32),
         ("use_graphs", C.c_int32),
-        ("reserved_i", C.c_int32 * 5),
+        ("plane_damping", C.c_int32),
+        ("reserved_i", C.c_int32 * 4),
         ("xlength", C.c_double), ("ylength", C.c_double),
         ("timestep", C.c_double),
         ("deltat", C.c_double),

@@ -54,126 +54,6 @@ __device__ __forceinline__ void load_tw2(C2<T>* tw2, const C2<T>* __restrict__ t
   }
 }
 
-// x-lines of length 256: 16 complex lines (32 real rows, or 16 row pairs of
-// a plane pair) per block of 256 threads, lane u = threadIdx % 16 of line
-// threadIdx / 16.  blockIdx.y selects the field.
-template <typename T, int MODE>
-__global__ __launch_bounds__(256) void k_x256(int nrows, int nlines, XArgs<T> X, T beta,
-                                              const T* __restrict__ mult,
-                                              const C2<T>* __restrict__ twg, int ny, int nz,
-                                              long nxy) {
-  constexpr int N = 256, LPB = 16, LSX = 16 * R16P;  // LSX = 16 (mod 32)
-  __shared__ C2<T> S[LPB * LSX];
-  __shared__ C2<T> tw2[256];
-  __shared__ T ml[256];
-  const int fld = blockIdx.y;
-  const T* __restrict__ a = X.a[fld];
-  const T* __restrict__ s = X.s[fld];
-  T* out = X.out[fld];
-  C2<T>* spec = X.spec[fld];
-  load_tw2(tw2, twg);
-  if (MODE == XM_DERIV || MODE == XM_FILTER)
-    for (int e = threadIdx.x; e < N; e += blockDim.x) ml[e] = mult[e];
-  const int l = threadIdx.x >> 4, u = threadIdx.x & 15;
-  const long L = (long)blockIdx.x * LPB + l;
-  const bool act = L < nlines;
-  C2<T>* Sl = S + l * LSX;
-  C2<T> v[16];
-  T ov0[MODE == XM_DAMP_INV ? 16 : 1], ov1[MODE == XM_DAMP_INV ? 16 : 1];
-  // plane-pair addressing of the damping modes
-  long o0 = 0;
-  bool w0 = false, w1 = false;
-  if (MODE == XM_DAMP_FWD || MODE == XM_DAMP_INV) {
-    const long q = L / ny, j = L - q * ny;
-    const long k0 = 2 * q, k1 = k0 + 1;
-    o0 = (long)N * j + nxy * k0;
-    if (MODE == XM_DAMP_FWD) {
-      w1 = k1 < nz;
-    } else {
-      w0 = X.edges || (k0 != 0 && k0 != nz - 1);
-      w1 = k1 < nz - 1 || (X.edges && k1 < nz);
-    }
-  }
-  const long r0 = 2 * L, r1 = r0 + 1;
-  const bool has1 = r1 < nrows;
-#pragma unroll
-  for (int n = 0; n < 16; ++n) {
-    const int x = 16 * n + u;
-    C2<T> z = {0, 0};
-    if (act) {
-      if (MODE == XM_DERIV || MODE == XM_FILTER) {
-        T re = a[r0 * N + x];
-        if (s) re = re + beta * s[r0 * N + x];
-        T im = 0;
-        if (has1) {
-          im = a[r1 * N + x];
-          if (s) im = im + beta * s[r1 * N + x];
-        }
-        z = {re, im};
-      } else if (MODE == XM_DAMP_FWD) {
-        T re = a[o0 + x];
-        if (s) re = re + beta * s[o0 + x];
-        T im = 0;
-        if (w1) {
-          im = a[o0 + nxy + x];
-          if (s) im = im + beta * s[o0 + nxy + x];
-        }
-        z = {re, im};
-      } else {  // XM_DAMP_INV: lane u loads X[u + 16 k2] and the RHS it updates
-        z = spec[L * N + u + 16 * n];
-        ov0[n] = w0 ? out[o0 + x] : (T)0;
-        ov1[n] = w1 ? out[o0 + nxy + x] : (T)0;
-      }
-    }
-    v[n] = z;
-  }
-  __syncthreads();  // tables
-  if (MODE != XM_DAMP_INV) {
-    dft_reg<T, 16, -1>(v);
-    twiddle16<T, -1>(v, tw2, u);
-    exchange16(v, Sl, u);
-    dft_reg<T, 16, -1>(v);  // v[k2] = X[u + 16 k2]
-    if (MODE == XM_DAMP_FWD) {
-      if (act) {
-#pragma unroll
-        for (int k = 0; k < 16; ++k) spec[L * N + u + 16 * k] = v[k];
-      }
-      return;
-    }
-#pragma unroll
-    for (int k = 0; k < 16; ++k) {
-      const T c = ml[u + 16 * k];
-      v[k] = (MODE == XM_DERIV) ? C2<T>{-c * v[k].y, c * v[k].x} : C2<T>{c * v[k].x, c * v[k].y};
-    }
-    __syncthreads();  // exchange tile reused
-  }
-  dft_reg<T, 16, +1>(v);
-  twiddle16<T, +1>(v, tw2, u);
-  exchange16(v, Sl, u);
-  dft_reg<T, 16, +1>(v);  // v[n2] = x[u + 16 n2]
-  if (!act) return;
-  if (MODE == XM_DERIV || MODE == XM_FILTER) {
-#pragma unroll
-    for (int n = 0; n < 16; ++n) {
-      const int x = u + 16 * n;
-      if (MODE == XM_DERIV && X.rhs) {  // row r = j + ny*k
-        out[r0 * N + x] = rhs_of(X, r0 / ny, r0 * N + x, nz, v[n].x);
-        if (has1) out[r1 * N + x] = rhs_of(X, r1 / ny, r1 * N + x, nz, v[n].y);
-      } else {
-        out[r0 * N + x] = v[n].x;
-        if (has1) out[r1 * N + x] = v[n].y;
-      }
-    }
-  } else {
-#pragma unroll
-    for (int n = 0; n < 16; ++n) {
-      const int x = u + 16 * n;
-      if (w0) out[o0 + x] = ov0[n] - v[n].x;
-      if (w1) out[o0 + nxy + x] = ov1[n] - v[n].y;
-    }
-  }
-}
-
 // y-lines of length 256 on one plane: 16 complex lines per block of 256
 // threads, lane w = threadIdx / 16 of line c = threadIdx % 16, so the 16
 // lines' lanes are adjacent in x.  DERIV/FILTER: line c packs columns
@@ -321,9 +201,6 @@ __device__ __forceinline__ void xw_swap(C2<T> (&v)[4], int lane) {
   }
 }
 
-#ifndef SPARC_XW_PL
-#define SPARC_XW_PL 3
-#endif
 // The same digit swaps for FP32 without the LDS permute: lane bits 5 and 4
 // by v_permlane32_swap / v_permlane16_swap (which swap the upper half of one
 // register's lane group with the lower half of the other's: exactly the
@@ -381,9 +258,9 @@ __device__ __forceinline__ void xw_swap_pl(C2<float> (&v)[4], int lane) {
 
 template <typename T, int B>
 __device__ __forceinline__ void xw_swap_any(C2<T> (&v)[4], int lane) {
-  // SPARC_XW_PL: 1 = lane bits 4, 5 by permlane swaps; 2 = lane bits 0-3
-  // by DPP; 3 = both
-  constexpr bool PL = sizeof(T) == 4 && (B == 16 ? (SPARC_XW_PL & 1) : (SPARC_XW_PL & 2));
+  // FP32: lane bits 4, 5 by permlane swaps, lane bits 0-3 by DPP; FP64 by
+  // the LDS-free shuffles of xw_swap
+  constexpr bool PL = sizeof(T) == 4;
   if constexpr (PL)
     xw_swap_pl<B>(reinterpret_cast<C2<float>(&)[4]>(v), lane);
   else

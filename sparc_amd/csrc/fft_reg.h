@@ -392,24 +392,18 @@ __global__ __launch_bounds__(NT) void k_xlines_n(int nrows, int nlines, XArgs<T>
 // a plane-pair spectrum (DAMP).  blockIdx.z selects the field.
 __host__ __device__ constexpr int pow2_floor(int v) { return v >= 2 ? 2 * pow2_floor(v / 2) : 1; }
 
-#ifndef SPARC_Y_TILE_BYTES
-#define SPARC_Y_TILE_BYTES 32768
-#endif
-#ifndef SPARC_Y_SEG_BYTES
-#define SPARC_Y_SEG_BYTES 128
-#endif
-#ifndef SPARC_YDAMP_NT
-#define SPARC_YDAMP_NT 512  // measured 112 vs 124 us with 256 at 256x256x300 FP32
-#endif
+// y tile of 32 KiB, rows read in 128-byte segments; the damping y pass takes
+// 512 threads (112 against 124 us with 256 at 256x256x300 FP32)
+constexpr int kYTileBytes = 32768, kYSegBytes = 128, kYDampNt = 512;
 template <typename T, int NY, int MODE>
 struct YCfg {
   // 32 KiB of complex values per block; row segments of 128 bytes at most;
   // derivative/filter tiles use 512 threads (measured 4.7 vs 3.7 TB/s with 256)
-  static constexpr int LINES = pow2_floor((SPARC_Y_TILE_BYTES / (int)sizeof(C2<T>)) / NY);
-  static constexpr int CAP = SPARC_Y_SEG_BYTES / (int)sizeof(C2<T>);  // complex lines per row segment
+  static constexpr int LINES = pow2_floor((kYTileBytes / (int)sizeof(C2<T>)) / NY);
+  static constexpr int CAP = kYSegBytes / (int)sizeof(C2<T>);  // complex lines per row segment
   static constexpr int NL = LINES < CAP ? LINES : CAP;
   static constexpr int TX = (MODE == YM_DAMP) ? NL : 2 * NL;
-  static constexpr int NT = MODE == YM_DAMP ? SPARC_YDAMP_NT : (NL * NY < 4096 ? 256 : 512);
+  static constexpr int NT = MODE == YM_DAMP ? kYDampNt : (NL * NY < 4096 ? 256 : 512);
 };
 
 template <typename T, int NY, int MODE, int NL = YCfg<T, NY, MODE>::NL,

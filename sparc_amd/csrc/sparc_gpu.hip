@@ -18,26 +18,13 @@
 #include "kernels.h"
 
 // Exact segmented z solve (k_zsolve_b: input and sweep results in LDS) for the
-// epilogues whose bit is set in SPARC_ZB_EPIS (bit EPI_*), when 2 x 64 x nz
-// values fit one block's LDS; otherwise the warm-up variants of k_zsolve.
-#ifndef SPARC_ZB_EPIS
-#define SPARC_ZB_EPIS 0x2a
-#endif
-#ifndef SPARC_ZB_NSEG
-#define SPARC_ZB_NSEG 16
-#endif
-// z-solve variant: ud in LDS (1) or in a global scratch array (0), and the
-// segments per column of the LDS variant.
-#ifndef SPARC_ZS_ULDS
-#define SPARC_ZS_ULDS 1
-#endif
-#ifndef SPARC_ZS_ULDS_NSEG
-#define SPARC_ZS_ULDS_NSEG 8
-#endif
-// FP64 takes 8 segments (512 threads): at 1024 threads its epilogues spill
-#ifndef SPARC_ZB_NSEG64
-#define SPARC_ZB_NSEG64 8
-#endif
+// epilogues that solve their own field (EPI_P3D, EPI_P2D, EPI_C2: bits of
+// kZbEpis), when 2 x 64 x nz values fit one block's LDS; otherwise k_zsolve.
+static constexpr int kZbEpis =
+    (1 << sparc::EPI_P3D) | (1 << sparc::EPI_P2D) | (1 << sparc::EPI_C2);
+// segments per column: k_zsolve_b 16 in FP32, 8 in FP64 (at 1024 threads its
+// epilogues spill); k_zsolve with ud in LDS 8
+static constexpr int kZbNseg = 16, kZbNseg64 = 8, kZsUldsNseg = 8;
 
 using namespace sparc;
 
@@ -86,8 +73,7 @@ struct sparc_ctx {
   void* rates_o = nullptr;  // damping rates as given: [m + (nx/2+1)*y]
   bool damping_set = false, filter_set = false;
   // option 4 on 256 x 256 FP32 planes: the damping by k_damp2d, one pass per
-  // plane, instead of the forward x, y and inverse x passes (SPARC_DAMP2D=1
-  // in the environment at sparc_create)
+  // plane, instead of the forward x, y and inverse x passes (cfg.plane_damping)
   bool damp2d = false;
   void* tri[4] = {};  // per ibc 0..3: lowr | invb | gam | fcum, gcum (2 partitions), nz each
   double tc[14] = {};
@@ -189,16 +175,14 @@ static std::mutex g_capture_mu;
 
 // Large allocations start on 2 MiB boundaries, so the same index of two
 // field arrays shares its low address bits.  The n-th allocation is moved
-// (n mod 16) * SPARC_STAGGER bytes past its start (a multiple of 256 keeps
-// every alignment the kernels use); allocs keeps the pointer hipMalloc gave.
-#ifndef SPARC_STAGGER
-#define SPARC_STAGGER 2304
-#endif
+// (n mod 16) * kStagger bytes past its start (a multiple of 256 keeps every
+// alignment the kernels use); allocs keeps the pointer hipMalloc gave.
+static constexpr size_t kStagger = 2304;
 static int dalloc(sparc_ctx* c, void** p, size_t bytes) {
-  static_assert(SPARC_STAGGER % 256 == 0, "stagger keeps 256-byte alignment");
+  static_assert(kStagger % 256 == 0, "stagger keeps 256-byte alignment");
   *p = nullptr;
   if (bytes == 0) bytes = 16;
-  const size_t off = (size_t)(c->allocs.size() % 16) * SPARC_STAGGER;
+  const size_t off = (size_t)(c->allocs.size() % 16) * kStagger;
   void* raw = nullptr;
   hipError_t e;
   {
@@ -363,7 +347,7 @@ static size_t footprint_bytes(const sparc_config* cfg) {
     b += 3 * spec_elems(cfg) * 2 * ts;      // damping spectra
   b += (size_t)(cfg->nx + cfg->ny) * 6 * ts + 4 * 7 * cfg->nz * ts + SPARC_PROF_COUNT * cfg->nz * ts;
   b += 2 * (size_t)(cfg->nx / 2 + 1) * cfg->ny * ts + 2 * nxy * ts;  // rates (two layouts)
-  b += 128 * 15 * (size_t)SPARC_STAGGER;  // allocation stagger, up to 128 allocations
+  b += 128 * 15 * kStagger;  // allocation stagger, up to 128 allocations
   return b;
 }
 
@@ -434,7 +418,7 @@ extern "C" int sparc_create(const sparc_config* cfg, sparc_ctx** out) {
   *out = nullptr;
   sparc_ctx* c = new sparc_ctx();
   c->cfg = *cfg;
-  if (const char* e = std::getenv("SPARC_DAMP2D")) c->damp2d = std::atoi(e) != 0;
+  c->damp2d = cfg->plane_damping != 0;
   if (c->cfg.field_scale == 0.0) c->cfg.field_scale = 1.0;
   std::string why;
   if (check_config(cfg, &why) != SPARC_OK) {
@@ -529,7 +513,7 @@ extern "C" int sparc_create(const sparc_config* cfg, sparc_ctx** out) {
     // into nseg segments of L rows: fcum[k] = prod_{i=first..k} (-lowr_i
     // invb_i), gcum[k] = prod_{i=k+1..end} (-gam_i), 0 in the last segment;
     // rows 3-4 for k_zsolve_b, rows 5-6 for the exact mode of k_zsolve
-    // (SPARC_ZS_ULDS_NSEG segments)
+    // (kZsUldsNseg segments)
     auto coupling = [&](int nseg, double* fc, double* gc) {
       const int n = cfg->nz, L = (n + nseg - 1) / nseg;
       for (int a = 0; a < n; a += L) {
@@ -540,9 +524,9 @@ extern "C" int sparc_create(const sparc_config* cfg, sparc_ctx** out) {
         for (int k = b - 1; k >= a; --k) gc[k] = (b == n) ? 0.0 : (p *= -gam[k + 1]);
       }
     };
-    coupling(c->fp32 ? SPARC_ZB_NSEG : SPARC_ZB_NSEG64, t.data() + 3 * cfg->nz,
+    coupling(c->fp32 ? kZbNseg : kZbNseg64, t.data() + 3 * cfg->nz,
              t.data() + 4 * cfg->nz);
-    coupling(SPARC_ZS_ULDS_NSEG, t.data() + 5 * cfg->nz, t.data() + 6 * cfg->nz);
+    coupling(kZsUldsNseg, t.data() + 5 * cfg->nz, t.data() + 6 * cfg->nz);
     if ((rc = dalloc(c, &c->tri[ibc], 7 * cfg->nz * ts))) return rc;
     if ((rc = upload(c, c->tri[ibc], t.data(), 7 * cfg->nz))) return rc;
     // the widest range around the middle row where lowr, invb and gam (as
@@ -858,66 +842,24 @@ static bool fast_size(int n) {
          n == 512;
 }
 
-#ifndef SPARC_XDAMP_LPB_DIV
-#define SPARC_XDAMP_LPB_DIV 1
-#endif
-// Length-256 lines use the register four-step kernels of fft_r16.h for the
-// y passes and the damping inverse x pass (x derivatives: k_xw256 below).  Measured at 256x256x300 FP32 (us
-// per launch, radix-16 Stockham of fft_reg.h -> four-step): y derivative
-// 59.3 -> 50.9, damping y 108.1 -> 97.9, damping inverse x 131.9 -> 125.0;
-// but x derivative 65.1 -> 78.6 and damping forward x 99.5 -> 124.4 (lanes
-// owning every 16th element read 64-byte pieces of a row instead of whole
-// rows), so those keep the Stockham kernels.
-#ifndef SPARC_R16
-#define SPARC_R16 1
-#endif
-
-// x derivatives of length 256: one wavefront per row pair with cross-lane
-// shuffles (k_xw256), SPARC_XW_LPW row pairs per wavefront.  Measured at
-// 256x256x300 FP32, us per launch averaged over the xi_x and p derivatives:
-// Stockham 67.6, k_xw256 56.4 (1 or 2 pairs per wavefront; 4 pairs 62.5,
-// 8 pairs 63.1).  DPP and v_permlane16/32_swap exchanges in place of the
-// shuffles measured the same (57.8): the kernel is not bound by them.  The
-// damping forward x pass also runs on it (SPARC_XW_DF): 103.3 -> 94.9 us,
-// and the damping inverse (SPARC_XW_DI, in place of k_x256): 125.4 -> 121.7.
-// With the loads ordered, 1 / 2 / 4 lines per wavefront measured 51.5 /
-// 50.0 / 55.7 us (derivative), 110.9 / 98.6 / 94.9 (damping forward) and
-// 137.6 / 122.8 / 126.8 (damping inverse): the forward damping pass takes 4.
-#ifndef SPARC_XW
-#define SPARC_XW 1
-#endif
-#ifndef SPARC_XW_LPW
-#define SPARC_XW_LPW 2
-#endif
-#ifndef SPARC_XW_DF
-#define SPARC_XW_DF 1
-#endif
-#ifndef SPARC_XW_DI
-#define SPARC_XW_DI 1
-#endif
-
+// Length-256 lines: the x derivatives, the damping forward x pass and the
+// damping inverse x pass run one wavefront per line with cross-lane shuffles
+// (k_xw256); the y passes run the register four-step k_y256.  Row pairs per
+// wavefront: 2 for the derivatives and the damping inverse, 4 for the
+// damping forward pass.  Other sizes take the Stockham kernels of
+// fft_reg.h.  Measured variants are listed in the design notes.
+static constexpr int kXwLpw = 2;
 template <typename T, int MODE, int N>
 static void launch_xn(sparc_ctx* c, int nf, const XArgs<T>& X, T beta, const T* mult, int nrows,
                       int nlines) {
-  if constexpr (N == 256 && SPARC_XW &&
-                (MODE == XM_DERIV || (SPARC_XW_DF && MODE == XM_DAMP_FWD) ||
-                 (SPARC_XW_DI && MODE == XM_DAMP_INV))) {
-    constexpr int LPW = MODE == XM_DAMP_FWD ? 2 * SPARC_XW_LPW : SPARC_XW_LPW;
+  if constexpr (N == 256 && (MODE == XM_DERIV || MODE == XM_DAMP_FWD || MODE == XM_DAMP_INV)) {
+    constexpr int LPW = MODE == XM_DAMP_FWD ? 2 * kXwLpw : kXwLpw;
     hipLaunchKernelGGL((k_xw256<T, MODE, LPW>), dim3(cdiv(nlines, 4 * LPW), nf), dim3(256), 0,
                        c->stream, nrows, nlines, X, beta, mult, (const C2<T>*)c->twx,
                        c->cfg.ny, c->cfg.nz, c->nxy);
     return;
   }
-  if constexpr (N == 256 && SPARC_R16 && MODE == XM_DAMP_INV) {
-    hipLaunchKernelGGL((k_x256<T, MODE>), dim3(cdiv(nlines, 16), nf), dim3(256), 0, c->stream,
-                       nrows, nlines, X, beta, mult, (const C2<T>*)c->twx, c->cfg.ny, c->cfg.nz,
-                       c->nxy);
-    return;
-  }
-  constexpr int LPB0 = XCfg<T, N>::LPB;
-  constexpr int LPB = (MODE == XM_DAMP_INV || MODE == XM_DAMP_FWD)
-                          ? (LPB0 / SPARC_XDAMP_LPB_DIV > 0 ? LPB0 / SPARC_XDAMP_LPB_DIV : 1)
-                          : LPB0;
+  constexpr int LPB = XCfg<T, N>::LPB;
   hipLaunchKernelGGL((k_xlines_n<T, N, MODE, LPB>), dim3(cdiv(nlines, LPB), nf), dim3(XCfg<T, N>::NT), 0,
                      c->stream, nrows, nlines, X, beta, mult, (const C2<T>*)c->twx, c->cfg.ny,
                      c->cfg.nz, c->nxy);
@@ -976,15 +918,6 @@ static int launch_xm(sparc_ctx* c, int nf, const XArgs<T>& X, T beta, const T* m
   return SPARC_OK;
 }
 
-// x derivative of p as the momentum RHS with the damping inverse of the
-// same component subtracted (k_xw256 XM_DERIV_DAMPINV): lines of 256 only
-#ifndef SPARC_XW_FUSE
-#define SPARC_XW_FUSE 1
-#endif
-// option 10: curl pieces combined as the line kernels produce them
-#ifndef SPARC_MHD_BACC
-#define SPARC_MHD_BACC 1
-#endif
 // the damping of three fields, one 256 x 256 plane per block (X.a the
 // velocities, X.out the RHS they are subtracted from)
 template <typename T>
@@ -1009,8 +942,8 @@ static int launch_x_rhs_damp(sparc_ctx* c, const XArgs<T>& X, const T* mult) {
   const long nrows = (long)c->cfg.ny * c->cfg.nz;
   // p, the damping spectrum, the RHS written
   ProfScope ps(c, SPARC_KC_XLINES, 3.0 * (double)c->N * sizeof(T));
-  hipLaunchKernelGGL((k_xw256<T, XM_DERIV_DAMPINV, SPARC_XW_LPW>),
-                     dim3(cdiv(nlines, 4 * SPARC_XW_LPW), 1), dim3(256), 0, c->stream, (int)nrows,
+  hipLaunchKernelGGL((k_xw256<T, XM_DERIV_DAMPINV, kXwLpw>),
+                     dim3(cdiv(nlines, 4 * kXwLpw), 1), dim3(256), 0, c->stream, (int)nrows,
                      (int)nlines, X, (T)0, mult, (const C2<T>*)c->twx, c->cfg.ny, c->cfg.nz,
                      c->nxy);
   LAUNCHCHK(c);
@@ -1032,7 +965,7 @@ template <typename T, int MODE, int NY>
 static void launch_yn(sparc_ctx* c, int nf, const XArgs<T>& X, T beta, const T* mult) {
   constexpr int TX = YCfg<T, NY, MODE>::TX;
   const int planes = (MODE == YM_DAMP) ? (c->cfg.nz + 1) / 2 : c->cfg.nz;
-  if constexpr (NY == 256 && SPARC_R16) {
+  if constexpr (NY == 256) {
     const int bx = MODE == YM_DAMP ? 16 : 32;
     if (c->cfg.nx % bx == 0) {
       hipLaunchKernelGGL((k_y256<T, MODE>), dim3(c->cfg.nx / bx, planes, nf), dim3(256), 0,
@@ -1102,10 +1035,8 @@ static int launch_y(sparc_ctx* c, const T* a, const T* s, T beta, T* out, const 
   return launch_ym<T, MODE>(c, 1, X, beta, mult);
 }
 
-// z-solve variants (SPARC_ZS_ULDS, SPARC_ZS_ULDS_NSEG above).  Measured at
-// 256x256x300 FP32 on MI355X (us per launch, momentum / pressure epilogue):
-// global ud with 4 segments 280 / 174, LDS with 4 segments 280 / 215 (two
-// blocks of four waves per CU), LDS with 8 segments 260 / 168.
+// z-solve variants: k_zsolve_b for the epilogues in kZbEpis, k_zsolve with
+// ud in LDS when it fits, else k_zsolve with a global ud and warm-ups.
 
 template <typename T, int EPI>
 static int launch_z(sparc_ctx* c, const Fld<T>& F, int ibc, const T* inA, const T* inS, T beta,
@@ -1142,8 +1073,8 @@ static int launch_z(sparc_ctx* c, const Fld<T>& F, int ibc, const T* inA, const 
   const TriCoef<T> tcv = make_tc<T>(c, ibc);
   const dim3 grid(cdiv(c->nxy, 64));
   const size_t lds_b = 2 * 64 * (size_t)nz * sizeof(T);
-  if (((SPARC_ZB_EPIS >> EPI) & 1) && nz >= 5 && lds_b <= (size_t)c->lds_max) {
-    constexpr int NS = sizeof(T) == 4 ? SPARC_ZB_NSEG : SPARC_ZB_NSEG64;
+  if (((kZbEpis >> EPI) & 1) && nz >= 5 && lds_b <= (size_t)c->lds_max) {
+    constexpr int NS = sizeof(T) == 4 ? kZbNseg : kZbNseg64;
     const bool bg3 = c->bg3d[SPARC_BG_C2RHO0] || c->bg3d[SPARC_BG_GRADP0_Z] ||
                      c->bg3d[SPARC_BG_RHO0] || c->bg3d[SPARC_BG_GRADRHO0_Z];
 #define ZB(G)                                                                                    \
@@ -1162,16 +1093,15 @@ static int launch_z(sparc_ctx* c, const Fld<T>& F, int ibc, const T* inA, const 
   // exact coupling any segment of at least 3 planes works, the warm-up
   // needs long columns (nz >= 192)
   const size_t lds_ud = 64 * (size_t)nz * sizeof(T);
-  const bool seg_ok = SPARC_ZS_EXACT ? (nz + SPARC_ZS_ULDS_NSEG - 1) / SPARC_ZS_ULDS_NSEG >= 3
-                                     : nseg == 4;
-  const bool ulds = SPARC_ZS_ULDS && seg_ok && lds_ud <= 81920 && lds_ud <= (size_t)c->lds_max;
+  const bool seg_ok = (nz + kZsUldsNseg - 1) / kZsUldsNseg >= 3;
+  const bool ulds = seg_ok && lds_ud <= 81920 && lds_ud <= (size_t)c->lds_max;
   const size_t lds = ulds ? lds_ud : 5 * (size_t)nz * sizeof(T);
 #define ZL(NS, U)                                                                                 \
   hipLaunchKernelGGL((k_zsolve<T, EPI, NS, U>), grid, dim3(64 * NS), lds, c->stream, F, tcv, tri, \
                      tri + nz, tri + 2 * nz, ibc, inA, inS, beta, (T*)c->W[7], prov, provscale,     \
                      out, tri + 5 * nz, tri + 6 * nz)
   if (ulds)
-    ZL(SPARC_ZS_ULDS_NSEG, true);
+    ZL(kZsUldsNseg, true);
   else if (nseg == 4)
     ZL(4, false);
   else if (nseg == 2)
@@ -1249,7 +1179,7 @@ static int run_stage(sparc_ctx* c, Fld<T>& F, int s) {
     // FP32 on 256 x 256 planes: the whole damping in one pass per plane
     const bool d2d = damp && c->damp2d && sizeof(T) == 4 && c->cfg.nx == 256 &&
                      c->cfg.ny == 256;
-    const bool fuse = damp && !d2d && SPARC_XW && SPARC_XW_FUSE && c->cfg.nx == 256;
+    const bool fuse = damp && !d2d && c->cfg.nx == 256;
     XArgs<T> G{};
     G.a[0] = F.W[3];
     G.rhs = 1;
@@ -1337,7 +1267,7 @@ static int run_stage(sparc_ctx* c, Fld<T>& F, int s) {
     // and k_mhd_rhs reads them through a remapped table.  Otherwise the six
     // pieces go to M6..M11 and k_mhd_b combines them into M0..M2.
     Fld<T> FR = F;  // the arrays k_mhd_rhs reads as M0..M11
-    const bool bacc = SPARC_MHD_BACC && fast_size(c->cfg.nx) && fast_size(c->cfg.ny);
+    const bool bacc = fast_size(c->cfg.nx) && fast_size(c->cfg.ny);
     auto xacc = [&](const T* in, T* out, T ald, T gout) -> int {
       XArgs<T> X{};
       X.a[0] = in;

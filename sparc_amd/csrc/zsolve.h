@@ -256,7 +256,7 @@ struct ZWarm {
 // ULDS: the forward-sweep result ud lives in LDS (64 x nz values per block)
 // instead of a global scratch array, and the per-plane tables are read with
 // wave-uniform (scalar) loads to leave the LDS to ud.  With ud in LDS the
-// segments are coupled exactly instead of by warm-up (SPARC_ZS_EXACT): the
+// segments are coupled exactly instead of by warm-up: the
 // forward sweep starts from u = 0 at its own first row, the true u on the
 // previous segment's last row is chained from the segment ends (fcum), the
 // local backward sweep runs in place in LDS, and the epilogue pass adds the
@@ -274,30 +274,16 @@ __global__ __launch_bounds__(64 * NSEG) __attribute__((amdgpu_waves_per_eu(4))) 
                                                       T* __restrict__ out,
                                                       const T* __restrict__ fcum,
                                                       const T* __restrict__ gcum) {
-#ifndef SPARC_ZS_UF
-#define SPARC_ZS_UF 8
-#endif
-#ifndef SPARC_ZS_UB
-#define SPARC_ZS_UB 3
-#endif
-#ifndef SPARC_ZS_UF64
-#define SPARC_ZS_UF64 8
-#endif
-#ifndef SPARC_ZS_EXACT
-#define SPARC_ZS_EXACT 1
-#endif
-#ifndef SPARC_ZS_UB64
-#define SPARC_ZS_UB64 1
-#endif
-  // forward chunk (planes loaded one chunk ahead) and backward chunk; FP64
-  // values take two registers each, so its chunks are shorter, and the
-  // global-ud variants keep 2 planes (3 spill there)
-  constexpr int UF = sizeof(T) == 4 ? SPARC_ZS_UF : SPARC_ZS_UF64;
-  constexpr int UB = sizeof(T) == 4 ? (ULDS ? SPARC_ZS_UB : 2) : SPARC_ZS_UB64;
+  // forward chunk (planes loaded one chunk ahead: 8) and backward chunk (3
+  // with ud in LDS; FP64 values take two registers each, so 1, and the
+  // global-ud variants keep 2 planes: 3 spill there)
+  constexpr int UF = 8;
+  constexpr int UB = sizeof(T) == 4 ? (ULDS ? 3 : 2) : 1;
   constexpr int ZW = ZWarm<T>::W;
-  // EX: segments coupled exactly (the products fcum, gcum of this partition)
-  // instead of by warm-up; needs ud in LDS and at least 3 planes per segment
-  constexpr bool EX = ULDS && SPARC_ZS_EXACT;
+  // EX: with ud in LDS the segments are coupled exactly (the products fcum,
+  // gcum of this partition, at least 3 planes per segment); the global-ud
+  // variants start each segment by a warm-up
+  constexpr bool EX = ULDS;
   extern __shared__ unsigned char zsm[];
   const int n = F.nz;
   constexpr bool HAS_G = (EPI == EPI_RHS3D || EPI == EPI_RHS2D || EPI == EPI_V2);
@@ -530,21 +516,6 @@ __global__ __launch_bounds__(64 * NSEG) __attribute__((amdgpu_waves_per_eu(4))) 
 //       last segment); epilogue.
 // In exact arithmetic this equals the sequential LU solve.  BKEEP epilogues
 // take the solved field's stage temp from the LDS copy of B.
-#ifndef SPARC_ZB_UB
-#define SPARC_ZB_UB 3
-#endif
-#ifndef SPARC_ZB_UBP
-#define SPARC_ZB_UBP 14
-#endif
-#ifndef SPARC_ZB_UBP3
-#define SPARC_ZB_UBP3 10
-#endif
-#ifndef SPARC_ZB_U1
-#define SPARC_ZB_U1 20
-#endif
-#ifndef SPARC_ZB_P4U
-#define SPARC_ZB_P4U 4
-#endif
 template <typename T, int EPI, int NSEG, bool NOBG3>
 __global__ __launch_bounds__(64 * NSEG) void k_zsolve_b(
     Fld<T> F, TriCoef<T> tc, const T* __restrict__ lowr, const T* __restrict__ invb,
@@ -553,9 +524,10 @@ __global__ __launch_bounds__(64 * NSEG) void k_zsolve_b(
     T provscale, T* __restrict__ out) {
   constexpr bool HAS_G = (EPI == EPI_RHS3D || EPI == EPI_RHS2D || EPI == EPI_V2);
   constexpr bool BKEEP = (EPI == EPI_P3D || EPI == EPI_P2D || EPI == EPI_C2);
-  constexpr int U1 = SPARC_ZB_U1;                       // P1 planes in flight
-  // P5 planes in flight (bounded by 128 VGPRs at 16 waves per CU)
-  constexpr int UB = BKEEP ? (NOBG3 ? SPARC_ZB_UBP : SPARC_ZB_UBP3) : SPARC_ZB_UB;
+  constexpr int U1 = 20;  // P1 planes in flight
+  // P5 planes in flight (bounded by 128 VGPRs at 16 waves per CU): 14 for the
+  // pressure epilogue, 10 when it also reads 3-D backgrounds, 3 otherwise
+  constexpr int UB = BKEEP ? (NOBG3 ? 14 : 10) : 3;
   extern __shared__ unsigned char zsm[];
   const int n = F.nz;
   T* sB = reinterpret_cast<T*>(zsm);  // B[k * 64 + lane]
@@ -679,7 +651,7 @@ __global__ __launch_bounds__(64 * NSEG) void k_zsolve_b(
   if (ks < ke) {
     T x = sX[(ke - 1) * 64 + lane] + fcum[ke - 1] * U;
     sX[(ke - 1) * 64 + lane] = x;
-#pragma unroll SPARC_ZB_P4U
+#pragma unroll 4
     for (int k = ke - 2; k >= ks; --k) {
       const T u = sX[k * 64 + lane] + fcum[k] * U;
       x = u - gam[k + 1] * x;

@@ -39,6 +39,7 @@ class Params:
     compensated: bool = True       # FP32: hi+lo state updated in FP64 arithmetic
     flow_kernels: bool = False     # snapshots keep v; kernel sums add K_v (kernels.f90:317-332)
     use_graphs: bool = True        # replay each step's stages + update as a captured HIP graph
+    plane_damping: bool = False    # option 4, FP32, 256 x 256 planes: one-pass damping (k_damp2d)
 
     @property
     def is2d(self) -> bool:
@@ -205,6 +206,7 @@ def make_config(P: Params, S: bgm.Setup, mode: int, nsteps_total: int) -> abi.Sp
     c.compensated = int(P.compensated and P.precision == abi.FP32)
     c.flow_kernels = int(P.flow_kernels)
     c.use_graphs = int(P.use_graphs)
+    c.plane_damping = int(P.plane_damping)
     return c
 
 

@@ -303,14 +303,16 @@ class FilteredResult:
 
 def filtered_solve(spec: SolveSpec, S: bgm.Setup, dat: np.ndarray, ridge: dict, bands: dict,
                    src_mm: tuple, eng_factory=None, windows: dict | None = None,
-                   receivers: np.ndarray | None = None, t1_min: float | None = None
-                   ) -> FilteredResult:
+                   receivers: np.ndarray | None = None, t1_min: float | None = None,
+                   adj: np.ndarray | None = None) -> FilteredResult:
     """The reference's kernel-mode chain with the filtered measurement:
     forward recording the observation plane, ADJOINT_SOURCE_FILT against the
     data planes `dat` (ridge windows params.N, frequency bands
     filter.params.N), and the adjoint run driven by that source as the
     reference reads it back (vr(:,:,1:forcing_length) = source.fits,
-    driver.f90:653-656), with the kernel sums accumulated in the loop."""
+    driver.f90:653-656), with the kernel sums accumulated in the loop.
+    `adj` (nx, ny, nt) replaces the measured source in the adjoint run (the
+    measurement is still made and returned)."""
     from . import measure_filt as MF
     P = spec.W.params
     fe, ae = make_engines(spec, S, eng_factory)
@@ -320,7 +322,7 @@ def filtered_solve(spec: SolveSpec, S: bgm.Setup, dat: np.ndarray, ridge: dict, 
                                      ridge, bands, t1_min=t1_min, receivers=receivers,
                                      windows=windows)
         vr = np.zeros((P.nx, P.ny, spec.nt + 8))
-        vr[:, :, :spec.nt] = res.adj
+        vr[:, :, :spec.nt] = res.adj if adj is None else adj
         if hasattr(ae, "kernel_accumulate_from"):
             adjoint_solve(ae, spec, vr, S.g_raw, src=fe)
         else:
