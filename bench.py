@@ -36,11 +36,12 @@ HBM_PEAK_GBS = 8000.0
 KERNEL_SYMBOLS = {
     "xlines": "sparc::k_xw256<float, 0 (XM_DERIV) | 4 (XM_DERIV_DAMPINV), 2> -- x derivative, one wavefront per line",
     "ylines": "sparc::k_y256<float, 0 (YM_DERIV)> -- y derivative, register four-step",
-    "damp": "sparc::k_xw256<float, 2 (XM_DAMP_FWD), 4> -- damping, forward x "
+    "damp": "sparc::k_xw256<float, 2 (XM_DAMP_FWD), 4> -- damping, forward x of the stage temp "
+            "a + beta scr of the velocities, also stored as RHS_xi "
             "(with --damp2d: sparc::k_damp2d<float>, the whole damping, one 256x256 plane per block)",
     "damp_y": "sparc::k_y256<float, 2 (YM_DAMP)> -- damping, y pass",
     "damp_xi": "sparc::k_xw256<float, 3 (XM_DAMP_INV), 2> -- damping, inverse x (v_y, v_z; v_x in XM_DERIV_DAMPINV)",
-    "zsolve": "sparc::k_zsolve<float, 2 (EPI_RHS3D), 8, true> -- z solve (ud in LDS) + momentum/continuity RHS",
+    "zsolve": "sparc::k_zsolve<float, 7 (EPI_RHS3N), 8, true> -- z solve of p (ud in LDS) + RHS_vz",
     "zsolve_p": "sparc::k_zsolve_b<float, 1 (EPI_P3D), 16, true> -- exact segmented z solve (column in LDS) + pressure/density",
     "update": "sparc::k_update_filter_c -- a += dt*scr with the z filter (hi+lo state), all fields per launch",
 }
@@ -142,8 +143,14 @@ def main() -> int:
         with open(args.traffic_json) as f:
             tj = json.load(f)
         ent = tj.get(f"{W.name}:{args.precision}:{dom}")
-        if ent:
+        # counter bytes of the same kernel only: an entry whose algorithmic
+        # bytes differ from this run's was measured on other code
+        if ent and abs(ent["algorithmic_bytes_per_launch"] * nl / by - 1.0) < 0.01:
             traffic = ent["hbm_bytes_per_launch"]
+        elif ent:
+            print(f"[bench] {args.traffic_json} entry for {dom} is stale "
+                  f"({ent['algorithmic_bytes_per_launch']:.4g} vs {by / nl:.4g} algorithmic bytes): "
+                  "traffic not reported", file=sys.stderr, flush=True)
     except (OSError, ValueError):
         pass
     roof = {"bound": "hbm", "kernel": dom, "kernel_symbol": KERNEL_SYMBOLS.get(dom, dom),

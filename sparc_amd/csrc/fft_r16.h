@@ -312,7 +312,8 @@ __device__ __forceinline__ void xw_inverse(C2<T> (&v)[4], const C2<T> (&w1)[3],
 // Same contracts as k_xlines_n<T, 256, MODE> for MODE = XM_DERIV (line L =
 // rows 2L, 2L+1, x fastest; out = i k mult(k) applied, or the momentum RHS
 // when X.rhs), XM_DAMP_FWD (line L = row j of planes 2q, 2q+1, L = j +
-// ny q; spec[L N + k] = the forward transform) and XM_DAMP_INV (the same
+// ny q; spec[L N + k] = the forward transform of the stage temp a + beta s,
+// which is also stored to X.cp when set) and XM_DAMP_INV (the same
 // rows of out minus the inverse transform of spec, planes 1 and nz skipped
 // unless X.edges).  XM_DERIV_DAMPINV takes the damping lines (row j of
 // planes 2q, 2q+1) of the input X.a (no stage temp) and writes
@@ -430,6 +431,19 @@ __global__ __launch_bounds__(256) void k_xw256(int nrows, int nlines, XArgs<T> X
       }
     }
     if (p + 1 < LPW && L + 1 < nlines) XW_LOAD(L + 1);
+    if constexpr (MODE == XM_DAMP_FWD) {
+      T* __restrict__ cp = X.cp[fld];
+      if (cp) {
+        long o, st;
+        bool h1;
+        rows(L, o, st, h1);
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+          cp[o + lane + 64 * j] = v[j].x;
+          if (h1) cp[o + st + lane + 64 * j] = v[j].y;
+        }
+      }
+    }
     // DD: rows k0, k1 of the pair, which of them take the RHS (not planes 1
     // and nz) and the damping, and rhoinv as a z profile.  When both rows
     // take the damping and rhoinv is a profile (constant along the line),

@@ -241,6 +241,9 @@ struct XArgs {
   // derivative combined into the output (curl assembly): out = ald d + gout out
   int acc;
   T ald, gout;
+  // damping forward pass of k_xw256: the stage temp a + beta s it transforms
+  // is also stored here (RHS_xi = temp v), when set
+  T* cp[3];
 };
 
 // plain derivative store: d, or combined with what out holds (X.acc)

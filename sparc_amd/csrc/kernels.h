@@ -12,7 +12,12 @@
 
 namespace sparc {
 
-enum { EPI_STORE = 0, EPI_P3D = 1, EPI_RHS3D = 2, EPI_P2D = 3, EPI_RHS2D = 4, EPI_C2 = 5, EPI_V2 = 6 };
+// EPI_RHS3N: EPI_RHS3D without RHS_xi = temp v on the interior planes, which
+// the damping forward x pass (k_xw256, XArgs::cp) writes instead
+enum {
+  EPI_STORE = 0, EPI_P3D = 1, EPI_RHS3D = 2, EPI_P2D = 3, EPI_RHS2D = 4, EPI_C2 = 5, EPI_V2 = 6,
+  EPI_RHS3N = 7
+};
 // XM_DERIV_DAMPINV: the momentum RHS from the x derivative with the damping
 // inverse of the same component subtracted, in one pass (k_xw256 only)
 enum { XM_DERIV = 0, XM_FILTER = 1, XM_DAMP_FWD = 2, XM_DAMP_INV = 3, XM_DERIV_DAMPINV = 4 };

@@ -879,7 +879,7 @@ static int launch_xm(sparc_ctx* c, int nf, const XArgs<T>& X, T beta, const T* m
   const double fb = (double)c->N * sizeof(T);
   double nb;
   if (MODE == XM_DERIV || MODE == XM_FILTER || MODE == XM_DAMP_FWD)
-    nb = (X.s[0] ? 2 : 1) + 1;
+    nb = (X.s[0] ? 2 : 1) + 1 + (X.cp[0] ? 1 : 0);
   else
     nb = 3;
   const int cls = MODE == XM_DAMP_FWD   ? SPARC_KC_DAMP
@@ -1059,6 +1059,8 @@ static int launch_z(sparc_ctx* c, const Fld<T>& F, int ibc, const T* inA, const 
     nb = in + 1 + nbg({SPARC_BG_C2RHO0, SPARC_BG_GRADP0_Z, SPARC_BG_RHO0, SPARC_BG_GRADRHO0_Z}) + 2;
   else if (EPI == EPI_RHS3D)  // rho temp, a (+ scr) of v; RHS of xi, v_z, rho
     nb = in + 1 + (F.beta != (T)0 ? 6 : 3) + nbg({SPARC_BG_RHOINV}) + 4;
+  else if (EPI == EPI_RHS3N)  // rho temp; RHS of v_z
+    nb = in + 1 + nbg({SPARC_BG_RHOINV}) + 1;
   else if (EPI == EPI_C2)  // v_z again (own plane), dvxdx, dvydy -> W4, W5
     nb = in + 2 + nbg({SPARC_BG_C2RHO0, SPARC_BG_GRADP0_Z, SPARC_BG_RHO0, SPARC_BG_GRADRHO0_Z}) + 2;
   else if (EPI == EPI_V2)  // gradp_x, gradp_y, W4, W5, rho temp -> scr of 5 fields
@@ -1168,18 +1170,40 @@ static int run_stage(sparc_ctx* c, Fld<T>& F, int s) {
     if ((rc = launch_x<T, XM_DERIV>(c, F.a[0], S(0), beta, F.W[0], dmx, nullptr))) return rc;
     if ((rc = launch_y<T, YM_DERIV>(c, F.a[1], S(1), beta, F.W[1], dmy, nullptr))) return rc;
     if ((rc = launch_z<T, EPI_P3D>(c, F, 1, F.a[2], S(2), beta, nullptr, 0, nullptr))) return rc;
-    // ddxyz(p, bc=0) (displacement.f90:547): the z part with the RHS epilogue
-    // (RHS_xi = temp v, RHS_vz, RHS_rho terms; reads the previous RHS_v), then
-    // the x and y parts written straight into RHS_vx, RHS_vy as -rhoinv*dp
-    if ((rc = launch_z<T, EPI_RHS3D>(c, F, 0, F.W[3], nullptr, 0, nullptr, 0, nullptr))) return rc;
     const bool damp = c->cfg.damp_waves && c->damping_set;
-    // with damping on 256-wide x lines, RHS_vx is written once, by the pass
-    // that also subtracts its damping (after the damping forward and y passes,
-    // which read RHS_xi only)
-    // FP32 on 256 x 256 planes: the whole damping in one pass per plane
+    // FP32 on 256 x 256 planes with cfg.plane_damping: the whole damping in
+    // one pass per plane
     const bool d2d = damp && c->damp2d && sizeof(T) == 4 && c->cfg.nx == 256 &&
                      c->cfg.ny == 256;
+    // with damping on 256-wide x lines: the damping forward pass reads the
+    // stage temp v = a + beta scr of the velocities itself and also stores
+    // it as RHS_xi (once the divergence passes above have read the previous
+    // RHS_xi), so the z solve of p below writes only RHS_vz; RHS_vx is
+    // written once, by the pass that also subtracts its damping
     const bool fuse = damp && !d2d && c->cfg.nx == 256;
+    XArgs<T> DX{};
+    for (int f = 0; f < 3; ++f) {
+      DX.out[f] = F.scr[3 + f];
+      DX.spec[f] = (C2<T>*)c->spec[f];
+    }
+    if (fuse) {
+      // DAMP_VELOCITY (damping.f90:125-185) forward x and y passes
+      for (int f = 0; f < 3; ++f) {
+        DX.a[f] = F.a[3 + f];
+        DX.s[f] = S(3 + f);
+        DX.cp[f] = F.scr[f];
+      }
+      if ((rc = launch_xm<T, XM_DAMP_FWD>(c, 3, DX, beta, nullptr))) return rc;
+      if ((rc = launch_ym<T, YM_DAMP>(c, 3, DX, 0, nullptr))) return rc;
+      // ddxyz(p, bc=0) (displacement.f90:547): the z part with RHS_vz
+      if ((rc = launch_z<T, EPI_RHS3N>(c, F, 0, F.W[3], nullptr, 0, nullptr, 0, nullptr))) return rc;
+    } else {
+      // ddxyz(p, bc=0) (displacement.f90:547): the z part with the RHS
+      // epilogue (RHS_xi = temp v, RHS_vz; reads the previous RHS_v)
+      if ((rc = launch_z<T, EPI_RHS3D>(c, F, 0, F.W[3], nullptr, 0, nullptr, 0, nullptr))) return rc;
+    }
+    // the x and y parts of grad p written straight into RHS_vx, RHS_vy as
+    // -rhoinv*dp
     XArgs<T> G{};
     G.a[0] = F.W[3];
     G.rhs = 1;
@@ -1191,36 +1215,28 @@ static int run_stage(sparc_ctx* c, Fld<T>& F, int s) {
     }
     G.out[0] = F.scr[SPARC_V_Y];
     if ((rc = launch_ym<T, YM_DERIV>(c, 1, G, 0, dmy))) return rc;
-    if (damp) {
+    if (fuse) {
+      XArgs<T> H = G;
+      H.out[0] = F.scr[SPARC_V_X];
+      H.spec[0] = DX.spec[0];
+      H.edges = DX.edges;
+      if ((rc = launch_x_rhs_damp<T>(c, H, dmx))) return rc;
+      XArgs<T> D2{};
+      for (int f = 0; f < 2; ++f) {
+        D2.out[f] = DX.out[f + 1];
+        D2.spec[f] = DX.spec[f + 1];
+      }
+      D2.edges = DX.edges;
+      if ((rc = launch_xm<T, XM_DAMP_INV>(c, 2, D2, 0, nullptr))) return rc;
+    } else if (damp) {
       // DAMP_VELOCITY (damping.f90:125-185) on the stage's temp v, which the
       // epilogue above has just stored as RHS_xi = v (zero on planes 1 and nz,
       // whose damping output is discarded)
-      XArgs<T> DX{};
-      for (int f = 0; f < 3; ++f) {
-        DX.a[f] = F.scr[f];
-        DX.out[f] = F.scr[3 + f];
-        DX.spec[f] = (C2<T>*)c->spec[f];
-      }
+      for (int f = 0; f < 3; ++f) DX.a[f] = F.scr[f];
       if (d2d) return launch_damp2d<T>(c, DX);
       if ((rc = launch_xm<T, XM_DAMP_FWD>(c, 3, DX, 0, nullptr))) return rc;
       if ((rc = launch_ym<T, YM_DAMP>(c, 3, DX, 0, nullptr))) return rc;
-      if (fuse) {
-        XArgs<T> H = G;
-        H.out[0] = F.scr[SPARC_V_X];
-        H.spec[0] = DX.spec[0];
-        H.edges = DX.edges;
-        if ((rc = launch_x_rhs_damp<T>(c, H, dmx))) return rc;
-        XArgs<T> D2{};
-        for (int f = 0; f < 2; ++f) {
-          D2.a[f] = DX.a[f + 1];
-          D2.out[f] = DX.out[f + 1];
-          D2.spec[f] = DX.spec[f + 1];
-        }
-        D2.edges = DX.edges;
-        if ((rc = launch_xm<T, XM_DAMP_INV>(c, 2, D2, 0, nullptr))) return rc;
-      } else if ((rc = launch_xm<T, XM_DAMP_INV>(c, 3, DX, 0, nullptr))) {
-        return rc;
-      }
+      if ((rc = launch_xm<T, XM_DAMP_INV>(c, 3, DX, 0, nullptr))) return rc;
     }
   } else if (c->cfg.option == SPARC_OPT_MHD_PML_3D_DISPL) {
     // MP_MHD_PML_3D_DISPL (displacement.f90:663-849): 18 derivative passes

@@ -64,6 +64,9 @@ __device__ __forceinline__ void load_epi(const Fld<T>& F, const T* __restrict__ 
       if (EPI == EPI_RHS3D) e.s1 = F.scr[SPARC_V_Y][q];
     }
     if (F.bg3d[SPARC_BG_RHOINV]) e.g0 = F.bg[SPARC_BG_RHOINV][q];
+  } else if (EPI == EPI_RHS3N) {
+    e.w4 = F.W[4][q];
+    if (F.bg3d[SPARC_BG_RHOINV]) e.g0 = F.bg[SPARC_BG_RHOINV][q];
   } else if (EPI == EPI_C2) {  // div v, RHScont, RHSp of option 2
     e.w0 = F.W[0][q];
     e.w1 = F.W[1][q];
@@ -194,7 +197,19 @@ __device__ __forceinline__ void epilogue(const Fld<T>& F, int c, int k, T val, T
     }
     const T ri = bg_or(F, SPARC_BG_RHOINV, e.g0, k);
     const T rho = e.w4;
-    if (EPI == EPI_RHS3D) {
+    if (EPI == EPI_RHS3N) {
+      T rvz = -ri * gz;
+      rvz = rvz - rho * gk * ri;
+      if (k == F.src_plane) rvz = rvz + F.forcing[c] / bgv(F, SPARC_BG_RHO0, c, k) * F.delta_width;
+      if (k == 0 || k == n - 1) {
+        F.scr[SPARC_XI_X][q] = 0;
+        F.scr[SPARC_XI_Y][q] = 0;
+        F.scr[SPARC_XI_Z][q] = 0;
+        F.scr[SPARC_V_Z][q] = 0;
+      } else {
+        F.scr[SPARC_V_Z][q] = rvz;
+      }
+    } else if (EPI == EPI_RHS3D) {
       // RHS_vx, RHS_vy = -rhoinv * (dp/dx, dp/dy) come from the x and y
       // derivative passes that follow this solve
       T rvz = -ri * gz;
@@ -278,7 +293,8 @@ __global__ __launch_bounds__(64 * NSEG) __attribute__((amdgpu_waves_per_eu(4))) 
   // with ud in LDS; FP64 values take two registers each, so 1, and the
   // global-ud variants keep 2 planes: 3 spill there)
   constexpr int UF = 8;
-  constexpr int UB = sizeof(T) == 4 ? (ULDS ? 3 : 2) : 1;
+  constexpr int UB = EPI == EPI_RHS3N ? (sizeof(T) == 4 ? (ULDS ? 6 : 4) : 2)
+                                       : (sizeof(T) == 4 ? (ULDS ? 3 : 2) : 1);
   constexpr int ZW = ZWarm<T>::W;
   // EX: with ud in LDS the segments are coupled exactly (the products fcum,
   // gcum of this partition, at least 3 planes per segment); the global-ud
@@ -286,7 +302,8 @@ __global__ __launch_bounds__(64 * NSEG) __attribute__((amdgpu_waves_per_eu(4))) 
   constexpr bool EX = ULDS;
   extern __shared__ unsigned char zsm[];
   const int n = F.nz;
-  constexpr bool HAS_G = (EPI == EPI_RHS3D || EPI == EPI_RHS2D || EPI == EPI_V2);
+  constexpr bool HAS_G =
+      (EPI == EPI_RHS3D || EPI == EPI_RHS3N || EPI == EPI_RHS2D || EPI == EPI_V2);
   T* s_st = reinterpret_cast<T*>(zsm);  // stretch
   T* s_gm = s_st + n;                   // gam
   T* s_lw = s_gm + n;                   // lowr
@@ -522,7 +539,8 @@ __global__ __launch_bounds__(64 * NSEG) void k_zsolve_b(
     const T* __restrict__ gam, const T* __restrict__ fcum, const T* __restrict__ gcum, int ibc,
     const T* __restrict__ inA, const T* __restrict__ inS, T beta_in, const T* __restrict__ prov,
     T provscale, T* __restrict__ out) {
-  constexpr bool HAS_G = (EPI == EPI_RHS3D || EPI == EPI_RHS2D || EPI == EPI_V2);
+  constexpr bool HAS_G =
+      (EPI == EPI_RHS3D || EPI == EPI_RHS3N || EPI == EPI_RHS2D || EPI == EPI_V2);
   constexpr bool BKEEP = (EPI == EPI_P3D || EPI == EPI_P2D || EPI == EPI_C2);
   constexpr int U1 = 20;  // P1 planes in flight
   // P5 planes in flight (bounded by 128 VGPRs at 16 waves per CU): 14 for the
