@@ -42,7 +42,9 @@ KERNEL_SYMBOLS = {
     "damp_y": "sparc::k_y256<float, 2 (YM_DAMP)> -- damping, y pass",
     "damp_xi": "sparc::k_xw256<float, 3 (XM_DAMP_INV), 2> -- damping, inverse x (v_y, v_z; v_x in XM_DERIV_DAMPINV)",
     "zsolve": "sparc::k_zsolve<float, 7 (EPI_RHS3N), 8, true> -- z solve of p (ud in LDS) + RHS_vz",
-    "zsolve_p": "sparc::k_zsolve_b<float, 1 (EPI_P3D), 16, true> -- exact segmented z solve (column in LDS) + pressure/density",
+    "zsolve_p": "sparc::k_zsolve_pv<float, 16, true, 20> -- exact segmented z solves (column in LDS): "
+                "d xi_z/dz with the pressure, then dp/dz on the LDS tile with RHS_vz "
+                "(without damping or on other line sizes: sparc::k_zsolve_b<float, 1 (EPI_P3D), 16, true>)",
     "update": "sparc::k_update_filter_c -- a += dt*scr with the z filter (hi+lo state), all fields per launch",
 }
 

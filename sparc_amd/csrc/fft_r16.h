@@ -313,12 +313,13 @@ __device__ __forceinline__ void xw_inverse(C2<T> (&v)[4], const C2<T> (&w1)[3],
 // rows 2L, 2L+1, x fastest; out = i k mult(k) applied, or the momentum RHS
 // when X.rhs), XM_DAMP_FWD (line L = row j of planes 2q, 2q+1, L = j +
 // ny q; spec[L N + k] = the forward transform of the stage temp a + beta s,
-// which is also stored to X.cp when set) and XM_DAMP_INV (the same
-// rows of out minus the inverse transform of spec, planes 1 and nz skipped
-// unless X.edges).  XM_DERIV_DAMPINV takes the damping lines (row j of
-// planes 2q, 2q+1) of the input X.a (no stage temp) and writes
-// -rhoinv d/dx minus the inverse transform of spec, which is what the
-// momentum-RHS derivative followed by the damping inverse leave there.
+// which is also stored to X.cp when set, zero on planes 1 and nz) and
+// XM_DAMP_INV (the same rows of out, or of X.rin when set, minus the inverse
+// transform of spec, into out; planes 1 and nz skipped unless X.edges).
+// XM_DERIV_DAMPINV takes the damping lines (row j of planes 2q, 2q+1) of
+// the input X.a (no stage temp) and writes -rhoinv d/dx minus the inverse
+// transform of spec, which is what the momentum-RHS derivative followed by
+// the damping inverse leave there.
 // LPW lines per wave, the next line's loads issued before the current
 // line's transform.
 template <typename T, int MODE, int LPW>
@@ -383,11 +384,12 @@ __global__ __launch_bounds__(256) void k_xw256(int nrows, int nlines, XArgs<T> X
     bool h1_, w0_, w1_;                                              \
     rows((LL), o_, st_, h1_);                                        \
     edge_w((LL), w0_, w1_);                                          \
+    const T* __restrict__ ri_ = X.rin[fld] ? X.rin[fld] : out;       \
     _Pragma("unroll") for (int j = 0; j < 4; ++j) {                  \
       const int x = lane + 64 * j;                                   \
       nv[j] = spec[(LL) * N + kl + 64 * j];                          \
-      no0[j] = w0_ ? out[o_ + x] : (T)0;                             \
-      no1[j] = w1_ ? out[o_ + st_ + x] : (T)0;                       \
+      no0[j] = w0_ ? ri_[o_ + x] : (T)0;                             \
+      no1[j] = w1_ ? ri_[o_ + st_ + x] : (T)0;                       \
     }                                                                \
   }
   // raw a and s values of the next line: the stage temp a + beta s is formed
@@ -437,10 +439,12 @@ __global__ __launch_bounds__(256) void k_xw256(int nrows, int nlines, XArgs<T> X
         long o, st;
         bool h1;
         rows(L, o, st, h1);
+        const int k0 = 2 * (int)((unsigned)L / (unsigned)ny);
+        const bool e0 = k0 == 0 || k0 == nz - 1, e1 = k0 + 1 == nz - 1;
 #pragma unroll
         for (int j = 0; j < 4; ++j) {
-          cp[o + lane + 64 * j] = v[j].x;
-          if (h1) cp[o + st + lane + 64 * j] = v[j].y;
+          cp[o + lane + 64 * j] = e0 ? (T)0 : v[j].x;
+          if (h1) cp[o + st + lane + 64 * j] = e1 ? (T)0 : v[j].y;
         }
       }
     }

@@ -242,8 +242,11 @@ struct XArgs {
   int acc;
   T ald, gout;
   // damping forward pass of k_xw256: the stage temp a + beta s it transforms
-  // is also stored here (RHS_xi = temp v), when set
+  // is also stored here (RHS_xi = temp v; zero on planes 1 and nz), when set
   T* cp[3];
+  // damping inverse pass of k_xw256: the RHS is read from here (the result
+  // still goes to out), when set
+  const T* rin[3];
 };
 
 // plain derivative store: d, or combined with what out holds (X.acc)

@@ -1115,6 +1115,53 @@ static int launch_z(sparc_ctx* c, const Fld<T>& F, int ibc, const T* inA, const 
   return SPARC_OK;
 }
 
+// The pressure and its z derivative in one launch (k_zsolve_pv): FP32
+// columns that fit the two-tile LDS layout of k_zsolve_b, at most
+// kPvLmax planes per segment.  Writes W3 = p, rvz = RHS_vz before the
+// damping (zero planes: RHS_vz itself) and both PML psi.
+static constexpr int kPvLmax = 20;
+template <typename T>
+static bool zpv_fits(const sparc_ctx* c) {
+  const int nz = c->cfg.nz, L = (nz + kZbNseg - 1) / kZbNseg;
+  return sizeof(T) == 4 && nz >= 5 && L <= kPvLmax &&
+         2 * 64 * (size_t)nz * sizeof(T) <= (size_t)c->lds_max;
+}
+
+template <typename T>
+static int launch_zpv(sparc_ctx* c, const Fld<T>& F, const T* inA, const T* inS, T beta, T* rvz) {
+  if constexpr (sizeof(T) != 4) {
+    return fail(c, SPARC_EINVAL, "k_zsolve_pv is FP32 only");
+  } else {
+    const int nz = c->cfg.nz;
+    auto nbg = [&](std::initializer_list<int> ids) {
+      double n = 0;
+      for (int id : ids) n += c->bg3d[id] ? 1.0 : 0.0;
+      return n;
+    };
+    // xi_z temp, the x and y divergence parts, backgrounds; p and RHS_vz
+    const double nb = (inS ? 2.0 : 1.0) + 2 +
+                      nbg({SPARC_BG_C2RHO0, SPARC_BG_GRADP0_Z, SPARC_BG_RHO0,
+                           SPARC_BG_GRADRHO0_Z, SPARC_BG_RHOINV}) + 2;
+    ProfScope ps(c, SPARC_KC_ZSOLVE_P, nb * (double)c->N * sizeof(T));
+    const bool bg3 = c->bg3d[SPARC_BG_C2RHO0] || c->bg3d[SPARC_BG_GRADP0_Z] ||
+                     c->bg3d[SPARC_BG_RHO0] || c->bg3d[SPARC_BG_GRADRHO0_Z];
+    const size_t lds = 2 * 64 * (size_t)nz * sizeof(T);
+    const T* t1 = (const T*)c->tri[1];
+    const T* t0 = (const T*)c->tri[0];
+#define ZPV(G)                                                                               \
+  hipLaunchKernelGGL((k_zsolve_pv<T, kZbNseg, G, kPvLmax>), dim3(cdiv(c->nxy, 64)),          \
+                     dim3(64 * kZbNseg), lds, c->stream, F, make_tc<T>(c, 1), t1, t0, inA,   \
+                     inS, beta, rvz)
+    if (bg3)
+      ZPV(false);
+    else
+      ZPV(true);
+#undef ZPV
+    LAUNCHCHK(c);
+    return SPARC_OK;
+  }
+}
+
 // FILTER_VAR_Z alone (kernels.f90:845-881) on one field, in place
 template <typename T>
 static void zfilter_only(sparc_ctx* c, T* f) {
@@ -1169,7 +1216,6 @@ static int run_stage(sparc_ctx* c, Fld<T>& F, int s) {
     // ddxyz(xi_x, xi_y, xi_z, bc=1) + PML + div, p, rho (displacement.f90:491-535)
     if ((rc = launch_x<T, XM_DERIV>(c, F.a[0], S(0), beta, F.W[0], dmx, nullptr))) return rc;
     if ((rc = launch_y<T, YM_DERIV>(c, F.a[1], S(1), beta, F.W[1], dmy, nullptr))) return rc;
-    if ((rc = launch_z<T, EPI_P3D>(c, F, 1, F.a[2], S(2), beta, nullptr, 0, nullptr))) return rc;
     const bool damp = c->cfg.damp_waves && c->damping_set;
     // FP32 on 256 x 256 planes with cfg.plane_damping: the whole damping in
     // one pass per plane
@@ -1181,10 +1227,19 @@ static int run_stage(sparc_ctx* c, Fld<T>& F, int s) {
     // RHS_xi), so the z solve of p below writes only RHS_vz; RHS_vx is
     // written once, by the pass that also subtracts its damping
     const bool fuse = damp && !d2d && c->cfg.nx == 256;
+    // with fuse, FP32 and columns that fit: p, rho' and the z part of grad p
+    // in one z-solve launch (k_zsolve_pv), RHS_vz staged in W4 for the
+    // damping inverse; the damping forward pass then follows it
+    const bool pv = fuse && zpv_fits<T>(c);
     XArgs<T> DX{};
     for (int f = 0; f < 3; ++f) {
       DX.out[f] = F.scr[3 + f];
       DX.spec[f] = (C2<T>*)c->spec[f];
+    }
+    if (pv) {
+      if ((rc = launch_zpv<T>(c, F, F.a[2], S(2), beta, F.W[4]))) return rc;
+    } else {
+      if ((rc = launch_z<T, EPI_P3D>(c, F, 1, F.a[2], S(2), beta, nullptr, 0, nullptr))) return rc;
     }
     if (fuse) {
       // DAMP_VELOCITY (damping.f90:125-185) forward x and y passes
@@ -1196,7 +1251,9 @@ static int run_stage(sparc_ctx* c, Fld<T>& F, int s) {
       if ((rc = launch_xm<T, XM_DAMP_FWD>(c, 3, DX, beta, nullptr))) return rc;
       if ((rc = launch_ym<T, YM_DAMP>(c, 3, DX, 0, nullptr))) return rc;
       // ddxyz(p, bc=0) (displacement.f90:547): the z part with RHS_vz
-      if ((rc = launch_z<T, EPI_RHS3N>(c, F, 0, F.W[3], nullptr, 0, nullptr, 0, nullptr))) return rc;
+      if (!pv &&
+          (rc = launch_z<T, EPI_RHS3N>(c, F, 0, F.W[3], nullptr, 0, nullptr, 0, nullptr)))
+        return rc;
     } else {
       // ddxyz(p, bc=0) (displacement.f90:547): the z part with the RHS
       // epilogue (RHS_xi = temp v, RHS_vz; reads the previous RHS_v)
@@ -1226,6 +1283,7 @@ static int run_stage(sparc_ctx* c, Fld<T>& F, int s) {
         D2.out[f] = DX.out[f + 1];
         D2.spec[f] = DX.spec[f + 1];
       }
+      if (pv) D2.rin[1] = F.W[4];  // RHS_vz before the damping
       D2.edges = DX.edges;
       if ((rc = launch_xm<T, XM_DAMP_INV>(c, 2, D2, 0, nullptr))) return rc;
     } else if (damp) {

@@ -515,6 +515,127 @@ __global__ __launch_bounds__(64 * NSEG) __attribute__((amdgpu_waves_per_eu(4))) 
   }
 }
 
+// Phases of the exact segmented solve on a column tile in LDS (64 columns x
+// n planes, value of plane k in column lane at [k * 64 + lane]); wave seg
+// owns planes [ks, ke) of L = ceil(n / NSEG).
+// P2: local forward sweep from u = 0 before the segment's first row over the
+// right-hand side of DBYD2 (dbyd2.f90:120-199) formed from sIn; u~ -> sX.
+template <typename T>
+__device__ __forceinline__ void zb_forward(const T* __restrict__ sIn, T* __restrict__ sX, int lane,
+                                           int c, bool active, int ks, int ke, int n, int ibc,
+                                           const TriCoef<T>& tc, const T* __restrict__ lowr,
+                                           const T* __restrict__ invb, const T* __restrict__ prov,
+                                           T provscale) {
+  if (ks >= ke) return;
+  auto Bv = [&](int k) -> T { return sIn[k * 64 + lane]; };
+  T u = 0;
+  int k = ks;
+  if (k < 2) {
+    // rows 0 and 1 (dbyd2.f90:120-146)
+    const T r1 = tc.c1 * Bv(0) + tc.c2 * Bv(1) + tc.c3 * Bv(2) + tc.c4 * Bv(3) + tc.c5 * Bv(4);
+    if (k == 0) {
+      T r0;
+      if (ibc == 0)
+        r0 = 0;
+      else if (ibc == 1 || ibc == 3)
+        r0 = tc.b1 * Bv(0) + tc.b2 * Bv(1) + tc.b3 * Bv(2) + tc.b4 * Bv(3);
+      else
+        r0 = (prov && active) ? prov[c] * provscale : (T)0;
+      if (ibc & 1) r0 = (tc.alpha2 * r0 - tc.beta1 * r1) * tc.cst;
+      u = r0 * invb[0];
+      sX[lane] = u;
+      k = 1;
+    }
+    if (k < ke) {
+      u = (r1 - lowr[1] * u) * invb[1];
+      sX[64 + lane] = u;
+      k = 2;
+    }
+  }
+  const int kend = min(ke, n - 2);
+#pragma unroll 4
+  for (; k < kend; ++k) {
+    const T r = tc.ci1 * (Bv(k + 1) - Bv(k - 1)) + tc.ci2 * (Bv(k + 2) - Bv(k - 2));
+    u = (r - lowr[k] * u) * invb[k];
+    sX[k * 64 + lane] = u;
+  }
+  if (k < ke) {
+    // rows n-2 and n-1
+    const T rn2 = -(tc.c1 * Bv(n - 1) + tc.c2 * Bv(n - 2) + tc.c3 * Bv(n - 3) + tc.c4 * Bv(n - 4) +
+                    tc.c5 * Bv(n - 5));
+    if (k == n - 2) {
+      u = (rn2 - lowr[n - 2] * u) * invb[n - 2];
+      sX[(n - 2) * 64 + lane] = u;
+      k = n - 1;
+    }
+    if (k < ke) {
+      T rn1;
+      if (ibc == 0)
+        rn1 = 0;
+      else if (ibc == 1 || ibc == 2) {
+        rn1 = -(tc.b1 * Bv(n - 1) + tc.b2 * Bv(n - 2) + tc.b3 * Bv(n - 3) + tc.b4 * Bv(n - 4));
+        rn1 = (tc.alpha2 * rn1 - tc.beta1 * rn2) * tc.cst;
+      } else {
+        rn1 = (prov && active) ? prov[c] * provscale : (T)0;
+      }
+      u = (rn1 - lowr[n - 1] * u) * invb[n - 1];
+      sX[(n - 1) * 64 + lane] = u;
+    }
+  }
+}
+
+// P3: U = the true u on the previous segment's last row, chained over the
+// earlier segments; P4: the corrected u and the local backward sweep, in
+// place in sX (x~).  Called by every thread of the block (barriers inside);
+// sX must be complete (a barrier after P2).
+// The chains run over a fixed trip count NSEG with a guard, so the LDS reads
+// of all segment ends are issued before the multiply-add chain.
+template <int NSEG, typename T>
+__device__ __forceinline__ void zb_backward(T* __restrict__ sX, int lane, int seg, int ks, int ke,
+                                            int L, const T* __restrict__ fcum,
+                                            const T* __restrict__ gam) {
+  T U = 0;
+  if (ks < ke) {
+#pragma unroll
+    for (int j = 0; j < NSEG - 1; ++j) {
+      if (j < seg) {
+        const int bj = (j + 1) * L - 1;
+        U = sX[bj * 64 + lane] + fcum[bj] * U;
+      }
+    }
+  }
+  __syncthreads();
+  if (ks < ke) {
+    T x = sX[(ke - 1) * 64 + lane] + fcum[ke - 1] * U;
+    sX[(ke - 1) * 64 + lane] = x;
+#pragma unroll 4
+    for (int k = ke - 2; k >= ks; --k) {
+      const T u = sX[k * 64 + lane] + fcum[k] * U;
+      x = u - gam[k + 1] * x;
+      sX[k * 64 + lane] = x;
+    }
+  }
+  __syncthreads();
+}
+
+// P5 (first part): X = the true x on the next segment's first row, chained
+// from the last segment down; the true x of plane k is then
+// sX[k] + gcum[k] X.
+template <int NSEG, typename T>
+__device__ __forceinline__ T zb_xnext(const T* __restrict__ sX, int lane, int seg, int L, int n,
+                                      const T* __restrict__ gcum) {
+  T X = 0;
+  const int nsu = (n + L - 1) / L;
+#pragma unroll
+  for (int j = NSEG - 1; j > 0; --j) {
+    if (j < nsu && j > seg) {
+      const int aj = j * L;
+      X = sX[aj * 64 + lane] + gcum[aj] * X;
+    }
+  }
+  return X;
+}
+
 // Exact segmented solve with the column's input and sweep results in LDS
 // (64 x n values each).  Block = 64 columns x NSEG waves; wave s owns planes
 // [s*L, min(n, (s+1)*L)).  Because the matrix is the same for every column,
@@ -598,93 +719,12 @@ __global__ __launch_bounds__(64 * NSEG) void k_zsolve_b(
     }
   }
   __syncthreads();
-  auto Bv = [&](int k) -> T { return sB[k * 64 + lane]; };
-  // ---------------- P2: local forward sweep
-  if (ks < ke) {
-    T u = 0;
-    int k = ks;
-    if (k < 2) {
-      // rows 0 and 1 (dbyd2.f90:120-146)
-      const T r1 = tc.c1 * Bv(0) + tc.c2 * Bv(1) + tc.c3 * Bv(2) + tc.c4 * Bv(3) + tc.c5 * Bv(4);
-      if (k == 0) {
-        T r0;
-        if (ibc == 0)
-          r0 = 0;
-        else if (ibc == 1 || ibc == 3)
-          r0 = tc.b1 * Bv(0) + tc.b2 * Bv(1) + tc.b3 * Bv(2) + tc.b4 * Bv(3);
-        else
-          r0 = (prov && active) ? prov[c] * provscale : (T)0;
-        if (ibc & 1) r0 = (tc.alpha2 * r0 - tc.beta1 * r1) * tc.cst;
-        u = r0 * invb[0];
-        sX[lane] = u;
-        k = 1;
-      }
-      if (k < ke) {
-        u = (r1 - lowr[1] * u) * invb[1];
-        sX[64 + lane] = u;
-        k = 2;
-      }
-    }
-    const int kend = min(ke, n - 2);
-    for (; k < kend; ++k) {
-      const T r = tc.ci1 * (Bv(k + 1) - Bv(k - 1)) + tc.ci2 * (Bv(k + 2) - Bv(k - 2));
-      u = (r - lowr[k] * u) * invb[k];
-      sX[k * 64 + lane] = u;
-    }
-    if (k < ke) {
-      // rows n-2 and n-1
-      const T rn2 = -(tc.c1 * Bv(n - 1) + tc.c2 * Bv(n - 2) + tc.c3 * Bv(n - 3) +
-                      tc.c4 * Bv(n - 4) + tc.c5 * Bv(n - 5));
-      if (k == n - 2) {
-        u = (rn2 - lowr[n - 2] * u) * invb[n - 2];
-        sX[(n - 2) * 64 + lane] = u;
-        k = n - 1;
-      }
-      if (k < ke) {
-        T rn1;
-        if (ibc == 0)
-          rn1 = 0;
-        else if (ibc == 1 || ibc == 2) {
-          rn1 = -(tc.b1 * Bv(n - 1) + tc.b2 * Bv(n - 2) + tc.b3 * Bv(n - 3) + tc.b4 * Bv(n - 4));
-          rn1 = (tc.alpha2 * rn1 - tc.beta1 * rn2) * tc.cst;
-        } else {
-          rn1 = (prov && active) ? prov[c] * provscale : (T)0;
-        }
-        u = (rn1 - lowr[n - 1] * u) * invb[n - 1];
-        sX[(n - 1) * 64 + lane] = u;
-      }
-    }
-  }
+  zb_forward(sB, sX, lane, c, active, ks, ke, n, ibc, tc, lowr, invb, prov, provscale);
   __syncthreads();
-  // ---------------- P3: true u on the previous segment's last row
-  T U = 0;
-  if (ks < ke) {
-    for (int j = 0; j < seg; ++j) {
-      const int bj = (j + 1) * L - 1;
-      U = sX[bj * 64 + lane] + fcum[bj] * U;
-    }
-  }
-  __syncthreads();
-  // ---------------- P4: corrected u, local backward sweep
-  if (ks < ke) {
-    T x = sX[(ke - 1) * 64 + lane] + fcum[ke - 1] * U;
-    sX[(ke - 1) * 64 + lane] = x;
-#pragma unroll 4
-    for (int k = ke - 2; k >= ks; --k) {
-      const T u = sX[k * 64 + lane] + fcum[k] * U;
-      x = u - gam[k + 1] * x;
-      sX[k * 64 + lane] = x;
-    }
-  }
-  __syncthreads();
+  zb_backward<NSEG>(sX, lane, seg, ks, ke, L, fcum, gam);
   // ---------------- P5: true x on the next segment's first row, epilogue
   if (active && ks < ke) {
-    T X = 0;
-    const int nsu = (n + L - 1) / L;
-    for (int j = nsu - 1; j > seg; --j) {
-      const int aj = j * L;
-      X = sX[aj * 64 + lane] + gcum[aj] * X;
-    }
+    const T X = zb_xnext<NSEG>(sX, lane, seg, L, n, gcum);
     const T* __restrict__ st = F.stretch;
     // ring of UB planes in flight: slot j is consumed, then refilled with
     // the plane UB further on
@@ -702,6 +742,157 @@ __global__ __launch_bounds__(64 * NSEG) void k_zsolve_b(
           }
           epilogue<T, EPI>(F, c, k, x * st[k], HAS_G ? F.g[k] : (T)0, cu, out);
         }
+      }
+    }
+  }
+}
+
+// The pressure of option 4 and its z derivative in one launch
+// (displacement.f90:495-535 then the z part of ddxyz(p, bc=0) with
+// :547-597): the exact segmented solve above for d xi_z / dz (ibc 1, the
+// column tile holding B = xi_z's stage temp) with the EPI_P3D arithmetic,
+// whose p goes to W3 (for the x and y derivative passes) and into the tile
+// in place of B, and whose rho' stays in registers; then the same solve of
+// dp/dz (ibc 0) on the tile, which reads nothing from HBM, with the
+// momentum epilogue: the PML psi of dp/dz, RHS_vz = -rhoinv (dp/dz + rho' g)
+// plus the source, into rvz (the damping inverse pass subtracts the
+// damping from it into RHS_vz); RHS_vz = 0 on planes 1 and nz.  tri0 and
+// tri1 hold lowr | invb | gam | fcum | gcum (n each) of ibc 0 and 1 for
+// this partition.  LMAX >= L = ceil(n / NSEG): each wave keeps the rho' of
+// its planes in registers.
+template <typename T, int NSEG, bool NOBG3, int LMAX>
+__global__ __launch_bounds__(64 * NSEG) void k_zsolve_pv(Fld<T> F, TriCoef<T> tc,
+                                                         const T* __restrict__ tri1,
+                                                         const T* __restrict__ tri0,
+                                                         const T* __restrict__ inA,
+                                                         const T* __restrict__ inS, T beta_in,
+                                                         T* __restrict__ rvz) {
+  constexpr int U1 = 20;               // P1 planes in flight
+  constexpr int UB = NOBG3 ? 14 : 6;   // P5 planes in flight
+  extern __shared__ unsigned char zsm[];
+  const int n = F.nz;
+  T* sB = reinterpret_cast<T*>(zsm);  // B, then p
+  T* sX = sB + 64 * n;                // u~, then x~ (both solves)
+  const int lane = threadIdx.x & 63;
+  const int seg = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+  const int c = blockIdx.x * 64 + lane;
+  const bool active = c < F.nxy;
+  const long nxy = F.nxy;
+  const int L = (n + NSEG - 1) / NSEG;
+  const int ks = seg * L;
+  const int ke = min(n, ks + L);
+  const bool st = F.beta != (T)0;
+  const T* __restrict__ stretch = F.stretch;
+  // ---------------- P1: B of the owned planes
+  {
+    T na[U1], ns[U1];
+    auto ld = [&](int kb) {
+#pragma unroll
+      for (int j = 0; j < U1; ++j) {
+        const int k = kb + j;
+        if (active && k < ke) {
+          const long q = c + nxy * k;
+          na[j] = inA[q];
+          if (inS) ns[j] = inS[q];
+        }
+      }
+    };
+    ld(ks);
+    for (int kb = ks; kb < ke; kb += U1) {
+      T ca[U1], cs[U1];
+#pragma unroll
+      for (int j = 0; j < U1; ++j) {
+        ca[j] = na[j];
+        cs[j] = ns[j];
+      }
+      if (kb + U1 < ke) ld(kb + U1);
+#pragma unroll
+      for (int j = 0; j < U1; ++j) {
+        const int k = kb + j;
+        if (k < ke) sB[k * 64 + lane] = active ? (inS ? ca[j] + beta_in * cs[j] : ca[j]) : (T)0;
+      }
+    }
+  }
+  EIn<T> nx[UB];
+#pragma unroll
+  for (int j = 0; j < UB; ++j) {
+    const int k = ks + j;
+    if (k < ke) load_epi<T, EPI_P3D, true, true, NOBG3>(F, nullptr, 0, 0, c, k, active, nx[j]);
+  }
+  __syncthreads();
+  // ---------------- d xi_z / dz
+  zb_forward(sB, sX, lane, c, active, ks, ke, n, 1, tc, tri1, tri1 + n, (const T*)nullptr,
+             (T)0);
+  __syncthreads();
+  zb_backward<NSEG>(sX, lane, seg, ks, ke, L, tri1 + 3 * n, tri1 + 2 * n);
+  T rh[LMAX];
+  {
+    const T* __restrict__ gcum = tri1 + 4 * n;
+    const T X = ks < ke ? zb_xnext<NSEG>(sX, lane, seg, L, n, gcum) : (T)0;
+#pragma unroll
+    for (int r = 0; r < LMAX; ++r) {
+      const int k = ks + r;
+      rh[r] = 0;
+      if (k < ke) {
+        const EIn<T> e = nx[r % UB];
+        if (k + UB < ke)
+          load_epi<T, EPI_P3D, true, true, NOBG3>(F, nullptr, 0, 0, c, k + UB, active, nx[r % UB]);
+        T p = 0;
+        if (active) {
+          const long q = c + nxy * k;
+          T dz = (sX[k * 64 + lane] + gcum[k] * X) * stretch[k];
+          const int pi = pml_index(k, n, F.npmlbot, F.npmltop);
+          if (pi >= 0) {
+            T psi = e.pv;
+            if (st) psi = psi + F.beta * e.ps;
+            F.scrpml[0][c + nxy * pi] = e.paz * dz + e.pbz * psi;
+            dz = dz + psi;
+          }
+          const T div = e.w0 + e.w1 + dz;
+          const T xz = sB[k * 64 + lane];
+          const T c2r = bg_or(F, SPARC_BG_C2RHO0, e.g0, k);
+          const T gp = bg_or(F, SPARC_BG_GRADP0_Z, e.g1, k);
+          const T r0 = bg_or(F, SPARC_BG_RHO0, e.g2, k);
+          const T gr = bg_or(F, SPARC_BG_GRADRHO0_Z, e.g3, k);
+          p = -c2r * div - xz * gp;
+          rh[r] = -r0 * div - xz * gr;
+          F.W[3][q] = p;
+        }
+        sB[k * 64 + lane] = p;  // own plane: B is no longer read
+      }
+    }
+  }
+  __syncthreads();
+  // ---------------- dp/dz and the momentum epilogue
+  zb_forward(sB, sX, lane, c, active, ks, ke, n, 0, tc, tri0, tri0 + n, (const T*)nullptr,
+             (T)0);
+  __syncthreads();
+  zb_backward<NSEG>(sX, lane, seg, ks, ke, L, tri0 + 3 * n, tri0 + 2 * n);
+  if (active && ks < ke) {
+    const T* __restrict__ gcum = tri0 + 4 * n;
+    const T X = zb_xnext<NSEG>(sX, lane, seg, L, n, gcum);
+#pragma unroll
+    for (int r = 0; r < LMAX; ++r) {
+      const int k = ks + r;
+      if (k < ke) {
+        const long q = c + nxy * k;
+        T gz = (sX[k * 64 + lane] + gcum[k] * X) * stretch[k];
+        const int pi = pml_index(k, n, F.npmlbot, F.npmltop);
+        if (pi >= 0) {
+          const long pq = c + nxy * pi;
+          T psi = F.pmlvars[1][pq];
+          if (st) psi = psi + F.beta * F.scrpml[1][pq];
+          F.scrpml[1][pq] = bgv(F, SPARC_BG_AZ, c, k) * gz + bgv(F, SPARC_BG_BZPML, c, k) * psi;
+          gz = gz + psi;
+        }
+        const T ri = bgv(F, SPARC_BG_RHOINV, c, k);
+        T v = -ri * gz;
+        v = v - rh[r] * F.g[k] * ri;
+        if (k == F.src_plane) v = v + F.forcing[c] / bgv(F, SPARC_BG_RHO0, c, k) * F.delta_width;
+        if (k == 0 || k == n - 1)
+          F.scr[SPARC_V_Z][q] = 0;
+        else
+          rvz[q] = v;
       }
     }
   }
