@@ -1116,9 +1116,9 @@ static int launch_z(sparc_ctx* c, const Fld<T>& F, int ibc, const T* inA, const 
 }
 
 // The pressure and its z derivative in one launch (k_zsolve_pv): FP32
-// columns that fit the two-tile LDS layout of k_zsolve_b, at most
-// kPvLmax planes per segment.  Writes W3 = p, rvz = RHS_vz before the
-// damping (zero planes: RHS_vz itself) and both PML psi.
+// columns whose two tiles of 64 columns fit the LDS, at most kPvLmax planes
+// per segment.  Writes W3 = p, rvz = RHS_vz before the damping (zero
+// planes: RHS_vz itself) and both PML psi.
 static constexpr int kPvLmax = 20;
 template <typename T>
 static bool zpv_fits(const sparc_ctx* c) {
@@ -1148,10 +1148,10 @@ static int launch_zpv(sparc_ctx* c, const Fld<T>& F, const T* inA, const T* inS,
     const size_t lds = 2 * 64 * (size_t)nz * sizeof(T);
     const T* t1 = (const T*)c->tri[1];
     const T* t0 = (const T*)c->tri[0];
-#define ZPV(G)                                                                               \
-  hipLaunchKernelGGL((k_zsolve_pv<T, kZbNseg, G, kPvLmax>), dim3(cdiv(c->nxy, 64)),          \
-                     dim3(64 * kZbNseg), lds, c->stream, F, make_tc<T>(c, 1), t1, t0, inA,   \
-                     inS, beta, rvz)
+#define ZPV(G)                                                                                \
+  hipLaunchKernelGGL((k_zsolve_pv<kZbNseg, G, kPvLmax>), dim3(cdiv(c->nxy, 64)),              \
+                     dim3(64 * kZbNseg), lds, c->stream, F, make_tc<T>(c, 1), t1, t0, inA, inS, \
+                     beta, rvz)
     if (bg3)
       ZPV(false);
     else

@@ -256,6 +256,16 @@ __device__ __forceinline__ void epilogue(const Fld<T>& F, int c, int k, T val, T
   }
 }
 
+// Block barrier that orders LDS accesses only: the waves' global loads in
+// flight (the epilogue's first planes, the per-plane tables) stay in flight
+// across it, where __syncthreads waits for every outstanding load.  The
+// column-tile kernels below share data between waves through LDS alone.
+__device__ __forceinline__ void lds_barrier() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+  __builtin_amdgcn_s_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
 // Segmented solve: NSEG waves per 64 columns, wave s owning planes
 // [s*L, min(n, (s+1)*L)).  The forward recurrence u_k = (r_k - lowr_k u_{k-1})
 // / betad_k contracts by |lowr/betad| ~ 0.382 per row and the backward one
@@ -446,7 +456,11 @@ __global__ __launch_bounds__(64 * NSEG) __attribute__((amdgpu_waves_per_eu(4))) 
       UDW(n - 1, u);  // x_{n-1}: read back by every wave that starts exactly
     }
   }
-  __syncthreads();
+  // ud in LDS: an LDS-only barrier; a global ud is read by other waves
+  if constexpr (ULDS)
+    lds_barrier();
+  else
+    __syncthreads();
   T XN = 0;  // EX: the true x on the next segment's first row
   if constexpr (EX) {
     // true u on the previous segment's last row, chained over the earlier
@@ -458,7 +472,7 @@ __global__ __launch_bounds__(64 * NSEG) __attribute__((amdgpu_waves_per_eu(4))) 
         const int bj = (j + 1) * L - 1;
         U = s_ud[bj * 64 + lane] + fcum[bj] * U;
       }
-    __syncthreads();
+    lds_barrier();
     if (active && ks < ke) {
       T x = s_ud[(ke - 1) * 64 + lane] + fcum[ke - 1] * U;
       s_ud[(ke - 1) * 64 + lane] = x;
@@ -467,7 +481,7 @@ __global__ __launch_bounds__(64 * NSEG) __attribute__((amdgpu_waves_per_eu(4))) 
         s_ud[k * 64 + lane] = x;
       }
     }
-    __syncthreads();
+    lds_barrier();
     if (active && ks < ke) {
       const int nsu = (n + L - 1) / L;
       for (int j = nsu - 1; j > seg; --j) XN = s_ud[j * L * 64 + lane] + gcum[j * L] * XN;
@@ -515,19 +529,19 @@ __global__ __launch_bounds__(64 * NSEG) __attribute__((amdgpu_waves_per_eu(4))) 
   }
 }
 
-// Phases of the exact segmented solve on a column tile in LDS (64 columns x
-// n planes, value of plane k in column lane at [k * 64 + lane]); wave seg
+// Phases of the exact segmented solve on a column tile in LDS (TS columns x
+// n planes, value of plane k in column col at [k * TS + col]); segment seg
 // owns planes [ks, ke) of L = ceil(n / NSEG).
 // P2: local forward sweep from u = 0 before the segment's first row over the
 // right-hand side of DBYD2 (dbyd2.f90:120-199) formed from sIn; u~ -> sX.
 template <typename T>
-__device__ __forceinline__ void zb_forward(const T* __restrict__ sIn, T* __restrict__ sX, int lane,
-                                           int c, bool active, int ks, int ke, int n, int ibc,
+__device__ __forceinline__ void zb_forward(const T* __restrict__ sIn, T* __restrict__ sX, int col,
+                                           int TS, int c, bool active, int ks, int ke, int n, int ibc,
                                            const TriCoef<T>& tc, const T* __restrict__ lowr,
                                            const T* __restrict__ invb, const T* __restrict__ prov,
                                            T provscale) {
   if (ks >= ke) return;
-  auto Bv = [&](int k) -> T { return sIn[k * 64 + lane]; };
+  auto Bv = [&](int k) -> T { return sIn[k * TS + col]; };
   T u = 0;
   int k = ks;
   if (k < 2) {
@@ -543,12 +557,12 @@ __device__ __forceinline__ void zb_forward(const T* __restrict__ sIn, T* __restr
         r0 = (prov && active) ? prov[c] * provscale : (T)0;
       if (ibc & 1) r0 = (tc.alpha2 * r0 - tc.beta1 * r1) * tc.cst;
       u = r0 * invb[0];
-      sX[lane] = u;
+      sX[col] = u;
       k = 1;
     }
     if (k < ke) {
       u = (r1 - lowr[1] * u) * invb[1];
-      sX[64 + lane] = u;
+      sX[TS + col] = u;
       k = 2;
     }
   }
@@ -557,7 +571,7 @@ __device__ __forceinline__ void zb_forward(const T* __restrict__ sIn, T* __restr
   for (; k < kend; ++k) {
     const T r = tc.ci1 * (Bv(k + 1) - Bv(k - 1)) + tc.ci2 * (Bv(k + 2) - Bv(k - 2));
     u = (r - lowr[k] * u) * invb[k];
-    sX[k * 64 + lane] = u;
+    sX[k * TS + col] = u;
   }
   if (k < ke) {
     // rows n-2 and n-1
@@ -565,7 +579,7 @@ __device__ __forceinline__ void zb_forward(const T* __restrict__ sIn, T* __restr
                     tc.c5 * Bv(n - 5));
     if (k == n - 2) {
       u = (rn2 - lowr[n - 2] * u) * invb[n - 2];
-      sX[(n - 2) * 64 + lane] = u;
+      sX[(n - 2) * TS + col] = u;
       k = n - 1;
     }
     if (k < ke) {
@@ -579,7 +593,7 @@ __device__ __forceinline__ void zb_forward(const T* __restrict__ sIn, T* __restr
         rn1 = (prov && active) ? prov[c] * provscale : (T)0;
       }
       u = (rn1 - lowr[n - 1] * u) * invb[n - 1];
-      sX[(n - 1) * 64 + lane] = u;
+      sX[(n - 1) * TS + col] = u;
     }
   }
 }
@@ -591,8 +605,8 @@ __device__ __forceinline__ void zb_forward(const T* __restrict__ sIn, T* __restr
 // The chains run over a fixed trip count NSEG with a guard, so the LDS reads
 // of all segment ends are issued before the multiply-add chain.
 template <int NSEG, typename T>
-__device__ __forceinline__ void zb_backward(T* __restrict__ sX, int lane, int seg, int ks, int ke,
-                                            int L, const T* __restrict__ fcum,
+__device__ __forceinline__ void zb_backward(T* __restrict__ sX, int col, int TS, int seg, int ks,
+                                            int ke, int L, const T* __restrict__ fcum,
                                             const T* __restrict__ gam) {
   T U = 0;
   if (ks < ke) {
@@ -600,29 +614,30 @@ __device__ __forceinline__ void zb_backward(T* __restrict__ sX, int lane, int se
     for (int j = 0; j < NSEG - 1; ++j) {
       if (j < seg) {
         const int bj = (j + 1) * L - 1;
-        U = sX[bj * 64 + lane] + fcum[bj] * U;
+        U = sX[bj * TS + col] + fcum[bj] * U;
       }
     }
   }
-  __syncthreads();
+  lds_barrier();
   if (ks < ke) {
-    T x = sX[(ke - 1) * 64 + lane] + fcum[ke - 1] * U;
-    sX[(ke - 1) * 64 + lane] = x;
+    T x = sX[(ke - 1) * TS + col] + fcum[ke - 1] * U;
+    sX[(ke - 1) * TS + col] = x;
 #pragma unroll 4
     for (int k = ke - 2; k >= ks; --k) {
-      const T u = sX[k * 64 + lane] + fcum[k] * U;
+      const T u = sX[k * TS + col] + fcum[k] * U;
       x = u - gam[k + 1] * x;
-      sX[k * 64 + lane] = x;
+      sX[k * TS + col] = x;
     }
   }
-  __syncthreads();
+  lds_barrier();
 }
 
 // P5 (first part): X = the true x on the next segment's first row, chained
 // from the last segment down; the true x of plane k is then
 // sX[k] + gcum[k] X.
 template <int NSEG, typename T>
-__device__ __forceinline__ T zb_xnext(const T* __restrict__ sX, int lane, int seg, int L, int n,
+__device__ __forceinline__ T zb_xnext(const T* __restrict__ sX, int col, int TS, int seg, int L,
+                                      int n,
                                       const T* __restrict__ gcum) {
   T X = 0;
   const int nsu = (n + L - 1) / L;
@@ -630,7 +645,7 @@ __device__ __forceinline__ T zb_xnext(const T* __restrict__ sX, int lane, int se
   for (int j = NSEG - 1; j > 0; --j) {
     if (j < nsu && j > seg) {
       const int aj = j * L;
-      X = sX[aj * 64 + lane] + gcum[aj] * X;
+      X = sX[aj * TS + col] + gcum[aj] * X;
     }
   }
   return X;
@@ -718,13 +733,13 @@ __global__ __launch_bounds__(64 * NSEG) void k_zsolve_b(
       if (k < ke) load_epi<T, EPI, true, BKEEP, NOBG3>(F, nullptr, 0, 0, c, k, true, nx[j]);
     }
   }
-  __syncthreads();
-  zb_forward(sB, sX, lane, c, active, ks, ke, n, ibc, tc, lowr, invb, prov, provscale);
-  __syncthreads();
-  zb_backward<NSEG>(sX, lane, seg, ks, ke, L, fcum, gam);
+  lds_barrier();
+  zb_forward(sB, sX, lane, 64, c, active, ks, ke, n, ibc, tc, lowr, invb, prov, provscale);
+  lds_barrier();
+  zb_backward<NSEG>(sX, lane, 64, seg, ks, ke, L, fcum, gam);
   // ---------------- P5: true x on the next segment's first row, epilogue
   if (active && ks < ke) {
-    const T X = zb_xnext<NSEG>(sX, lane, seg, L, n, gcum);
+    const T X = zb_xnext<NSEG>(sX, lane, 64, seg, L, n, gcum);
     const T* __restrict__ st = F.stretch;
     // ring of UB planes in flight: slot j is consumed, then refilled with
     // the plane UB further on
@@ -747,6 +762,13 @@ __global__ __launch_bounds__(64 * NSEG) void k_zsolve_b(
   }
 }
 
+// Value of lane r (compile-time r in the unrolled plane loops) to the
+// whole wave: the per-plane tables of a wave's planes are loaded once, lane r
+// holding plane ks + r, instead of one dependent scalar load per plane.
+__device__ __forceinline__ float lane_at(float v, int r) {
+  return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), r));
+}
+
 // The pressure of option 4 and its z derivative in one launch
 // (displacement.f90:495-535 then the z part of ddxyz(p, bc=0) with
 // :547-597): the exact segmented solve above for d xi_z / dz (ibc 1, the
@@ -758,17 +780,23 @@ __global__ __launch_bounds__(64 * NSEG) void k_zsolve_b(
 // plus the source, into rvz (the damping inverse pass subtracts the
 // damping from it into RHS_vz); RHS_vz = 0 on planes 1 and nz.  tri0 and
 // tri1 hold lowr | invb | gam | fcum | gcum (n each) of ibc 0 and 1 for
-// this partition.  LMAX >= L = ceil(n / NSEG): each wave keeps the rho' of
-// its planes in registers.
-template <typename T, int NSEG, bool NOBG3, int LMAX>
-__global__ __launch_bounds__(64 * NSEG) void k_zsolve_pv(Fld<T> F, TriCoef<T> tc,
-                                                         const T* __restrict__ tri1,
-                                                         const T* __restrict__ tri0,
-                                                         const T* __restrict__ inA,
-                                                         const T* __restrict__ inS, T beta_in,
-                                                         T* __restrict__ rvz) {
-  constexpr int U1 = 20;               // P1 planes in flight
-  constexpr int UB = NOBG3 ? 14 : 6;   // P5 planes in flight
+// this partition; the stencil coefficients tc are those of every ibc.
+// FP32; 64 columns per block, wave s owns segment s; LMAX >= L =
+// ceil(n / NSEG), and each lane keeps the rho' of its planes in registers.
+// Measured per block (256 x 256 x 300): without the per-wave tables the two
+// epilogues took 13 and 16 us of 54, waiting on one scalar load after
+// another per plane.
+template <int NSEG, bool NOBG3, int LMAX>
+__global__ __launch_bounds__(64 * NSEG) void k_zsolve_pv(Fld<float> F, TriCoef<float> tc,
+                                                         const float* __restrict__ tri1,
+                                                         const float* __restrict__ tri0,
+                                                         const float* __restrict__ inA,
+                                                         const float* __restrict__ inS,
+                                                         float beta_in, float* __restrict__ rvz) {
+  using T = float;
+  static_assert(LMAX <= 64, "one lane per plane of a segment");
+  constexpr int U1 = 20;                // P1 planes in flight
+  constexpr int UB = NOBG3 ? 14 : 6;    // P5 planes in flight
   extern __shared__ unsigned char zsm[];
   const int n = F.nz;
   T* sB = reinterpret_cast<T*>(zsm);  // B, then p
@@ -782,7 +810,19 @@ __global__ __launch_bounds__(64 * NSEG) void k_zsolve_pv(Fld<T> F, TriCoef<T> tc
   const int ks = seg * L;
   const int ke = min(n, ks + L);
   const bool st = F.beta != (T)0;
-  const T* __restrict__ stretch = F.stretch;
+  // lane r: plane ks + r of a per-plane table (z profile)
+  const int kl = ks + lane;
+  const bool lk = lane < LMAX && kl < ke;
+  auto tab = [&](const T* __restrict__ t) -> T { return lk ? t[kl] : (T)0; };
+  const T v_st = tab(F.stretch);
+  T v_gc = tab(tri1 + 4 * n);
+  T v_c2r = 0, v_gp = 0, v_r0 = 0, v_gr = 0;
+  if (NOBG3) {
+    v_c2r = tab(F.bg[SPARC_BG_C2RHO0]);
+    v_gp = tab(F.bg[SPARC_BG_GRADP0_Z]);
+    v_r0 = tab(F.bg[SPARC_BG_RHO0]);
+    v_gr = tab(F.bg[SPARC_BG_GRADRHO0_Z]);
+  }
   // ---------------- P1: B of the owned planes
   {
     T na[U1], ns[U1];
@@ -819,16 +859,15 @@ __global__ __launch_bounds__(64 * NSEG) void k_zsolve_pv(Fld<T> F, TriCoef<T> tc
     const int k = ks + j;
     if (k < ke) load_epi<T, EPI_P3D, true, true, NOBG3>(F, nullptr, 0, 0, c, k, active, nx[j]);
   }
-  __syncthreads();
+  lds_barrier();
   // ---------------- d xi_z / dz
-  zb_forward(sB, sX, lane, c, active, ks, ke, n, 1, tc, tri1, tri1 + n, (const T*)nullptr,
+  zb_forward(sB, sX, lane, 64, c, active, ks, ke, n, 1, tc, tri1, tri1 + n, (const T*)nullptr,
              (T)0);
-  __syncthreads();
-  zb_backward<NSEG>(sX, lane, seg, ks, ke, L, tri1 + 3 * n, tri1 + 2 * n);
+  lds_barrier();
+  zb_backward<NSEG>(sX, lane, 64, seg, ks, ke, L, tri1 + 3 * n, tri1 + 2 * n);
   T rh[LMAX];
   {
-    const T* __restrict__ gcum = tri1 + 4 * n;
-    const T X = ks < ke ? zb_xnext<NSEG>(sX, lane, seg, L, n, gcum) : (T)0;
+    const T X = ks < ke ? zb_xnext<NSEG>(sX, lane, 64, seg, L, n, tri1 + 4 * n) : (T)0;
 #pragma unroll
     for (int r = 0; r < LMAX; ++r) {
       const int k = ks + r;
@@ -840,7 +879,7 @@ __global__ __launch_bounds__(64 * NSEG) void k_zsolve_pv(Fld<T> F, TriCoef<T> tc
         T p = 0;
         if (active) {
           const long q = c + nxy * k;
-          T dz = (sX[k * 64 + lane] + gcum[k] * X) * stretch[k];
+          T dz = (sX[k * 64 + lane] + lane_at(v_gc, r) * X) * lane_at(v_st, r);
           const int pi = pml_index(k, n, F.npmlbot, F.npmltop);
           if (pi >= 0) {
             T psi = e.pv;
@@ -850,10 +889,18 @@ __global__ __launch_bounds__(64 * NSEG) void k_zsolve_pv(Fld<T> F, TriCoef<T> tc
           }
           const T div = e.w0 + e.w1 + dz;
           const T xz = sB[k * 64 + lane];
-          const T c2r = bg_or(F, SPARC_BG_C2RHO0, e.g0, k);
-          const T gp = bg_or(F, SPARC_BG_GRADP0_Z, e.g1, k);
-          const T r0 = bg_or(F, SPARC_BG_RHO0, e.g2, k);
-          const T gr = bg_or(F, SPARC_BG_GRADRHO0_Z, e.g3, k);
+          T c2r, gp, r0, gr;
+          if (NOBG3) {
+            c2r = lane_at(v_c2r, r);
+            gp = lane_at(v_gp, r);
+            r0 = lane_at(v_r0, r);
+            gr = lane_at(v_gr, r);
+          } else {
+            c2r = bg_or(F, SPARC_BG_C2RHO0, e.g0, k);
+            gp = bg_or(F, SPARC_BG_GRADP0_Z, e.g1, k);
+            r0 = bg_or(F, SPARC_BG_RHO0, e.g2, k);
+            gr = bg_or(F, SPARC_BG_GRADRHO0_Z, e.g3, k);
+          }
           p = -c2r * div - xz * gp;
           rh[r] = -r0 * div - xz * gr;
           F.W[3][q] = p;
@@ -862,37 +909,64 @@ __global__ __launch_bounds__(64 * NSEG) void k_zsolve_pv(Fld<T> F, TriCoef<T> tc
       }
     }
   }
-  __syncthreads();
+  lds_barrier();
   // ---------------- dp/dz and the momentum epilogue
-  zb_forward(sB, sX, lane, c, active, ks, ke, n, 0, tc, tri0, tri0 + n, (const T*)nullptr,
+  // tables of the second epilogue, in flight during the second solve
+  v_gc = tab(tri0 + 4 * n);
+  const T v_g = tab(F.g);
+  const bool ri3 = F.bg3d[SPARC_BG_RHOINV] != 0;
+  const T v_ri = ri3 ? (T)0 : tab(F.bg[SPARC_BG_RHOINV]);
+  const bool az3 = F.bg3d[SPARC_BG_AZ] != 0, bz3 = F.bg3d[SPARC_BG_BZPML] != 0;
+  const T v_az = az3 ? (T)0 : tab(F.bg[SPARC_BG_AZ]);
+  const T v_bz = bz3 ? (T)0 : tab(F.bg[SPARC_BG_BZPML]);
+  // the PML psi of dp/dz on this wave's PML planes (the first and last
+  // segments), whose loads would otherwise wait one after another
+  T pv[LMAX], pz[LMAX];
+#pragma unroll
+  for (int r = 0; r < LMAX; ++r) {
+    const int k = ks + r;
+    pv[r] = 0;
+    pz[r] = 0;
+    const int pi = k < ke ? pml_index(k, n, F.npmlbot, F.npmltop) : -1;
+    if (active && pi >= 0) {
+      const long pq = c + nxy * pi;
+      pv[r] = F.pmlvars[1][pq];
+      if (st) pz[r] = F.scrpml[1][pq];
+    }
+  }
+  zb_forward(sB, sX, lane, 64, c, active, ks, ke, n, 0, tc, tri0, tri0 + n, (const T*)nullptr,
              (T)0);
-  __syncthreads();
-  zb_backward<NSEG>(sX, lane, seg, ks, ke, L, tri0 + 3 * n, tri0 + 2 * n);
-  if (active && ks < ke) {
-    const T* __restrict__ gcum = tri0 + 4 * n;
-    const T X = zb_xnext<NSEG>(sX, lane, seg, L, n, gcum);
+  lds_barrier();
+  zb_backward<NSEG>(sX, lane, 64, seg, ks, ke, L, tri0 + 3 * n, tri0 + 2 * n);
+  if (ks < ke) {
+    const T X = zb_xnext<NSEG>(sX, lane, 64, seg, L, n, tri0 + 4 * n);
 #pragma unroll
     for (int r = 0; r < LMAX; ++r) {
       const int k = ks + r;
       if (k < ke) {
-        const long q = c + nxy * k;
-        T gz = (sX[k * 64 + lane] + gcum[k] * X) * stretch[k];
-        const int pi = pml_index(k, n, F.npmlbot, F.npmltop);
-        if (pi >= 0) {
-          const long pq = c + nxy * pi;
-          T psi = F.pmlvars[1][pq];
-          if (st) psi = psi + F.beta * F.scrpml[1][pq];
-          F.scrpml[1][pq] = bgv(F, SPARC_BG_AZ, c, k) * gz + bgv(F, SPARC_BG_BZPML, c, k) * psi;
-          gz = gz + psi;
+        const T gc = lane_at(v_gc, r), sk = lane_at(v_st, r), gk = lane_at(v_g, r);
+        const T rz = lane_at(v_ri, r);
+        if (active) {
+          const long q = c + nxy * k;
+          T gz = (sX[k * 64 + lane] + gc * X) * sk;
+          const int pi = pml_index(k, n, F.npmlbot, F.npmltop);
+          if (pi >= 0) {
+            T psi = pv[r];
+            if (st) psi = psi + F.beta * pz[r];
+            const T az = az3 ? F.bg[SPARC_BG_AZ][q] : lane_at(v_az, r);
+            const T bz = bz3 ? F.bg[SPARC_BG_BZPML][q] : lane_at(v_bz, r);
+            F.scrpml[1][c + nxy * pi] = az * gz + bz * psi;
+            gz = gz + psi;
+          }
+          const T ri = ri3 ? F.bg[SPARC_BG_RHOINV][q] : rz;
+          T v = -ri * gz;
+          v = v - rh[r] * gk * ri;
+          if (k == F.src_plane) v = v + F.forcing[c] / bgv(F, SPARC_BG_RHO0, c, k) * F.delta_width;
+          if (k == 0 || k == n - 1)
+            F.scr[SPARC_V_Z][q] = 0;
+          else
+            rvz[q] = v;
         }
-        const T ri = bgv(F, SPARC_BG_RHOINV, c, k);
-        T v = -ri * gz;
-        v = v - rh[r] * F.g[k] * ri;
-        if (k == F.src_plane) v = v + F.forcing[c] / bgv(F, SPARC_BG_RHO0, c, k) * F.delta_width;
-        if (k == 0 || k == n - 1)
-          F.scr[SPARC_V_Z][q] = 0;
-        else
-          rvz[q] = v;
       }
     }
   }
