@@ -75,6 +75,12 @@ def main() -> int:
                     help="one forward + adjoint + kernel solve of STEPS steps on the workload "
                          "(observed data from a +2%% sound-speed bump, 8 receivers on a 10 Mm "
                          "ring) instead of the step benchmark")
+    ap.add_argument("--traffic-from", default="", metavar="DIR",
+                    help="write the counter traffic per launch (--traffic-out) from rocprofv3 "
+                         "--pmc csv output: DIR/fetch and DIR/write (FETCH_SIZE, WRITE_SIZE over "
+                         "a bench run), DIR/copy_fetch and DIR/copy_write (the same counters "
+                         "over --copy-probe 256), with this run's algorithmic bytes per class")
+    ap.add_argument("--traffic-out", default=os.path.join(ROOT, "profiles", "traffic.json"))
     ap.add_argument("--damp2d", action="store_true",
                     help="the damping of 256x256 FP32 planes by k_damp2d, one pass per plane "
                          "(cfg.plane_damping)")
@@ -136,7 +142,7 @@ def main() -> int:
     kernels = {k: {"ms_total": v[0], "launches": v[2],
                    "us_per_launch": 1e3 * v[0] / v[2] if v[2] else None,
                    "gbs": v[1] / (v[0] * 1e-3) / 1e9 if v[0] > 0 else None}
-               for k, v in prof.items()}
+               for k, v in prof.items() if v[2]}
     dom = max(prof, key=lambda k: prof[k][0])
     ms, by, nl = prof[dom]
     achieved = by / (ms * 1e-3) / 1e9
@@ -206,6 +212,8 @@ def main() -> int:
         "roofline": roof,
         "cpu_baseline": cpu,
     }
+    if args.traffic_from:
+        write_traffic(args.traffic_from, args.traffic_out, f"{W.name}:{args.precision}", prof)
     if fp64 is not None:
         line["fp64_step"] = fp64
     if serving is not None:
@@ -319,6 +327,73 @@ def serving_bench(n: int, rate: float) -> int:
     }
     print(json.dumps(line))
     return 0
+
+
+# the kernels of each timed class on the default path of the 256x256x300 FP32
+# box, as rocprofv3 names them
+CLASS_KERNELS = {
+    "zsolve_p": ("k_zsolve_pv<16, true, 20>",),
+    "damp": ("k_xw256<float, 2, 4>",),
+    "damp_y": ("k_y256<float, 2>",),
+    "damp_xi": ("k_xw256<float, 3, 2>",),
+    "update": ("k_update_filter_c",),
+    "xlines": ("k_xw256<float, 0, 2>", "k_xw256<float, 4, 2>"),
+    "ylines": ("k_y256<float, 0>",),
+}
+
+
+def _pmc_per_kernel(directory: str) -> dict:
+    """{kernel name: (dispatches, mean counter value per dispatch)} from the
+    *counter_collection.csv files rocprofv3 writes under directory."""
+    import csv
+    import glob
+    per = {}
+    for f in glob.glob(os.path.join(directory, "**", "*counter_collection.csv"), recursive=True):
+        with open(f, newline="") as fh:
+            for row in csv.DictReader(fh):
+                key = (row["Kernel_Name"], row.get("Dispatch_Id", row.get("Correlation_Id")))
+                per[key] = per.get(key, 0.0) + float(row["Counter_Value"])
+    out = {}
+    for (name, _), v in per.items():
+        n, t = out.get(name, (0, 0.0))
+        out[name] = (n + 1, t + v)
+    return {k: (n, t / n) for k, (n, t) in out.items()}
+
+
+def write_traffic(directory: str, out_path: str, prefix: str, prof: dict) -> None:
+    """HBM bytes per launch of each class: FETCH_SIZE and WRITE_SIZE (KiB per
+    dispatch) scaled by what they read for a device-to-device copy of known
+    size, averaged over the class's dispatches."""
+    fe, wr = _pmc_per_kernel(os.path.join(directory, "fetch")), _pmc_per_kernel(
+        os.path.join(directory, "write"))
+    cf, cw = _pmc_per_kernel(os.path.join(directory, "copy_fetch")), _pmc_per_kernel(
+        os.path.join(directory, "copy_write"))
+    copy_bytes = 256 * (1 << 20)
+    ck = [k for k in cf if "copyBuffer" in k]
+    if not ck:
+        raise RuntimeError(f"no copy kernel under {directory}/copy_fetch")
+    fscale = copy_bytes / (1024.0 * cf[ck[0]][1])
+    wscale = copy_bytes / (1024.0 * cw[ck[0]][1])
+    res = {"_calibration": {"copy_bytes": copy_bytes, "fetch_scale": fscale, "write_scale": wscale,
+                            "note": "FETCH_SIZE and WRITE_SIZE (KiB per dispatch) scaled by what "
+                                    "they read for a 256 MiB device-to-device copy"}}
+    for cls, names in CLASS_KERNELS.items():
+        if cls not in prof or not prof[cls][2]:
+            continue
+        rows = [(k, fe[k], wr.get(k)) for k in fe if any(nm in k for nm in names)]
+        if not rows or any(w is None for _, _, w in rows):
+            continue
+        nd = sum(f[0] for _, f, _ in rows)
+        fb = sum(f[0] * f[1] for _, f, _ in rows) / nd * 1024.0 * fscale
+        wb = sum(w[0] * w[1] for _, _, w in rows) / nd * 1024.0 * wscale
+        ms, by, nl = prof[cls]
+        res[f"{prefix}:{cls}"] = {"kernel": " + ".join(names), "hbm_bytes_per_launch": fb + wb,
+                                  "fetch_bytes": fb, "write_bytes": wb,
+                                  "algorithmic_bytes_per_launch": by / nl, "dispatches": nd}
+    with open(out_path, "w") as f:
+        json.dump(res, f, indent=1)
+    print(f"[bench] counter traffic of {len(res) - 1} classes -> {out_path}", file=sys.stderr,
+          flush=True)
 
 
 def copy_probe(mib: int) -> int:
