@@ -151,13 +151,15 @@ def main() -> int:
         with open(args.traffic_json) as f:
             tj = json.load(f)
         ent = tj.get(f"{W.name}:{args.precision}:{dom}")
-        # counter bytes of the same kernel only: an entry whose algorithmic
-        # bytes differ from this run's was measured on other code
-        if ent and abs(ent["algorithmic_bytes_per_launch"] * nl / by - 1.0) < 0.01:
+        # counter bytes of the same kernel only: an entry naming other
+        # kernels, or whose algorithmic bytes differ from this run's, was
+        # measured on other code
+        same = ent and ent.get("kernel") == " + ".join(CLASS_KERNELS.get(dom, ()))
+        if same and abs(ent["algorithmic_bytes_per_launch"] * nl / by - 1.0) < 0.01:
             traffic = ent["hbm_bytes_per_launch"]
         elif ent:
-            print(f"[bench] {args.traffic_json} entry for {dom} is stale "
-                  f"({ent['algorithmic_bytes_per_launch']:.4g} vs {by / nl:.4g} algorithmic bytes): "
+            print(f"[bench] {args.traffic_json} entry for {dom} is stale ({ent.get('kernel')}, "
+                  f"{ent['algorithmic_bytes_per_launch']:.4g} vs {by / nl:.4g} algorithmic bytes): "
                   "traffic not reported", file=sys.stderr, flush=True)
     except (OSError, ValueError):
         pass
