@@ -60,16 +60,17 @@ __device__ __forceinline__ void load_tw2(C2<T>* tw2, const C2<T>* __restrict__ t
 // i0 + 2c, i0 + 2c + 1 (i0 = 32 * blockIdx.x).  DAMP: line c is x-bin
 // i0 + c (i0 = 16 * blockIdx.x) of a plane pair's spectrum, times the damping
 // rates held as rates_o[m + (nx/2+1) * y].  blockIdx.z selects the field.
+constexpr int kY256Lsy = 18 * R16P - 16;  // 290 = 2 (mod 32)
+
+// One block of k_y256: x-chunk bx, plane (pair) k, field fld; S, tw2 and ml
+// are the block's LDS tiles.
 template <typename T, int MODE>
-__global__ __launch_bounds__(256) void k_y256(int nx, int nz, XArgs<T> X, T beta,
-                                              const T* __restrict__ mult,
-                                              const C2<T>* __restrict__ twg,
-                                              const T* __restrict__ rates_o) {
-  constexpr int NY = 256, LSY = 18 * R16P - 16;  // 290 = 2 (mod 32)
-  __shared__ C2<T> S[16 * LSY];
-  __shared__ C2<T> tw2[256];
-  __shared__ T ml[256];
-  const int fld = blockIdx.z;
+__device__ __forceinline__ void y256_body(int nx, int nz, const XArgs<T>& X, T beta,
+                                          const T* __restrict__ mult,
+                                          const C2<T>* __restrict__ twg,
+                                          const T* __restrict__ rates_o, C2<T>* S, C2<T>* tw2,
+                                          T* ml, int bx, long k, int fld) {
+  constexpr int NY = 256, LSY = kY256Lsy;
   const T* __restrict__ a = X.a[fld];
   const T* __restrict__ s = X.s[fld];
   T* out = X.out[fld];
@@ -79,18 +80,17 @@ __global__ __launch_bounds__(256) void k_y256(int nx, int nz, XArgs<T> X, T beta
     for (int e = threadIdx.x; e < NY; e += blockDim.x) ml[e] = mult[e];
   const int c = threadIdx.x & 15, w = threadIdx.x >> 4;
   const long nxy = (long)nx * NY;
-  const long k = blockIdx.y;
   C2<T>* Sl = S + c * LSY;
   C2<T> v[16];
   if (MODE == YM_DAMP) {
-    const int i0 = blockIdx.x * 16;
+    const int i0 = bx * 16;
 #pragma unroll
     for (int n = 0; n < 16; ++n) v[n] = spec[(k * NY + 16 * n + w) * nx + i0 + c];
   } else {
     // all 16 (or 32) loads are issued before the stage temp a + beta s is
     // formed; combining them inside the loop made every row wait for its
     // own loads
-    const long x0 = (long)blockIdx.x * 32 + 2 * c;
+    const long x0 = (long)bx * 32 + 2 * c;
     if (s) {
       C2<T> t[16];
 #pragma unroll
@@ -115,7 +115,7 @@ __global__ __launch_bounds__(256) void k_y256(int nx, int nz, XArgs<T> X, T beta
   exchange16(v, Sl, w);
   dft_reg<T, 16, -1>(v);  // v[k2] = Y[w + 16 k2]
   if (MODE == YM_DAMP) {
-    const int m = blockIdx.x * 16 + c;
+    const int m = bx * 16 + c;
     const int mi = (m <= nx / 2) ? m : nx - m;
     const long nh = nx / 2 + 1;
 #pragma unroll
@@ -137,11 +137,11 @@ __global__ __launch_bounds__(256) void k_y256(int nx, int nz, XArgs<T> X, T beta
   exchange16(v, Sl, w);
   dft_reg<T, 16, +1>(v);  // v[n2] = y-line value at w + 16 n2
   if (MODE == YM_DAMP) {
-    const int i0 = blockIdx.x * 16;
+    const int i0 = bx * 16;
 #pragma unroll
     for (int n = 0; n < 16; ++n) spec[(k * NY + w + 16 * n) * nx + i0 + c] = v[n];
   } else {
-    const long x0 = (long)blockIdx.x * 32 + 2 * c;
+    const long x0 = (long)bx * 32 + 2 * c;
 #pragma unroll
     for (int n = 0; n < 16; ++n) {
       const long o = x0 + (long)nx * (w + 16 * n) + nxy * k;
@@ -154,6 +154,43 @@ __global__ __launch_bounds__(256) void k_y256(int nx, int nz, XArgs<T> X, T beta
       }
       *reinterpret_cast<C2<T>*>(out + o) = z;
     }
+  }
+}
+
+template <typename T, int MODE>
+__global__ __launch_bounds__(256) void k_y256(int nx, int nz, XArgs<T> X, T beta,
+                                              const T* __restrict__ mult,
+                                              const C2<T>* __restrict__ twg,
+                                              const T* __restrict__ rates_o) {
+  __shared__ C2<T> S[16 * kY256Lsy];
+  __shared__ C2<T> tw2[256];
+  __shared__ T ml[256];
+  y256_body<T, MODE>(nx, nz, X, beta, mult, twg, rates_o, S, tw2, ml, blockIdx.x, blockIdx.y,
+                     blockIdx.z);
+}
+
+// The damping y pass of three fields (D: nbd blocks, x-chunks of 16 bins
+// fastest, then plane pairs, then fields) and the y derivative of one field
+// (G, the rest: x-chunks of 32 columns, then planes) in one launch, so the
+// blocks of the lighter derivative pass fill the damping pass's tail.
+template <typename T>
+__global__ __launch_bounds__(256) void k_y256_damp_deriv(int nx, int nz, XArgs<T> D, XArgs<T> G,
+                                                         const T* __restrict__ mult,
+                                                         const C2<T>* __restrict__ twg,
+                                                         const T* __restrict__ rates_o,
+                                                         int nbd) {
+  __shared__ C2<T> S[16 * kY256Lsy];
+  __shared__ C2<T> tw2[256];
+  __shared__ T ml[256];
+  int b = blockIdx.x;
+  if (b < nbd) {
+    const int nbx = nx / 16, np = (nz + 1) / 2;
+    const int bx = b % nbx, t = b / nbx;
+    y256_body<T, YM_DAMP>(nx, nz, D, (T)0, mult, twg, rates_o, S, tw2, ml, bx, t % np, t / np);
+  } else {
+    b -= nbd;
+    const int nbx = nx / 32;
+    y256_body<T, YM_DERIV>(nx, nz, G, (T)0, mult, twg, rates_o, S, tw2, ml, b % nbx, b / nbx, 0);
   }
 }
 
@@ -322,17 +359,18 @@ __device__ __forceinline__ void xw_inverse(C2<T> (&v)[4], const C2<T> (&w1)[3],
 // the damping inverse leave there.
 // LPW lines per wave, the next line's loads issued before the current
 // line's transform.
+// One 256-thread block of k_xw256: field fld; wave w takes lines
+// LPW (4 bx + w) .. LPW (4 bx + w) + LPW - 1.
 template <typename T, int MODE, int LPW>
-__global__ __launch_bounds__(256) void k_xw256(int nrows, int nlines, XArgs<T> X, T beta,
-                                               const T* __restrict__ mult,
-                                               const C2<T>* __restrict__ twg, int ny, int nz,
-                                               long nxy) {
+__device__ __forceinline__ void xw256_body(int nrows, int nlines, const XArgs<T>& X, T beta,
+                                           const T* __restrict__ mult,
+                                           const C2<T>* __restrict__ twg, int ny, int nz,
+                                           long nxy, int fld, int bx) {
   static_assert(MODE == XM_DERIV || MODE == XM_DAMP_FWD || MODE == XM_DAMP_INV ||
                     MODE == XM_DERIV_DAMPINV,
                 "k_xw256 modes");
   constexpr bool DD = MODE == XM_DERIV_DAMPINV;
   constexpr int N = 256;
-  const int fld = blockIdx.y;
   const T* __restrict__ a = X.a[fld];
   const T* __restrict__ s = X.s[fld];
   T* __restrict__ out = X.out[fld];
@@ -353,7 +391,7 @@ __global__ __launch_bounds__(256) void k_xw256(int nrows, int nlines, XArgs<T> X
   };
   const int lane = threadIdx.x & 63;
   const int wv = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
-  const long L0 = ((long)blockIdx.x * 4 + wv) * LPW;
+  const long L0 = ((long)bx * 4 + wv) * LPW;
   // per-lane twiddles of the three twiddled stages and the multiplier of the
   // four spectral values this lane holds
   C2<T> w1[3], w2[3], w3[3];
@@ -560,6 +598,31 @@ __global__ __launch_bounds__(256) void k_xw256(int nrows, int nlines, XArgs<T> X
   }
 #undef XW_LOAD
 #undef XW_LOAD_INV
+}
+
+template <typename T, int MODE, int LPW>
+__global__ __launch_bounds__(256) void k_xw256(int nrows, int nlines, XArgs<T> X, T beta,
+                                               const T* __restrict__ mult,
+                                               const C2<T>* __restrict__ twg, int ny, int nz,
+                                               long nxy) {
+  xw256_body<T, MODE, LPW>(nrows, nlines, X, beta, mult, twg, ny, nz, nxy, blockIdx.y, blockIdx.x);
+}
+
+// The last two x passes of the option-4 stage in one launch, so the blocks of
+// the lighter pass fill the other's tail: blockIdx.y = 0 the x derivative of
+// p as the momentum RHS with the damping inverse of v_x (XM_DERIV_DAMPINV on
+// H), 1 and 2 the damping inverse of fields 0 and 1 of D (XM_DAMP_INV).
+template <typename T, int LPW>
+__global__ __launch_bounds__(256) void k_xw256_rhs_dampinv(int nrows, int nlines, XArgs<T> H,
+                                                           XArgs<T> D, const T* __restrict__ mult,
+                                                           const C2<T>* __restrict__ twg, int ny,
+                                                           int nz, long nxy) {
+  if (blockIdx.y == 0)
+    xw256_body<T, XM_DERIV_DAMPINV, LPW>(nrows, nlines, H, (T)0, mult, twg, ny, nz, nxy, 0,
+                                         blockIdx.x);
+  else
+    xw256_body<T, XM_DAMP_INV, LPW>(nrows, nlines, D, (T)0, mult, twg, ny, nz, nxy,
+                                    blockIdx.y - 1, blockIdx.x);
 }
 
 // DAMP_VELOCITY (damping.f90:125-185) on one 256 x 256 plane per block:

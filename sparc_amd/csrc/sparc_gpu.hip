@@ -936,16 +936,38 @@ static int launch_damp2d(sparc_ctx* c, const XArgs<T>& X) {
   }
 }
 
+// the damping y pass of three fields (D) and the y derivative of p as the
+// momentum RHS (G) in one launch (256-long y lines, nx a multiple of 32)
 template <typename T>
-static int launch_x_rhs_damp(sparc_ctx* c, const XArgs<T>& X, const T* mult) {
+static bool y_damp_deriv_fits(const sparc_ctx* c) {
+  return c->cfg.ny == 256 && c->cfg.nx % 32 == 0;
+}
+
+template <typename T>
+static int launch_y_damp_deriv(sparc_ctx* c, const XArgs<T>& D, const XArgs<T>& G, const T* mult) {
+  const int nbd = (c->cfg.nx / 16) * ((c->cfg.nz + 1) / 2) * 3;
+  const int nbg = (c->cfg.nx / 32) * c->cfg.nz;
+  // three spectra read and written; p read, RHS_vy written
+  ProfScope ps(c, SPARC_KC_DAMP_Y, 8.0 * (double)c->N * sizeof(T));
+  hipLaunchKernelGGL((k_y256_damp_deriv<T>), dim3(nbd + nbg), dim3(256), 0, c->stream,
+                     c->cfg.nx, c->cfg.nz, D, G, mult, (const C2<T>*)c->twy,
+                     (const T*)c->rates_o, nbd);
+  LAUNCHCHK(c);
+  return SPARC_OK;
+}
+
+// the x derivative of p as the momentum RHS with the v_x damping inverse (H)
+// and the damping inverse of two more fields (D) in one launch
+template <typename T>
+static int launch_x_rhs_dampinv(sparc_ctx* c, const XArgs<T>& H, const XArgs<T>& D,
+                                const T* mult) {
   const long nlines = (long)((c->cfg.nz + 1) / 2) * c->cfg.ny;
   const long nrows = (long)c->cfg.ny * c->cfg.nz;
-  // p, the damping spectrum, the RHS written
-  ProfScope ps(c, SPARC_KC_XLINES, 3.0 * (double)c->N * sizeof(T));
-  hipLaunchKernelGGL((k_xw256<T, XM_DERIV_DAMPINV, kXwLpw>),
-                     dim3(cdiv(nlines, 4 * kXwLpw), 1), dim3(256), 0, c->stream, (int)nrows,
-                     (int)nlines, X, (T)0, mult, (const C2<T>*)c->twx, c->cfg.ny, c->cfg.nz,
-                     c->nxy);
+  // p, the damping spectrum, the RHS written; two spectra, two RHS read and written
+  ProfScope ps(c, SPARC_KC_DAMP_XI, 9.0 * (double)c->N * sizeof(T));
+  hipLaunchKernelGGL((k_xw256_rhs_dampinv<T, kXwLpw>), dim3(cdiv(nlines, 4 * kXwLpw), 3),
+                     dim3(256), 0, c->stream, (int)nrows, (int)nlines, H, D, mult,
+                     (const C2<T>*)c->twx, c->cfg.ny, c->cfg.nz, c->nxy);
   LAUNCHCHK(c);
   return SPARC_OK;
 }
@@ -1231,6 +1253,7 @@ static int run_stage(sparc_ctx* c, Fld<T>& F, int s) {
     // in one z-solve launch (k_zsolve_pv), RHS_vz staged in W4 for the
     // damping inverse; the damping forward pass then follows it
     const bool pv = fuse && zpv_fits<T>(c);
+    const bool ydd = pv && y_damp_deriv_fits<T>(c);
     XArgs<T> DX{};
     for (int f = 0; f < 3; ++f) {
       DX.out[f] = F.scr[3 + f];
@@ -1249,7 +1272,9 @@ static int run_stage(sparc_ctx* c, Fld<T>& F, int s) {
         DX.cp[f] = F.scr[f];
       }
       if ((rc = launch_xm<T, XM_DAMP_FWD>(c, 3, DX, beta, nullptr))) return rc;
-      if ((rc = launch_ym<T, YM_DAMP>(c, 3, DX, 0, nullptr))) return rc;
+      // with the fused z solve the damping y pass runs with the y derivative
+      // of p below
+      if (!ydd && (rc = launch_ym<T, YM_DAMP>(c, 3, DX, 0, nullptr))) return rc;
       // ddxyz(p, bc=0) (displacement.f90:547): the z part with RHS_vz
       if (!pv &&
           (rc = launch_z<T, EPI_RHS3N>(c, F, 0, F.W[3], nullptr, 0, nullptr, 0, nullptr)))
@@ -1271,13 +1296,16 @@ static int run_stage(sparc_ctx* c, Fld<T>& F, int s) {
       if ((rc = launch_xm<T, XM_DERIV>(c, 1, G, 0, dmx))) return rc;
     }
     G.out[0] = F.scr[SPARC_V_Y];
-    if ((rc = launch_ym<T, YM_DERIV>(c, 1, G, 0, dmy))) return rc;
+    if (ydd) {
+      if ((rc = launch_y_damp_deriv<T>(c, DX, G, dmy))) return rc;
+    } else if ((rc = launch_ym<T, YM_DERIV>(c, 1, G, 0, dmy))) {
+      return rc;
+    }
     if (fuse) {
       XArgs<T> H = G;
       H.out[0] = F.scr[SPARC_V_X];
       H.spec[0] = DX.spec[0];
       H.edges = DX.edges;
-      if ((rc = launch_x_rhs_damp<T>(c, H, dmx))) return rc;
       XArgs<T> D2{};
       for (int f = 0; f < 2; ++f) {
         D2.out[f] = DX.out[f + 1];
@@ -1285,7 +1313,7 @@ static int run_stage(sparc_ctx* c, Fld<T>& F, int s) {
       }
       if (pv) D2.rin[1] = F.W[4];  // RHS_vz before the damping
       D2.edges = DX.edges;
-      if ((rc = launch_xm<T, XM_DAMP_INV>(c, 2, D2, 0, nullptr))) return rc;
+      if ((rc = launch_x_rhs_dampinv<T>(c, H, D2, dmx))) return rc;
     } else if (damp) {
       // DAMP_VELOCITY (damping.f90:125-185) on the stage's temp v, which the
       // epilogue above has just stored as RHS_xi = v (zero on planes 1 and nz,
