@@ -1142,11 +1142,23 @@ static int launch_z(sparc_ctx* c, const Fld<T>& F, int ibc, const T* inA, const 
 // per segment.  Writes W3 = p, rvz = RHS_vz before the damping (zero
 // planes: RHS_vz itself) and both PML psi.
 static constexpr int kPvLmax = 20;
+// The streaming form (k_zstream_pv): whole blocks of 64-column waves, its
+// plane table and rings within the LDS, every background of the two
+// epilogues a z profile.
+static bool zstream_fits(const sparc_ctx* c) {
+  for (int id : {SPARC_BG_C2RHO0, SPARC_BG_GRADP0_Z, SPARC_BG_RHO0, SPARC_BG_GRADRHO0_Z,
+                 SPARC_BG_RHOINV, SPARC_BG_AZ, SPARC_BG_BZPML})
+    if (c->bg3d[id] || !c->bg[id]) return false;
+  return c->fp32 && c->nxy % (64 * kZsWaves) == 0 && c->cfg.nz >= 8 &&
+         (size_t)c->N * 4 < ((size_t)1 << 31) &&
+         zstream_lds_bytes(c->cfg.nz) + kZsRingBytes <= (size_t)c->lds_max;
+}
 template <typename T>
 static bool zpv_fits(const sparc_ctx* c) {
   const int nz = c->cfg.nz, L = (nz + kZbNseg - 1) / kZbNseg;
-  return sizeof(T) == 4 && nz >= 5 && L <= kPvLmax &&
-         2 * 64 * (size_t)nz * sizeof(T) <= (size_t)c->lds_max;
+  return sizeof(T) == 4 && nz >= 5 &&
+         (zstream_fits(c) ||
+          (L <= kPvLmax && 2 * 64 * (size_t)nz * sizeof(T) <= (size_t)c->lds_max));
 }
 
 template <typename T>
@@ -1167,9 +1179,63 @@ static int launch_zpv(sparc_ctx* c, const Fld<T>& F, const T* inA, const T* inS,
     ProfScope ps(c, SPARC_KC_ZSOLVE_P, nb * (double)c->N * sizeof(T));
     const bool bg3 = c->bg3d[SPARC_BG_C2RHO0] || c->bg3d[SPARC_BG_GRADP0_Z] ||
                      c->bg3d[SPARC_BG_RHO0] || c->bg3d[SPARC_BG_GRADRHO0_Z];
-    const size_t lds = 2 * 64 * (size_t)nz * sizeof(T);
     const T* t1 = (const T*)c->tri[1];
     const T* t0 = (const T*)c->tri[0];
+    if (zstream_fits(c)) {
+      ZsArgs Z{};
+      Z.a = inA;
+      Z.s = inS;
+      Z.w0 = F.W[0];
+      Z.w1 = F.W[1];
+      Z.p = F.W[3];
+      Z.rvz = rvz;
+      Z.scrvz = F.scr[SPARC_V_Z];
+      for (int w = 0; w < 2; ++w) {
+        Z.pv[w] = F.pmlvars[w];
+        Z.ps[w] = F.scrpml[w];
+      }
+      Z.tab[ZT_ST] = F.stretch;
+      Z.tab[ZT_C2R] = F.bg[SPARC_BG_C2RHO0];
+      Z.tab[ZT_GP] = F.bg[SPARC_BG_GRADP0_Z];
+      Z.tab[ZT_R0] = F.bg[SPARC_BG_RHO0];
+      Z.tab[ZT_GR] = F.bg[SPARC_BG_GRADRHO0_Z];
+      Z.tab[ZT_G] = F.g;
+      Z.tab[ZT_AZ] = F.bg[SPARC_BG_AZ];
+      Z.tab[ZT_BZ] = F.bg[SPARC_BG_BZPML];
+      Z.tab[ZT_LOWR1] = t1;
+      Z.tab[ZT_INVB1] = t1 + nz;
+      Z.tab[ZT_LOWR0] = t0;
+      Z.tab[ZT_INVB0] = t0 + nz;
+      Z.tab[ZT_GAM1] = t1 + 2 * nz;
+      Z.tab[ZT_GAM0] = t0 + 2 * nz;
+      Z.tab[ZT_RI] = F.bg[SPARC_BG_RHOINV];
+      Z.forcing = F.forcing;
+      Z.nbytes = (unsigned)(c->N * 4);
+      Z.pbytes = (unsigned)(c->nxy * (long)c->nzpml * 4);
+      Z.nxy = (int)c->nxy;
+      Z.n = nz;
+      Z.npmlbot = F.npmlbot;
+      Z.npmltop = F.npmltop;
+      Z.src_plane = F.src_plane;
+      Z.beta = F.beta;
+      Z.beta_in = beta;
+      Z.delta_width = F.delta_width;
+      for (int ibc = 0; ibc < 2; ++ibc) {
+        const TriCoef<T> tq = make_tc<T>(c, ibc);
+        Z.kc0[ibc] = tq.kc0;
+        Z.kc1[ibc] = tq.kc1;
+        Z.lw[ibc] = tq.lw;
+        Z.ib[ibc] = tq.ib;
+        Z.gm[ibc] = tq.gm;
+      }
+      // columns long enough for two waves each (one from each end)
+      Z.split = nz >= 96;
+      hipLaunchKernelGGL(k_zstream_pv, dim3(cdiv(c->nxy, 64 * kZsWaves)), dim3(128 * kZsWaves),
+                         zstream_lds_bytes(nz), c->stream, Z, make_tc<T>(c, 1));
+      LAUNCHCHK(c);
+      return SPARC_OK;
+    }
+    const size_t lds = 2 * 64 * (size_t)nz * sizeof(T);
 #define ZPV(G)                                                                                \
   hipLaunchKernelGGL((k_zsolve_pv<kZbNseg, G, kPvLmax>), dim3(cdiv(c->nxy, 64)),              \
                      dim3(64 * kZbNseg), lds, c->stream, F, make_tc<T>(c, 1), t1, t0, inA, inS, \

@@ -971,3 +971,451 @@ __global__ __launch_bounds__(64 * NSEG) void k_zsolve_pv(Fld<float> F, TriCoef<f
     }
   }
 }
+
+// ---------------------------------------------------------------------------
+// Streaming z solves of option 4's pressure and dp/dz (FP32).
+//
+// The same two solves and epilogues as k_zsolve_pv, organised so that one
+// wave walks its 64 columns from the first plane to the last with the HBM
+// loads of the next chunk in flight behind the arithmetic of the current one
+// (the column-tile kernels instead run their load, sweep and epilogue phases
+// one after another, so HBM idles during the sweeps).
+//
+// DBYD2 + MTRIDAG (dbyd2.f90:88-199, mtridag.f90:94-111) per column:
+//   forward  u_k = (r_k - lowr_k u_{k-1}) / betad_k   from row 0, exact;
+//   backward x_k = u_k - gam_{k+1} x_{k+1}            from row n-1.
+// The backward sweep runs per chunk of C output planes: it starts G planes
+// below the chunk's last plane from x = u there (exact when that plane is
+// row n-1).  The start error decays by gam ~ 0.382 per plane, so the outputs
+// differ from the sequential solve by <= 0.382^20 ~ 4e-9 relative, below
+// FP32 rounding.
+//
+// Iteration with chunk start k0 (planes k0 .. k0+C-1):
+//   fwd1  forward sweep of d xi_z/dz (ibc 1) over planes [k0+G, k0+G+C); the
+//         five-point right-hand side takes B = a + beta s of xi_z from a
+//         register window; B is also kept in an LDS ring for the epilogue
+//   bwd1  backward sweep over [k0, k0+C+G) -> x of planes [k0, k0+C)
+//   ep1   the pressure epilogue (displacement.f90:495-535): PML psi of
+//         dxi_z/dz, p -> W3, rho' -> LDS ring
+//   fwd2  forward sweep of dp/dz (ibc 0) over [k0-2, k0+C-2) (p of the
+//         planes up to k0+C-1 is known)
+//   bwd2  backward sweep over [k0-D, k0-D+C+G), D = G+2 -> planes
+//         [k0-D, k0-D+C)
+//   ep2   the momentum epilogue (displacement.f90:547-597): PML psi of dp/dz,
+//         RHS_vz = -rhoinv (dp/dz + rho' g) + source -> rvz; planes 1 and nz:
+//         RHS_vz = 0
+// Per-plane coefficients (LU factors of both solves, stretch, background
+// profiles, g, rhoinv, PML profiles) sit in an LDS table shared by the
+// block's waves.  The main loop body holds two iterations so the register
+// buffers of the next chunk's loads alternate without copies.
+//
+// Every HBM access is a buffer access whose plane offset is a scalar; planes
+// outside the column (and the scratch of the first stage, and the PML psi of
+// non-PML planes) get an offset beyond the buffer, so the load returns 0 and
+// the store is dropped by the hardware range check.  The sequence of memory
+// instructions is then the same on every path through an iteration, which
+// keeps the compiler's count of loads in flight exact: a wave waits only for
+// the load it is about to use, never for the next chunk's.
+static constexpr int kZsC = 10;     // output planes per chunk
+static constexpr int kZsG = 20;     // truncated backward start, planes
+static constexpr int kZsRing = 64;  // LDS ring slots (planes) per wave for B of xi_z
+static constexpr int kZsWaves = 4;  // column groups per block (two waves each)
+static constexpr int kZsTab = 16;   // table floats per plane
+// table slots per plane
+enum { ZT_ST, ZT_C2R, ZT_GP, ZT_R0, ZT_GR, ZT_G, ZT_AZ, ZT_BZ, ZT_LOWR1, ZT_INVB1, ZT_LOWR0,
+       ZT_INVB0, ZT_GAM1, ZT_GAM0, ZT_RI, ZT_NUM };
+
+// dynamic LDS: the table (the rings are a static array)
+__host__ __device__ constexpr size_t zstream_lds_bytes(int n) {
+  return (size_t)n * kZsTab * sizeof(float);
+}
+static constexpr size_t kZsRingBytes = (size_t)2 * kZsWaves * kZsRing * 64 * sizeof(float);
+
+// Buffer view of an array of `bytes` bytes (0: every access out of range).
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t zrsrc(const void* p, unsigned bytes) {
+  return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(p), (short)0, (int)bytes,
+                                           0x00020000);
+}
+__device__ __forceinline__ float zbl(__amdgpu_buffer_rsrc_t r, unsigned voff, unsigned soff) {
+  return __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(r, (int)voff, (int)soff, 0));
+}
+__device__ __forceinline__ void zbs(__amdgpu_buffer_rsrc_t r, unsigned voff, unsigned soff,
+                                    float v) {
+  __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(v), r, (int)voff, (int)soff, 0);
+}
+
+// Arguments of k_zstream_pv: the arrays it streams and the launch geometry.
+struct ZsArgs {
+  const float* a;     // xi_z
+  const float* s;     // its scratch (null in the first stage)
+  const float* w0;    // d xi_x/dx
+  const float* w1;    // d xi_y/dy
+  float* p;           // W3
+  float* rvz;         // RHS_vz before the damping
+  float* scrvz;       // RHS_vz itself (zero planes 1 and nz)
+  const float* pv[2]; // PML psi of d xi_z/dz, dp/dz
+  float* ps[2];       // their scratch (read in stages 2-5, written always)
+  const float* tab[ZT_NUM];  // per-plane table sources (n values each)
+  const float* forcing;
+  unsigned nbytes;    // bytes of one 3-D field
+  unsigned pbytes;    // bytes of one PML psi array
+  int nxy, n, npmlbot, npmltop, src_plane;
+  int split;          // 1: two waves per column group, one from each end
+  float beta, beta_in, delta_width;
+  // per ibc (0: dp/dz, 1: d xi_z/dz): rows [kc0, kc1) where the LU factors
+  // are one value each (lowr = lw, 1/betad = ib, gam = gm, bitwise)
+  int kc0[2], kc1[2];
+  float lw[2], ib[2], gm[2];
+};
+
+// LDS of k_zstream_pv: the per-plane table (dynamic, n rows of kZsTab) and
+// the B rings of the block's waves (static)
+extern __shared__ __align__(16) float zs_tab[];
+static __shared__ float zs_ring[2 * kZsWaves * kZsRing * 64];
+
+// One wave's walk along its column (DIR 0: from plane 1 down the index,
+// DIR 1: from plane nz up, on the reflected column; local plane t is global
+// plane t or nz-1-t).  h: planes the wave owns (local 0 .. h-1).
+template <int DIR>
+__device__ __forceinline__ void zs_walk(const ZsArgs& Z, const TriCoef<float>& tc, int c, int wv,
+                                        int h) {
+  using T = float;
+  constexpr int C = kZsC, G = kZsG, D = G + 2, WIN = C + G;
+  const int n = Z.n;
+  const int lane = threadIdx.x & 63;
+  const int oxz = wv * (kZsRing * 64) + lane;  // this wave's B ring
+  const bool st_on = Z.beta != (T)0;
+  const T beta = Z.beta, beta_in = Z.beta_in;
+  const __amdgpu_buffer_rsrc_t rA = zrsrc(Z.a, Z.nbytes), rS = zrsrc(Z.s, Z.s ? Z.nbytes : 0u),
+                               rW0 = zrsrc(Z.w0, Z.nbytes), rW1 = zrsrc(Z.w1, Z.nbytes),
+                               rP = zrsrc(Z.p, Z.nbytes), rV = zrsrc(Z.rvz, Z.nbytes),
+                               rVe = zrsrc(Z.scrvz, Z.nbytes);
+  const unsigned colb = (unsigned)c * 4u, planeb = (unsigned)Z.nxy * 4u;
+  constexpr unsigned OOB = 0x7ffffff0u;
+  auto kg = [&](int t) -> int { return DIR ? n - 1 - t : t; };
+  // scalar byte offsets of local plane t: loads (out of range outside the
+  // column), stores (outside the planes the wave owns)
+  auto so = [&](int t) -> unsigned { return (t >= 0 && t < n) ? (unsigned)kg(t) * planeb : OOB; };
+  auto sown = [&](int t) -> unsigned {
+    return (t >= 0 && t < h) ? (unsigned)kg(t) * planeb : OOB;
+  };
+  auto pmlg = [&](int t) -> int {
+    return (t >= 0 && t < n) ? pml_index(kg(t), n, Z.npmlbot, Z.npmltop) : -1;
+  };
+  // PML extents in local planes: at the wave's starting end, at the far end
+  const int pnear = DIR ? Z.npmltop : Z.npmlbot, pfar = DIR ? Z.npmlbot : Z.npmltop;
+  auto no_pml = [&](int t0) { return t0 >= pnear && t0 + C <= n - pfar; };
+  // the source term of the source plane (displacement.f90:625-632)
+  const T fsrc = Z.src_plane >= 0 ? Z.forcing[c] / Z.tab[ZT_R0][Z.src_plane] * Z.delta_width
+                                  : (T)0;
+  const int tsrc = Z.src_plane >= 0 ? kg(Z.src_plane) : -1000;  // local source plane
+  const T sg = DIR ? (T)-1 : (T)1;  // d/dz of the reflected column is -d/dz
+
+  // ---- per-lane state
+  // U1, U2: forward values, then backward results, of the window planes;
+  // Bc: B of rows f-4 .. f+3 (f: first forward row of the next iteration);
+  // P: p of local planes k0-5 .. k0+C-1; R: rho' of local planes
+  // k0-D .. k0+C-1; A, S, X, Y: loads of the current and the next chunk
+  T U1[WIN], U2[WIN], P[5 + C], Bc[8], R[C + D];
+  T A[2][C], S[2][C], X[2][C], Y[2][C];
+#pragma unroll
+  for (int j = 0; j < WIN; ++j) U1[j] = U2[j] = 0;
+#pragma unroll
+  for (int j = 0; j < 5 + C; ++j) P[j] = 0;
+#pragma unroll
+  for (int j = 0; j < C + D; ++j) R[j] = 0;
+  // prologue: B of rows 0..3 and the stream of the first iteration (rows 4..C+3)
+#pragma unroll
+  for (int i = 0; i < C; ++i) {
+    A[0][i] = zbl(rA, colb, so(4 + i));
+    S[0][i] = zbl(rS, colb, so(4 + i));
+  }
+#pragma unroll
+  for (int j = 0; j < 4; ++j) {
+    Bc[j] = 0;
+    Bc[4 + j] = zbl(rA, colb, so(j)) + beta_in * zbl(rS, colb, so(j));
+    zs_ring[oxz + j * 64] = Bc[4 + j];
+  }
+  T u1p = 0, u2p = 0, rn2 = 0;
+
+  auto iter = [&](int k0, auto par_c) {
+    constexpr int PA = decltype(par_c)::value, PB = 1 - PA;
+    // ======== fwd1: rows f .. f+C-1 of d xi_z/dz (ibc 1)
+    const int f = k0 + G;
+    T Bx[C + 8];  // B of rows f-4 .. f+C+3
+#pragma unroll
+    for (int j = 0; j < 8; ++j) Bx[j] = Bc[j];
+#pragma unroll
+    for (int i = 0; i < C; ++i) {
+      const int kb = f + 4 + i;
+      const T Bn = A[PA][i] + beta_in * S[PA][i];  // 0 beyond the column
+      zs_ring[oxz + (kb & (kZsRing - 1)) * 64] = Bn;
+      Bx[8 + i] = Bn;
+      A[PB][i] = zbl(rA, colb, so(kb + C));
+      S[PB][i] = zbl(rS, colb, so(kb + C));
+    }
+#pragma unroll
+    for (int j = 0; j < 8; ++j) Bc[j] = Bx[C + j];
+    if (f >= max(2, Z.kc0[1]) && f + C <= min(n - 2, Z.kc1[1])) {
+      // interior rows, constant LU factors
+#pragma unroll
+      for (int i = 0; i < C; ++i) {
+        const T r = tc.ci1 * (Bx[i + 5] - Bx[i + 3]) + tc.ci2 * (Bx[i + 6] - Bx[i + 2]);
+        u1p = (r - Z.lw[1] * u1p) * Z.ib[1];
+        U1[G + i] = u1p;
+      }
+    } else {
+#pragma unroll
+      for (int i = 0; i < C; ++i) {
+        const int k = f + i;
+        if (k < n) {
+          T r;
+          if (k >= 2 && k <= n - 3) {
+            r = tc.ci1 * (Bx[i + 5] - Bx[i + 3]) + tc.ci2 * (Bx[i + 6] - Bx[i + 2]);
+          } else if (k == 0) {
+            const T r1 = tc.c1 * Bx[i + 4] + tc.c2 * Bx[i + 5] + tc.c3 * Bx[i + 6] +
+                         tc.c4 * Bx[i + 7] + tc.c5 * Bx[i + 8];
+            const T r0 = tc.b1 * Bx[i + 4] + tc.b2 * Bx[i + 5] + tc.b3 * Bx[i + 6] +
+                         tc.b4 * Bx[i + 7];
+            r = (tc.alpha2 * r0 - tc.beta1 * r1) * tc.cst;
+          } else if (k == 1) {
+            r = tc.c1 * Bx[i + 3] + tc.c2 * Bx[i + 4] + tc.c3 * Bx[i + 5] + tc.c4 * Bx[i + 6] +
+                tc.c5 * Bx[i + 7];
+          } else if (k == n - 2) {
+            rn2 = -(tc.c1 * Bx[i + 5] + tc.c2 * Bx[i + 4] + tc.c3 * Bx[i + 3] +
+                    tc.c4 * Bx[i + 2] + tc.c5 * Bx[i + 1]);
+            r = rn2;
+          } else {  // k == n - 1
+            const T rn1 = -(tc.b1 * Bx[i + 4] + tc.b2 * Bx[i + 3] + tc.b3 * Bx[i + 2] +
+                            tc.b4 * Bx[i + 1]);
+            r = (tc.alpha2 * rn1 - tc.beta1 * rn2) * tc.cst;
+          }
+          const float2 lv = *reinterpret_cast<const float2*>(&zs_tab[k * kZsTab + ZT_LOWR1]);
+          u1p = (r - lv.x * u1p) * lv.y;
+          U1[G + i] = u1p;
+        }
+      }
+    }
+    // PML psi of the two epilogues, requested before the backward sweeps
+    const bool pml1 = k0 + C > 0 && k0 < h && !no_pml(k0);
+    const bool pml2 = k0 - D + C > 0 && k0 - D < h && !no_pml(k0 - D);
+    T pv1[C], ps1[C], pv2[C], ps2[C];
+    int pi1[C], pi2[C];
+    auto pml_loads = [&](int t0, int w, T* pv, T* ps, int* pi) {
+      const __amdgpu_buffer_rsrc_t r = zrsrc(Z.pv[w], Z.pbytes),
+                                   q = zrsrc(Z.ps[w], st_on ? Z.pbytes : 0u);
+#pragma unroll
+      for (int i = 0; i < C; ++i) {
+        pi[i] = pmlg(t0 + i);
+        const unsigned o = pi[i] >= 0 ? (unsigned)pi[i] * planeb : OOB;
+        pv[i] = zbl(r, colb, o);
+        ps[i] = zbl(q, colb, o);
+      }
+    };
+    if (pml1) pml_loads(k0, 0, pv1, ps1, pi1);
+    if (pml2) pml_loads(k0 - D, 1, pv2, ps2, pi2);
+    // ======== bwd1 over the window [k0, k0+WIN) -> x of [k0, k0+C)
+    if (k0 + C > 0) {
+      if (k0 >= 0 && k0 + WIN < n && k0 + 1 >= Z.kc0[1] && k0 + WIN <= Z.kc1[1]) {
+        T x = U1[WIN - 1];
+#pragma unroll
+        for (int j = WIN - 2; j >= 0; --j) {
+          x = U1[j] - Z.gm[1] * x;
+          if (j < C) U1[j] = x;
+        }
+      } else {
+        const int e = min(k0 + WIN, n) - 1;
+        T x = 0;
+#pragma unroll
+        for (int j = WIN - 1; j >= 0; --j) {
+          const int k = k0 + j;
+          if (k >= 0 && k <= e) {
+            x = (k == e) ? U1[j] : U1[j] - zs_tab[(k + 1) * kZsTab + ZT_GAM1] * x;
+            if (j < C) U1[j] = x;
+          }
+        }
+      }
+    }
+    // ======== ep1: planes [k0, k0+C)
+    {
+      T pout[C];
+      if (k0 >= 0 && k0 + C <= h && !pml1) {
+#pragma unroll
+        for (int i = 0; i < C; ++i) {
+          const int k = k0 + i, kt = kg(k);
+          const float4 t4 = *reinterpret_cast<const float4*>(&zs_tab[kt * kZsTab]);  // st c2r gp r0
+          const T gr = zs_tab[kt * kZsTab + ZT_GR];
+          const T dz = U1[i] * (sg * t4.x);
+          const T div = X[PA][i] + Y[PA][i] + dz;
+          const T xz = zs_ring[oxz + (k & (kZsRing - 1)) * 64];
+          pout[i] = -t4.y * div - xz * t4.z;
+          R[D + i] = -t4.w * div - xz * gr;
+        }
+      } else if (k0 + C > 0 && k0 < n) {
+#pragma unroll
+        for (int i = 0; i < C; ++i) {
+          const int k = k0 + i, kt = kg(min(max(k, 0), n - 1));
+          const float4 t4 = *reinterpret_cast<const float4*>(&zs_tab[kt * kZsTab]);
+          const T gr = zs_tab[kt * kZsTab + ZT_GR];
+          T dz = U1[i] * (sg * t4.x);
+          if (pml1) {
+            const T psi = pv1[i] + beta * ps1[i];
+            const float2 ab = *reinterpret_cast<const float2*>(&zs_tab[kt * kZsTab + ZT_AZ]);
+            zbs(zrsrc(Z.ps[0], Z.pbytes), colb,
+                (pi1[i] >= 0 && k < h) ? (unsigned)pi1[i] * planeb : OOB,
+                ab.x * dz + ab.y * psi);
+            if (pi1[i] >= 0) dz = dz + psi;
+          }
+          const T div = X[PA][i] + Y[PA][i] + dz;
+          const T xz = zs_ring[oxz + (k & (kZsRing - 1)) * 64];
+          pout[i] = -t4.y * div - xz * t4.z;
+          R[D + i] = -t4.w * div - xz * gr;
+        }
+      }
+#pragma unroll
+      for (int i = 0; i < C; ++i) {
+        const int k = k0 + i;
+        zbs(rP, colb, sown(k), pout[i]);
+        P[5 + i] = pout[i];
+        X[PB][i] = zbl(rW0, colb, so(k + C));
+        Y[PB][i] = zbl(rW1, colb, so(k + C));
+      }
+    }
+    // ======== fwd2: rows m = k0-2+i of dp/dz (ibc 0); P[j] = p of plane k0-5+j
+    const int f2 = k0 - 2;
+    if (f2 >= max(2, Z.kc0[0]) && f2 + C <= min(n - 2, Z.kc1[0])) {
+#pragma unroll
+      for (int i = 0; i < C; ++i) {
+        const T r = tc.ci1 * (P[i + 4] - P[i + 2]) + tc.ci2 * (P[i + 5] - P[i + 1]);
+        u2p = (r - Z.lw[0] * u2p) * Z.ib[0];
+        U2[G + i] = u2p;
+      }
+    } else if (f2 + C > 0) {
+#pragma unroll
+      for (int i = 0; i < C; ++i) {
+        const int m = f2 + i;
+        if (m >= 0 && m < n) {
+          T r = 0;
+          if (m >= 2 && m <= n - 3) {
+            r = tc.ci1 * (P[i + 4] - P[i + 2]) + tc.ci2 * (P[i + 5] - P[i + 1]);
+          } else if (m == 1) {
+            // (only i = 3 of k0 = 0 reaches this row; the index bound keeps
+            // the unrolled copies of the other i in range)
+            r = tc.c1 * P[i + 2] + tc.c2 * P[i + 3] + tc.c3 * P[i + 4] + tc.c4 * P[i + 5] +
+                tc.c5 * P[i + 6 < 5 + C ? i + 6 : 4 + C];
+          } else if (m == n - 2) {
+            r = -(tc.c1 * P[i + 4] + tc.c2 * P[i + 3] + tc.c3 * P[i + 2] + tc.c4 * P[i + 1] +
+                  tc.c5 * P[i]);
+          }  // rows 0 and n-1: r = 0 (ibc 0)
+          const float2 lv = *reinterpret_cast<const float2*>(&zs_tab[m * kZsTab + ZT_LOWR0]);
+          u2p = (r - lv.x * u2p) * lv.y;
+          U2[G + i] = u2p;
+        }
+      }
+    }
+#pragma unroll
+    for (int j = 0; j < 5; ++j) P[j] = P[C + j];
+    // ======== bwd2 over [k0-D, k0-D+WIN) -> x of [k0-D, k0-D+C); U2[j] = plane k0-D+j
+    const int e0 = k0 - D;
+    if (e0 + C > 0) {
+      if (e0 >= 0 && e0 + WIN < n && e0 + 1 >= Z.kc0[0] && e0 + WIN <= Z.kc1[0]) {
+        T x = U2[WIN - 1];
+#pragma unroll
+        for (int j = WIN - 2; j >= 0; --j) {
+          x = U2[j] - Z.gm[0] * x;
+          if (j < C) U2[j] = x;
+        }
+      } else {
+        const int e = min(e0 + WIN, n) - 1;
+        T x = 0;
+#pragma unroll
+        for (int j = WIN - 1; j >= 0; --j) {
+          const int k = e0 + j;
+          if (k >= 0 && k <= e) {
+            x = (k == e) ? U2[j] : U2[j] - zs_tab[(k + 1) * kZsTab + ZT_GAM0] * x;
+            if (j < C) U2[j] = x;
+          }
+        }
+      }
+      // ======== ep2: planes [k0-D, k0-D+C)
+      T vout[C];
+      bool edge[C];
+      if (e0 >= 1 && e0 + C <= min(h, n - 1) && !pml2) {
+#pragma unroll
+        for (int i = 0; i < C; ++i) {
+          const int k = e0 + i, kt = kg(k);
+          const float2 sg2 = *reinterpret_cast<const float2*>(&zs_tab[kt * kZsTab + ZT_ST]);
+          const T gz = U2[i] * (sg * sg2.x);
+          const T ri = zs_tab[kt * kZsTab + ZT_RI];
+          T v = -ri * gz;
+          v = v - R[i] * zs_tab[kt * kZsTab + ZT_G] * ri;
+          vout[i] = v + (k == tsrc ? fsrc : (T)0);
+          edge[i] = false;
+        }
+      } else {
+#pragma unroll
+        for (int i = 0; i < C; ++i) {
+          const int k = e0 + i, kt = kg(min(max(k, 0), n - 1));
+          T gz = U2[i] * (sg * zs_tab[kt * kZsTab + ZT_ST]);
+          if (pml2) {
+            const T psi = pv2[i] + beta * ps2[i];
+            const float2 ab = *reinterpret_cast<const float2*>(&zs_tab[kt * kZsTab + ZT_AZ]);
+            zbs(zrsrc(Z.ps[1], Z.pbytes), colb,
+                (pi2[i] >= 0 && k < h) ? (unsigned)pi2[i] * planeb : OOB,
+                ab.x * gz + ab.y * psi);
+            if (pi2[i] >= 0) gz = gz + psi;
+          }
+          const T ri = zs_tab[kt * kZsTab + ZT_RI];
+          T v = -ri * gz;
+          v = v - R[i] * zs_tab[kt * kZsTab + ZT_G] * ri;
+          vout[i] = v + (k == tsrc ? fsrc : (T)0);
+          edge[i] = (k == 0 || k == n - 1);
+        }
+      }
+#pragma unroll
+      for (int i = 0; i < C; ++i)
+        zbs(edge[i] ? rVe : rV, colb, sown(e0 + i), edge[i] ? 0.0f : vout[i]);
+    }
+#pragma unroll
+    for (int j = 0; j < G; ++j) {
+      U1[j] = U1[C + j];
+      U2[j] = U2[C + j];
+    }
+#pragma unroll
+    for (int j = 0; j < D; ++j) R[j] = R[C + j];
+  };
+
+  using I0 = std::integral_constant<int, 0>;
+  using I1 = std::integral_constant<int, 1>;
+  for (int k0 = -G; k0 - D < h; k0 += 2 * C) {
+    iter(k0, I0{});
+    iter(k0 + C, I1{});
+  }
+}
+
+__global__ __launch_bounds__(128 * kZsWaves) void k_zstream_pv(ZsArgs Z, TriCoef<float> tc) {
+  static_assert(kZsG % kZsC == 0, "the first iteration starts at k0 = -G");
+  const int n = Z.n;
+  const int wv = threadIdx.x >> 6;
+  // column group of the wave and its direction: waves 0..kZsWaves-1 walk
+  // their columns from plane 1, the others from plane nz on the reflected
+  // column.  Each owns half of the column and runs on past it only as far
+  // as its own outputs need (2G + 6 planes).  The DBYD2 matrix of ibc 0 and
+  // 1 is centrosymmetric and its right-hand side odd under the reflection,
+  // so the reflected walk runs the same recurrences and negates the
+  // derivatives.
+  const int grp = wv % kZsWaves, dir = wv / kZsWaves;
+  const int c = (blockIdx.x * kZsWaves + grp) * 64 + (threadIdx.x & 63);
+  const int ntop = Z.split ? n / 2 : n;
+  const int h = dir ? n - ntop : ntop;
+  // ---- per-plane coefficient table (block-shared)
+#pragma unroll
+  for (int t = 0; t < ZT_NUM; ++t)
+    for (int k = threadIdx.x; k < n; k += blockDim.x) zs_tab[k * kZsTab + t] = Z.tab[t][k];
+  __syncthreads();
+  if (h <= 0) return;
+  if (dir)
+    zs_walk<1>(Z, tc, c, wv, h);
+  else
+    zs_walk<0>(Z, tc, c, wv, h);
+}
