@@ -1094,10 +1094,17 @@ __device__ __forceinline__ void zs_walk(const ZsArgs& Z, const TriCoef<float>& t
   constexpr unsigned OOB = 0x7ffffff0u;
   auto kg = [&](int t) -> int { return DIR ? n - 1 - t : t; };
   // scalar byte offsets of local plane t: loads (out of range outside the
-  // column), stores (outside the planes the wave owns)
-  auto so = [&](int t) -> unsigned { return (t >= 0 && t < n) ? (unsigned)kg(t) * planeb : OOB; };
+  // column), stores (outside the planes the wave owns).  nn and pb are the
+  // column length and plane bytes laundered at each iteration, so that the
+  // compiler computes the offsets in the loop instead of keeping one
+  // register per unrolled plane for the whole walk.
+  int nn = n, hh = h;
+  unsigned pb = planeb;
+  auto so = [&](int t) -> unsigned {
+    return (unsigned)t < (unsigned)nn ? (unsigned)(DIR ? nn - 1 - t : t) * pb : OOB;
+  };
   auto sown = [&](int t) -> unsigned {
-    return (t >= 0 && t < h) ? (unsigned)kg(t) * planeb : OOB;
+    return (unsigned)t < (unsigned)hh ? (unsigned)(DIR ? nn - 1 - t : t) * pb : OOB;
   };
   auto pmlg = [&](int t) -> int {
     return (t >= 0 && t < n) ? pml_index(kg(t), n, Z.npmlbot, Z.npmltop) : -1;
@@ -1140,6 +1147,11 @@ __device__ __forceinline__ void zs_walk(const ZsArgs& Z, const TriCoef<float>& t
 
   auto iter = [&](int k0, auto par_c) {
     constexpr int PA = decltype(par_c)::value, PB = 1 - PA;
+    nn = Z.n;
+    hh = h;
+    pb = planeb;
+    __asm__ volatile("" : "+s"(nn), "+s"(hh), "+s"(pb));
+    const int n = nn, h = hh;  // (the laundered values, for every test below)
     // ======== fwd1: rows f .. f+C-1 of d xi_z/dz (ibc 1)
     const int f = k0 + G;
     T Bx[C + 8];  // B of rows f-4 .. f+C+3
@@ -1396,7 +1408,7 @@ __device__ __forceinline__ void zs_walk(const ZsArgs& Z, const TriCoef<float>& t
 __global__ __launch_bounds__(128 * kZsWaves) void k_zstream_pv(ZsArgs Z, TriCoef<float> tc) {
   static_assert(kZsG % kZsC == 0, "the first iteration starts at k0 = -G");
   const int n = Z.n;
-  const int wv = threadIdx.x >> 6;
+  const int wv = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));  // wave-uniform
   // column group of the wave and its direction: waves 0..kZsWaves-1 walk
   // their columns from plane 1, the others from plane nz on the reflected
   // column.  Each owns half of the column and runs on past it only as far
