@@ -1072,6 +1072,8 @@ struct ZsArgs {
 // the B rings of the block's waves (static)
 extern __shared__ __align__(16) float zs_tab[];
 static __shared__ float zs_ring[2 * kZsWaves * kZsRing * 64];
+// DBYD2's boundary-row coefficients c1..c5, b1..b4, alpha2, beta1, cst
+static __shared__ float zs_bc[16];
 
 // One wave's walk along its column (DIR 0: from plane 1 down the index,
 // DIR 1: from plane nz up, on the reflected column; local plane t is global
@@ -1185,22 +1187,22 @@ __device__ __forceinline__ void zs_walk(const ZsArgs& Z, const TriCoef<float>& t
           if (k >= 2 && k <= n - 3) {
             r = tc.ci1 * (Bx[i + 5] - Bx[i + 3]) + tc.ci2 * (Bx[i + 6] - Bx[i + 2]);
           } else if (k == 0) {
-            const T r1 = tc.c1 * Bx[i + 4] + tc.c2 * Bx[i + 5] + tc.c3 * Bx[i + 6] +
-                         tc.c4 * Bx[i + 7] + tc.c5 * Bx[i + 8];
-            const T r0 = tc.b1 * Bx[i + 4] + tc.b2 * Bx[i + 5] + tc.b3 * Bx[i + 6] +
-                         tc.b4 * Bx[i + 7];
-            r = (tc.alpha2 * r0 - tc.beta1 * r1) * tc.cst;
+            const T r1 = zs_bc[0] * Bx[i + 4] + zs_bc[1] * Bx[i + 5] + zs_bc[2] * Bx[i + 6] +
+                         zs_bc[3] * Bx[i + 7] + zs_bc[4] * Bx[i + 8];
+            const T r0 = zs_bc[5] * Bx[i + 4] + zs_bc[6] * Bx[i + 5] + zs_bc[7] * Bx[i + 6] +
+                         zs_bc[8] * Bx[i + 7];
+            r = (zs_bc[9] * r0 - zs_bc[10] * r1) * zs_bc[11];
           } else if (k == 1) {
-            r = tc.c1 * Bx[i + 3] + tc.c2 * Bx[i + 4] + tc.c3 * Bx[i + 5] + tc.c4 * Bx[i + 6] +
-                tc.c5 * Bx[i + 7];
+            r = zs_bc[0] * Bx[i + 3] + zs_bc[1] * Bx[i + 4] + zs_bc[2] * Bx[i + 5] + zs_bc[3] * Bx[i + 6] +
+                zs_bc[4] * Bx[i + 7];
           } else if (k == n - 2) {
-            rn2 = -(tc.c1 * Bx[i + 5] + tc.c2 * Bx[i + 4] + tc.c3 * Bx[i + 3] +
-                    tc.c4 * Bx[i + 2] + tc.c5 * Bx[i + 1]);
+            rn2 = -(zs_bc[0] * Bx[i + 5] + zs_bc[1] * Bx[i + 4] + zs_bc[2] * Bx[i + 3] +
+                    zs_bc[3] * Bx[i + 2] + zs_bc[4] * Bx[i + 1]);
             r = rn2;
           } else {  // k == n - 1
-            const T rn1 = -(tc.b1 * Bx[i + 4] + tc.b2 * Bx[i + 3] + tc.b3 * Bx[i + 2] +
-                            tc.b4 * Bx[i + 1]);
-            r = (tc.alpha2 * rn1 - tc.beta1 * rn2) * tc.cst;
+            const T rn1 = -(zs_bc[5] * Bx[i + 4] + zs_bc[6] * Bx[i + 3] + zs_bc[7] * Bx[i + 2] +
+                            zs_bc[8] * Bx[i + 1]);
+            r = (zs_bc[9] * rn1 - zs_bc[10] * rn2) * zs_bc[11];
           }
           const float2 lv = *reinterpret_cast<const float2*>(&zs_tab[k * kZsTab + ZT_LOWR1]);
           u1p = (r - lv.x * u1p) * lv.y;
@@ -1313,11 +1315,11 @@ __device__ __forceinline__ void zs_walk(const ZsArgs& Z, const TriCoef<float>& t
           } else if (m == 1) {
             // (only i = 3 of k0 = 0 reaches this row; the index bound keeps
             // the unrolled copies of the other i in range)
-            r = tc.c1 * P[i + 2] + tc.c2 * P[i + 3] + tc.c3 * P[i + 4] + tc.c4 * P[i + 5] +
-                tc.c5 * P[i + 6 < 5 + C ? i + 6 : 4 + C];
+            r = zs_bc[0] * P[i + 2] + zs_bc[1] * P[i + 3] + zs_bc[2] * P[i + 4] + zs_bc[3] * P[i + 5] +
+                zs_bc[4] * P[i + 6 < 5 + C ? i + 6 : 4 + C];
           } else if (m == n - 2) {
-            r = -(tc.c1 * P[i + 4] + tc.c2 * P[i + 3] + tc.c3 * P[i + 2] + tc.c4 * P[i + 1] +
-                  tc.c5 * P[i]);
+            r = -(zs_bc[0] * P[i + 4] + zs_bc[1] * P[i + 3] + zs_bc[2] * P[i + 2] + zs_bc[3] * P[i + 1] +
+                  zs_bc[4] * P[i]);
           }  // rows 0 and n-1: r = 0 (ibc 0)
           const float2 lv = *reinterpret_cast<const float2*>(&zs_tab[m * kZsTab + ZT_LOWR0]);
           u2p = (r - lv.x * u2p) * lv.y;
@@ -1420,10 +1422,18 @@ __global__ __launch_bounds__(128 * kZsWaves) void k_zstream_pv(ZsArgs Z, TriCoef
   const int c = (blockIdx.x * kZsWaves + grp) * 64 + (threadIdx.x & 63);
   const int ntop = Z.split ? n / 2 : n;
   const int h = dir ? n - ntop : ntop;
-  // ---- per-plane coefficient table (block-shared)
+  // ---- per-plane coefficient table (block-shared) and the boundary-row
+  // coefficients (read from LDS in the rare boundary rows instead of being
+  // held in scalar registers for the whole walk)
 #pragma unroll
   for (int t = 0; t < ZT_NUM; ++t)
     for (int k = threadIdx.x; k < n; k += blockDim.x) zs_tab[k * kZsTab + t] = Z.tab[t][k];
+  if (threadIdx.x == 0) {
+    const float v[12] = {tc.c1, tc.c2, tc.c3, tc.c4, tc.c5, tc.b1,
+                         tc.b2, tc.b3, tc.b4, tc.alpha2, tc.beta1, tc.cst};
+#pragma unroll
+    for (int j = 0; j < 12; ++j) zs_bc[j] = v[j];
+  }
   __syncthreads();
   if (h <= 0) return;
   if (dir)
