@@ -1209,6 +1209,7 @@ static int launch_zpv(sparc_ctx* c, const Fld<T>& F, const T* inA, const T* inS,
       Z.tab[ZT_GAM1] = t1 + 2 * nz;
       Z.tab[ZT_GAM0] = t0 + 2 * nz;
       Z.tab[ZT_RI] = F.bg[SPARC_BG_RHOINV];
+      Z.tab[ZT_ST2] = F.stretch;
       Z.forcing = F.forcing;
       Z.nbytes = (unsigned)(c->N * 4);
       Z.pbytes = (unsigned)(c->nxy * (long)c->nzpml * 4);

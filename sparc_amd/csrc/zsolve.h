@@ -1022,8 +1022,9 @@ static constexpr int kZsRing = 64;  // LDS ring slots (planes) per wave for B of
 static constexpr int kZsWaves = 4;  // column groups per block (two waves each)
 static constexpr int kZsTab = 16;   // table floats per plane
 // table slots per plane
-enum { ZT_ST, ZT_C2R, ZT_GP, ZT_R0, ZT_GR, ZT_G, ZT_AZ, ZT_BZ, ZT_LOWR1, ZT_INVB1, ZT_LOWR0,
-       ZT_INVB0, ZT_GAM1, ZT_GAM0, ZT_RI, ZT_NUM };
+// (the pressure epilogue reads slots 0-3 and 6, the momentum epilogue 4-7)
+enum { ZT_ST, ZT_C2R, ZT_GP, ZT_R0, ZT_G, ZT_RI, ZT_GR, ZT_ST2, ZT_LOWR1, ZT_INVB1, ZT_LOWR0,
+       ZT_INVB0, ZT_GAM1, ZT_GAM0, ZT_AZ, ZT_BZ, ZT_NUM };
 
 // dynamic LDS: the table (the rings are a static array)
 __host__ __device__ constexpr size_t zstream_lds_bytes(int n) {
@@ -1179,6 +1180,12 @@ __device__ __forceinline__ void zs_walk(const ZsArgs& Z, const TriCoef<float>& t
         U1[G + i] = u1p;
       }
     } else {
+      // LU factors of the rows read up front (rows past the column clamped,
+      // unused), so the rows wait on the LDS once
+      float2 lvs[C];
+#pragma unroll
+      for (int i = 0; i < C; ++i)
+        lvs[i] = *reinterpret_cast<const float2*>(&zs_tab[min(f + i, n - 1) * kZsTab + ZT_LOWR1]);
 #pragma unroll
       for (int i = 0; i < C; ++i) {
         const int k = f + i;
@@ -1204,8 +1211,7 @@ __device__ __forceinline__ void zs_walk(const ZsArgs& Z, const TriCoef<float>& t
                             zs_bc[8] * Bx[i + 1]);
             r = (zs_bc[9] * rn1 - zs_bc[10] * rn2) * zs_bc[11];
           }
-          const float2 lv = *reinterpret_cast<const float2*>(&zs_tab[k * kZsTab + ZT_LOWR1]);
-          u1p = (r - lv.x * u1p) * lv.y;
+          u1p = (r - lvs[i].x * u1p) * lvs[i].y;
           U1[G + i] = u1p;
         }
       }
@@ -1238,15 +1244,19 @@ __device__ __forceinline__ void zs_walk(const ZsArgs& Z, const TriCoef<float>& t
           if (j < C) U1[j] = x;
         }
       } else {
+        // window [k0, k0+WIN) from its last row inside the column (exact at
+        // row n-1); rows above the column are passed over
         const int e = min(k0 + WIN, n) - 1;
+        T gv[WIN];
+#pragma unroll
+        for (int j = 0; j < WIN; ++j)
+          gv[j] = zs_tab[min(max(k0 + j + 1, 0), n - 1) * kZsTab + ZT_GAM1];
         T x = 0;
 #pragma unroll
         for (int j = WIN - 1; j >= 0; --j) {
           const int k = k0 + j;
-          if (k >= 0 && k <= e) {
-            x = (k == e) ? U1[j] : U1[j] - zs_tab[(k + 1) * kZsTab + ZT_GAM1] * x;
-            if (j < C) U1[j] = x;
-          }
+          x = (k < e) ? U1[j] - gv[j] * x : (k == e ? U1[j] : x);
+          if (j < C) U1[j] = x;
         }
       }
     }
@@ -1266,25 +1276,34 @@ __device__ __forceinline__ void zs_walk(const ZsArgs& Z, const TriCoef<float>& t
           R[D + i] = -t4.w * div - xz * gr;
         }
       } else if (k0 + C > 0 && k0 < n) {
+        // planes outside the column or not owned (table rows clamped, values
+        // unused), PML planes
+        auto ep1g = [&](auto pml_c) {
+          constexpr bool PM = decltype(pml_c)::value;
 #pragma unroll
-        for (int i = 0; i < C; ++i) {
-          const int k = k0 + i, kt = kg(min(max(k, 0), n - 1));
-          const float4 t4 = *reinterpret_cast<const float4*>(&zs_tab[kt * kZsTab]);
-          const T gr = zs_tab[kt * kZsTab + ZT_GR];
-          T dz = U1[i] * (sg * t4.x);
-          if (pml1) {
-            const T psi = pv1[i] + beta * ps1[i];
-            const float2 ab = *reinterpret_cast<const float2*>(&zs_tab[kt * kZsTab + ZT_AZ]);
-            zbs(zrsrc(Z.ps[0], Z.pbytes), colb,
-                (pi1[i] >= 0 && k < h) ? (unsigned)pi1[i] * planeb : OOB,
-                ab.x * dz + ab.y * psi);
-            if (pi1[i] >= 0) dz = dz + psi;
+          for (int i = 0; i < C; ++i) {
+            const int k = k0 + i, kt = kg(min(max(k, 0), n - 1));
+            const float4 t4 = *reinterpret_cast<const float4*>(&zs_tab[kt * kZsTab]);
+            const T gr = zs_tab[kt * kZsTab + ZT_GR];
+            T dz = U1[i] * (sg * t4.x);
+            if constexpr (PM) {
+              const T psi = pv1[i] + beta * ps1[i];
+              const float2 ab = *reinterpret_cast<const float2*>(&zs_tab[kt * kZsTab + ZT_AZ]);
+              zbs(zrsrc(Z.ps[0], Z.pbytes), colb,
+                  (pi1[i] >= 0 && k < h) ? (unsigned)pi1[i] * planeb : OOB,
+                  ab.x * dz + ab.y * psi);
+              dz = pi1[i] >= 0 ? dz + psi : dz;
+            }
+            const T div = X[PA][i] + Y[PA][i] + dz;
+            const T xz = zs_ring[oxz + (k & (kZsRing - 1)) * 64];
+            pout[i] = -t4.y * div - xz * t4.z;
+            R[D + i] = -t4.w * div - xz * gr;
           }
-          const T div = X[PA][i] + Y[PA][i] + dz;
-          const T xz = zs_ring[oxz + (k & (kZsRing - 1)) * 64];
-          pout[i] = -t4.y * div - xz * t4.z;
-          R[D + i] = -t4.w * div - xz * gr;
-        }
+        };
+        if (pml1)
+          ep1g(std::integral_constant<bool, true>{});
+        else
+          ep1g(std::integral_constant<bool, false>{});
       }
 #pragma unroll
       for (int i = 0; i < C; ++i) {
@@ -1305,6 +1324,11 @@ __device__ __forceinline__ void zs_walk(const ZsArgs& Z, const TriCoef<float>& t
         U2[G + i] = u2p;
       }
     } else if (f2 + C > 0) {
+      float2 lvs[C];
+#pragma unroll
+      for (int i = 0; i < C; ++i)
+        lvs[i] = *reinterpret_cast<const float2*>(
+            &zs_tab[min(max(f2 + i, 0), n - 1) * kZsTab + ZT_LOWR0]);
 #pragma unroll
       for (int i = 0; i < C; ++i) {
         const int m = f2 + i;
@@ -1321,8 +1345,7 @@ __device__ __forceinline__ void zs_walk(const ZsArgs& Z, const TriCoef<float>& t
             r = -(zs_bc[0] * P[i + 4] + zs_bc[1] * P[i + 3] + zs_bc[2] * P[i + 2] + zs_bc[3] * P[i + 1] +
                   zs_bc[4] * P[i]);
           }  // rows 0 and n-1: r = 0 (ibc 0)
-          const float2 lv = *reinterpret_cast<const float2*>(&zs_tab[m * kZsTab + ZT_LOWR0]);
-          u2p = (r - lv.x * u2p) * lv.y;
+          u2p = (r - lvs[i].x * u2p) * lvs[i].y;
           U2[G + i] = u2p;
         }
       }
@@ -1341,14 +1364,16 @@ __device__ __forceinline__ void zs_walk(const ZsArgs& Z, const TriCoef<float>& t
         }
       } else {
         const int e = min(e0 + WIN, n) - 1;
+        T gv[WIN];
+#pragma unroll
+        for (int j = 0; j < WIN; ++j)
+          gv[j] = zs_tab[min(max(e0 + j + 1, 0), n - 1) * kZsTab + ZT_GAM0];
         T x = 0;
 #pragma unroll
         for (int j = WIN - 1; j >= 0; --j) {
           const int k = e0 + j;
-          if (k >= 0 && k <= e) {
-            x = (k == e) ? U2[j] : U2[j] - zs_tab[(k + 1) * kZsTab + ZT_GAM0] * x;
-            if (j < C) U2[j] = x;
-          }
+          x = (k < e) ? U2[j] - gv[j] * x : (k == e ? U2[j] : x);
+          if (j < C) U2[j] = x;
         }
       }
       // ======== ep2: planes [k0-D, k0-D+C)
@@ -1358,33 +1383,39 @@ __device__ __forceinline__ void zs_walk(const ZsArgs& Z, const TriCoef<float>& t
 #pragma unroll
         for (int i = 0; i < C; ++i) {
           const int k = e0 + i, kt = kg(k);
-          const float2 sg2 = *reinterpret_cast<const float2*>(&zs_tab[kt * kZsTab + ZT_ST]);
-          const T gz = U2[i] * (sg * sg2.x);
-          const T ri = zs_tab[kt * kZsTab + ZT_RI];
-          T v = -ri * gz;
-          v = v - R[i] * zs_tab[kt * kZsTab + ZT_G] * ri;
+          const float4 t4 = *reinterpret_cast<const float4*>(&zs_tab[kt * kZsTab + ZT_G]);
+          const T gz = U2[i] * (sg * t4.w);  // g rhoinv gr st
+          T v = -t4.y * gz;
+          v = v - R[i] * t4.x * t4.y;
           vout[i] = v + (k == tsrc ? fsrc : (T)0);
           edge[i] = false;
         }
       } else {
+        auto ep2g = [&](auto pml_c) {
+          constexpr bool PM = decltype(pml_c)::value;
 #pragma unroll
-        for (int i = 0; i < C; ++i) {
-          const int k = e0 + i, kt = kg(min(max(k, 0), n - 1));
-          T gz = U2[i] * (sg * zs_tab[kt * kZsTab + ZT_ST]);
-          if (pml2) {
-            const T psi = pv2[i] + beta * ps2[i];
-            const float2 ab = *reinterpret_cast<const float2*>(&zs_tab[kt * kZsTab + ZT_AZ]);
-            zbs(zrsrc(Z.ps[1], Z.pbytes), colb,
-                (pi2[i] >= 0 && k < h) ? (unsigned)pi2[i] * planeb : OOB,
-                ab.x * gz + ab.y * psi);
-            if (pi2[i] >= 0) gz = gz + psi;
+          for (int i = 0; i < C; ++i) {
+            const int k = e0 + i, kt = kg(min(max(k, 0), n - 1));
+            const float4 t4 = *reinterpret_cast<const float4*>(&zs_tab[kt * kZsTab + ZT_G]);
+            T gz = U2[i] * (sg * t4.w);  // g rhoinv gr st
+            if constexpr (PM) {
+              const T psi = pv2[i] + beta * ps2[i];
+              const float2 ab = *reinterpret_cast<const float2*>(&zs_tab[kt * kZsTab + ZT_AZ]);
+              zbs(zrsrc(Z.ps[1], Z.pbytes), colb,
+                  (pi2[i] >= 0 && k < h) ? (unsigned)pi2[i] * planeb : OOB,
+                  ab.x * gz + ab.y * psi);
+              gz = pi2[i] >= 0 ? gz + psi : gz;
+            }
+            T v = -t4.y * gz;
+            v = v - R[i] * t4.x * t4.y;
+            vout[i] = v + (k == tsrc ? fsrc : (T)0);
+            edge[i] = (k == 0 || k == n - 1);
           }
-          const T ri = zs_tab[kt * kZsTab + ZT_RI];
-          T v = -ri * gz;
-          v = v - R[i] * zs_tab[kt * kZsTab + ZT_G] * ri;
-          vout[i] = v + (k == tsrc ? fsrc : (T)0);
-          edge[i] = (k == 0 || k == n - 1);
-        }
+        };
+        if (pml2)
+          ep2g(std::integral_constant<bool, true>{});
+        else
+          ep2g(std::integral_constant<bool, false>{});
       }
 #pragma unroll
       for (int i = 0; i < C; ++i)
