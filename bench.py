@@ -34,16 +34,18 @@ sys.path.insert(0, ROOT)
 HBM_PEAK_GBS = 8000.0
 # the kernel behind each timed class in the forward step of the 3-D box
 KERNEL_SYMBOLS = {
-    "xlines": "sparc::k_xw256<float, 0 (XM_DERIV) | 4 (XM_DERIV_DAMPINV), 2> -- x derivative, one wavefront per line",
-    "ylines": "sparc::k_y256<float, 0 (YM_DERIV)> -- y derivative, register four-step",
+    "xlines": "sparc::k_xw256<float, 0 (XM_DERIV), 2> -- x derivative of xi_x, one wavefront per line pair",
+    "ylines": "sparc::k_y256<float, 0 (YM_DERIV)> -- y derivative of xi_y, register four-step",
     "damp": "sparc::k_xw256<float, 2 (XM_DAMP_FWD), 4> -- damping, forward x of the stage temp "
             "a + beta scr of the velocities, also stored as RHS_xi "
             "(with --damp2d: sparc::k_damp2d<float>, the whole damping, one 256x256 plane per block)",
-    "damp_y": "sparc::k_y256<float, 2 (YM_DAMP)> -- damping, y pass",
-    "damp_xi": "sparc::k_xw256<float, 3 (XM_DAMP_INV), 2> -- damping, inverse x (v_y, v_z; v_x in XM_DERIV_DAMPINV)",
+    "damp_y": "sparc::k_y256_damp_deriv<float> -- damping y pass of v_x, v_y, v_z and the y derivative "
+              "of p (RHS_vy) in one launch",
+    "damp_xi": "sparc::k_xw256_rhs_dampinv<float, 2> -- x derivative of p (RHS_vx) and the damping "
+               "inverse x of v_x, v_y, v_z subtracted from RHS_v in one launch",
     "zsolve": "sparc::k_zsolve<float, 7 (EPI_RHS3N), 8, true> -- z solve of p (ud in LDS) + RHS_vz",
-    "zsolve_p": "sparc::k_zsolve_pv<float, 16, true, 20> -- exact segmented z solves (column in LDS): "
-                "d xi_z/dz with the pressure, then dp/dz on the LDS tile with RHS_vz "
+    "zsolve_p": "sparc::k_zstream_pv -- streaming z solves, two walks per 64-column group (one "
+                "from each end): d xi_z/dz with the pressure, then dp/dz with RHS_vz "
                 "(without damping or on other line sizes: sparc::k_zsolve_b<float, 1 (EPI_P3D), 16, true>)",
     "update": "sparc::k_update_filter_c -- a += dt*scr with the z filter (hi+lo state), all fields per launch",
 }
@@ -334,12 +336,12 @@ def serving_bench(n: int, rate: float) -> int:
 # the kernels of each timed class on the default path of the 256x256x300 FP32
 # box, as rocprofv3 names them
 CLASS_KERNELS = {
-    "zsolve_p": ("k_zsolve_pv<16, true, 20>",),
+    "zsolve_p": ("k_zstream_pv",),
     "damp": ("k_xw256<float, 2, 4>",),
-    "damp_y": ("k_y256<float, 2>",),
-    "damp_xi": ("k_xw256<float, 3, 2>",),
+    "damp_y": ("k_y256_damp_deriv<float>",),
+    "damp_xi": ("k_xw256_rhs_dampinv<float, 2>",),
     "update": ("k_update_filter_c",),
-    "xlines": ("k_xw256<float, 0, 2>", "k_xw256<float, 4, 2>"),
+    "xlines": ("k_xw256<float, 0, 2>",),
     "ylines": ("k_y256<float, 0>",),
 }
 

@@ -27,7 +27,8 @@ def test_traffic_from_counters(tmp_path):
     _write(tmp_path / "copy_write", [{"Dispatch_Id": i, "Kernel_Name": "__amd_rocclr_copyBuffer",
                                       "Counter_Name": "WRITE_SIZE", "Counter_Value": kib}
                                      for i in range(5)])
-    a, b = "void sparc::k_xw256<float, 0, 2>(int)", "void sparc::k_xw256<float, 4, 2>(int)"
+    # two dispatches of the class's kernel, averaged
+    a = b = "void sparc::k_xw256<float, 0, 2>(int)"
     fe = [{"Dispatch_Id": 1, "Kernel_Name": a, "Counter_Name": "FETCH_SIZE", "Counter_Value": 100},
           {"Dispatch_Id": 2, "Kernel_Name": b, "Counter_Name": "FETCH_SIZE", "Counter_Value": 300}]
     wr = [{"Dispatch_Id": 1, "Kernel_Name": a, "Counter_Name": "WRITE_SIZE", "Counter_Value": 50},

@@ -283,3 +283,31 @@ def test_generic_line_lengths_match_oracle(case, prec):
     finally:
         eng.close()
         o.close()
+
+
+# ---------------------------------------------------------------------------
+# the streaming z solve's column splits (256-wide FP32 boxes)
+# ---------------------------------------------------------------------------
+
+@pytest.mark.timeout(300)
+@pytest.mark.parametrize("nz", [96, 101, 128], ids=["split96", "split_odd101", "split128"])
+def test_streaming_zsolve_column_splits_match_oracle(nz):
+    """Option 4 on 256 x 256 x nz in FP32: the pressure and dp/dz z solves run
+    as two walks per column, one from each end, each owning half of the
+    column (equal halves at 96 and 128 planes, 50 + 51 at 101) and carrying
+    its truncated backward sweep 2G + 6 planes into the other half; the PML
+    planes at both ends are each handled by one walk."""
+    W = _grid_workload(256, 256, nz)
+    S = D.setup_for(W)
+    eng, o, nf = _forward_pair_from_state(W, S, abi.FP32, 2, 500)
+    try:
+        for f in range(nf):
+            _check(eng.get(abi.ARR_A, f), o.get(abi.ARR_A, f), abi.FP32, what=("a", f))
+            _check(eng.get(abi.ARR_SCR, f), o.get(abi.ARR_SCR, f), abi.FP32, tol32=TOL32_KERN,
+                   what=("scr", f))
+        for f in range(2):
+            _check(eng.get(abi.ARR_PMLVARS, f), o.get(abi.ARR_PMLVARS, f), abi.FP32,
+                   tol32=TOL32_KERN, what=("pmlvars", f))
+    finally:
+        eng.close()
+        o.close()
