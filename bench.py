@@ -69,6 +69,10 @@ def main() -> int:
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run 5 device-to-device copies of MIB MiB (calibrates the "
                          "FETCH_SIZE/WRITE_SIZE counters against known bytes) and exit")
+    ap.add_argument("--dft-probe", type=int, default=0, metavar="REPS",
+                    help="only time the 128-wide x derivative of the 128x128x150 box as the "
+                         "engine's FFT line kernel and as a dense FP32 (and bf16) GEMM on the "
+                         "matrix cores, REPS launches each, and exit")
     ap.add_argument("--serving", type=int, default=0, metavar="N",
                     help="serving benchmark instead: the first N requests of the arrival "
                          "stream through the FP32 serving loop, then through the plain FP64 path")
@@ -89,6 +93,8 @@ def main() -> int:
     args = ap.parse_args()
     if args.copy_probe:
         return copy_probe(args.copy_probe)
+    if args.dft_probe:
+        return dft_probe(args.dft_probe)
     if args.serving:
         if args.gpus != 1:
             print("bench.py measures one GPU only", file=sys.stderr)
@@ -398,6 +404,67 @@ def write_traffic(directory: str, out_path: str, prefix: str, prof: dict) -> Non
         json.dump(res, f, indent=1)
     print(f"[bench] counter traffic of {len(res) - 1} classes -> {out_path}", file=sys.stderr,
           flush=True)
+
+
+def dft_probe(reps: int) -> int:
+    """The x derivative of the 128 x 128 x 150 box (19200 lines of 128) two
+    ways: the engine's FFT line kernel (HIP events around each launch), and
+    the same operator as a dense product Y = X D^T on the matrix cores
+    (torch.matmul, FP32 and bf16), D being the engine's own derivative matrix
+    (the operator applied to unit impulses).  Prints one JSON line with the
+    times per launch and the FP32 product's difference from the FFT result."""
+    import numpy as np
+    import torch
+
+    from sparc_amd import abi
+    from sparc_amd import driver as D
+
+    W = D.workload("box2s", precision=abi.FP32)
+    S = D.setup_for(W)
+    eng, _ = D.make_engine(W, S, nsteps=2)
+    nx, ny, nz = eng.shape
+    # D[:, j] = d/dx of the unit impulse at x = j (line y = j of plane 0)
+    imp = np.zeros((nx, ny, nz))
+    for j in range(min(nx, ny)):
+        imp[j, j, 0] = 1.0
+    d_op = eng.apply_op(abi.OP_DDX, imp)[:, :nx, 0]
+    rng = np.random.default_rng(3)
+    f = rng.standard_normal((nx, ny, nz))
+    g = eng.apply_op(abi.OP_DDX, f)
+    eng.profile(True)
+    for _ in range(reps):
+        eng.apply_op(abi.OP_DDX, f)
+    prof = eng.profile_read()
+    eng.profile(False)
+    eng.close()
+    ms, by, nl = prof["xlines"]
+    us_fft = 1e3 * ms / nl
+    # lines as rows: X[l, i] = f[i, y, z], l = y + ny z
+    X = torch.tensor(f.transpose(1, 2, 0).reshape(-1, nx), dtype=torch.float32, device="cuda")
+    Dt = torch.tensor(d_op.T.copy(), dtype=torch.float32, device="cuda")
+    res = {}
+    for name, dt in (("fp32", torch.float32), ("bf16", torch.bfloat16)):
+        Xd, Dd = X.to(dt), Dt.to(dt).contiguous()
+        Y = torch.matmul(Xd, Dd)
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(reps):
+            Y = torch.matmul(Xd, Dd)
+        e1.record()
+        torch.cuda.synchronize()
+        us = 1e3 * e0.elapsed_time(e1) / reps
+        y = Y.float().cpu().numpy().reshape(ny, nz, nx).transpose(2, 0, 1)
+        err = float(np.linalg.norm(y - g) / np.linalg.norm(g))
+        flops = 2.0 * X.shape[0] * nx * nx
+        res[name] = {"us_per_launch": round(us, 2), "rel_l2_vs_fft_kernel": err,
+                     "tflops": round(flops / (us * 1e-6) / 1e12, 1)}
+    line = {"probe": "x derivative of 128x128x150 (19200 lines of 128)",
+            "fft_line_kernel": {"us_per_launch": round(us_fft, 2),
+                                "gbs": round(by / (ms * 1e-3) / 1e9, 1), "launches": nl},
+            "dense_gemm": res, "reps": reps}
+    print(json.dumps(line))
+    return 0
 
 
 def copy_probe(mib: int) -> int:
