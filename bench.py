@@ -259,9 +259,10 @@ def fp64_step_rate(W, S, warmup: int, steps: int) -> dict:
 
 
 def serving_mix(nsteps: int) -> dict:
-    """Solves/hour of the FP32 serving loop and of the plain FP64 path (the
-    same requests one at a time) on serve.mix_sample: one request per grid of
-    the mix, 384x384x300 included."""
+    """Solves/hour of the FP32 serving loop and of the plain FP64 and FP32
+    paths (the same requests one at a time) on serve.mix_sample: one request
+    per grid of the mix, 384x384x300 included."""
+    from sparc_amd import abi
     from sparc_amd import serve as SV
 
     reqs = SV.mix_sample(nsteps)
@@ -275,6 +276,8 @@ def serving_mix(nsteps: int) -> dict:
     print(f"[bench] serving loop: {fast}", file=sys.stderr, flush=True)
     plain = SV.plain_fp64(reqs, cache)
     print(f"[bench] plain FP64: {plain}", file=sys.stderr, flush=True)
+    plain32 = SV.plain_sequential(reqs, cache, abi.FP32)
+    print(f"[bench] plain FP32 (sequential): {plain32}", file=sys.stderr, flush=True)
     sph = fast.get("solves_per_hour", 0.0)
     return {"metric": "forward+adjoint+kernel solves/hour", "unit": "solves/h",
             "requests": [f"{r.workload} x {r.nsteps} steps" for r in reqs],
@@ -286,7 +289,11 @@ def serving_mix(nsteps: int) -> dict:
             "plain_fp64": {"solves_per_hour": round(plain["solves_per_hour"], 1),
                            "wall_s": round(plain["wall_s"], 2),
                            "per_solve_s": [round(x, 2) for x in plain["per_solve_s"]]},
-            "speedup_vs_plain_fp64": round(sph / plain["solves_per_hour"], 3)}
+            "plain_fp32": {"solves_per_hour": round(plain32["solves_per_hour"], 1),
+                           "wall_s": round(plain32["wall_s"], 2),
+                           "per_solve_s": [round(x, 2) for x in plain32["per_solve_s"]]},
+            "speedup_vs_plain_fp64": round(sph / plain["solves_per_hour"], 3),
+            "speedup_vs_plain_fp32": round(sph / plain32["solves_per_hour"], 3)}
 
 
 def serving_bench(n: int, rate: float) -> int:

@@ -321,12 +321,19 @@ def summarize(results: list[Result]) -> dict:
 def plain_fp64(requests: list[Request], cache: ModelCache | None = None) -> dict:
     """The plain path: the same requests one after another in FP64 with the
     full snapshot bank, nothing overlapped."""
+    return plain_sequential(requests, cache, abi.FP64)
+
+
+def plain_sequential(requests: list[Request], cache: ModelCache | None = None,
+                     precision: int = abi.FP64) -> dict:
+    """The same requests one after another in the given precision with the
+    full snapshot bank, nothing overlapped (no serving loop)."""
     cache = cache or ModelCache()
     t0 = time.perf_counter()
     per = []
     for r in requests:
         t = time.perf_counter()
-        solve_request(replace(r, precision=abi.FP64), cache, "snapshots")
+        solve_request(replace(r, precision=precision), cache, "snapshots")
         per.append(time.perf_counter() - t)
     wall = time.perf_counter() - t0
     return {"solves": len(requests), "wall_s": wall, "solves_per_hour": 3600.0 * len(requests) / wall,
