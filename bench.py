@@ -34,7 +34,8 @@ sys.path.insert(0, ROOT)
 HBM_PEAK_GBS = 8000.0
 # the kernel behind each timed class in the forward step of the 3-D box
 KERNEL_SYMBOLS = {
-    "xlines": "sparc::k_xw256<float, 0 (XM_DERIV), 2> -- x derivative of xi_x, one wavefront per line pair",
+    "xlines": "sparc::k_xy256_div<float, 2> -- x derivative of xi_x (one wavefront per line pair) and "
+              "y derivative of xi_y (register four-step) in one launch",
     "ylines": "sparc::k_y256<float, 0 (YM_DERIV)> -- y derivative of xi_y, register four-step",
     "damp": "sparc::k_xw256<float, 2 (XM_DAMP_FWD), 4> -- damping, forward x of the stage temp "
             "a + beta scr of the velocities, also stored as RHS_xi "
@@ -354,7 +355,7 @@ CLASS_KERNELS = {
     "damp_y": ("k_y256_damp_deriv<float>",),
     "damp_xi": ("k_xw256_rhs_dampinv<float, 2>",),
     "update": ("k_update_filter_c",),
-    "xlines": ("k_xw256<float, 0, 2>",),
+    "xlines": ("k_xy256_div<float, 2>",),
     "ylines": ("k_y256<float, 0>",),
 }
 

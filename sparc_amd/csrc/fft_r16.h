@@ -169,6 +169,32 @@ __global__ __launch_bounds__(256) void k_y256(int nx, int nz, XArgs<T> X, T beta
                      blockIdx.z);
 }
 
+// The two horizontal parts of div xi in one launch (displacement.f90:491-
+// 513): blocks [0, nbx) the x derivative of X.a[0] (x lines of 256, one
+// wavefront per row pair), the rest the y derivative of Y.a[0] (register
+// four-step, x-chunks of 32 columns, then planes), each of the stage temp
+// a + beta scr; neither pass's tail idles the device.
+template <typename T, int LPW>
+__global__ __launch_bounds__(256) void k_xy256_div(int nrows, int nlines, XArgs<T> X, XArgs<T> Y,
+                                                   T beta, const T* __restrict__ multx,
+                                                   const T* __restrict__ multy,
+                                                   const C2<T>* __restrict__ twx,
+                                                   const C2<T>* __restrict__ twy, int nx, int ny,
+                                                   int nz, long nxy, int nbx) {
+  __shared__ C2<T> S[16 * kY256Lsy];
+  __shared__ C2<T> tw2[256];
+  __shared__ T ml[256];
+  int b = blockIdx.x;
+  if (b < nbx) {
+    xw256_body<T, XM_DERIV, LPW>(nrows, nlines, X, beta, multx, twx, ny, nz, nxy, 0, b);
+  } else {
+    b -= nbx;
+    const int nbc = nx / 32;
+    y256_body<T, YM_DERIV>(nx, nz, Y, beta, multy, twy, (const T*)nullptr, S, tw2, ml, b % nbc,
+                           b / nbc, 0);
+  }
+}
+
 // The damping y pass of three fields (D: nbd blocks, x-chunks of 16 bins
 // fastest, then plane pairs, then fields) and the y derivative of one field
 // (G, the rest: x-chunks of 32 columns, then planes) in one launch, so the

@@ -1303,8 +1303,26 @@ static int run_stage(sparc_ctx* c, Fld<T>& F, int s) {
   int rc;
   if (c->cfg.option == SPARC_OPT_QUIET_PML_3D_DISPL) {
     // ddxyz(xi_x, xi_y, xi_z, bc=1) + PML + div, p, rho (displacement.f90:491-535)
-    if ((rc = launch_x<T, XM_DERIV>(c, F.a[0], S(0), beta, F.W[0], dmx, nullptr))) return rc;
-    if ((rc = launch_y<T, YM_DERIV>(c, F.a[1], S(1), beta, F.W[1], dmy, nullptr))) return rc;
+    if (c->cfg.nx == 256 && c->cfg.ny == 256) {
+      // the x and y derivatives in one launch
+      XArgs<T> X{}, Y{};
+      X.a[0] = F.a[0];
+      X.s[0] = S(0);
+      X.out[0] = F.W[0];
+      Y.a[0] = F.a[1];
+      Y.s[0] = S(1);
+      Y.out[0] = F.W[1];
+      const long nrows = (long)c->cfg.ny * c->cfg.nz, nlines = (nrows + 1) / 2;
+      const int nbx = cdiv(nlines, 4 * kXwLpw), nby = (c->cfg.nx / 32) * c->cfg.nz;
+      ProfScope ps(c, SPARC_KC_XLINES, (first ? 4.0 : 6.0) * (double)c->N * sizeof(T));
+      hipLaunchKernelGGL((k_xy256_div<T, kXwLpw>), dim3(nbx + nby), dim3(256), 0, c->stream,
+                         (int)nrows, (int)nlines, X, Y, beta, dmx, dmy, (const C2<T>*)c->twx,
+                         (const C2<T>*)c->twy, c->cfg.nx, c->cfg.ny, c->cfg.nz, c->nxy, nbx);
+      LAUNCHCHK(c);
+    } else {
+      if ((rc = launch_x<T, XM_DERIV>(c, F.a[0], S(0), beta, F.W[0], dmx, nullptr))) return rc;
+      if ((rc = launch_y<T, YM_DERIV>(c, F.a[1], S(1), beta, F.W[1], dmy, nullptr))) return rc;
+    }
     const bool damp = c->cfg.damp_waves && c->damping_set;
     // FP32 on 256 x 256 planes with cfg.plane_damping: the whole damping in
     // one pass per plane
