@@ -28,7 +28,7 @@ def test_traffic_from_counters(tmp_path):
                                       "Counter_Name": "WRITE_SIZE", "Counter_Value": kib}
                                      for i in range(5)])
     # two dispatches of the class's kernel, averaged
-    a = b = "void sparc::k_xw256<float, 0, 2>(int)"
+    a = b = f"void sparc::{bench.CLASS_KERNELS['xlines'][0]}(int)"
     fe = [{"Dispatch_Id": 1, "Kernel_Name": a, "Counter_Name": "FETCH_SIZE", "Counter_Value": 100},
           {"Dispatch_Id": 2, "Kernel_Name": b, "Counter_Name": "FETCH_SIZE", "Counter_Value": 300}]
     wr = [{"Dispatch_Id": 1, "Kernel_Name": a, "Counter_Name": "WRITE_SIZE", "Counter_Value": 50},
