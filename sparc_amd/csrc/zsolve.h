@@ -986,9 +986,12 @@ __global__ __launch_bounds__(64 * NSEG) void k_zsolve_pv(Fld<float> F, TriCoef<f
 //   backward x_k = u_k - gam_{k+1} x_{k+1}            from row n-1.
 // The backward sweep runs per chunk of C output planes: it starts G planes
 // below the chunk's last plane from x = u there (exact when that plane is
-// row n-1).  The start error decays by gam ~ 0.382 per plane, so the outputs
-// differ from the sequential solve by <= 0.382^20 ~ 4e-9 relative, below
-// FP32 rounding.
+// row n-1).  The start error, |x - u| <= 0.4 |x| there, decays by gam ~
+// 0.382 per plane, so the outputs differ from the sequential solve by
+// <= 0.4 * 0.382^18 ~ 1.2e-8 relative, below FP32 rounding (6e-8).  Chunks
+// of 6 planes with an 18-plane start ran the box fastest among C = 3..12,
+// G = 18..32 (117 us per launch at 256 x 256 x 300; 125 us at C = 10,
+// G = 20).
 //
 // Iteration with chunk start k0 (planes k0 .. k0+C-1):
 //   fwd1  forward sweep of d xi_z/dz (ibc 1) over planes [k0+G, k0+G+C); the
@@ -1016,8 +1019,8 @@ __global__ __launch_bounds__(64 * NSEG) void k_zsolve_pv(Fld<float> F, TriCoef<f
 // instructions is then the same on every path through an iteration, which
 // keeps the compiler's count of loads in flight exact: a wave waits only for
 // the load it is about to use, never for the next chunk's.
-static constexpr int kZsC = 10;     // output planes per chunk
-static constexpr int kZsG = 20;     // truncated backward start, planes
+static constexpr int kZsC = 6;      // output planes per chunk
+static constexpr int kZsG = 18;     // truncated backward start, planes
 static constexpr int kZsRing = 64;  // LDS ring slots (planes) per wave for B of xi_z
 static constexpr int kZsWaves = 4;  // column groups per block (two waves each)
 static constexpr int kZsTab = 16;   // table floats per plane
@@ -1445,7 +1448,7 @@ __global__ __launch_bounds__(128 * kZsWaves) void k_zstream_pv(ZsArgs Z, TriCoef
   // column group of the wave and its direction: waves 0..kZsWaves-1 walk
   // their columns from plane 1, the others from plane nz on the reflected
   // column.  Each owns half of the column and runs on past it only as far
-  // as its own outputs need (2G + 6 planes).  The DBYD2 matrix of ibc 0 and
+  // as its own outputs need (B up to 2G + 4 planes past it).  The DBYD2 matrix of ibc 0 and
   // 1 is centrosymmetric and its right-hand side odd under the reflection,
   // so the reflected walk runs the same recurrences and negates the
   // derivatives.

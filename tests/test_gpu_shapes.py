@@ -295,7 +295,7 @@ def test_streaming_zsolve_column_splits_match_oracle(nz):
     """Option 4 on 256 x 256 x nz in FP32: the pressure and dp/dz z solves run
     as two walks per column, one from each end, each owning half of the
     column (equal halves at 96 and 128 planes, 50 + 51 at 101) and carrying
-    its truncated backward sweep 2G + 6 planes into the other half; the PML
+    its truncated backward sweeps 2G + 4 planes into the other half; the PML
     planes at both ends are each handled by one walk."""
     W = _grid_workload(256, 256, nz)
     S = D.setup_for(W)
