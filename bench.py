@@ -178,11 +178,12 @@ def main() -> int:
             "traffic": traffic, "algorithmic_bytes_per_launch": by / nl,
             "us_per_launch": 1e3 * ms / nl, "kernels": kernels}
     # bytes per point and step: 18 field arrays moved per field and step, 5
-    # 3-D background arrays per stage (15 with a magnetic field), 2 more per
-    # field for the compensated state
+    # 3-D background arrays per stage (15 with a magnetic field), and the
+    # compensated state's 2-byte low words read and written once per field
     mhd = P.option in (abi.OPT_MHD, abi.OPT_MHD8)
     nbg = 15 if mhd else 5
-    step_algo = (18 * 6 + (12 if cfg.compensated else 0) + 5 * nbg) * (4 if prec == abi.FP32 else 8)
+    step_algo = ((18 * 6 + 5 * nbg) * (4 if prec == abi.FP32 else 8)
+                 + (6 * 2 * 2 if cfg.compensated else 0))
     opt = "option 10" if P.option == abi.OPT_MHD else ("option 8" if P.option == abi.OPT_MHD8
                                                         else "option 4")
     eng.close()

@@ -78,8 +78,10 @@ extern "C" {
                                 (initialize.f90:485-565)                        */
 #define SPARC_ARR_SCRPML 3   /* scrpml(:,:,:,f)                                 */
 #define SPARC_ARR_ALO 4      /* low-order part of a(:,:,:,f) when the FP32 state is
-                                compensated (cfg.compensated); get/put of
-                                SPARC_ARR_A then move hi+lo as one FP64 value    */
+                                compensated (cfg.compensated), as FP64 values;
+                                put rounds them to the 16-bit low words under the
+                                current high words.  get/put of SPARC_ARR_A then
+                                move hi+lo as one FP64 value                    */
 
 /* 3-D background fields (sparc_set_background).  n = nx*ny*nz for a full
  * 3-D field or n = nz for a horizontally uniform z-profile. */
@@ -182,9 +184,11 @@ typedef struct sparc_config {
   int32_t cadforcing_step;/* floor(cadforcing/timestep) (initialize.f90:977)    */
   int32_t forcing_length; /* source samples (initialize.f90:981)                */
   int32_t num_steps;      /* Lagrange end guard (driver.f90:280-281)            */
-  int32_t compensated;    /* FP32 only: hold the state as hi+lo FP32 pairs and run
-                             a += deltat*scr and the z filter in FP64 (the
-                             reference state is FP64); 0 = plain FP32 state     */
+  int32_t compensated;    /* FP32 only: hold the state as an FP32 word plus a
+                             16-bit low word in steps of its ulp/2^16 (40
+                             significant bits) and run a += deltat*scr and the
+                             z filter in FP64 (the reference state is FP64);
+                             0 = plain FP32 state                               */
   int32_t flow_kernels;   /* snapshots also keep v (9 fields) and the kernel sums
                              add the flow kernels SPARC_KERN_VX..VZ            */
   int32_t use_graphs;     /* replay each step's five stages and update as a
@@ -308,7 +312,8 @@ int sparc_kernel_accumulate_from(sparc_ctx *ctx, const sparc_ctx *src, int slot)
 /* ---- forward-wavefield checkpoints (restart state of driver.f90:489-524,
  * WRITE_OUT_FULL_STATE all_modules.f90:623-694, kept in device memory) ---- */
 
-/* Reserves nslots restart states: a(1..6) and pmlvars(1..2).  The next step
+/* Reserves nslots restart states: a(1..6) (with its low words when the
+ * state is compensated) and pmlvars(1..2).  The next step
  * needs nothing else (the first LDDRK stage uses a alone, step.f90:52-56). */
 int sparc_checkpoints_alloc(sparc_ctx *ctx, int nslots);
 int sparc_checkpoint_save(sparc_ctx *ctx, int slot);

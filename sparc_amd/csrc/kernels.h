@@ -273,7 +273,7 @@ __global__ __launch_bounds__(256) void k_ylines(int nx, int ny, int nz, int TX,
 template <typename T>
 struct UpdArgs {
   T* a[SPARC_MAXFIELDS];
-  float* lo[SPARC_MAXFIELDS];
+  short* lo[SPARC_MAXFIELDS];  // low words of the compensated FP32 state
   const T* scr[SPARC_MAXFIELDS];
 };
 
@@ -344,22 +344,47 @@ __global__ __launch_bounds__(64) void k_update_filter(UpdArgs<T> A, T dt, int nx
   }
 }
 
+// Low word of the compensated FP32 state: v = hi + q * 2^(e - 166), with q a
+// 16-bit integer and e = max(biased exponent of hi, 1), i.e. q counts steps of
+// ulp(hi) / 2^16. hi = fl32(v) leaves |v - hi| <= ulp(hi) / 2, so |q| <= 2^15
+// (clamped to 32767) and hi + lo carries 40 significant bits. The scale is a
+// power of two built from its exponent field, so encoding and decoding are
+// exact apart from the rounding of q.
+__host__ __device__ inline int lo_exponent(float hi) {
+  union { float f; unsigned u; } b{hi};
+  const int e = (int)((b.u >> 23) & 255u);
+  return e < 1 ? 1 : e;
+}
+__host__ __device__ inline double lo_pow2(int k) {  // 2^k for -1022 <= k <= 1023
+  union { unsigned long long u; double d; } b{(unsigned long long)(k + 1023) << 52};
+  return b.d;
+}
+__host__ __device__ inline double lo_decode(float hi, short q) {
+  return (double)q * lo_pow2(lo_exponent(hi) - 166);
+}
+__host__ __device__ inline short lo_encode(float hi, double v) {
+  double r = rint((v - (double)hi) * lo_pow2(166 - lo_exponent(hi)));
+  r = r > 32767.0 ? 32767.0 : (r < -32767.0 ? -32767.0 : r);
+  return (short)r;
+}
+
 // Compensated form for an FP32 state held as hi + lo: the update and the
 // filter run in FP64 on hi + lo + dt*scr and the result is split back into
-// hi = fl(v), lo = fl(v - hi), so the state carries ~48 significant bits and
-// the per-step rounding of the sum does not accumulate over a solve.
+// hi = fl32(v) and the 16-bit low word of v - hi, so the state carries 40
+// significant bits and the per-step rounding of the sum does not accumulate
+// over a solve.
 __global__ __launch_bounds__(64) void k_update_filter_c(UpdArgs<float> A, double dt, int nxy,
                                                         int nz, int kst) {
   constexpr int U = 8;
   const int c = blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= nxy) return;
   float* a = A.a[blockIdx.y];
-  float* lo = A.lo[blockIdx.y];
+  short* lo = A.lo[blockIdx.y];
   const float* __restrict__ scr = A.scr[blockIdx.y];
   const double c0 = 0.75390625, h1 = 0.5 * 0.41015625, h2 = 0.5 * -0.234375,
                h3 = 0.5 * 0.087890625, h4 = 0.5 * -0.01953125, h5 = 0.5 * 0.001953125;
-  auto V = [&](float x, float l, float s) -> double {
-    return (double)x + (double)l + dt * (double)s;
+  auto V = [&](float x, short l, float s) -> double {
+    return (double)x + lo_decode(x, l) + dt * (double)s;
   };
   double w[11];
 #pragma unroll
@@ -370,7 +395,8 @@ __global__ __launch_bounds__(64) void k_update_filter_c(UpdArgs<float> A, double
     const long q = c + (long)nxy * k;
     w[i] = (k < nz) ? V(a[q], lo[q], scr[q]) : 0.0;
   }
-  float na[U], nl[U], ns[U];
+  float na[U], ns[U];
+  short nl[U];
   auto load_chunk = [&](int kb) {
 #pragma unroll
     for (int j = 0; j < U; ++j) {
@@ -385,7 +411,8 @@ __global__ __launch_bounds__(64) void k_update_filter_c(UpdArgs<float> A, double
   };
   load_chunk(6);
   for (int k0 = 0; k0 < nz; k0 += U) {
-    float ca[U], cl[U], cs[U];
+    float ca[U], cs[U];
+    short cl[U];
 #pragma unroll
     for (int j = 0; j < U; ++j) {
       ca[j] = na[j];
@@ -410,7 +437,7 @@ __global__ __launch_bounds__(64) void k_update_filter_c(UpdArgs<float> A, double
         const long q = c + (long)nxy * k;
         const float hi = (float)o;
         a[q] = hi;
-        lo[q] = (float)(o - (double)hi);
+        lo[q] = lo_encode(hi, o);
 #pragma unroll
         for (int i = 0; i < 10; ++i) w[i] = w[i + 1];
         w[10] = (k + 6 < nz) ? V(ca[j], cl[j], cs[j]) : 0.0;

@@ -45,7 +45,7 @@ struct sparc_ctx {
   int nzpml = 0;
   void* a[SPARC_MAXFIELDS] = {};
   void* scr[SPARC_MAXFIELDS] = {};
-  void* alo[SPARC_MAXFIELDS] = {};  // low-order state words (compensated FP32 state)
+  void* alo[SPARC_MAXFIELDS] = {};  // 16-bit low state words (compensated FP32 state)
   int nf = SPARC_NFIELDS;           // dimfive: 5 (option 2), 6 (displacement), 8 (option 8)
   bool comp = false;
   void* pmlvars[6] = {};
@@ -337,7 +337,7 @@ static size_t footprint_bytes(const sparc_config* cfg) {
   size_t b = 0;
   const size_t nf = nfields_of(cfg->option);
   b += 2 * nf * N * ts;                     // a, scr
-  if (cfg->precision == SPARC_FP32 && cfg->compensated) b += nf * N * ts;  // alo
+  if (cfg->precision == SPARC_FP32 && cfg->compensated) b += nf * N * 2;  // alo
   const size_t npv = is_mhd(cfg->option) ? 6 : 2;
   b += 2 * npv * NP * ts;                   // pmlvars, scrpml
   if (is_mhd(cfg->option)) b += 12 * N * ts + cfg->nx * ts;  // MHD work, 2-D damping
@@ -446,7 +446,7 @@ extern "C" int sparc_create(const sparc_config* cfg, sparc_ctx** out) {
   for (int f = 0; f < c->nf; ++f) {
     if ((rc = dalloc(c, &c->a[f], N * ts))) return rc;
     if ((rc = dalloc(c, &c->scr[f], N * ts))) return rc;
-    if (c->comp && (rc = dalloc(c, &c->alo[f], N * ts))) return rc;
+    if (c->comp && (rc = dalloc(c, &c->alo[f], N * 2))) return rc;
   }
   c->npv = is_mhd(cfg->option) ? 6 : 2;
   for (int f = 0; f < c->npv; ++f) {
@@ -626,7 +626,7 @@ extern "C" int sparc_reset(sparc_ctx* c) {
   for (int f = 0; f < c->nf; ++f) {
     HIPCHK(c, hipMemsetAsync(c->a[f], 0, c->N * c->ts, c->stream));
     HIPCHK(c, hipMemsetAsync(c->scr[f], 0, c->N * c->ts, c->stream));
-    if (c->comp) HIPCHK(c, hipMemsetAsync(c->alo[f], 0, c->N * c->ts, c->stream));
+    if (c->comp) HIPCHK(c, hipMemsetAsync(c->alo[f], 0, c->N * 2, c->stream));
   }
   for (int f = 0; f < c->npv; ++f) {
     HIPCHK(c, hipMemsetAsync(c->pmlvars[f], 0, c->NP * c->ts, c->stream));
@@ -1724,19 +1724,19 @@ static int step_body(sparc_ctx* c, Fld<T>& F) {
   const int nfld = c->nf;  // dimfive
   {
     // all fields in one launch (blockIdx.y = field)
+    // a, scr read and a written; 2-byte low words read and written
     ProfScope ps(c, SPARC_KC_UPDATE,
-                 nfld * (c->comp ? 5.0 : 3.0) * (double)c->N * sizeof(T));
+                 nfld * (3.0 * sizeof(T) + (c->comp ? 4.0 : 0.0)) * (double)c->N);
     UpdArgs<T> A{};
     for (int f = 0; f < nfld; ++f) {
       A.a[f] = F.a[f];
-      A.lo[f] = (float*)c->alo[f];
       A.scr[f] = (const T*)F.scr[f];
     }
     if (c->comp) {
       UpdArgs<float> Af{};
       for (int f = 0; f < nfld; ++f) {
         Af.a[f] = (float*)F.a[f];
-        Af.lo[f] = (float*)c->alo[f];
+        Af.lo[f] = (short*)c->alo[f];
         Af.scr[f] = (const float*)F.scr[f];
       }
       hipLaunchKernelGGL(k_update_filter_c, dim3(cdiv(nxy, 64), nfld), dim3(64), 0, c->stream,
@@ -1893,16 +1893,39 @@ static void* arr_ptr(sparc_ctx* c, int arr, int field, size_t* n) {
   return nullptr;
 }
 
+// raw hi words and 16-bit low words of field f of the compensated state
+static int download_hilo(sparc_ctx* c, int f, std::vector<float>& hi, std::vector<short>& lo) {
+  hi.resize(c->N);
+  lo.resize(c->N);
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  HIPCHK(c, hipMemcpyAsync(hi.data(), c->a[f], c->N * 4, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipMemcpyAsync(lo.data(), c->alo[f], c->N * 2, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return SPARC_OK;
+}
+
+static int upload_raw(sparc_ctx* c, void* dst, const void* h, size_t bytes) {
+  HIPCHK(c, hipMemcpyAsync(dst, h, bytes, hipMemcpyHostToDevice, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));  // h is released by the caller
+  return SPARC_OK;
+}
+
 extern "C" int sparc_get_state(sparc_ctx* c, int arr, int field, double* h) {
   if (!c || !h) return SPARC_EINVAL;
   size_t n;
   void* p = arr_ptr(c, arr, field, &n);
   if (!p) return fail(c, SPARC_EINVAL, "bad array %d / field %d", arr, field);
-  int rc = download(c, h, p, n, 1.0 / c->cfg.field_scale);
-  if (rc || !(c->comp && arr == SPARC_ARR_A)) return rc;
-  std::vector<double> lo(n);
-  if ((rc = download(c, lo.data(), c->alo[field], n, 1.0 / c->cfg.field_scale))) return rc;
-  for (size_t i = 0; i < n; ++i) h[i] += lo[i];
+  const double s = 1.0 / c->cfg.field_scale;
+  if (!(c->comp && (arr == SPARC_ARR_A || arr == SPARC_ARR_ALO))) return download(c, h, p, n, s);
+  // hi + lo of the compensated state (or lo alone), decoded in FP64
+  std::vector<float> hi;
+  std::vector<short> lo;
+  int rc = download_hilo(c, field, hi, lo);
+  if (rc) return rc;
+  for (size_t i = 0; i < n; ++i) {
+    const double l = lo_decode(hi[i], lo[i]);
+    h[i] = (arr == SPARC_ARR_A ? (double)hi[i] + l : l) * s;
+  }
   return SPARC_OK;
 }
 
@@ -1915,16 +1938,23 @@ extern "C" int sparc_put_state(sparc_ctx* c, int arr, int field, const double* h
   std::vector<double> t(h, h + n);
   if (c->cfg.field_scale != 1.0)
     for (auto& v : t) v *= c->cfg.field_scale;
-  if (!(c->comp && arr == SPARC_ARR_A)) return upload(c, p, t.data(), n);
-  // split each FP64 value into hi = fl(v) and lo = fl(v - hi)
-  std::vector<double> lo(n);
-  for (size_t i = 0; i < n; ++i) {
-    const double hi = (double)(float)t[i];
-    lo[i] = t[i] - hi;
-    t[i] = hi;
+  if (!(c->comp && (arr == SPARC_ARR_A || arr == SPARC_ARR_ALO))) return upload(c, p, t.data(), n);
+  std::vector<float> hi(n);
+  std::vector<short> lo(n);
+  int rc;
+  if (arr == SPARC_ARR_A) {
+    // split each FP64 value into hi = fl32(v) and the low word of v - hi
+    for (size_t i = 0; i < n; ++i) {
+      hi[i] = (float)t[i];
+      lo[i] = lo_encode(hi[i], t[i]);
+    }
+    if ((rc = upload_raw(c, c->a[field], hi.data(), n * 4))) return rc;
+  } else {
+    // new low parts under the high words already held
+    if ((rc = download_hilo(c, field, hi, lo))) return rc;
+    for (size_t i = 0; i < n; ++i) lo[i] = lo_encode(hi[i], (double)hi[i] + t[i]);
   }
-  int rc = upload(c, p, t.data(), n);
-  return rc ? rc : upload(c, c->alo[field], lo.data(), n);
+  return upload_raw(c, c->alo[field], lo.data(), n * 2);
 }
 
 extern "C" int sparc_get_plane(sparc_ctx* c, int field, int k, double* h) {
@@ -2287,12 +2317,16 @@ extern "C" int sparc_kernel_accumulate(sparc_ctx* c, int slot) {
   return sparc_kernel_accumulate_from(c, c, slot);
 }
 
+static size_t ckpt_slot_bytes(const sparc_ctx* c) {
+  return c->nf * (size_t)c->N * c->ts + (c->comp ? c->nf * (size_t)c->N * 2 : 0) +
+         c->npv * (size_t)c->NP * c->ts;
+}
+
 extern "C" int sparc_checkpoints_alloc(sparc_ctx* c, int nslots) {
   if (!c || nslots < 0) return SPARC_EINVAL;
   if (nslots <= c->nckpt) return SPARC_OK;
   void* p;
-  const size_t per = ((c->comp ? 2 : 1) * c->nf * (size_t)c->N + c->npv * (size_t)c->NP) * c->ts;
-  int rc = dalloc(c, &p, (size_t)nslots * per);
+  int rc = dalloc(c, &p, (size_t)nslots * ckpt_slot_bytes(c));
   if (rc) return rc;
   c->ckpt = p;
   c->nckpt = nslots;
@@ -2301,21 +2335,23 @@ extern "C" int sparc_checkpoints_alloc(sparc_ctx* c, int nslots) {
 
 static int ckpt_copy(sparc_ctx* c, int slot, bool save) {
   if (!c || slot < 0 || slot >= c->nckpt) return fail(c, SPARC_EINVAL, "bad checkpoint %d", slot);
-  const size_t fb = c->N * c->ts, pb = c->NP * c->ts;
-  const int nf = c->comp ? 2 * c->nf : c->nf;
-  char* base = (char*)c->ckpt + (size_t)slot * (nf * fb + c->npv * pb);
-  for (int f = 0; f < nf; ++f) {
-    void* dev = f < c->nf ? c->a[f] : c->alo[f - c->nf];
-    void* d = save ? (void*)(base + f * fb) : dev;
-    const void* s = save ? (const void*)dev : (const void*)(base + f * fb);
-    HIPCHK(c, hipMemcpyAsync(d, s, fb, hipMemcpyDeviceToDevice, c->stream));
-  }
-  for (int f = 0; f < c->npv; ++f) {
-    char* pp = base + nf * fb + f * pb;
-    void* d = save ? (void*)pp : c->pmlvars[f];
-    const void* s = save ? (const void*)c->pmlvars[f] : (const void*)pp;
-    HIPCHK(c, hipMemcpyAsync(d, s, pb, hipMemcpyDeviceToDevice, c->stream));
-  }
+  // slot layout: a(:,:,:,f) for every field, their 16-bit low words when the
+  // state is compensated, then pmlvars
+  char* pp = (char*)c->ckpt + (size_t)slot * ckpt_slot_bytes(c);
+  auto move = [&](void* dev, size_t bytes) -> int {
+    void* d = save ? (void*)pp : dev;
+    const void* s = save ? (const void*)dev : (const void*)pp;
+    HIPCHK(c, hipMemcpyAsync(d, s, bytes, hipMemcpyDeviceToDevice, c->stream));
+    pp += bytes;
+    return SPARC_OK;
+  };
+  int rc;
+  for (int f = 0; f < c->nf; ++f)
+    if ((rc = move(c->a[f], c->N * c->ts))) return rc;
+  for (int f = 0; c->comp && f < c->nf; ++f)
+    if ((rc = move(c->alo[f], c->N * 2))) return rc;
+  for (int f = 0; f < c->npv; ++f)
+    if ((rc = move(c->pmlvars[f], c->NP * c->ts))) return rc;
   return SPARC_OK;
 }
 
