@@ -38,12 +38,14 @@ KERNEL_SYMBOLS = {
               "y derivative of xi_y (register four-step) in one launch",
     "ylines": "sparc::k_y256<float, 0 (YM_DERIV)> -- y derivative of xi_y, register four-step",
     "damp": "sparc::k_xw256<float, 2 (XM_DAMP_FWD), 4> -- damping, forward x of the stage temp "
-            "a + beta scr of the velocities, also stored as RHS_xi "
+            "a + beta scr of the velocities, also stored as RHS_xi; first stage only, the "
+            "last pass of the previous stage does it for the other four "
             "(with --damp2d: sparc::k_damp2d<float>, the whole damping, one 256x256 plane per block)",
     "damp_y": "sparc::k_y256_damp_deriv<float> -- damping y pass of v_x, v_y, v_z and the y derivative "
               "of p (RHS_vy) in one launch",
-    "damp_xi": "sparc::k_xw256_rhs_dampinv<float, 2> -- x derivative of p (RHS_vx) and the damping "
-               "inverse x of v_x, v_y, v_z subtracted from RHS_v in one launch",
+    "damp_xi": "sparc::k_xw256_rhs_dampinv<float, 2, true|false> -- x derivative of p (RHS_vx) and "
+               "the damping inverse x of v_x, v_y, v_z subtracted from RHS_v in one launch; in "
+               "stages 1-4 (true) also the next stage's damping forward x pass and RHS_xi",
     "zsolve": "sparc::k_zsolve<float, 7 (EPI_RHS3N), 8, true> -- z solve of p (ud in LDS) + RHS_vz",
     "zsolve_p": "sparc::k_zstream_pv -- streaming z solves, two walks per 64-column group (one "
                 "from each end): d xi_z/dz with the pressure, then dp/dz with RHS_vz "
@@ -354,7 +356,7 @@ CLASS_KERNELS = {
     "zsolve_p": ("k_zstream_pv",),
     "damp": ("k_xw256<float, 2, 4>",),
     "damp_y": ("k_y256_damp_deriv<float>",),
-    "damp_xi": ("k_xw256_rhs_dampinv<float, 2>",),
+    "damp_xi": ("k_xw256_rhs_dampinv<float, 2, true>", "k_xw256_rhs_dampinv<float, 2, false>"),
     "update": ("k_update_filter_c",),
     "xlines": ("k_xy256_div<float, 2>",),
     "ylines": ("k_y256<float, 0>",),

@@ -383,11 +383,14 @@ __device__ __forceinline__ void xw_inverse(C2<T> (&v)[4], const C2<T> (&w1)[3],
 // the input X.a (no stage temp) and writes -rhoinv d/dx minus the inverse
 // transform of spec, which is what the momentum-RHS derivative followed by
 // the damping inverse leave there.
+// PREP (XM_DERIV_DAMPINV and XM_DAMP_INV): each line then also does the
+// next stage's XM_DAMP_FWD from the RHS rows it has just written (X.pa,
+// X.pbeta, X.pcp; the spectrum goes over the one the line read).
 // LPW lines per wave, the next line's loads issued before the current
 // line's transform.
 // One 256-thread block of k_xw256: field fld; wave w takes lines
 // LPW (4 bx + w) .. LPW (4 bx + w) + LPW - 1.
-template <typename T, int MODE, int LPW>
+template <typename T, int MODE, int LPW, bool PREP = false>
 __device__ __forceinline__ void xw256_body(int nrows, int nlines, const XArgs<T>& X, T beta,
                                            const T* __restrict__ mult,
                                            const C2<T>* __restrict__ twg, int ny, int nz,
@@ -395,6 +398,8 @@ __device__ __forceinline__ void xw256_body(int nrows, int nlines, const XArgs<T>
   static_assert(MODE == XM_DERIV || MODE == XM_DAMP_FWD || MODE == XM_DAMP_INV ||
                     MODE == XM_DERIV_DAMPINV,
                 "k_xw256 modes");
+  static_assert(!PREP || MODE == XM_DAMP_INV || MODE == XM_DERIV_DAMPINV,
+                "the next stage is prepared by the damping inverse passes");
   constexpr bool DD = MODE == XM_DERIV_DAMPINV;
   constexpr int N = 256;
   const T* __restrict__ a = X.a[fld];
@@ -435,6 +440,7 @@ __device__ __forceinline__ void xw256_body(int nrows, int nlines, const XArgs<T>
   }
   C2<T> nv[4];
   T no0[MODE == XM_DAMP_INV ? 4 : 1], no1[MODE == XM_DAMP_INV ? 4 : 1];
+  C2<T> npa[PREP ? 4 : 1];  // PREP: rows of the next stage's a of the line
   // damping inverse: lane l takes X[kl + 64 j], the layout the inverse
   // stages start from, and the RHS values it updates
   auto edge_w = [&](long L, bool& w0, bool& w1) {
@@ -442,19 +448,24 @@ __device__ __forceinline__ void xw256_body(int nrows, int nlines, const XArgs<T>
     w0 = X.edges || (k0 != 0 && k0 != nz - 1);
     w1 = k1 < nz - 1 || (X.edges && k1 < nz);
   };
-#define XW_LOAD_INV(LL)                                              \
-  {                                                                  \
-    long o_, st_;                                                    \
-    bool h1_, w0_, w1_;                                              \
-    rows((LL), o_, st_, h1_);                                        \
-    edge_w((LL), w0_, w1_);                                          \
-    const T* __restrict__ ri_ = X.rin[fld] ? X.rin[fld] : out;       \
-    _Pragma("unroll") for (int j = 0; j < 4; ++j) {                  \
-      const int x = lane + 64 * j;                                   \
-      nv[j] = spec[(LL) * N + kl + 64 * j];                          \
-      no0[j] = w0_ ? ri_[o_ + x] : (T)0;                             \
-      no1[j] = w1_ ? ri_[o_ + st_ + x] : (T)0;                       \
-    }                                                                \
+#define XW_LOAD_INV(LL)                                                      \
+  {                                                                          \
+    long o_, st_;                                                            \
+    bool h1_, w0_, w1_;                                                      \
+    rows((LL), o_, st_, h1_);                                                \
+    edge_w((LL), w0_, w1_);                                                  \
+    const T* __restrict__ ri_ = X.rin[fld] ? X.rin[fld] : out;               \
+    _Pragma("unroll") for (int j = 0; j < 4; ++j) {                          \
+      const int x = lane + 64 * j;                                           \
+      nv[j] = spec[(LL) * N + kl + 64 * j];                                  \
+      /* PREP: rows the pass leaves alone keep the RHS held in out */        \
+      no0[j] = w0_ ? ri_[o_ + x] : (PREP ? out[o_ + x] : (T)0);              \
+      no1[j] = w1_ ? ri_[o_ + st_ + x] : ((PREP && h1_) ? out[o_ + st_ + x] : (T)0); \
+      if constexpr (PREP) {                                                  \
+        npa[j].x = X.pa[fld][o_ + x];                                        \
+        npa[j].y = h1_ ? X.pa[fld][o_ + st_ + x] : (T)0;                     \
+      }                                                                      \
+    }                                                                        \
   }
   // raw a and s values of the next line: the stage temp a + beta s is formed
   // when the line is used, so the loads stay in flight meanwhile
@@ -477,18 +488,44 @@ __device__ __forceinline__ void xw256_body(int nrows, int nlines, const XArgs<T>
         nv[j].y = h1_ ? a0_[st_ + x] : (T)0;                       \
         if (s0_) ns[j].y = h1_ ? s0_[st_ + x] : (T)0;              \
         if constexpr (DD) nsp[j] = spec[(LL) * N + kl + 64 * j];   \
+        if constexpr (PREP) {                                      \
+          npa[j].x = X.pa[fld][o_ + x];                            \
+          npa[j].y = h1_ ? X.pa[fld][o_ + st_ + x] : (T)0;         \
+        }                                                          \
       }                                                            \
     }                                                              \
   } while (0)
+  // PREP: the next stage temp pa + pbeta r of line L from the RHS rows r0,
+  // r1 just written, as XM_DAMP_FWD forms, stores and transforms it
+  auto prep_line = [&](long L, const C2<T> (&pa)[4], const T (&r0)[4], const T (&r1)[4]) {
+    long o, st;
+    bool h1;
+    rows(L, o, st, h1);
+    const int k0 = 2 * (int)((unsigned)L / (unsigned)ny);
+    const bool e0 = k0 == 0 || k0 == nz - 1, e1 = k0 + 1 == nz - 1;
+    T* __restrict__ cp = X.pcp[fld];
+    C2<T> u[4];
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      u[j] = C2<T>{pa[j].x + X.pbeta * r0[j], pa[j].y + X.pbeta * r1[j]};
+      cp[o + lane + 64 * j] = e0 ? (T)0 : u[j].x;
+      if (h1) cp[o + st + lane + 64 * j] = e1 ? (T)0 : u[j].y;
+    }
+    xw_forward(u, w1, w2, w3, lane);
+#pragma unroll
+    for (int j = 0; j < 4; ++j) spec[L * N + kl + 64 * j] = u[j];
+  };
   if (L0 < nlines) XW_LOAD(L0);
   for (int p = 0; p < LPW; ++p) {
     const long L = L0 + p;
     if (L >= nlines) break;
-    C2<T> v[4], sp[DD ? 4 : 1];
+    C2<T> v[4], sp[DD ? 4 : 1], pa[PREP ? 4 : 1];
     T ov0[MODE == XM_DAMP_INV ? 4 : 1], ov1[MODE == XM_DAMP_INV ? 4 : 1];
+    T r0a[PREP ? 4 : 1], r1a[PREP ? 4 : 1];
 #pragma unroll
     for (int j = 0; j < 4; ++j) {
       v[j] = nv[j];
+      if constexpr (PREP) pa[j] = npa[j];
       if constexpr (DD) sp[j] = nsp[j];
       if (MODE != XM_DAMP_INV && s) v[j] = C2<T>{v[j].x + beta * ns[j].x, v[j].y + beta * ns[j].y};
       if (MODE == XM_DAMP_INV) {
@@ -560,7 +597,12 @@ __device__ __forceinline__ void xw256_body(int nrows, int nlines, const XArgs<T>
           const int x = lane + 64 * j;
           out[dd_o + x] = v[j].x;
           if (dd_h1) out[dd_o + dd_st + x] = v[j].y;
+          if constexpr (PREP) {
+            r0a[j] = v[j].x;
+            r1a[j] = dd_h1 ? v[j].y : (T)0;
+          }
         }
+        if constexpr (PREP) prep_line(L, pa, r0a, r1a);
         continue;
       }
       xw_inverse(sp, w1, w2, w3, lane);
@@ -571,13 +613,19 @@ __device__ __forceinline__ void xw256_body(int nrows, int nlines, const XArgs<T>
         T r0 = dd_e0 ? (T)0 : -ri0 * v[j].x;
         if (dd_w0) r0 = r0 - sp[j].x;
         out[dd_o + x] = r0;
+        T r1 = (T)0;
         if (dd_h1) {
           const T ri1 = X.r3 ? X.r3[dd_o + dd_st + x] : dd_z1;
-          T r1 = dd_e1 ? (T)0 : -ri1 * v[j].y;
+          r1 = dd_e1 ? (T)0 : -ri1 * v[j].y;
           if (dd_w1) r1 = r1 - sp[j].y;
           out[dd_o + dd_st + x] = r1;
         }
+        if constexpr (PREP) {
+          r0a[j] = r0;
+          r1a[j] = r1;
+        }
       }
+      if constexpr (PREP) prep_line(L, pa, r0a, r1a);
       continue;
     }
     if (MODE == XM_DAMP_INV) {
@@ -590,7 +638,12 @@ __device__ __forceinline__ void xw256_body(int nrows, int nlines, const XArgs<T>
         const int x = lane + 64 * j;
         if (w0) out[o + x] = ov0[j] - v[j].x;
         if (w1) out[o + st + x] = ov1[j] - v[j].y;
+        if constexpr (PREP) {
+          r0a[j] = w0 ? ov0[j] - v[j].x : ov0[j];
+          r1a[j] = w1 ? ov1[j] - v[j].y : ov1[j];
+        }
       }
+      if constexpr (PREP) prep_line(L, pa, r0a, r1a);
       continue;
     }
     const long r0 = 2 * L;
@@ -638,17 +691,19 @@ __global__ __launch_bounds__(256) void k_xw256(int nrows, int nlines, XArgs<T> X
 // the lighter pass fill the other's tail: blockIdx.y = 0 the x derivative of
 // p as the momentum RHS with the damping inverse of v_x (XM_DERIV_DAMPINV on
 // H), 1 and 2 the damping inverse of fields 0 and 1 of D (XM_DAMP_INV).
-template <typename T, int LPW>
+// PREP: each of them also does the next stage's damping forward pass of its
+// velocity (H field 0, D fields 0 and 1) from the RHS it has just written.
+template <typename T, int LPW, bool PREP>
 __global__ __launch_bounds__(256) void k_xw256_rhs_dampinv(int nrows, int nlines, XArgs<T> H,
                                                            XArgs<T> D, const T* __restrict__ mult,
                                                            const C2<T>* __restrict__ twg, int ny,
                                                            int nz, long nxy) {
   if (blockIdx.y == 0)
-    xw256_body<T, XM_DERIV_DAMPINV, LPW>(nrows, nlines, H, (T)0, mult, twg, ny, nz, nxy, 0,
-                                         blockIdx.x);
+    xw256_body<T, XM_DERIV_DAMPINV, LPW, PREP>(nrows, nlines, H, (T)0, mult, twg, ny, nz, nxy,
+                                               0, blockIdx.x);
   else
-    xw256_body<T, XM_DAMP_INV, LPW>(nrows, nlines, D, (T)0, mult, twg, ny, nz, nxy,
-                                    blockIdx.y - 1, blockIdx.x);
+    xw256_body<T, XM_DAMP_INV, LPW, PREP>(nrows, nlines, D, (T)0, mult, twg, ny, nz, nxy,
+                                          blockIdx.y - 1, blockIdx.x);
 }
 
 // DAMP_VELOCITY (damping.f90:125-185) on one 256 x 256 plane per block:

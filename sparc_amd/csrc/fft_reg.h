@@ -247,6 +247,14 @@ struct XArgs {
   // damping inverse pass of k_xw256: the RHS is read from here (the result
   // still goes to out), when set
   const T* rin[3];
+  // damping inverse passes of k_xw256_rhs_dampinv<..., true>, which also do
+  // the next stage's damping forward pass: with the RHS r each line leaves in
+  // out, the next stage temp pa + pbeta r is stored to pcp (RHS_xi, zero on
+  // planes 1 and nz) and its forward transform to spec, over the spectrum
+  // the line has just read
+  const T* pa[3];
+  T* pcp[3];
+  T pbeta;
 };
 
 // plain derivative store: d, or combined with what out holds (X.acc)

@@ -960,14 +960,21 @@ static int launch_y_damp_deriv(sparc_ctx* c, const XArgs<T>& D, const XArgs<T>& 
 // and the damping inverse of two more fields (D) in one launch
 template <typename T>
 static int launch_x_rhs_dampinv(sparc_ctx* c, const XArgs<T>& H, const XArgs<T>& D,
-                                const T* mult) {
+                                const T* mult, bool prep) {
   const long nlines = (long)((c->cfg.nz + 1) / 2) * c->cfg.ny;
   const long nrows = (long)c->cfg.ny * c->cfg.nz;
-  // p, the damping spectrum, the RHS written; two spectra, two RHS read and written
-  ProfScope ps(c, SPARC_KC_DAMP_XI, 9.0 * (double)c->N * sizeof(T));
-  hipLaunchKernelGGL((k_xw256_rhs_dampinv<T, kXwLpw>), dim3(cdiv(nlines, 4 * kXwLpw), 3),
-                     dim3(256), 0, c->stream, (int)nrows, (int)nlines, H, D, mult,
-                     (const C2<T>*)c->twx, c->cfg.ny, c->cfg.nz, c->nxy);
+  // p, the damping spectrum, the RHS written; two spectra, two RHS read and
+  // written; prep: the three velocities read, their RHS_xi and spectra written
+  ProfScope ps(c, SPARC_KC_DAMP_XI, (prep ? 18.0 : 9.0) * (double)c->N * sizeof(T));
+  const dim3 grid(cdiv(nlines, 4 * kXwLpw), 3);
+  if (prep)
+    hipLaunchKernelGGL((k_xw256_rhs_dampinv<T, kXwLpw, true>), grid, dim3(256), 0, c->stream,
+                       (int)nrows, (int)nlines, H, D, mult, (const C2<T>*)c->twx, c->cfg.ny,
+                       c->cfg.nz, c->nxy);
+  else
+    hipLaunchKernelGGL((k_xw256_rhs_dampinv<T, kXwLpw, false>), grid, dim3(256), 0, c->stream,
+                       (int)nrows, (int)nlines, H, D, mult, (const C2<T>*)c->twx, c->cfg.ny,
+                       c->cfg.nz, c->nxy);
   LAUNCHCHK(c);
   return SPARC_OK;
 }
@@ -1297,7 +1304,29 @@ static int run_stage(sparc_ctx* c, Fld<T>& F, int s) {
   const T beta = (T)c->betas[s];
   F.beta = beta;
   const bool first = (s == 0);
-  auto S = [&](int f) -> const T* { return first ? nullptr : (const T*)c->scr[f]; };
+  // Option 4 with the damping on 256 x 256 planes, the fused z solve and the
+  // fused damping y pass: the last pass of stages 0-3 also does the next
+  // stage's damping forward pass (chain). It writes RHS_xi of stage s + 1
+  // while the divergence passes of stage s + 1 have still to read RHS_xi of
+  // stage s, so RHS_xi alternates between scr (even stages) and W2, W5, W6
+  // (odd stages, idle in this path); stage 4's, which the update reads,
+  // lands in scr.
+  bool fuse = false, pv = false, ydd = false, chain = false;
+  if (c->cfg.option == SPARC_OPT_QUIET_PML_3D_DISPL) {
+    const bool damp = c->cfg.damp_waves && c->damping_set;
+    const bool d2d = damp && c->damp2d && sizeof(T) == 4 && c->cfg.nx == 256 &&
+                     c->cfg.ny == 256;
+    fuse = damp && !d2d && c->cfg.nx == 256;
+    pv = fuse && zpv_fits<T>(c);
+    ydd = pv && y_damp_deriv_fits<T>(c);
+    chain = ydd && c->cfg.ny == 256;
+  }
+  T* const xi_alt[3] = {F.W[2], F.W[5], F.W[6]};
+  auto xi_buf = [&](int stage, int f) -> T* { return (stage & 1) ? xi_alt[f] : F.scr[f]; };
+  auto S = [&](int f) -> const T* {
+    if (first) return nullptr;
+    return (chain && f < 3) ? xi_buf(s - 1, f) : (const T*)c->scr[f];
+  };
   const T* dmx = (const T*)c->dmulx;
   const T* dmy = (const T*)c->dmuly;
   int rc;
@@ -1328,17 +1357,14 @@ static int run_stage(sparc_ctx* c, Fld<T>& F, int s) {
     // one pass per plane
     const bool d2d = damp && c->damp2d && sizeof(T) == 4 && c->cfg.nx == 256 &&
                      c->cfg.ny == 256;
-    // with damping on 256-wide x lines: the damping forward pass reads the
+    // fuse (damping on 256-wide x lines): the damping forward pass reads the
     // stage temp v = a + beta scr of the velocities itself and also stores
     // it as RHS_xi (once the divergence passes above have read the previous
     // RHS_xi), so the z solve of p below writes only RHS_vz; RHS_vx is
-    // written once, by the pass that also subtracts its damping
-    const bool fuse = damp && !d2d && c->cfg.nx == 256;
-    // with fuse, FP32 and columns that fit: p, rho' and the z part of grad p
-    // in one z-solve launch (k_zsolve_pv), RHS_vz staged in W4 for the
+    // written once, by the pass that also subtracts its damping.
+    // pv (fuse, FP32 and columns that fit): p, rho' and the z part of grad p
+    // in one z-solve launch (k_zstream_pv), RHS_vz staged in W4 for the
     // damping inverse; the damping forward pass then follows it
-    const bool pv = fuse && zpv_fits<T>(c);
-    const bool ydd = pv && y_damp_deriv_fits<T>(c);
     XArgs<T> DX{};
     for (int f = 0; f < 3; ++f) {
       DX.out[f] = F.scr[3 + f];
@@ -1350,13 +1376,16 @@ static int run_stage(sparc_ctx* c, Fld<T>& F, int s) {
       if ((rc = launch_z<T, EPI_P3D>(c, F, 1, F.a[2], S(2), beta, nullptr, 0, nullptr))) return rc;
     }
     if (fuse) {
-      // DAMP_VELOCITY (damping.f90:125-185) forward x and y passes
+      // DAMP_VELOCITY (damping.f90:125-185) forward x and y passes; with
+      // chain, stages 1-4 have the x pass done by the previous stage's last
+      // pass
       for (int f = 0; f < 3; ++f) {
         DX.a[f] = F.a[3 + f];
         DX.s[f] = S(3 + f);
         DX.cp[f] = F.scr[f];
       }
-      if ((rc = launch_xm<T, XM_DAMP_FWD>(c, 3, DX, beta, nullptr))) return rc;
+      if ((!chain || first) && (rc = launch_xm<T, XM_DAMP_FWD>(c, 3, DX, beta, nullptr)))
+        return rc;
       // with the fused z solve the damping y pass runs with the y derivative
       // of p below
       if (!ydd && (rc = launch_ym<T, YM_DAMP>(c, 3, DX, 0, nullptr))) return rc;
@@ -1398,7 +1427,19 @@ static int run_stage(sparc_ctx* c, Fld<T>& F, int s) {
       }
       if (pv) D2.rin[1] = F.W[4];  // RHS_vz before the damping
       D2.edges = DX.edges;
-      if ((rc = launch_x_rhs_dampinv<T>(c, H, D2, dmx))) return rc;
+      const bool prep = chain && s < 4;
+      if (prep) {
+        // the next stage's temp v = a + beta' RHS_v, its x spectrum and RHS_xi
+        H.pa[0] = F.a[3];
+        H.pcp[0] = xi_buf(s + 1, 0);
+        H.pbeta = (T)c->betas[s + 1];
+        for (int f = 0; f < 2; ++f) {
+          D2.pa[f] = F.a[4 + f];
+          D2.pcp[f] = xi_buf(s + 1, 1 + f);
+        }
+        D2.pbeta = H.pbeta;
+      }
+      if ((rc = launch_x_rhs_dampinv<T>(c, H, D2, dmx, prep))) return rc;
     } else if (damp) {
       // DAMP_VELOCITY (damping.f90:125-185) on the stage's temp v, which the
       // epilogue above has just stored as RHS_xi = v (zero on planes 1 and nz,
