@@ -20,12 +20,13 @@ bump, radius 5 Mm, 3 Mm deep).
   roundings (0.092 and 0.125); the FP64 differences are free of it.
 * Peak device memory of the checkpointed solve is reported.
 
-Measured: misfits 7.570765 (FP64) and 7.570474 (FP32); adjoint against the
+Measured: misfits 7.570765 (FP64) and 7.570786 (FP32), FP32 gradient 6.8e-5
+relative L2 from the FP64 one; adjoint against the
 FP64 differences 8.9e-2 relative L2 over the 16 directions for the FP64 and
 the FP32 adjoint alike, the large ones within 0.79-1.18 (the kernel formula
 is the adjoint of the continuous equations, so the agreement is at
 discretisation level; the outliers are the directions whose misfit change is
-under 1 % of the largest); peak device memory 62.9 GiB (a 100-step segment
+under 1 % of the largest); peak device memory 59.2 GiB (a 100-step segment
 of per-step snapshots plus the checkpoints).
 
 1401 steps (16.3 min of solar time) instead of the configuration's 2000
