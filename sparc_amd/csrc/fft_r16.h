@@ -35,14 +35,64 @@ __device__ __forceinline__ void twiddle16(C2<T> (&v)[16], const C2<T>* tw2, int 
   }
 }
 
-// lane u holds row u's 16 entries after the exchange: writes (k, u), reads (u, k)
+// lane u holds row u's 16 entries after the exchange: writes (k, u), reads
+// (u, k); real parts first, then imaginary parts through the same tile of T,
+// which halves the block's LDS (four blocks of 40 KB filled a CU; this leaves
+// room for six)
 template <typename T>
-__device__ __forceinline__ void exchange16(C2<T> (&v)[16], C2<T>* Sl, int u) {
+__device__ __forceinline__ void exchange16(C2<T> (&v)[16], T* Sl, int u) {
 #pragma unroll
-  for (int k = 0; k < 16; ++k) Sl[k * R16P + u] = v[k];
+  for (int k = 0; k < 16; ++k) Sl[k * R16P + u] = v[k].x;
   __syncthreads();
 #pragma unroll
-  for (int k = 0; k < 16; ++k) v[k] = Sl[u * R16P + k];
+  for (int k = 0; k < 16; ++k) v[k].x = Sl[u * R16P + k];
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < 16; ++k) Sl[k * R16P + u] = v[k].y;
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < 16; ++k) v[k].y = Sl[u * R16P + k];
+}
+
+// Buffer view of `bytes` bytes at p, and complex loads and stores through it
+// at a per-lane offset plus a wave-uniform one: the row offsets of a y line
+// stay in scalar registers instead of one 64-bit address per row.
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t ybuf(const void* p, unsigned bytes) {
+  return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(p), (short)0, (int)bytes,
+                                           0x00020000);
+}
+typedef unsigned int ybu2 __attribute__((ext_vector_type(2)));
+typedef unsigned int ybu4 __attribute__((ext_vector_type(4)));
+template <typename T>
+__device__ __forceinline__ C2<T> ybl(__amdgpu_buffer_rsrc_t r, unsigned vo, unsigned so) {
+  if constexpr (sizeof(T) == 4) {
+    const ybu2 u = __builtin_amdgcn_raw_buffer_load_b64(r, (int)vo, (int)so, 0);
+    return C2<T>{__uint_as_float(u.x), __uint_as_float(u.y)};
+  } else {
+    const ybu4 u = __builtin_amdgcn_raw_buffer_load_b128(r, (int)vo, (int)so, 0);
+    return C2<T>{__hiloint2double((int)u.y, (int)u.x), __hiloint2double((int)u.w, (int)u.z)};
+  }
+}
+template <typename T>
+__device__ __forceinline__ void ybs(__amdgpu_buffer_rsrc_t r, unsigned vo, unsigned so,
+                                    C2<T> v) {
+  if constexpr (sizeof(T) == 4) {
+    const ybu2 u = {__float_as_uint(v.x), __float_as_uint(v.y)};
+    __builtin_amdgcn_raw_buffer_store_b64(u, r, (int)vo, (int)so, 0);
+  } else {
+    const long long x = __double_as_longlong(v.x), y = __double_as_longlong(v.y);
+    const ybu4 u = {(unsigned)x, (unsigned)(x >> 32), (unsigned)y, (unsigned)(y >> 32)};
+    __builtin_amdgcn_raw_buffer_store_b128(u, r, (int)vo, (int)so, 0);
+  }
+}
+template <typename T>
+__device__ __forceinline__ T ybl1(__amdgpu_buffer_rsrc_t r, unsigned vo, unsigned so) {
+  if constexpr (sizeof(T) == 4) {
+    return __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(r, (int)vo, (int)so, 0));
+  } else {
+    const ybu2 u = __builtin_amdgcn_raw_buffer_load_b64(r, (int)vo, (int)so, 0);
+    return __hiloint2double((int)u.y, (int)u.x);
+  }
 }
 
 // block table W256^(u*k) at [k*16 + u]
@@ -60,7 +110,9 @@ __device__ __forceinline__ void load_tw2(C2<T>* tw2, const C2<T>* __restrict__ t
 // i0 + 2c, i0 + 2c + 1 (i0 = 32 * blockIdx.x).  DAMP: line c is x-bin
 // i0 + c (i0 = 16 * blockIdx.x) of a plane pair's spectrum, times the damping
 // rates held as rates_o[m + (nx/2+1) * y].  blockIdx.z selects the field.
-constexpr int kY256Lsy = 18 * R16P - 16;  // 290 = 2 (mod 32)
+// line stride of the exchange tile in T: 292 = 4 (mod 32), so both the
+// column writes and the row reads of a wave spread over all 32 banks
+constexpr int kY256Lsy = 292;
 
 // One block of k_y256: x-chunk bx, plane (pair) k, field fld; S, tw2 and ml
 // are the block's LDS tiles.
@@ -68,7 +120,7 @@ template <typename T, int MODE>
 __device__ __forceinline__ void y256_body(int nx, int nz, const XArgs<T>& X, T beta,
                                           const T* __restrict__ mult,
                                           const C2<T>* __restrict__ twg,
-                                          const T* __restrict__ rates_o, C2<T>* S, C2<T>* tw2,
+                                          const T* __restrict__ rates_o, T* S, C2<T>* tw2,
                                           T* ml, int bx, long k, int fld) {
   constexpr int NY = 256, LSY = kY256Lsy;
   const T* __restrict__ a = X.a[fld];
@@ -80,33 +132,39 @@ __device__ __forceinline__ void y256_body(int nx, int nz, const XArgs<T>& X, T b
     for (int e = threadIdx.x; e < NY; e += blockDim.x) ml[e] = mult[e];
   const int c = threadIdx.x & 15, w = threadIdx.x >> 4;
   const long nxy = (long)nx * NY;
-  C2<T>* Sl = S + c * LSY;
+  T* Sl = S + c * LSY;
   C2<T> v[16];
+  // one plane (pair) per block: buffers over it, lane offset vo, row n at
+  // the scalar offset n * rs
+  const int i0 = MODE == YM_DAMP ? bx * 16 : bx * 32;
+  const unsigned vo = MODE == YM_DAMP ? (unsigned)((w * nx + i0 + c) * sizeof(C2<T>))
+                                      : (unsigned)((w * nx + i0 + 2 * c) * sizeof(T));
+  const unsigned rs = MODE == YM_DAMP ? (unsigned)(16 * nx * sizeof(C2<T>))
+                                      : (unsigned)(16 * nx * sizeof(T));
+  const __amdgpu_buffer_rsrc_t rsp =
+      MODE == YM_DAMP ? ybuf(spec + k * NY * nx, (unsigned)(NY * nx * sizeof(C2<T>)))
+                      : ybuf(out + nxy * k, (unsigned)(nxy * sizeof(T)));
   if (MODE == YM_DAMP) {
-    const int i0 = bx * 16;
 #pragma unroll
-    for (int n = 0; n < 16; ++n) v[n] = spec[(k * NY + 16 * n + w) * nx + i0 + c];
+    for (int n = 0; n < 16; ++n) v[n] = ybl<T>(rsp, vo, n * rs);
   } else {
     // all 16 (or 32) loads are issued before the stage temp a + beta s is
     // formed; combining them inside the loop made every row wait for its
     // own loads
-    const long x0 = (long)bx * 32 + 2 * c;
+    const __amdgpu_buffer_rsrc_t ra = ybuf(a + nxy * k, (unsigned)(nxy * sizeof(T)));
     if (s) {
+      const __amdgpu_buffer_rsrc_t rsc = ybuf(s + nxy * k, (unsigned)(nxy * sizeof(T)));
       C2<T> t[16];
 #pragma unroll
       for (int n = 0; n < 16; ++n) {
-        const long o = x0 + (long)nx * (16 * n + w) + nxy * k;
-        v[n] = *reinterpret_cast<const C2<T>*>(a + o);
-        t[n] = *reinterpret_cast<const C2<T>*>(s + o);
+        v[n] = ybl<T>(ra, vo, n * rs);
+        t[n] = ybl<T>(rsc, vo, n * rs);
       }
 #pragma unroll
       for (int n = 0; n < 16; ++n) v[n] = {v[n].x + beta * t[n].x, v[n].y + beta * t[n].y};
     } else {
 #pragma unroll
-      for (int n = 0; n < 16; ++n) {
-        const long o = x0 + (long)nx * (16 * n + w) + nxy * k;
-        v[n] = *reinterpret_cast<const C2<T>*>(a + o);
-      }
+      for (int n = 0; n < 16; ++n) v[n] = ybl<T>(ra, vo, n * rs);
     }
   }
   __syncthreads();  // tables
@@ -117,10 +175,12 @@ __device__ __forceinline__ void y256_body(int nx, int nz, const XArgs<T>& X, T b
   if (MODE == YM_DAMP) {
     const int m = bx * 16 + c;
     const int mi = (m <= nx / 2) ? m : nx - m;
-    const long nh = nx / 2 + 1;
+    const int nh = nx / 2 + 1;
+    const __amdgpu_buffer_rsrc_t rr = ybuf(rates_o, (unsigned)(nh * NY * sizeof(T)));
 #pragma unroll
     for (int q = 0; q < 16; ++q) {
-      const T r = rates_o[mi + nh * (w + 16 * q)];
+      const T r = ybl1<T>(rr, (unsigned)((mi + nh * w) * sizeof(T)),
+                          (unsigned)(16 * q * nh * sizeof(T)));
       v[q] = {r * v[q].x, r * v[q].y};
     }
   } else {
@@ -137,22 +197,21 @@ __device__ __forceinline__ void y256_body(int nx, int nz, const XArgs<T>& X, T b
   exchange16(v, Sl, w);
   dft_reg<T, 16, +1>(v);  // v[n2] = y-line value at w + 16 n2
   if (MODE == YM_DAMP) {
-    const int i0 = bx * 16;
 #pragma unroll
-    for (int n = 0; n < 16; ++n) spec[(k * NY + w + 16 * n) * nx + i0 + c] = v[n];
+    for (int n = 0; n < 16; ++n) ybs<T>(rsp, vo, n * rs, v[n]);
   } else {
-    const long x0 = (long)bx * 32 + 2 * c;
+    const long x0 = (long)i0 + 2 * c;
 #pragma unroll
     for (int n = 0; n < 16; ++n) {
-      const long o = x0 + (long)nx * (w + 16 * n) + nxy * k;
       C2<T> z = v[n];
       if (MODE == YM_DERIV && X.rhs) {
+        const long o = x0 + (long)nx * (w + 16 * n) + nxy * k;
         z = {rhs_of(X, k, o, nz, z.x), rhs_of(X, k, o + 1, nz, z.y)};
       } else if (X.acc) {
-        const C2<T> w = *reinterpret_cast<const C2<T>*>(out + o);
-        z = {X.ald * z.x + X.gout * w.x, X.ald * z.y + X.gout * w.y};
+        const C2<T> u = ybl<T>(rsp, vo, n * rs);
+        z = {X.ald * z.x + X.gout * u.x, X.ald * z.y + X.gout * u.y};
       }
-      *reinterpret_cast<C2<T>*>(out + o) = z;
+      ybs<T>(rsp, vo, n * rs, z);
     }
   }
 }
@@ -162,7 +221,7 @@ __global__ __launch_bounds__(256) void k_y256(int nx, int nz, XArgs<T> X, T beta
                                               const T* __restrict__ mult,
                                               const C2<T>* __restrict__ twg,
                                               const T* __restrict__ rates_o) {
-  __shared__ C2<T> S[16 * kY256Lsy];
+  __shared__ T S[16 * kY256Lsy];
   __shared__ C2<T> tw2[256];
   __shared__ T ml[256];
   y256_body<T, MODE>(nx, nz, X, beta, mult, twg, rates_o, S, tw2, ml, blockIdx.x, blockIdx.y,
@@ -181,7 +240,7 @@ __global__ __launch_bounds__(256) void k_xy256_div(int nrows, int nlines, XArgs<
                                                    const C2<T>* __restrict__ twx,
                                                    const C2<T>* __restrict__ twy, int nx, int ny,
                                                    int nz, long nxy, int nbx) {
-  __shared__ C2<T> S[16 * kY256Lsy];
+  __shared__ T S[16 * kY256Lsy];
   __shared__ C2<T> tw2[256];
   __shared__ T ml[256];
   int b = blockIdx.x;
@@ -205,7 +264,7 @@ __global__ __launch_bounds__(256) void k_y256_damp_deriv(int nx, int nz, XArgs<T
                                                          const C2<T>* __restrict__ twg,
                                                          const T* __restrict__ rates_o,
                                                          int nbd) {
-  __shared__ C2<T> S[16 * kY256Lsy];
+  __shared__ T S[16 * kY256Lsy];
   __shared__ C2<T> tw2[256];
   __shared__ T ml[256];
   int b = blockIdx.x;
