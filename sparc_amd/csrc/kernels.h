@@ -537,9 +537,13 @@ __device__ __forceinline__ T tmp_of(const Fld<T>& F, int f, long q) {
   return v;
 }
 
-// flux = xi x B0 (displacement.f90:673-675: flux1 = -(xi_z boy - xi_y boz), ...)
+// div xi -> p, rho with the horizontal background gradients (:694-700):
+// W0 + W1 + W2 = div, out p -> W3, rho -> W4; and, from the same stage temp
+// xi, flux = xi x B0 -> M0..M2 (:673-675: flux1 = -(xi_z boy - xi_y boz),
+// ...), which the reference forms before the divergence: nothing between
+// the two reads or writes M0..M2, and xi is read once
 template <typename T>
-__global__ void k_mhd_flux(Fld<T> F, long n) {
+__global__ void k_mhd_pr(Fld<T> F, long n) {
   const long q = blockIdx.x * (long)blockDim.x + threadIdx.x;
   if (q >= n) return;
   const int k = (int)(q / F.nxy), c = (int)(q - (long)k * F.nxy);
@@ -548,16 +552,6 @@ __global__ void k_mhd_flux(Fld<T> F, long n) {
   F.M[0][q] = -(xz * by - xy * bz);
   F.M[1][q] = -(xx * bz - xz * bx);
   F.M[2][q] = -(xy * bx - xx * by);
-}
-
-// div xi -> p, rho with the horizontal background gradients (:694-700):
-// W0 + W1 + W2 = div, out p -> W3, rho -> W4
-template <typename T>
-__global__ void k_mhd_pr(Fld<T> F, long n) {
-  const long q = blockIdx.x * (long)blockDim.x + threadIdx.x;
-  if (q >= n) return;
-  const int k = (int)(q / F.nxy), c = (int)(q - (long)k * F.nxy);
-  const T xx = tmp_of(F, SPARC_XI_X, q), xy = tmp_of(F, SPARC_XI_Y, q), xz = tmp_of(F, SPARC_XI_Z, q);
   const T div = F.W[0][q] + F.W[1][q] + F.W[2][q];
   F.W[3][q] = -bgv(F, SPARC_BG_C2RHO0, c, k) * div - xz * bgv(F, SPARC_BG_GRADP0_Z, c, k) -
               xx * bgv(F, SPARC_BG_GRADP0_X, c, k) - xy * bgv(F, SPARC_BG_GRADP0_Y, c, k);
@@ -593,14 +587,25 @@ __global__ void k_mhd_b(Fld<T> F, long n) {
 // J = curl(b), the Lorentz force J x B0 + J0 x b, grad p, sponge, gravity,
 // RHSxi = v, source, zeroed planes 1 and nz (:776-846).  gradp in M3..M5,
 // b in M0..M2, curl pieces dx by, dy bz, dz bx, dx bz, dy bx, dz by in M6..M11,
-// rho in W4.  The damping of v is subtracted afterwards.
+// rho in W4.  The damping of v is subtracted afterwards.  vin: the damping
+// forward pass has already stored the stage temp v as RHS_xi (zero on
+// planes 1 and nz), so v is read from there and RHS_xi is not written.
 template <typename T>
-__global__ void k_mhd_rhs(Fld<T> F, const T* __restrict__ sponge, long n) {
+__global__ void k_mhd_rhs(Fld<T> F, const T* __restrict__ sponge, long n, int vin) {
   const long q = blockIdx.x * (long)blockDim.x + threadIdx.x;
   if (q >= n) return;
   const int nz = F.nz;
   const int k = (int)(q / F.nxy), c = (int)(q - (long)k * F.nxy);
-  const T vx = tmp_of(F, SPARC_V_X, q), vy = tmp_of(F, SPARC_V_Y, q), vz = tmp_of(F, SPARC_V_Z, q);
+  T vx, vy, vz;
+  if (vin) {
+    vx = F.scr[SPARC_XI_X][q];
+    vy = F.scr[SPARC_XI_Y][q];
+    vz = F.scr[SPARC_XI_Z][q];
+  } else {
+    vx = tmp_of(F, SPARC_V_X, q);
+    vy = tmp_of(F, SPARC_V_Y, q);
+    vz = tmp_of(F, SPARC_V_Z, q);
+  }
   if (k == 0 || k == nz - 1) {
 #pragma unroll
     for (int f = 0; f < SPARC_NFIELDS; ++f) F.scr[f][q] = 0;
@@ -626,9 +631,11 @@ __global__ void k_mhd_rhs(Fld<T> F, const T* __restrict__ sponge, long n) {
   }
   rz = rz - ri * F.W[4][q] * F.g[k];
   rz = rz + mhd_source(F, c, k);
-  F.scr[SPARC_XI_X][q] = vx;
-  F.scr[SPARC_XI_Y][q] = vy;
-  F.scr[SPARC_XI_Z][q] = vz;
+  if (!vin) {
+    F.scr[SPARC_XI_X][q] = vx;
+    F.scr[SPARC_XI_Y][q] = vy;
+    F.scr[SPARC_XI_Z][q] = vz;
+  }
   F.scr[SPARC_V_X][q] = rx;
   F.scr[SPARC_V_Y][q] = ry;
   F.scr[SPARC_V_Z][q] = rz;

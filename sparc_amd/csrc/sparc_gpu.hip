@@ -1467,18 +1467,13 @@ static int run_stage(sparc_ctx* c, Fld<T>& F, int s) {
       return launch_z<T, EPI_STORE>(c, F, ibc, in, s, b, nullptr, 0, out);
     };
     T** M = F.M;
-    {
-      ProfScope ps(c, SPARC_KC_OTHER, 9.0 * (double)N * sizeof(T));
-      hipLaunchKernelGGL((k_mhd_flux<T>), gN, b256, 0, c->stream, F, N);
-      LAUNCHCHK(c);
-    }
-    // div xi (bc=1) with psivz; p, rho
+    // div xi (bc=1) with psivz; p, rho and flux = xi x B0 in one pass
     if ((rc = launch_x<T, XM_DERIV>(c, F.a[0], S(0), beta, F.W[0], dmx, nullptr))) return rc;
     if ((rc = launch_y<T, YM_DERIV>(c, F.a[1], S(1), beta, F.W[1], dmy, nullptr))) return rc;
     if ((rc = dz(F.a[2], S(2), beta, 1, F.W[2]))) return rc;
     if ((rc = pmlb(F.W[2], 0))) return rc;
     {
-      ProfScope ps(c, SPARC_KC_OTHER, 11.0 * (double)N * sizeof(T));
+      ProfScope ps(c, SPARC_KC_OTHER, 14.0 * (double)N * sizeof(T));
       hipLaunchKernelGGL((k_mhd_pr<T>), gN, b256, 0, c->stream, F, N);
       LAUNCHCHK(c);
     }
@@ -1567,14 +1562,20 @@ static int run_stage(sparc_ctx* c, Fld<T>& F, int s) {
       DX.out[f] = F.scr[3 + f];
       DX.spec[f] = (C2<T>*)c->spec[f];
     }
+    // on 256-wide x lines (k_xw256) the damping forward pass also stores the
+    // stage temp v as RHS_xi (every read of the previous RHS_xi is above),
+    // which k_mhd_rhs then reads
+    const bool vin = damp && c->cfg.nx == 256;
     if (damp) {
+      if (vin)
+        for (int f = 0; f < 3; ++f) DX.cp[f] = F.scr[f];
       if ((rc = launch_xm<T, XM_DAMP_FWD>(c, 3, DX, beta, nullptr))) return rc;
       if ((rc = launch_ym<T, YM_DAMP>(c, 3, DX, 0, nullptr))) return rc;
     }
     {
-      ProfScope ps(c, SPARC_KC_OTHER, 25.0 * (double)N * sizeof(T));
+      ProfScope ps(c, SPARC_KC_OTHER, (vin ? 19.0 : 25.0) * (double)N * sizeof(T));
       hipLaunchKernelGGL((k_mhd_rhs<T>), gN, b256, 0, c->stream, FR,
-                         (const T*)c->prof[SPARC_PROF_SPONGE], N);
+                         (const T*)c->prof[SPARC_PROF_SPONGE], N, vin ? 1 : 0);
       LAUNCHCHK(c);
     }
     if (damp && (rc = launch_xm<T, XM_DAMP_INV>(c, 3, DX, 0, nullptr))) return rc;
