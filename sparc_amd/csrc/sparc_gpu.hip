@@ -979,6 +979,25 @@ static int launch_x_rhs_dampinv(sparc_ctx* c, const XArgs<T>& H, const XArgs<T>&
   return SPARC_OK;
 }
 
+// x derivative of X.a[0] and y derivative of Y.a[0] (each of a + beta s,
+// stored or combined into out as X.acc / Y.acc say) in one launch on
+// 256 x 256 planes: the blocks of either part fill the other's tail.
+template <typename T>
+static int launch_xy256(sparc_ctx* c, const XArgs<T>& X, const XArgs<T>& Y, T beta) {
+  const long nrows = (long)c->cfg.ny * c->cfg.nz, nlines = (nrows + 1) / 2;
+  const int nbx = cdiv(nlines, 4 * kXwLpw), nby = (c->cfg.nx / 32) * c->cfg.nz;
+  auto arrays = [](const XArgs<T>& A) {
+    return 2.0 + (A.s[0] ? 1.0 : 0.0) + (A.acc ? 1.0 : 0.0);
+  };
+  ProfScope ps(c, SPARC_KC_XLINES, (arrays(X) + arrays(Y)) * (double)c->N * sizeof(T));
+  hipLaunchKernelGGL((k_xy256_div<T, kXwLpw>), dim3(nbx + nby), dim3(256), 0, c->stream,
+                     (int)nrows, (int)nlines, X, Y, beta, (const T*)c->dmulx,
+                     (const T*)c->dmuly, (const C2<T>*)c->twx, (const C2<T>*)c->twy, c->cfg.nx,
+                     c->cfg.ny, c->cfg.nz, c->nxy, nbx);
+  LAUNCHCHK(c);
+  return SPARC_OK;
+}
+
 template <typename T, int MODE>
 static int launch_x(sparc_ctx* c, const T* a, const T* s, T beta, T* out, const T* mult,
                     C2<T>* spec) {
@@ -1341,13 +1360,7 @@ static int run_stage(sparc_ctx* c, Fld<T>& F, int s) {
       Y.a[0] = F.a[1];
       Y.s[0] = S(1);
       Y.out[0] = F.W[1];
-      const long nrows = (long)c->cfg.ny * c->cfg.nz, nlines = (nrows + 1) / 2;
-      const int nbx = cdiv(nlines, 4 * kXwLpw), nby = (c->cfg.nx / 32) * c->cfg.nz;
-      ProfScope ps(c, SPARC_KC_XLINES, (first ? 4.0 : 6.0) * (double)c->N * sizeof(T));
-      hipLaunchKernelGGL((k_xy256_div<T, kXwLpw>), dim3(nbx + nby), dim3(256), 0, c->stream,
-                         (int)nrows, (int)nlines, X, Y, beta, dmx, dmy, (const C2<T>*)c->twx,
-                         (const C2<T>*)c->twy, c->cfg.nx, c->cfg.ny, c->cfg.nz, c->nxy, nbx);
-      LAUNCHCHK(c);
+      if ((rc = launch_xy256<T>(c, X, Y, beta))) return rc;
     } else {
       if ((rc = launch_x<T, XM_DERIV>(c, F.a[0], S(0), beta, F.W[0], dmx, nullptr))) return rc;
       if ((rc = launch_y<T, YM_DERIV>(c, F.a[1], S(1), beta, F.W[1], dmy, nullptr))) return rc;
@@ -1467,9 +1480,33 @@ static int run_stage(sparc_ctx* c, Fld<T>& F, int s) {
       return launch_z<T, EPI_STORE>(c, F, ibc, in, s, b, nullptr, 0, out);
     };
     T** M = F.M;
+    // on 256 x 256 planes independent x and y derivative passes share a
+    // launch (launch_xy256)
+    const bool xy = c->cfg.nx == 256 && c->cfg.ny == 256;
+    auto xypair = [&](const T* ax, const T* sx, T* ox, T aldx, T goutx, const T* ay,
+                      const T* sy, T* oy, T aldy, T gouty, T b) -> int {
+      XArgs<T> X{}, Y{};
+      X.a[0] = ax;
+      X.s[0] = sx;
+      X.out[0] = ox;
+      X.acc = aldx != (T)0;
+      X.ald = aldx;
+      X.gout = goutx;
+      Y.a[0] = ay;
+      Y.s[0] = sy;
+      Y.out[0] = oy;
+      Y.acc = aldy != (T)0;
+      Y.ald = aldy;
+      Y.gout = gouty;
+      return launch_xy256<T>(c, X, Y, b);
+    };
     // div xi (bc=1) with psivz; p, rho and flux = xi x B0 in one pass
-    if ((rc = launch_x<T, XM_DERIV>(c, F.a[0], S(0), beta, F.W[0], dmx, nullptr))) return rc;
-    if ((rc = launch_y<T, YM_DERIV>(c, F.a[1], S(1), beta, F.W[1], dmy, nullptr))) return rc;
+    if (xy) {
+      if ((rc = xypair(F.a[0], S(0), F.W[0], 0, 0, F.a[1], S(1), F.W[1], 0, 0, beta))) return rc;
+    } else {
+      if ((rc = launch_x<T, XM_DERIV>(c, F.a[0], S(0), beta, F.W[0], dmx, nullptr))) return rc;
+      if ((rc = launch_y<T, YM_DERIV>(c, F.a[1], S(1), beta, F.W[1], dmy, nullptr))) return rc;
+    }
     if ((rc = dz(F.a[2], S(2), beta, 1, F.W[2]))) return rc;
     if ((rc = pmlb(F.W[2], 0))) return rc;
     {
@@ -1478,8 +1515,12 @@ static int run_stage(sparc_ctx* c, Fld<T>& F, int s) {
       LAUNCHCHK(c);
     }
     // grad p (bc=0) with psip
-    if ((rc = launch_x<T, XM_DERIV>(c, F.W[3], nullptr, 0, M[3], dmx, nullptr))) return rc;
-    if ((rc = launch_y<T, YM_DERIV>(c, F.W[3], nullptr, 0, M[4], dmy, nullptr))) return rc;
+    if (xy) {
+      if ((rc = xypair(F.W[3], nullptr, M[3], 0, 0, F.W[3], nullptr, M[4], 0, 0, 0))) return rc;
+    } else {
+      if ((rc = launch_x<T, XM_DERIV>(c, F.W[3], nullptr, 0, M[3], dmx, nullptr))) return rc;
+      if ((rc = launch_y<T, YM_DERIV>(c, F.W[3], nullptr, 0, M[4], dmy, nullptr))) return rc;
+    }
     if ((rc = dz(F.W[3], nullptr, 0, 0, M[5]))) return rc;
     if ((rc = pmlb(M[5], 1))) return rc;
     // b = curl(flux): dx f3, dy f3, dz f2 (psiinductionbx), dx f2, dy f1, dz f1
@@ -1509,7 +1550,27 @@ static int run_stage(sparc_ctx* c, Fld<T>& F, int s) {
       X.gout = gout;
       return launch_ym<T, YM_DERIV>(c, 1, X, 0, dmy);
     };
-    if (bacc) {
+    if (bacc && xy) {
+      // the same passes, the independent x/y pairs in one launch each
+      if ((rc = dz(M[1], nullptr, 0, 1, M[9]))) return rc;
+      if ((rc = pmlb(M[9], 4))) return rc;  // psiinductionbx
+      if ((rc = dz(M[0], nullptr, 0, 1, M[10]))) return rc;
+      if ((rc = pmlb(M[10], 5))) return rc;  // psiinductionby
+      // by = dz f1 - dx f3 (x part), bx = dy f3 - dz f2 (y part)
+      if ((rc = xypair(M[2], nullptr, M[10], -1, 1, M[2], nullptr, M[9], 1, -1, 0))) return rc;
+      if ((rc = launch_x<T, XM_DERIV>(c, M[1], nullptr, 0, M[11], dmx, nullptr))) return rc;
+      if ((rc = yacc(M[0], M[11], -1, 1))) return rc;  // bz = dx f2 - dy f1
+      // J = curl(b) pieces dx by, dy bz, dz bx, dx bz, dy bx, dz by into M0,
+      // M1, M2, M6, M7, M8 (flux no longer needed)
+      if ((rc = xypair(M[10], nullptr, M[0], 0, 0, M[11], nullptr, M[1], 0, 0, 0))) return rc;
+      if ((rc = dz(M[9], nullptr, 0, 1, M[2]))) return rc;
+      if ((rc = pmlb(M[2], 2))) return rc;  // psidzbx
+      if ((rc = xypair(M[11], nullptr, M[6], 0, 0, M[9], nullptr, M[7], 0, 0, 0))) return rc;
+      if ((rc = dz(M[10], nullptr, 0, 1, M[8]))) return rc;
+      if ((rc = pmlb(M[8], 3))) return rc;  // psidzby
+      T* const perm[12] = {M[9], M[10], M[11], M[3], M[4], M[5], M[0], M[1], M[2], M[6], M[7], M[8]};
+      for (int w = 0; w < 12; ++w) FR.M[w] = perm[w];
+    } else if (bacc) {
       if ((rc = dz(M[1], nullptr, 0, 1, M[9]))) return rc;
       if ((rc = pmlb(M[9], 4))) return rc;  // psiinductionbx
       if ((rc = yacc(M[2], M[9], 1, -1))) return rc;   // bx = dy f3 - dz f2
