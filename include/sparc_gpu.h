@@ -80,7 +80,8 @@ extern "C" {
 #define SPARC_ARR_ALO 4      /* low-order part of a(:,:,:,f) when the FP32 state is
                                 compensated (cfg.compensated), as FP64 values;
                                 put rounds them to the 16-bit low words under the
-                                current high words.  get/put of SPARC_ARR_A then
+                                current high words (clamped to +-32767 steps of
+                                ulp/2^16).  get/put of SPARC_ARR_A then
                                 move hi+lo as one FP64 value                    */
 
 /* 3-D background fields (sparc_set_background).  n = nx*ny*nz for a full

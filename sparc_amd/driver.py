@@ -36,7 +36,7 @@ class Params:
     obsheight: float = 0.0         # cm above the photosphere
     precision: int = abi.FP32
     field_scale: float = 1.0
-    compensated: bool = True       # FP32: hi+lo state updated in FP64 arithmetic
+    compensated: bool = True       # FP32: hi + 16-bit lo state updated in FP64 arithmetic
     flow_kernels: bool = False     # snapshots keep v; kernel sums add K_v (kernels.f90:317-332)
     use_graphs: bool = True        # replay each step's stages + update as a captured HIP graph
     plane_damping: bool = False    # option 4, FP32, 256 x 256 planes: one-pass damping (k_damp2d)

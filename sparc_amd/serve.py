@@ -198,8 +198,10 @@ def footprint(spec: PL.SolveSpec, mode: str, checkpoint_every: int = 100) -> int
         extra = (spec.maxtime // spec.steps + 1) * 6 * N * ts
     else:
         nck = spec.maxtime // checkpoint_every + 1
-        nf = 12 if c.compensated else 6
-        extra = nck * (nf * N + 2 * NP) * ts + (checkpoint_every // spec.steps + 2) * 6 * N * ts
+        # a checkpoint slot: the six fields, their 2-byte low words when the
+        # state is compensated, the two PML fields
+        slot = (6 * N + 2 * NP) * ts + (6 * N * 2 if c.compensated else 0)
+        extra = nck * slot + (checkpoint_every // spec.steps + 2) * 6 * N * ts
     return 2 * ctx + extra
 
 
