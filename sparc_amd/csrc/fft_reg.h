@@ -319,11 +319,13 @@ __global__ __launch_bounds__(NT) void k_xlines_n(int nrows, int nlines, XArgs<T>
       } else {
         sv[it] = spec[L * N + x];
         if constexpr (MODE == XM_DAMP_INV) {
+          // the RHS is read from X.rin when set (the result still goes to out)
+          const T* __restrict__ ri = X.rin[fld] ? X.rin[fld] : out;
           const long q = L / ny, j = L - q * ny;
           const long k0 = 2 * q, k1 = k0 + 1;
           const long o0 = x + (long)N * j + nxy * k0;
-          ov0[it] = (X.edges || (k0 != 0 && k0 != nz - 1)) ? out[o0] : (T)0;
-          ov1[it] = (k1 < nz - 1 || (X.edges && k1 < nz)) ? out[o0 + nxy] : (T)0;
+          ov0[it] = (X.edges || (k0 != 0 && k0 != nz - 1)) ? ri[o0] : (T)0;
+          ov1[it] = (k1 < nz - 1 || (X.edges && k1 < nz)) ? ri[o0 + nxy] : (T)0;
         }
       }
     }
@@ -339,6 +341,16 @@ __global__ __launch_bounds__(NT) void k_xlines_n(int nrows, int nlines, XArgs<T>
           v = s ? C2<T>{ra[it] + beta * rs[it], ia[it] + beta * is[it]} : C2<T>{ra[it], ia[it]};
         else
           v = sv[it];
+        if constexpr (MODE == XM_DAMP_FWD) {
+          // the stage temp is also stored to X.cp when set, zero on planes 1 and nz
+          T* __restrict__ cp = X.cp[fld];
+          if (cp) {
+            const long L = L0 + l, q = L / ny, j = L - q * ny, k0 = 2 * q;
+            const long o0 = x + (long)N * j + nxy * k0;
+            cp[o0] = (k0 == 0 || k0 == nz - 1) ? (T)0 : v.x;
+            if (k0 + 1 < nz) cp[o0 + nxy] = (k0 + 1 == nz - 1) ? (T)0 : v.y;
+          }
+        }
       }
       b0[l * LS + lpad(x)] = v;
     }

@@ -41,6 +41,7 @@ struct sparc_ctx {
   hipGraphExec_t gexec[2] = {nullptr, nullptr};
   int device = 0;
   int lds_max = 65536;  // dynamic LDS bytes one block may take
+  int cus = 256;        // compute units of the device
   long N = 0, nxy = 0, NP = 0;
   int nzpml = 0;
   void* a[SPARC_MAXFIELDS] = {};
@@ -436,6 +437,7 @@ extern "C" int sparc_create(const sparc_config* cfg, sparc_ctx** out) {
   HIPCHK(c, hipGetDevice(&c->device));
   HIPCHK(c, hipDeviceGetAttribute(&c->lds_max, hipDeviceAttributeMaxSharedMemoryPerBlock,
                                   c->device));
+  HIPCHK(c, hipDeviceGetAttribute(&c->cus, hipDeviceAttributeMultiprocessorCount, c->device));
   HIPCHK(c, hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
   HIPCHK(c, hipEventCreateWithFlags(&c->xev, hipEventDisableTiming));
   const size_t ts = c->ts, N = c->N;
@@ -1175,7 +1177,10 @@ static bool zstream_fits(const sparc_ctx* c) {
   for (int id : {SPARC_BG_C2RHO0, SPARC_BG_GRADP0_Z, SPARC_BG_RHO0, SPARC_BG_GRADRHO0_Z,
                  SPARC_BG_RHOINV, SPARC_BG_AZ, SPARC_BG_BZPML})
     if (c->bg3d[id] || !c->bg[id]) return false;
-  return c->fp32 && c->nxy % (64 * kZsWaves) == 0 && c->cfg.nz >= 8 &&
+  // one block per 64 * kZsWaves columns: below one block per CU the
+  // segmented column kernel runs faster (128 x 128 x 150: 35 against 65 us)
+  return c->fp32 && c->nxy % (64 * kZsWaves) == 0 && c->nxy / (64 * kZsWaves) >= c->cus &&
+         c->cfg.nz >= 8 &&
          (size_t)c->N * 4 < ((size_t)1 << 31) &&
          zstream_lds_bytes(c->cfg.nz) + kZsRingBytes <= (size_t)c->lds_max;
 }
@@ -1335,10 +1340,10 @@ static int run_stage(sparc_ctx* c, Fld<T>& F, int s) {
     const bool damp = c->cfg.damp_waves && c->damping_set;
     const bool d2d = damp && c->damp2d && sizeof(T) == 4 && c->cfg.nx == 256 &&
                      c->cfg.ny == 256;
-    fuse = damp && !d2d && c->cfg.nx == 256;
+    fuse = damp && !d2d && fast_size(c->cfg.nx);
     pv = fuse && zpv_fits<T>(c);
     ydd = pv && y_damp_deriv_fits<T>(c);
-    chain = ydd && c->cfg.ny == 256;
+    chain = ydd && c->cfg.nx == 256 && c->cfg.ny == 256;
   }
   T* const xi_alt[3] = {F.W[2], F.W[5], F.W[6]};
   auto xi_buf = [&](int stage, int f) -> T* { return (stage & 1) ? xi_alt[f] : F.scr[f]; };
@@ -1370,11 +1375,12 @@ static int run_stage(sparc_ctx* c, Fld<T>& F, int s) {
     // one pass per plane
     const bool d2d = damp && c->damp2d && sizeof(T) == 4 && c->cfg.nx == 256 &&
                      c->cfg.ny == 256;
-    // fuse (damping on 256-wide x lines): the damping forward pass reads the
-    // stage temp v = a + beta scr of the velocities itself and also stores
-    // it as RHS_xi (once the divergence passes above have read the previous
-    // RHS_xi), so the z solve of p below writes only RHS_vz; RHS_vx is
-    // written once, by the pass that also subtracts its damping.
+    // fuse (damping on x lines of a fast length): the damping forward pass
+    // reads the stage temp v = a + beta scr of the velocities itself and
+    // also stores it as RHS_xi (once the divergence passes above have read
+    // the previous RHS_xi), so the z solve of p below writes only RHS_vz; on
+    // 256-wide lines RHS_vx is written once, by the pass that also
+    // subtracts its damping.
     // pv (fuse, FP32 and columns that fit): p, rho' and the z part of grad p
     // in one z-solve launch (k_zstream_pv), RHS_vz staged in W4 for the
     // damping inverse; the damping forward pass then follows it
@@ -1418,7 +1424,8 @@ static int run_stage(sparc_ctx* c, Fld<T>& F, int s) {
     G.rhs = 1;
     G.rz = F.bg3d[SPARC_BG_RHOINV] ? nullptr : F.bg[SPARC_BG_RHOINV];
     G.r3 = F.bg3d[SPARC_BG_RHOINV] ? F.bg[SPARC_BG_RHOINV] : nullptr;
-    if (!fuse) {
+    const bool x256 = fuse && c->cfg.nx == 256;
+    if (!x256) {
       G.out[0] = F.scr[SPARC_V_X];
       if ((rc = launch_xm<T, XM_DERIV>(c, 1, G, 0, dmx))) return rc;
     }
@@ -1428,7 +1435,18 @@ static int run_stage(sparc_ctx* c, Fld<T>& F, int s) {
     } else if ((rc = launch_ym<T, YM_DERIV>(c, 1, G, 0, dmy))) {
       return rc;
     }
-    if (fuse) {
+    if (fuse && !x256) {
+      // DAMP_VELOCITY's inverse x pass on the RHS the passes above wrote
+      // (RHS_vz from W4 when the fused z solve staged it there)
+      XArgs<T> DI{};
+      for (int f = 0; f < 3; ++f) {
+        DI.out[f] = DX.out[f];
+        DI.spec[f] = DX.spec[f];
+      }
+      if (pv) DI.rin[2] = F.W[4];
+      DI.edges = DX.edges;
+      if ((rc = launch_xm<T, XM_DAMP_INV>(c, 3, DI, 0, nullptr))) return rc;
+    } else if (fuse) {
       XArgs<T> H = G;
       H.out[0] = F.scr[SPARC_V_X];
       H.spec[0] = DX.spec[0];
