@@ -270,7 +270,11 @@ __device__ __forceinline__ T rhs_of(const XArgs<T>& X, long k, long o, int nz, T
   return -ri * d;
 }
 
-template <typename T, int N, int MODE, int LPB = XCfg<T, N>::LPB, int NT = XCfg<T, N>::NT>
+// PREP (XM_DAMP_INV only): each line then also does the next stage's
+// XM_DAMP_FWD from the RHS rows it has just written (X.pa, X.pbeta, X.pcp;
+// the spectrum goes over the one the line read).
+template <typename T, int N, int MODE, int LPB = XCfg<T, N>::LPB, int NT = XCfg<T, N>::NT,
+          bool PREP = false>
 __global__ __launch_bounds__(NT) void k_xlines_n(int nrows, int nlines, XArgs<T> X, T beta,
                                                  const T* __restrict__ mult,
                                                  const C2<T>* __restrict__ twg, int ny, int nz,
@@ -290,7 +294,9 @@ __global__ __launch_bounds__(NT) void k_xlines_n(int nrows, int nlines, XArgs<T>
   // Every load is issued before the first use (a + beta s, the LDS store).
   constexpr int IT = (LPB * N + NT - 1) / NT;
   constexpr bool RL = MODE == XM_DERIV || MODE == XM_FILTER || MODE == XM_DAMP_FWD;
+  static_assert(!PREP || MODE == XM_DAMP_INV, "the next stage is prepared by the inverse pass");
   T ov0[MODE == XM_DAMP_INV ? IT : 1], ov1[MODE == XM_DAMP_INV ? IT : 1];
+  T pa0[PREP ? IT : 1], pa1[PREP ? IT : 1];
   T ra[RL ? IT : 1], rs[RL ? IT : 1], ia[RL ? IT : 1], is[RL ? IT : 1];
   C2<T> sv[RL ? 1 : IT];
 #pragma unroll
@@ -324,8 +330,14 @@ __global__ __launch_bounds__(NT) void k_xlines_n(int nrows, int nlines, XArgs<T>
           const long q = L / ny, j = L - q * ny;
           const long k0 = 2 * q, k1 = k0 + 1;
           const long o0 = x + (long)N * j + nxy * k0;
-          ov0[it] = (X.edges || (k0 != 0 && k0 != nz - 1)) ? ri[o0] : (T)0;
-          ov1[it] = (k1 < nz - 1 || (X.edges && k1 < nz)) ? ri[o0 + nxy] : (T)0;
+          // PREP: rows the pass leaves alone keep the RHS held in out
+          ov0[it] = (X.edges || (k0 != 0 && k0 != nz - 1)) ? ri[o0] : (PREP ? out[o0] : (T)0);
+          ov1[it] = (k1 < nz - 1 || (X.edges && k1 < nz)) ? ri[o0 + nxy]
+                                                          : ((PREP && k1 < nz) ? out[o0 + nxy] : (T)0);
+          if constexpr (PREP) {
+            pa0[it] = X.pa[fld][o0];
+            pa1[it] = k1 < nz ? X.pa[fld][o0 + nxy] : (T)0;
+          }
         }
       }
     }
@@ -407,8 +419,27 @@ __global__ __launch_bounds__(NT) void k_xlines_n(int nrows, int nlines, XArgs<T>
       const long k0 = 2 * q, k1 = k0 + 1;
       const long o0 = x + (long)N * j + nxy * k0;
       const C2<T> v = r[l * LS + lpad(x)];
-      if (X.edges || (k0 != 0 && k0 != nz - 1)) out[o0] = ov0[it] - v.x;
-      if (k1 < nz - 1 || (X.edges && k1 < nz)) out[o0 + nxy] = ov1[it] - v.y;
+      const bool w0 = X.edges || (k0 != 0 && k0 != nz - 1);
+      const bool w1 = k1 < nz - 1 || (X.edges && k1 < nz);
+      if (w0) out[o0] = ov0[it] - v.x;
+      if (w1) out[o0 + nxy] = ov1[it] - v.y;
+      if constexpr (PREP) {
+        // the next stage temp pa + pbeta r, as XM_DAMP_FWD forms and stores it
+        const T r0 = w0 ? ov0[it] - v.x : ov0[it], r1 = w1 ? ov1[it] - v.y : ov1[it];
+        const C2<T> u = {pa0[it] + X.pbeta * r0, pa1[it] + X.pbeta * r1};
+        T* __restrict__ cp = X.pcp[fld];
+        cp[o0] = (k0 == 0 || k0 == nz - 1) ? (T)0 : u.x;
+        if (k1 < nz) cp[o0 + nxy] = (k1 == nz - 1) ? (T)0 : u.y;
+        r[l * LS + lpad(x)] = u;
+      }
+    }
+    if constexpr (PREP) {
+      __syncthreads();
+      rfft<T, N, -1, LPB, NT>(b0, tw);
+      for (int e = threadIdx.x; e < nl * N; e += NT) {
+        const int l = e / N, x = e - l * N;
+        spec[(long)L0 * N + e] = r[l * LS + lpad(x)];
+      }
     }
   }
 }

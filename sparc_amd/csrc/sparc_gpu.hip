@@ -862,6 +862,15 @@ static void launch_xn(sparc_ctx* c, int nf, const XArgs<T>& X, T beta, const T* 
     return;
   }
   constexpr int LPB = XCfg<T, N>::LPB;
+  if constexpr (MODE == XM_DAMP_INV) {
+    if (X.pcp[0]) {  // with the next stage's damping forward pass
+      hipLaunchKernelGGL((k_xlines_n<T, N, MODE, LPB, XCfg<T, N>::NT, true>),
+                         dim3(cdiv(nlines, LPB), nf), dim3(XCfg<T, N>::NT), 0, c->stream, nrows,
+                         nlines, X, beta, mult, (const C2<T>*)c->twx, c->cfg.ny, c->cfg.nz,
+                         c->nxy);
+      return;
+    }
+  }
   hipLaunchKernelGGL((k_xlines_n<T, N, MODE, LPB>), dim3(cdiv(nlines, LPB), nf), dim3(XCfg<T, N>::NT), 0,
                      c->stream, nrows, nlines, X, beta, mult, (const C2<T>*)c->twx, c->cfg.ny,
                      c->cfg.nz, c->nxy);
@@ -883,7 +892,7 @@ static int launch_xm(sparc_ctx* c, int nf, const XArgs<T>& X, T beta, const T* m
   if (MODE == XM_DERIV || MODE == XM_FILTER || MODE == XM_DAMP_FWD)
     nb = (X.s[0] ? 2 : 1) + 1 + (X.cp[0] ? 1 : 0);
   else
-    nb = 3;
+    nb = X.pcp[0] ? 6 : 3;  // with the next stage: its a read, RHS_xi and spectrum written
   const int cls = MODE == XM_DAMP_FWD   ? SPARC_KC_DAMP
                   : MODE == XM_DAMP_INV ? SPARC_KC_DAMP_XI
                   : MODE == XM_FILTER   ? SPARC_KC_FILTER
@@ -1328,9 +1337,9 @@ static int run_stage(sparc_ctx* c, Fld<T>& F, int s) {
   const T beta = (T)c->betas[s];
   F.beta = beta;
   const bool first = (s == 0);
-  // Option 4 with the damping on 256 x 256 planes, the fused z solve and the
-  // fused damping y pass: the last pass of stages 0-3 also does the next
-  // stage's damping forward pass (chain). It writes RHS_xi of stage s + 1
+  // Option 4 with the damping on x lines of a fast length and the fused z
+  // solve: the last pass of stages 0-3 also does the next stage's damping
+  // forward pass (chain). It writes RHS_xi of stage s + 1
   // while the divergence passes of stage s + 1 have still to read RHS_xi of
   // stage s, so RHS_xi alternates between scr (even stages) and W2, W5, W6
   // (odd stages, idle in this path); stage 4's, which the update reads,
@@ -1343,7 +1352,7 @@ static int run_stage(sparc_ctx* c, Fld<T>& F, int s) {
     fuse = damp && !d2d && fast_size(c->cfg.nx);
     pv = fuse && zpv_fits<T>(c);
     ydd = pv && y_damp_deriv_fits<T>(c);
-    chain = ydd && c->cfg.nx == 256 && c->cfg.ny == 256;
+    chain = pv;
   }
   T* const xi_alt[3] = {F.W[2], F.W[5], F.W[6]};
   auto xi_buf = [&](int stage, int f) -> T* { return (stage & 1) ? xi_alt[f] : F.scr[f]; };
@@ -1445,6 +1454,14 @@ static int run_stage(sparc_ctx* c, Fld<T>& F, int s) {
       }
       if (pv) DI.rin[2] = F.W[4];
       DI.edges = DX.edges;
+      if (chain && s < 4) {
+        // the next stage's temp v = a + beta' RHS_v, its x spectrum and RHS_xi
+        for (int f = 0; f < 3; ++f) {
+          DI.pa[f] = F.a[3 + f];
+          DI.pcp[f] = xi_buf(s + 1, f);
+        }
+        DI.pbeta = (T)c->betas[s + 1];
+      }
       if ((rc = launch_xm<T, XM_DAMP_INV>(c, 3, DI, 0, nullptr))) return rc;
     } else if (fuse) {
       XArgs<T> H = G;
