@@ -241,17 +241,18 @@ struct XArgs {
   // derivative combined into the output (curl assembly): out = ald d + gout out
   int acc;
   T ald, gout;
-  // damping forward pass of k_xw256: the stage temp a + beta s it transforms
-  // is also stored here (RHS_xi = temp v; zero on planes 1 and nz), when set
+  // damping forward pass (k_xw256, k_xlines_n): the stage temp a + beta s it
+  // transforms is also stored here (RHS_xi = temp v; zero on planes 1 and
+  // nz), when set
   T* cp[3];
-  // damping inverse pass of k_xw256: the RHS is read from here (the result
-  // still goes to out), when set
+  // damping inverse pass (k_xw256, k_xlines_n): the RHS is read from here
+  // (the result still goes to out), when set
   const T* rin[3];
-  // damping inverse passes of k_xw256_rhs_dampinv<..., true>, which also do
-  // the next stage's damping forward pass: with the RHS r each line leaves in
-  // out, the next stage temp pa + pbeta r is stored to pcp (RHS_xi, zero on
-  // planes 1 and nz) and its forward transform to spec, over the spectrum
-  // the line has just read
+  // damping inverse passes that also do the next stage's damping forward
+  // pass (k_xw256_rhs_dampinv<..., true>, k_xlines_n<..., XM_DAMP_INV, ...,
+  // true>): with the RHS r each line leaves in out, the next stage temp
+  // pa + pbeta r is stored to pcp (RHS_xi, zero on planes 1 and nz) and its
+  // forward transform to spec, over the spectrum the line has just read
   const T* pa[3];
   T* pcp[3];
   T pbeta;
