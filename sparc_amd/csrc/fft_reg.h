@@ -456,12 +456,14 @@ constexpr int kYTileBytes = 32768, kYSegBytes = 128, kYDampNt = 512;
 template <typename T, int NY, int MODE>
 struct YCfg {
   // 32 KiB of complex values per block; row segments of 128 bytes at most;
-  // derivative/filter tiles use 512 threads (measured 4.7 vs 3.7 TB/s with 256)
+  // derivative/filter tiles of 3072 values and more use 512 threads (measured
+  // 4.7 vs 3.7 TB/s with 256 on 4096; on 384-point lines 141 vs 190 us, the
+  // 256-thread tile at 138 VGPRs)
   static constexpr int LINES = pow2_floor((kYTileBytes / (int)sizeof(C2<T>)) / NY);
   static constexpr int CAP = kYSegBytes / (int)sizeof(C2<T>);  // complex lines per row segment
   static constexpr int NL = LINES < CAP ? LINES : CAP;
   static constexpr int TX = (MODE == YM_DAMP) ? NL : 2 * NL;
-  static constexpr int NT = MODE == YM_DAMP ? kYDampNt : (NL * NY < 4096 ? 256 : 512);
+  static constexpr int NT = MODE == YM_DAMP ? kYDampNt : (NL * NY < 3072 ? 256 : 512);
 };
 
 template <typename T, int NY, int MODE, int NL = YCfg<T, NY, MODE>::NL,
