@@ -13,6 +13,9 @@ are the ones checked:
   384-wide line kernels (radix 3) of its large grid.
 * 288-wide x lines and 288-long y lines: the generic (LDS) line kernels, with
   option 2's damping of the boundary planes.
+* 64x64x41 and 96x64x40: the fused damping route on the register-free line
+  kernels (odd nz, radix 3), with the next stage's damping forward pass done
+  by the inverse pass in FP32.
 
 Tolerances: FP64 engine <= 1e-10 max-abs relative; FP32 engine <= 1e-5
 relative L2 on the wavefields and <= 1e-4 relative L2 on the kernels (the
@@ -280,6 +283,35 @@ def test_generic_line_lengths_match_oracle(case, prec):
         assert W.params.damp_waves
         for f in range(nf):
             _check(eng.get(abi.ARR_A, f), o.get(abi.ARR_A, f), prec, what=("a", f))
+    finally:
+        eng.close()
+        o.close()
+
+
+# ---------------------------------------------------------------------------
+# the fused damping route on line lengths other than 256
+# ---------------------------------------------------------------------------
+
+@pytest.mark.timeout(300)
+@pytest.mark.parametrize("prec", [abi.FP64, abi.FP32], ids=["fp64", "fp32"])
+@pytest.mark.parametrize("dims", ["64x64x41", "96x64x40"])
+def test_fused_damping_route_generic_lengths_match_oracle(dims, prec):
+    """Option 4 with damping on fast line lengths other than 256: the damping
+    forward pass stores RHS_xi, and in FP32 the segmented column z solve
+    (too few columns for the streaming one) stages RHS_vz and the generic
+    x-line damping inverse does the next stage's forward pass, RHS_xi
+    alternating between two buffers over the five stages; an odd nz (the
+    last plane pair has one plane) and 96-point x lines (radix 3)."""
+    nx, ny, nz = (int(v) for v in dims.split("x"))
+    W = _grid_workload(nx, ny, nz)
+    S = D.setup_for(W)
+    eng, o, nf = _forward_pair_from_state(W, S, prec, 3, 500)
+    try:
+        assert W.params.damp_waves
+        for f in range(nf):
+            _check(eng.get(abi.ARR_A, f), o.get(abi.ARR_A, f), prec, what=("a", f))
+            _check(eng.get(abi.ARR_SCR, f), o.get(abi.ARR_SCR, f), prec, tol32=TOL32_KERN,
+                   what=("scr", f))
     finally:
         eng.close()
         o.close()
