@@ -871,7 +871,10 @@ static void launch_xn(sparc_ctx* c, int nf, const XArgs<T>& X, T beta, const T* 
       return;
     }
   }
-  hipLaunchKernelGGL((k_xlines_n<T, N, MODE, LPB>), dim3(cdiv(nlines, LPB), nf), dim3(XCfg<T, N>::NT), 0,
+  // 384-point derivative lines at 512 threads: 165 against 172 us (the
+  // damping passes keep 256: 637 against 541 us with the next stage)
+  constexpr int NT = (N == 384 && MODE == XM_DERIV) ? 512 : XCfg<T, N>::NT;
+  hipLaunchKernelGGL((k_xlines_n<T, N, MODE, LPB, NT>), dim3(cdiv(nlines, LPB), nf), dim3(NT), 0,
                      c->stream, nrows, nlines, X, beta, mult, (const C2<T>*)c->twx, c->cfg.ny,
                      c->cfg.nz, c->nxy);
 }
