@@ -50,7 +50,7 @@ KERNEL_SYMBOLS = {
     "zsolve_p": "sparc::k_zstream_pv -- streaming z solves, two walks per 64-column group (one "
                 "from each end): d xi_z/dz with the pressure, then dp/dz with RHS_vz "
                 "(without damping or on other line sizes: sparc::k_zsolve_b<float, 1 (EPI_P3D), 16, true>)",
-    "update": "sparc::k_update_filter_c -- a += dt*scr with the z filter (hi+lo state), all fields per launch",
+    "update": "sparc::k_update_filter_c -- a += dt*scr with the z filter (FP32 high words, 16-bit low words), all fields per launch",
 }
 
 

@@ -77,7 +77,7 @@ def test_pipeline_fp64_engine_matches_oracle():
 
 def test_fp32_kernel_matches_fp64(column):
     """Same adjoint source into both precisions (the FP64 solve's); the FP32
-    state is the compensated hi+lo form.  The PML layers are excluded from
+    state is the compensated hi + 16-bit lo form.  The PML layers are excluded from
     the kernel norm: the model is held fixed there."""
     W, S0, rec, maxtime, obs = column
     P = W.params

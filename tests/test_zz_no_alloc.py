@@ -39,7 +39,7 @@ def test_sample_noalloc_matches_sample(prec):
                 got = eng.sample(f, ijk, noalloc=True)
                 assert np.array_equal(got, want), (n, f)
                 a = eng.get(abi.ARR_A, f)
-                if cfg.compensated:  # sampling reads the high FP32 word of the hi+lo state
+                if cfg.compensated:  # sampling reads the high FP32 word of the hi + 16-bit lo state
                     a = a - eng.get(abi.ARR_ALO, f)
                 assert np.array_equal(got, a[ijk[:, 0], ijk[:, 1], ijk[:, 2] - 1])
         too_many = _points(eng.shape, 65537, 4)
