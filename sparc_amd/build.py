@@ -18,7 +18,8 @@ HIPFLAGS = ["--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-shared",
             "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
 SOURCES = [os.path.join(HERE, "csrc", "sparc_gpu.hip")]
 DEPS = SOURCES + [os.path.join(HERE, "csrc", f)
-                  for f in ("kernels.h", "fft_lds.h", "fft_reg.h", "fft_r16.h", "zsolve.h")] + [
+                  for f in ("kernels.h", "fft_lds.h", "fft_reg.h", "fft_r16.h", "zsolve.h",
+                            "lowword.h")] + [
     os.path.join(ROOT, "include", "sparc_gpu.h")]
 
 
